@@ -1,0 +1,154 @@
+"""ctypes wrapper for oracle/_ref/libvbref.so: the reference's own Fortran step, run in place.
+
+TEST INFRASTRUCTURE ONLY.  Used by tests/, by the smoke check and by the CPU-baseline leg of
+bench.py, always as the checker or the timed CPU baseline, never as part of the product path.
+The library holds the reference's module globals, so one process has exactly one reference state.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_ref", "libvbref.so")
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="F_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="F_CONTIGUOUS")
+_lib = None
+
+
+def available() -> bool:
+    return os.path.exists(LIB_PATH)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not available():
+            raise RuntimeError(f"reference harness not built: {LIB_PATH} (run make -C oracle ref)")
+        L = C.CDLL(LIB_PATH)
+        L.vbref_setup.argtypes = [_ip, _dp, C.c_int, _dp, _ip, _dp, _dp, _ip, C.c_int, _dp, _dp, _dp, _dp, _dp]
+        L.vbref_set_particles.argtypes = [C.c_int] + [_dp] * 8 + [_ip]
+        L.vbref_get_particles.argtypes = [C.c_int] + [_dp] * 8 + [_ip]
+        L.vbref_get_scalars.argtypes = [_ip, _dp]
+        L.vbref_phase.argtypes = [C.c_int]
+        L.vbref_get_mirrors.argtypes = [C.c_int, _ip, _ip, _dp]
+        L.vbref_get_neighbours.argtypes = [C.c_int, C.c_int, _ip, _ip]
+        L.vbref_get_weights.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp]
+        L.vbref_get_ushift.argtypes = [C.c_int, _dp]
+        L.vbref_get_grad_p.argtypes = [C.c_int, _dp]
+        L.vbref_bicgstab_nonull.argtypes = [C.c_int, C.c_int, _dp, _ip, _dp, _dp, C.c_int, C.c_double,
+                                            C.POINTER(C.c_int), C.POINTER(C.c_double)]
+        _lib = L
+    return _lib
+
+
+def _f(a, dt=np.float64):
+    return np.asfortranarray(np.asarray(a, dtype=dt))
+
+
+class RefSolver:
+    """The reference step on a Case; state lives in the Fortran module globals."""
+
+    def __init__(self, case, time=0.0, itime=0, state=None):
+        L = lib()
+        self.case = case
+        nb = case.nb_patches
+        nc = case.nb_circles
+        iprm = _f([case.npfb, case.nbS, case.i_open_domain, case.i_shift_coeff,
+                   int(case.npfb * 1.2) + 1000, itime], np.int32)
+        prm = _f(list(case.params_vector()) + [time])
+        one = lambda n, k=None: _f(np.zeros((max(n, 1),) if k is None else (max(n, 1), k)))
+        if nc:
+            cc, cr, co, cv, ct = (_f(case.c_centre), _f(case.c_radius), _f(case.c_omega),
+                                  _f(case.c_vel), _f(case.c_thermal))
+        else:
+            cc, cr, co, cv, ct = one(0, 2), one(0), one(0), one(0, 2), one(0)
+        L.vbref_setup(iprm, prm, nb, _f(case.b_node), _f(case.b_type, np.int32), _f(case.b_vel),
+                      _f(case.b_thermal), _f(case.b_periodic_parent, np.int32), nc, cc, cr, co, cv, ct)
+        n = case.npfb
+        if state is None:
+            z = np.zeros(n)
+            tau = np.zeros((n, 4))
+            state = dict(x=case.x, y=case.y, u=case.u, v=case.v, theta=case.theta, tau=tau,
+                         trace=np.full(n, 2.0), p=z, inbin=np.zeros(n, np.int32))
+        self.set_state(state)
+
+    def set_state(self, s):
+        n = self.case.npfb
+        lib().vbref_set_particles(n, _f(s["x"]), _f(s["y"]), _f(s["u"]), _f(s["v"]), _f(s["theta"]),
+                                  _f(s["tau"]), _f(s["trace"]), _f(s["p"]), _f(s["inbin"], np.int32))
+
+    def get_state(self, n=None):
+        n = self.case.npfb if n is None else n
+        out = {k: np.zeros(n, order="F") for k in ("x", "y", "u", "v", "theta", "trace", "p")}
+        tau = np.zeros((n, 4), order="F")
+        inb = np.zeros(n, np.int32, order="F")
+        lib().vbref_get_particles(n, out["x"], out["y"], out["u"], out["v"], out["theta"], tau,
+                                  out["trace"], out["p"], inb)
+        out["tau"] = tau
+        out["inbin"] = inb
+        return out
+
+    def scalars(self):
+        ints = np.zeros(11, np.int32, order="F")
+        dbls = np.zeros(8, order="F")
+        lib().vbref_get_scalars(ints, dbls)
+        keys_i = ("np", "npfb", "nbS", "nmirror", "maxneighbours", "LS_iters", "itime", "n_inbin",
+                  "ncx", "ncy", "nct")
+        keys_d = ("dt", "LS_residual", "time", "umax", "x_mint", "x_maxt", "y_mint", "y_maxt")
+        d = {k: int(v) for k, v in zip(keys_i, ints)}
+        d.update({k: float(v) for k, v in zip(keys_d, dbls)})
+        return d
+
+    def step(self):
+        lib().vbref_step()
+
+    def phase(self, k):
+        lib().vbref_phase(k)
+
+    def mirrors(self):
+        m = self.scalars()["nmirror"]
+        irel = np.zeros(m, np.int32, order="F"); vrel = np.zeros(m, np.int32, order="F")
+        dp = np.zeros(m, order="F")
+        lib().vbref_get_mirrors(m, irel, vrel, dp)
+        return irel, vrel, dp
+
+    def neighbours(self, w=NotImplemented):
+        n = self.case.npfb
+        from vebisph_amd.cases import NPLINK
+        w = NPLINK if w is NotImplemented else w
+        cnt = np.zeros(n, np.int32, order="F"); link = np.zeros((n, w), np.int32, order="F")
+        lib().vbref_get_neighbours(n, w, cnt, link)
+        return cnt, link
+
+    def weights(self, w):
+        n = self.case.npfb
+        wg = np.zeros((n, w, 2), order="F"); wl = np.zeros((n, w), order="F")
+        kg = np.zeros((n, 4), order="F")
+        lib().vbref_get_weights(n, w, wg, wl, kg)
+        return wg, wl, kg
+
+    def ushift(self):
+        n = self.case.npfb
+        us = np.zeros((n, 2), order="F")
+        lib().vbref_get_ushift(n, us)
+        return us
+
+    def grad_p(self):
+        n = self.case.npfb
+        gp = np.zeros((n, 2), order="F")
+        lib().vbref_get_grad_p(n, gp)
+        return gp
+
+
+def bicgstab_nonull(a, ija, b, x0=None, itermax=1000, tol=1e-8):
+    """Reference SPARSE_PBICGSTAB_nonull on a 1-based row-indexed (Numerical Recipes) matrix."""
+    n = b.shape[0]
+    x = _f(np.zeros(n) if x0 is None else x0)
+    it = C.c_int(0); res = C.c_double(0.0)
+    lib().vbref_bicgstab_nonull(n, a.shape[0], _f(a), _f(ija, np.int32), x, _f(b), itermax, tol,
+                                C.byref(it), C.byref(res))
+    return x, it.value, res.value
