@@ -1,0 +1,160 @@
+/* vebisph_gpu.h — C ABI of the MI355X (gfx950) implementation of one VEBISPH time step.
+ *
+ * The reference (jrcking/vebisph, Fortran 90 + OpenMP) runs one step as div_free
+ * (source/base/div_free.F90:22-124), a sequence of argument-less subroutines working on module
+ * globals.  Each entry point below replaces one of those calls; a context owns the particle state in
+ * HBM between calls, so a Fortran driver keeps the reference's call sequence and calls these through
+ * iso_c_binding (see vebisph_amd/fortran/vebisph_gpu.F90).
+ *
+ * Conventions
+ *   - plain C types only: double, float, int32_t, pointers and sizes; no framework types;
+ *   - host arrays are per-component columns, i.e. the reference's column-major (npar,2) arrays are
+ *     passed as two pointers r(:,1), r(:,2); tau_p(npar,2,2) as four: tau(:,1,1), (:,1,2), (:,2,1),
+ *     (:,2,2);
+ *   - particle indices handed out (neighbour lists, mirror parents) are 1-based, like the
+ *     reference's; neighbour lists are slot-major ("ij_link(npfb,nplink)", leading dimension ld);
+ *   - every call returns a vb_status; VB_OK is 0.  vb_last_error() gives a message;
+ *   - one host thread per context; all work goes to the context's HIP stream, and a call returns
+ *     once its kernels are enqueued, except where it hands back host data (then it synchronises).
+ */
+#ifndef VEBISPH_GPU_H
+#define VEBISPH_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  VB_OK = 0,
+  VB_ERR_OUT_OF_GRID = 1,        /* particle outside the cell grid: neighbour_finding_mod.F90:82-88 STOP */
+  VB_ERR_CELL_OVERFLOW = 2,      /* more than nplink particles in a cell: neighbour_finding_mod.F90:93-103 STOP */
+  VB_ERR_ALL_BINNED = 3,         /* every fluid particle in the bin: put_in_bin.F90:51-54 STOP */
+  VB_ERR_CAPACITY = 4,           /* mirror (2*npfb) or neighbour (nplink) capacity exceeded */
+  VB_WARN_BICGSTAB_BREAKDOWN = 5,/* rho == 0: linear_solver.F90:332-335 prints and returns */
+  VB_WARN_MAXITER = 6,           /* itermax reached without convergence (ppe_...:144-149) */
+  VB_ERR_HIP = 7,                /* a HIP runtime call failed */
+  VB_ERR_ARG = 8,                /* invalid argument */
+} vb_status;
+
+enum { VB_NPLINK = 75 };         /* common_parameter.F90:5, neighbour / cell cap */
+enum { VB_MAX_PATCHES = 16, VB_MAX_CIRCLES = 8 };
+
+typedef enum { VB_PREC_FP64 = 0, VB_PREC_MIXED = 1 } vb_precision;
+
+/* Run parameters: the INDAT values of input.F90:18-37 that the step uses. */
+typedef struct {
+  double h, dx, dv;              /* smoothing length, particle spacing, particle volume (dx*dx) */
+  double Ra, Pr, El, beta;       /* dimensionless groups; Wi = El*sqrt(Ra/Pr) */
+  double body_force[2];
+  double dt_coef_cfl, dt_coef_visc;
+  double ve_nonlinearity;        /* eps of linear PTT; 0 = Oldroyd-B */
+  double time;                   /* start time */
+  int32_t itime;                 /* steps already taken */
+  int32_t i_open_domain;         /* must be 0 (closed domain); the open-domain branch is not built */
+  int32_t i_shift_coeff;         /* must be 0 (D = 0.25 h^2) */
+  int32_t precision;             /* vb_precision */
+  int32_t reference_kick;        /* 1: reproduce predictor_mod.F90:369-371 as the reference runs it */
+} vb_params;
+
+/* Boundary patches and circles: IBOUND, input.F90:51-81.  b_node is (nb_patches,2) row-major;
+ * b_edge(ib) = b_node(ib+1) - b_node(ib).  Circle arrays are (nb_circles[,2]) row-major. */
+typedef struct {
+  int32_t nb_patches;
+  const double* b_node;
+  const int32_t* b_type;         /* 0 none, 1 wall, 2 periodic (3/4 inflow/outflow unsupported) */
+  const double* b_vel;
+  const double* b_thermal;
+  const int32_t* b_periodic_parent; /* 1-based */
+  int32_t nb_circles;
+  const double* c_centre;
+  const double* c_radius;        /* > 0: solid disc; < 0: fluid inside */
+  const double* c_omega;
+  const double* c_vel;
+  const double* c_thermal;
+} vb_boundary;
+
+typedef struct vb_ctx vb_ctx;
+
+/* Context: replaces the module globals of common_2d.F90:20-80 and the allocations of
+ * input.F90:105-114.  capacity rows = npfb + mirror capacity (2*npfb, 10*npfb when npfb <= 500,
+ * mirror_boundaries_mod.F90:53-54). */
+vb_status vb_create(const vb_params* prm, const vb_boundary* bnd, int32_t npfb, int32_t nbS, vb_ctx** out);
+vb_status vb_destroy(vb_ctx* ctx);
+const char* vb_last_error(const vb_ctx* ctx);
+
+/* State transfer (explicit; never done inside a step).  Rows 1..n of the reference arrays r, u,
+ * theta, tau_p, trace_conf, p, inbin (common_2d.F90:49-61).  download may ask for n up to np to
+ * include the mirror rows of the last step. */
+vb_status vb_upload_state(vb_ctx* ctx, int32_t n, const double* x, const double* y, const double* u,
+                          const double* v, const double* theta, const double* tau11, const double* tau12,
+                          const double* tau21, const double* tau22, const double* trace_conf,
+                          const double* p, const int32_t* inbin);
+vb_status vb_download_state(vb_ctx* ctx, int32_t n, double* x, double* y, double* u, double* v,
+                            double* theta, double* tau11, double* tau12, double* tau21, double* tau22,
+                            double* trace_conf, double* p, int32_t* inbin);
+
+/* ---- one time step, phase by phase (the call sequence of div_free.F90:38-122) ---- */
+/* div_free.F90:39-53 with set_time_step (:126-146): r0,u0 copies, dt, r = r0 + u0 dt */
+vb_status vb_set_time_step(vb_ctx* ctx);
+/* put_in_bin.F90:1-56 (x_mint.. are the padded box of sph2D_incom.F90:64-67, held by the context) */
+vb_status vb_put_in_bin(vb_ctx* ctx);
+/* create_mirror_particles, mirror_boundaries_mod.F90:18-334 */
+vb_status vb_create_mirror_particles(vb_ctx* ctx);
+/* particle_allocation: divide + neighboring_list_parallel, neighbour_finding_mod.F90:15-127,261-365 */
+vb_status vb_particle_allocation(vb_ctx* ctx);
+/* kernel_calculation.F90:2-75 */
+vb_status vb_kernel_calculation(vb_ctx* ctx);
+/* locate_free_surf_part, closed-domain branch, freesurf_mod.F90:39-45 */
+vb_status vb_locate_free_surf_part(vb_ctx* ctx);
+/* new_fick_shift.F90:15-92,117-210 and ushift, div_free.F90:65-78 */
+vb_status vb_new_fick_shift(vb_ctx* ctx);
+/* prediction_step, predictor_mod.F90:27-139 (log-conformation :141-331, mirror stress
+ * mirror_boundaries_mod.F90:586-653) */
+vb_status vb_prediction_step(vb_ctx* ctx);
+/* mirror_velocities, mirror_boundaries_mod.F90:509-584 */
+vb_status vb_mirror_velocities(vb_ctx* ctx);
+/* ppe_solve_in_house_simple_dirichlet.F90:1-158 with SPARSE_PBICGSTAB_nonull
+ * (linear_solver.F90:275-425).  tol <= 0 / itermax <= 0 select the reference's 1e-8 / 1000.
+ * iters and residual may be NULL; when given, the call synchronises to return them. */
+vb_status vb_ppe_solve(vb_ctx* ctx, double tol, int32_t itermax, int32_t* iters, double* residual);
+/* calc_pressure_gradient (gradient_calculation.F90:16-48) and div_free.F90:93-102 */
+vb_status vb_pressure_update(vb_ctx* ctx);
+/* div_free.F90:111-117 final advection; advances time */
+vb_status vb_final_advection(vb_ctx* ctx);
+/* the whole step: the calls above in order (one synchronisation for the step's scalars) */
+vb_status vb_div_free(vb_ctx* ctx);
+
+/* ---- scalars and exports (for the host driver and parity checks; these synchronise) ---- */
+/* ints: np, npfb, nbS, nmirror, maxneighbours, LS_iters, itime, n_inbin, ncx, ncy, nct, status
+ * dbls: dt, LS_residual, time, umax, x_mint, x_maxt, y_mint, y_maxt */
+vb_status vb_get_scalars(vb_ctx* ctx, int32_t* ints12, double* dbls8);
+/* irelation, vrelation, dP_mp of the m mirrors (rows npfb+1..np), 1-based parents */
+vb_status vb_export_mirrors(vb_ctx* ctx, int32_t m, int32_t* irelation, int32_t* vrelation, double* dP_mp);
+/* ij_count(1..n) and ij_link(1..n, 1..w), slot-major with leading dimension n, 1-based, 0-padded */
+vb_status vb_export_neighbours(vb_ctx* ctx, int32_t n, int32_t w, int32_t* ij_count, int32_t* ij_link);
+/* ij_w_G(n,w,2), ij_w_L(n,w) slot-major; kgcm as four columns K11,K12,K21,K22 (n each) */
+vb_status vb_export_weights(vb_ctx* ctx, int32_t n, int32_t w, double* w_G, double* w_L, double* kgcm);
+vb_status vb_export_ushift(vb_ctx* ctx, int32_t n, double* ushift);      /* (n,2) column-major */
+vb_status vb_export_grad_p(vb_ctx* ctx, int32_t n, double* grad_p);      /* (n,2) column-major */
+/* PPE before the solve, ELL form: diag(n), col(n,w)/val(n,w) slot-major (1-based columns; slots
+ * beyond ij_count are 0), rhs(n).  In the reference's Numerical-Recipes storage row i holds
+ * A(i)=diag and, at ija(i)+k-1, the k-th slot. */
+vb_status vb_export_ppe(vb_ctx* ctx, int32_t n, int32_t w, double* diag, int32_t* col, double* val, double* rhs);
+
+/* ---- stand-alone device kernels on caller-owned device memory (stream 0) ---- */
+/* nnf_models_mod.F90:8-110 (no call sites in the reference): model 0 newtonian (copies mu_or_sr),
+ * 1 power law, 2 Bingham, 3 Cross, 4 Carreau; d_sr and d_mu_eff are device pointers of n doubles */
+vb_status vb_nnf(int32_t model, int32_t n, const double* d_sr, double mu, double* d_mu_eff);
+/* fac and Wab of sphtools.F90:11-76 on n device values */
+vb_status vb_kernel_functions(int32_t n, const double* d_q, double ad_7, double ad_7h, double* d_fac, double* d_wab);
+/* log-conformation update of n particles (predictor_mod.F90:165-245): tau, grad_vel as (n,4)
+ * row-major device arrays, trace in/out; kick selects the reference's B perturbation */
+vb_status vb_logconf(int32_t n, const double* d_tau, const double* d_trace, const double* d_gradvel,
+                     double dt, double wi, double eps_ptt, int32_t kick, double* d_tau_out, double* d_trace_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -50,6 +50,7 @@ contains
     real(rkind) :: p_dist, tmp1
 
     call vbref_free()
+    call vbref_set_threads(1)
 
     h = prm(1); dx = prm(2); dv = prm(3); Ra = prm(4); Pr = prm(5); El = prm(6); beta = prm(7)
     body_force(1) = prm(8); body_force(2) = prm(9)
@@ -142,6 +143,15 @@ contains
     nsheet_m = nct
     next_dump = 1.0d300; next_dump2 = 1.0d300
   end subroutine vbref_setup
+
+  !! OpenMP thread count of the reference loops.  1 (set by vbref_setup) makes the reference
+  !! deterministic: its reductions then sum in index order and its random_number draws
+  !! (predictor_mod.F90:370, inside a parallel loop) go to particles in index order.
+  subroutine vbref_set_threads(n) bind(C, name="vbref_set_threads")
+    use omp_lib
+    integer(c_int), value :: n
+    call omp_set_num_threads(n)
+  end subroutine vbref_set_threads
 
   subroutine vbref_free() bind(C, name="vbref_free")
     if(allocated(r)) deallocate(r)

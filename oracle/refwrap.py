@@ -34,6 +34,7 @@ def lib():
         L.vbref_get_particles.argtypes = [C.c_int] + [_dp] * 8 + [_ip]
         L.vbref_get_scalars.argtypes = [_ip, _dp]
         L.vbref_phase.argtypes = [C.c_int]
+        L.vbref_set_threads.argtypes = [C.c_int]
         L.vbref_get_mirrors.argtypes = [C.c_int, _ip, _ip, _dp]
         L.vbref_get_neighbours.argtypes = [C.c_int, C.c_int, _ip, _ip]
         L.vbref_get_weights.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp]
@@ -102,6 +103,10 @@ class RefSolver:
         d = {k: int(v) for k, v in zip(keys_i, ints)}
         d.update({k: float(v) for k, v in zip(keys_d, dbls)})
         return d
+
+    def set_threads(self, n):
+        """OpenMP threads of the reference loops (1 after construction: deterministic)."""
+        lib().vbref_set_threads(int(n))
 
     def step(self):
         lib().vbref_step()

@@ -1,0 +1,36 @@
+import os
+import subprocess
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP path)")
+    config.addinivalue_line("markers", "slow: long-running CPU test")
+    # The CPU checkers are test infrastructure; build them only when they are missing.
+    oracle_so = os.path.join(ROOT, "oracle", "liboracle.so")
+    if not os.path.exists(oracle_so):
+        subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "oracle"], check=False,
+                       stdout=subprocess.DEVNULL)
+    ref_so = os.path.join(ROOT, "oracle", "_ref", "libvbref.so")
+    if not os.path.exists(ref_so) and os.path.isdir("/root/reference/source"):
+        subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "ref"], check=False,
+                       stdout=subprocess.DEVNULL)
+
+
+def have_ref():
+    from oracle import refwrap
+    return refwrap.available()
+
+
+@pytest.fixture(scope="session")
+def gpu_lib():
+    """The HIP library; GPU tests fail (not skip) when it is missing or no device is visible."""
+    from vebisph_amd import _lib
+    L = _lib.lib()
+    return L

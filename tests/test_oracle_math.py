@@ -1,0 +1,238 @@
+"""Pins of the oracle that do not depend on the reference binary: closed forms, invariants,
+brute force and textbook routines, each chosen so a plausible slip (a wrong coefficient, sign,
+index or operand order) fails."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oraclewrap as ow
+from vebisph_amd import cases
+
+
+@pytest.fixture(scope="module")
+def c1_after_ppe():
+    """Poiseuille C1 after two whole steps and the phases of a third up to the PPE assembly."""
+    c = cases.make("poiseuille_2500")
+    O = ow.OracleSolver(c)
+    O.step(); O.step()
+    for k in range(1, 11):
+        O.phase(k)
+    return c, O
+
+
+def _all_pairs(x, y, npfb, inbin, sup2):
+    pairs = set()
+    for i in range(npfb):
+        if inbin[i]:
+            continue
+        rx = x[i] - x
+        ry = y[i] - y
+        rr2 = rx * rx + ry * ry
+        js = np.nonzero((rr2 > 0.0) & (rr2 <= sup2))[0]
+        pairs.update((i, int(j)) for j in js)
+    return pairs
+
+
+def test_pair_set_equals_brute_force(c1_after_ppe):
+    c, O = c1_after_ppe
+    s = O.scalars()
+    st = O.get_state(s["np"])
+    cnt, link = O.neighbours()
+    got = {(i, int(link[i, k]) - 1) for i in range(c.npfb) for k in range(cnt[i])}
+    sup2 = 9.0 * (c.h * c.h)
+    want = _all_pairs(st["x"], st["y"], c.npfb, np.r_[st["inbin"], np.zeros(s["np"] - c.npfb, int)], sup2)
+    assert got == want
+    assert len(got) == int(cnt.sum())        # no duplicates in any list
+
+
+def test_binning_is_stable_counting_sort(c1_after_ppe):
+    c, O = c1_after_ppe
+    cp, ist, ip = O.cells()
+    order = np.argsort(cp, kind="stable") + 1
+    assert np.array_equal(ip, order)
+    counts = np.bincount(cp, minlength=ist.shape[0])[1:]
+    assert np.array_equal(np.diff(ist), counts[: ist.shape[0] - 1])
+    assert ist[0] == 1
+
+
+def test_kernel_closed_forms():
+    h = 0.013
+    ad7 = 7.0 / 478.0 / math.pi / h / h
+    ad7h = ad7 / h
+    q = np.linspace(0.0, 3.5, 70001)
+    f, w = ow.fac_wab(q, ad7, ad7h)
+    assert w[0] == pytest.approx(66.0 * ad7, rel=1e-15)          # 3^5 - 6 2^5 + 15
+    assert np.all(w[q >= 3.0] == 0.0) and np.all(f[q >= 3.0] == 0.0)
+    # normalisation: int W dA = 1 over the support disc of radius 3h
+    r = q * h
+    integral = np.trapezoid(w * 2.0 * math.pi * r, r)
+    assert integral == pytest.approx(1.0, rel=1e-8)
+    # fac is dW/dr (dW/dq / h): compare with a central difference
+    dw = np.gradient(w, r)
+    inner = (q > 0.05) & (q < 2.95) & (np.abs(q - 1.0) > 1e-3) & (np.abs(q - 2.0) > 1e-3)
+    assert np.allclose(f[inner], dw[inner], rtol=1e-5, atol=1e-6 * np.max(np.abs(f)))
+    # continuity at the branch points q = 1, 2
+    for qb in (1.0, 2.0):
+        fl, wl = ow.fac_wab(np.array([qb - 1e-12, qb]), ad7, ad7h)
+        assert wl[0] == pytest.approx(wl[1], rel=1e-9) and fl[0] == pytest.approx(fl[1], rel=1e-9)
+
+
+def test_kgc_makes_linear_gradients_exact(c1_after_ppe):
+    c, O = c1_after_ppe
+    s = O.scalars()
+    st = O.get_state(s["np"])
+    cnt, link = O.neighbours()
+    w = s["maxneighbours"]
+    wg, wl, kg = O.weights(w)
+    a, b0, b1 = 0.3, -1.7, 2.9
+    phi = a + b0 * st["x"] + b1 * st["y"]
+    for i in range(0, c.npfb, 7):
+        js = link[i, :cnt[i]] - 1
+        g = ((phi[js] - phi[i])[:, None] * wg[i, :cnt[i], :]).sum(axis=0)
+        K = kg[i].reshape(2, 2)
+        grad = K @ g
+        assert grad == pytest.approx([b0, b1], rel=1e-9, abs=1e-9)
+
+
+def test_laplacian_weight_closed_form(c1_after_ppe):
+    """w_L = 2 rij.w_G / (r^2 + 0.01f h^2) with 0.01 a single-precision literal."""
+    c, O = c1_after_ppe
+    s = O.scalars()
+    st = O.get_state(s["np"])
+    cnt, link = O.neighbours()
+    wg, wl, kg = O.weights(s["maxneighbours"])
+    eta = float(np.float32(0.01)) * c.h * c.h
+    for i in range(0, c.npfb, 11):
+        js = link[i, :cnt[i]] - 1
+        rx = st["x"][i] - st["x"][js]; ry = st["y"][i] - st["y"][js]
+        want = 2.0 * (rx * wg[i, :cnt[i], 0] + ry * wg[i, :cnt[i], 1]) / (rx * rx + ry * ry + eta)
+        assert np.allclose(wl[i, :cnt[i]], want, rtol=1e-14, atol=0)
+
+
+def test_eigens_closed_form():
+    rng = np.random.default_rng(5)
+    a11 = rng.uniform(0.5, 3.0, 200); a22 = rng.uniform(0.5, 3.0, 200); a12 = rng.uniform(-1, 1, 200)
+    A = np.stack([a11, a12, a12, a22], axis=1)
+    lam, R = ow.calc_eigens(A)
+    for k in range(200):
+        Rm = R[k].reshape(2, 2)
+        assert np.allclose(Rm @ Rm.T, np.eye(2), atol=1e-14)
+        assert np.allclose(Rm @ np.diag(lam[k]) @ Rm.T, A[k].reshape(2, 2), atol=1e-13)
+        assert lam[k, 0] >= lam[k, 1]
+    lam, R = ow.calc_eigens(np.array([[2.0, 0.0, 0.0, 3.0]]))      # diagonal branch keeps order
+    assert np.array_equal(lam[0], [2.0, 3.0]) and np.array_equal(R[0], [1.0, 0.0, 0.0, 1.0])
+
+
+def test_logconf_pure_relaxation():
+    """grad u = 0, eps = 0: A_{n+1} = I + (A_n - I) exp(-dt/Wi) exactly (analytic integration)."""
+    rng = np.random.default_rng(1)
+    n = 64
+    tau = np.zeros((n, 4))
+    tau[:, 0] = rng.uniform(-0.3, 0.5, n); tau[:, 3] = rng.uniform(-0.3, 0.5, n)
+    tau[:, 1] = tau[:, 2] = rng.uniform(-0.2, 0.2, n)
+    dt, wi = 1e-3, 0.7
+    out, tr = ow.logconf(tau, 2.0 + tau[:, 0] + tau[:, 3], np.zeros((n, 4)), dt, wi)
+    # the eigenvector formula (L1 - A22, A21) loses digits when A21 is small, ~1e-13 absolute here
+    assert np.allclose(out, tau * math.exp(-dt / wi), rtol=0, atol=1e-11)
+    assert np.allclose(tr, 2.0 + (tau[:, 0] + tau[:, 3]) * math.exp(-dt / wi), rtol=1e-12)
+
+
+def test_logconf_simple_shear_steady_state():
+    """Oldroyd-B in steady simple shear: A_xy = Wi g, A_xx = 1 + 2 (Wi g)^2, A_yy = 1."""
+    wi, gd = 0.5, 0.8
+    G = np.array([[0.0, gd, 0.0, 0.0]])
+    # start off A = I: with A exactly I the decomposition keeps only diag(grad u), so A would never
+    # leave I (the reference relies on its B perturbation to get started)
+    tau = np.array([[0.01, 0.005, 0.005, 0.0]]); tr = np.array([2.01])
+    dt = 2e-3 * wi
+    for _ in range(6000):
+        tau, tr = ow.logconf(tau, tr, G, dt, wi)
+    A = tau[0] + np.array([1.0, 0.0, 0.0, 1.0])
+    assert A[1] == pytest.approx(wi * gd, rel=3e-3)
+    assert A[0] == pytest.approx(1.0 + 2.0 * (wi * gd) ** 2, rel=3e-3)
+    assert A[3] == pytest.approx(1.0, abs=3e-3)
+
+
+def test_ppe_row_sums_vanish(c1_after_ppe):
+    """Closed domain: constants are the null space, so every assembled row sums to zero."""
+    c, O = c1_after_ppe
+    a, ija, b = O.ppe()
+    n = c.npfb
+    for i in range(n):
+        row = a[ija[i] - 1: ija[i + 1] - 1]
+        assert abs(a[i] + row.sum()) <= 1e-11 * abs(a[i])
+    assert abs(b.sum()) <= 1e-9 * np.abs(b).sum()     # the mean was removed before the wall terms
+
+
+def test_bicgstab_against_dense_solve(c1_after_ppe):
+    c, O = c1_after_ppe
+    a, ija, b = O.ppe()
+    n = c.npfb
+    A = np.zeros((n, n))
+    A[np.arange(n), np.arange(n)] = a[:n]
+    for i in range(n):
+        for k in range(ija[i], ija[i + 1]):
+            A[i, ija[k - 1] - 1] += a[k - 1]
+    x, it, res, st = ow.bicgstab_nonull(a, ija, b)
+    assert st == 0 and res <= 1e-8 and 1 < it < 1000
+    # With A 1 = 0 the system is singular and b need not lie in range(A): the DC projections of p and
+    # s make the method converge to A x = b - c 1, the residual left in the constant direction.
+    # Dense answer: least squares over [A | 1], i.e. min |A x + c 1 - b|, then the mean removed.
+    sol = np.linalg.lstsq(np.c_[A, np.ones(n)], b, rcond=None)[0]
+    xd, cd = sol[:n] - sol[:n].mean(), sol[n]
+    rt = b - A @ x
+    assert np.linalg.norm(rt - rt.mean()) <= 1e-7 * np.linalg.norm(b)
+    assert rt.mean() == pytest.approx(cd, rel=1e-4)
+    assert np.linalg.norm(x - x.mean() - xd) <= 1e-5 * np.linalg.norm(xd)
+    x2, it2, res2, st2 = ow.bicgstab_nonull(a, ija, b, tol=1e-10, itermax=5000)
+    assert np.linalg.norm(x2 - x2.mean() - xd) <= 1e-7 * np.linalg.norm(xd)
+    # the SpMV in row-indexed storage equals the dense product
+    v = np.random.default_rng(2).standard_normal(n)
+    assert np.allclose(ow.spmv_nr(a, ija, v), A @ v, rtol=1e-12, atol=1e-12 * np.abs(A).max())
+
+
+def test_mirror_geometry(c1_after_ppe):
+    """Wall mirrors are reflections through the wall, periodic mirrors are translates by L, and
+    dP_mp = body_force . (r_mirror - r_parent)."""
+    c, O = c1_after_ppe
+    s = O.scalars()
+    st = O.get_state(s["np"])
+    irel, vrel, dp = O.mirrors()
+    par = irel - 1
+    k = np.arange(c.npfb, s["np"])
+    x, y = st["x"], st["y"]
+    bot, top = vrel == 1, vrel == 3
+    assert np.allclose(y[k[bot]], -y[par[bot]], atol=1e-15) and np.allclose(x[k[bot]], x[par[bot]], atol=1e-15)
+    assert np.allclose(y[k[top]], 2.0 - y[par[top]], atol=1e-15)
+    per = (vrel == 2) | (vrel == 4)
+    assert np.allclose(np.abs(x[k[per]] - x[par[per]]), 1.0, atol=1e-14)
+    assert np.allclose(y[k[per]], y[par[per]], atol=1e-14)
+    wall = bot | top
+    assert np.allclose(dp[wall], 8.0 * (x[k[wall]] - x[par[wall]]), rtol=1e-14, atol=1e-15)
+    assert np.all(dp[per] == 0.0)                     # pressure itself is periodic
+
+
+def test_kick_stream_is_minstd():
+    """The per-particle perturbation uses Park-Miller minstd (a = 48271): its 10000th draw from the
+    default state is 399268537 (the C++ standard's check value for minstd_rand)."""
+    x = 1
+    for _ in range(10000):
+        x = (x * 48271) % 2147483647
+    assert x == 399268537
+    # first real(8) value the reference's runtime returns after random_seed() (observed: 2.2478015192228806e-05)
+    a = 48271; b = (a * 48271) % 2147483647
+    f = ((a << 30) | ((b - 1) & ((1 << 30) - 1))) >> 7
+    assert math.ldexp(float(f), -54) == 2.2478015192228806e-05
+
+
+def test_nnf_closed_forms():
+    sr = np.array([0.0, 0.5, 3.0, 1e4])
+    mu = 2.0
+    assert np.array_equal(ow.nnf(0, sr, mu), sr)
+    assert np.allclose(ow.nnf(1, sr, mu), mu * np.sqrt(sr + 1.0), rtol=1e-15)
+    bing = ow.nnf(2, sr, mu)
+    assert bing[0] == mu * 100.0 and bing[3] == pytest.approx(mu + 1e-2 / 1e4, rel=1e-15)
+    assert ow.nnf(3, sr, mu)[0] == mu and ow.nnf(4, sr, mu)[0] == pytest.approx(mu, rel=1e-15)
+    cr = ow.nnf(3, sr, mu)
+    assert cr[3] == pytest.approx(0.05 * mu + 0.95 * mu / (1.0 + 1000.0), rel=1e-14)

@@ -1,0 +1,79 @@
+// Host-side context of the gfx950 step: device buffers (capacity-sized, allocated once), the
+// stream, and the launchers each phase file provides.
+#pragma once
+
+#include <string>
+
+#include "vb_common.h"
+
+struct vb_ctx {
+  vb_params prm;
+  vb::Geom g;
+  int npfb, nbS, cap, mcap;          // rows: cap = npfb + mcap, mcap = mirror capacity
+  int itime;
+  double time, dt_visc;
+  hipStream_t stream;
+  std::string err;
+
+  vb::State st;                      // current particle arrays
+  double *u_alt, *v_alt, *th_alt;    // prediction writes u*, v*, theta* here; then swapped in
+  int* nsurf;
+
+  // mirrors: segment compaction scratch
+  int* seg_counts;                   // per-block counts / offsets
+  int seg_blocks;
+
+  // cells and neighbours
+  int *cellkey, *cellslot, *nc, *ist, *ip;
+  int *cnt, *link;                   // link[k*npfb + i], 0-based
+  double *wg1, *wg2, *wl;            // slot-major like link
+  double *k11, *k12, *k21, *k22;
+  float *wg1f, *wg2f, *wlf;          // FP32 copies for the mixed-precision sums
+  double *usx, *usy;
+  double* kick;                      // per-particle B perturbation (reference_kick)
+  double *gpx, *gpy;
+
+  // PPE (ELL, slot-major) and BiCGSTAB vectors
+  double *a_diag, *a_val, *rhs;
+  int* a_col;
+  float *a_valf, *a_diagf;           // FP32 matrix for the mixed inner solve
+  double *r, *rb, *vv, *pp, *pn, *pinv, *ss, *s2, *sinv, *tt;
+  float* fbuf;                       // FP32 work vectors of the inner solve (10 * npfb)
+
+  double* partials;                  // reduction partials (4 * kMaxBlocks)
+  vb::Scal* scal;                    // device scalars
+  vb::Scal* scal_host;               // pinned mirror
+  int nmirror_host, np_host;
+};
+
+namespace vb {
+// phase launchers (vb_step.hip)
+vb_status launch_set_time_step(vb_ctx* c);
+vb_status launch_put_in_bin(vb_ctx* c);
+vb_status launch_mirrors(vb_ctx* c);
+vb_status launch_neighbours(vb_ctx* c);
+vb_status launch_weights(vb_ctx* c);
+vb_status launch_freesurf(vb_ctx* c);
+vb_status launch_shift(vb_ctx* c);
+vb_status launch_predict(vb_ctx* c);
+vb_status launch_mirror_vel(vb_ctx* c);
+vb_status launch_pressure_update(vb_ctx* c);
+vb_status launch_final(vb_ctx* c);
+// PPE (vb_ppe.hip)
+vb_status launch_ppe(vb_ctx* c, double tol, int itermax);
+// helpers (vb_capi.hip)
+vb_status hip_check(vb_ctx* c, hipError_t e, const char* what);
+vb_status sync_scalars(vb_ctx* c);
+}  // namespace vb
+
+#define VB_HIP(c, call)                                                     \
+  do {                                                                      \
+    vb_status _s = vb::hip_check((c), (call), #call);                       \
+    if (_s != VB_OK) return _s;                                             \
+  } while (0)
+
+#define VB_LAUNCH(c, kern, grid, ...)                                       \
+  do {                                                                      \
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(vb::kBlock), 0, (c)->stream, __VA_ARGS__); \
+    VB_HIP(c, hipGetLastError());                                           \
+  } while (0)

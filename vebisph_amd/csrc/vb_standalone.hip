@@ -1,0 +1,108 @@
+// Stand-alone device kernels of the C ABI: elementwise pieces of the step that callers (and the
+// parity tests) can run on their own device arrays.
+#include "vb_ctx.h"
+#include "vb_logconf.h"
+
+namespace vb {
+
+// nnf_models_mod.F90:8-110.  The reference keeps these with no call sites; they are elementwise.
+__global__ void k_nnf(int model, int n, const double* sr, double mu, double* out) {
+  int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  double s = sr[i], r;
+  switch (model) {
+    case 0: r = s; break;                                               // :8-21 copies mu(i)
+    case 1: r = mu * pow(s + 1.0, 1.5 - 1.0); break;                    // :23-39, n = 1.5
+    case 2: {                                                           // :41-61
+      double ty = 1.0e-2, al = 1.0e2;
+      r = (s <= ty / (mu * al)) ? mu * al : mu + ty / s;
+    } break;
+    case 3: {                                                           // :63-82
+      double m0 = mu, mi = 0.05 * m0;
+      r = mi + (m0 - mi) / (1.0 + pow(1.0e-3 * s, 3.0));
+    } break;
+    case 4: {                                                           // :84-108
+      double m0 = mu, mi = 1.0e-3;
+      r = mi + (m0 - mi) * pow(1.0 + pow(0.14 * s, 1.5), (0.1 - 1.0) / 1.5);
+    } break;
+    default: r = 0.0;
+  }
+  out[i] = r;
+}
+
+__global__ void k_kernel_functions(int n, const double* q, double ad7, double ad7h, double* f, double* w) {
+  int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  f[i] = fac(q[i], ad7h);
+  w[i] = wab(q[i], ad7);
+}
+
+__global__ void k_logconf_n(int n, const double* tau, const double* trc, const double* gv, double dt,
+                            double wi, double eps, const double* kick, int use_kick, double* tout,
+                            double* trout) {
+  int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  double t[4] = {tau[4 * i], tau[4 * i + 1], tau[4 * i + 2], tau[4 * i + 3]};
+  double G[4] = {gv[4 * i], gv[4 * i + 1], gv[4 * i + 2], gv[4 * i + 3]};
+  double o[4], tr;
+  logconf(t, trc[i], G, dt, wi, eps, use_kick ? kick[i] : 0.0, use_kick != 0, o, &tr);
+  tout[4 * i] = o[0]; tout[4 * i + 1] = o[1]; tout[4 * i + 2] = o[2]; tout[4 * i + 3] = o[3];
+  trout[i] = tr;
+}
+
+// minstd stream of predictor_mod.F90:357,369-371 (see vb_capi.hip), computed on the device by
+// jumping the generator ahead: x_k = 48271^k mod (2^31-1).
+__device__ inline uint64_t mulmod(uint64_t a, uint64_t b) { return (a * b) % 2147483647ull; }
+__device__ inline uint64_t powmod(uint64_t e) {
+  uint64_t r = 1, b = 48271;
+  while (e) { if (e & 1) r = mulmod(r, b); b = mulmod(b, b); e >>= 1; }
+  return r;
+}
+__global__ void k_kick_table(int n, double* kick) {
+  int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  uint64_t a = powmod(2ull * i + 1), b = mulmod(a, 48271);
+  uint64_t f = ((a << 30) | ((b - 1) & ((1ull << 30) - 1))) >> 7;
+  kick[i] = (1.0e13 * kEps) * (ldexp((double)f, -54) - 0.5);
+}
+
+}  // namespace vb
+
+using namespace vb;
+
+extern "C" {
+
+vb_status vb_nnf(int32_t model, int32_t n, const double* d_sr, double mu, double* d_out) {
+  if (n < 0 || model < 0 || model > 4) return VB_ERR_ARG;
+  if (n == 0) return VB_OK;
+  hipLaunchKernelGGL(k_nnf, dim3(grid_1d(n)), dim3(kBlock), 0, 0, model, n, d_sr, mu, d_out);
+  if (hipGetLastError() != hipSuccess || hipStreamSynchronize(0) != hipSuccess) return VB_ERR_HIP;
+  return VB_OK;
+}
+
+vb_status vb_kernel_functions(int32_t n, const double* d_q, double ad7, double ad7h, double* d_f, double* d_w) {
+  if (n < 0) return VB_ERR_ARG;
+  if (n == 0) return VB_OK;
+  hipLaunchKernelGGL(k_kernel_functions, dim3(grid_1d(n)), dim3(kBlock), 0, 0, n, d_q, ad7, ad7h, d_f, d_w);
+  if (hipGetLastError() != hipSuccess || hipStreamSynchronize(0) != hipSuccess) return VB_ERR_HIP;
+  return VB_OK;
+}
+
+vb_status vb_logconf(int32_t n, const double* d_tau, const double* d_trace, const double* d_gradvel,
+                     double dt, double wi, double eps_ptt, int32_t kick, double* d_tau_out, double* d_trace_out) {
+  if (n < 0) return VB_ERR_ARG;
+  if (n == 0) return VB_OK;
+  double* dk = nullptr;
+  if (kick) {
+    if (hipMalloc((void**)&dk, (size_t)n * 8) != hipSuccess) return VB_ERR_HIP;
+    hipLaunchKernelGGL(k_kick_table, dim3(grid_1d(n)), dim3(kBlock), 0, 0, n, dk);
+  }
+  hipLaunchKernelGGL(k_logconf_n, dim3(grid_1d(n)), dim3(kBlock), 0, 0, n, d_tau, d_trace, d_gradvel, dt,
+                     wi, eps_ptt, dk, kick, d_tau_out, d_trace_out);
+  hipError_t e = hipGetLastError();
+  hipError_t e2 = hipStreamSynchronize(0);
+  if (dk) (void)hipFree(dk);
+  return (e != hipSuccess || e2 != hipSuccess) ? VB_ERR_HIP : VB_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,922 @@
+// gfx950 kernels for the per-step phases of div_free (everything except the PPE).
+//
+// One thread per particle (or per mirror / candidate / cell); SoA FP64 arrays; neighbour lists and
+// pair weights slot-major (link[k*npfb + i]) so that a wavefront reading slot k of 64 consecutive
+// particles issues one coalesced load.  Every kernel mirrors one reference loop; each carries the
+// file:line it restates.
+#include "vb_ctx.h"
+#include "vb_logconf.h"
+
+namespace vb {
+
+// ------------------------------------------------------------------------------------------------
+// Phase 1: set_time_step (div_free.F90:126-146) and r = r0 + u0 dt (:39-53)
+// ------------------------------------------------------------------------------------------------
+__global__ void k_umax(State st, int nbS, int npfb, double cfl_h, double dt_visc, Scal* sc,
+                       double* partials) {
+  double m = 1.0e-10;
+  for (int i = nbS + blockIdx.x * kBlock + threadIdx.x; i < npfb; i += gridDim.x * kBlock) {
+    double ua = sqrt(st.u[i] * st.u[i] + st.v[i] * st.v[i]);
+    if (ua > m) m = ua;
+  }
+  m = wave_max(m);
+  __shared__ double sh[kBlock / 64];
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
+  __syncthreads();
+  double v[1] = {0.0};
+  if (threadIdx.x == 0) {
+    double b = sh[0];
+    for (int w = 1; w < kBlock / 64; w++) b = sh[w] > b ? sh[w] : b;
+    v[0] = b;
+  }
+  if (publish_partials<1>(v, partials, &sc->counter)) {
+    if (threadIdx.x == 0) {
+      double um = 1.0e-10;
+      for (int b = 0; b < (int)gridDim.x; b++) {
+        double w = __hip_atomic_load(&partials[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (w > um) um = w;
+      }
+      sc->umax = um;
+      double dt_cfl = cfl_h / um;
+      sc->dt = dt_cfl < dt_visc ? dt_cfl : dt_visc;
+      sc->n_inbin = 0;
+      sc->err = 0;
+      sc->nmirror = 0;
+      sc->maxneigh = 0;
+      sc->counter = 0;
+    }
+  }
+}
+
+__global__ void k_advect(State st, int nbS, int npfb, const Scal* sc) {
+  int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= npfb) return;
+  double x = st.x[i], y = st.y[i], u = st.u[i], v = st.v[i];
+  st.x0[i] = x; st.y0[i] = y; st.u0[i] = u; st.v0[i] = v;
+  if (i >= nbS) {
+    double dt = sc->dt;
+    st.x[i] = x + u * dt;
+    st.y[i] = y + v * dt;
+  }
+}
+
+// Phase 2: put_in_bin.F90:9-54
+__global__ void k_put_in_bin(Geom g, State st, int nbS, int npfb, Scal* sc) {
+  int i = nbS + blockIdx.x * kBlock + threadIdx.x;
+  if (i >= npfb) return;
+  bool b = st.inbin[i] != 0;
+  double x = st.x[i], y = st.y[i];
+  if (!b) {
+    if (x < g.xmint || x > g.xmaxt || y < g.ymint || y > g.ymaxt) b = true;
+    if (isnan(x) || isnan(y)) b = true;
+  }
+  if (b) {
+    st.inbin[i] = 1;
+    st.x[i] = 2.0 * g.xmint - g.xmaxt;
+    st.y[i] = 2.0 * g.ymint - g.ymaxt;
+    st.u[i] = 0.0; st.v[i] = 0.0;
+    atomicAdd(&sc->n_inbin, 1);
+  }
+}
+
+__global__ void k_check_all_binned(int nfluid, Scal* sc) {
+  if (threadIdx.x == 0 && blockIdx.x == 0 && sc->n_inbin == nfluid) sc->err |= kErrAllBinned;
+}
+
+// ------------------------------------------------------------------------------------------------
+// Phase 3: create_mirror_particles (mirror_boundaries_mod.F90:18-334)
+// ------------------------------------------------------------------------------------------------
+__device__ inline double bound_dist(const Geom& g, double px, double py, int ib) {   // :447-462
+  int ip1 = (ib + 1) % g.nbp;
+  double pd = g.bedge[ib][1] * px - g.bedge[ib][0] * py + g.bnode[ip1][0] * g.bnode[ib][1] -
+              g.bnode[ip1][1] * g.bnode[ib][0];
+  return -1.0 * pd / g.blen[ib];
+}
+
+__device__ inline double pos_along(const Geom& g, double px, double py, int ib, double pd) {  // :476-491
+  if (fabs(g.bedge[ib][0]) > fabs(g.bedge[ib][1]))
+    return (px - g.bnode[ib][0] - pd * g.bnorm[ib][0]) / g.bedge[ib][0];
+  return (py - g.bnode[ib][1] - pd * g.bnorm[ib][1]) / g.bedge[ib][1];
+}
+
+// replace_escapees, :336-407.  Per particle the circle and patch tests run in the reference's order.
+__global__ void k_escapees(Geom g, State st, const Scal* sc) {
+  int i = g.nbS + blockIdx.x * kBlock + threadIdx.x;
+  if (i >= g.npfb) return;
+  double x = st.x[i], y = st.y[i];
+  for (int ib = 0; ib < g.ncirc; ib++) {
+    double R = g.crad[ib];
+    double rx = x - g.ccen[ib][0], ry = y - g.ccen[ib][1];
+    double rad = sqrt(rx * rx + ry * ry);
+    if (rad < fabs(R) && R > 0.0) { x = x + (rx / rad) * 2.0 * (R - rad); y = y + (ry / rad) * 2.0 * (R - rad); }
+    if (rad > fabs(R) && R < 0.0) { x = x - (rx / rad) * 2.0 * (rad + R); y = y - (ry / rad) * 2.0 * (rad + R); }
+  }
+  bool moved = false;
+  if (!st.inbin[i]) {
+    for (int ib = 0; ib < g.nbp; ib++) {
+      int t = g.btype[ib];
+      if (t != 1 && t != 2) continue;
+      double pd = bound_dist(g, x, y, ib);
+      if (pd > 0.0) continue;
+      if (fabs(pd) > g.sup + 1.0e-10) continue;
+      double tmp = pos_along(g, x, y, ib, pd);
+      if (t == 1) {
+        if (tmp < 0.0 || tmp > 1.0) continue;
+        x = x - 2.0 * pd * g.bnorm[ib][0];
+        y = y - 2.0 * pd * g.bnorm[ib][1];
+      } else {
+        int ibp = g.bpp[ib];
+        double tx = -1.0 * tmp * g.bedge[ibp][0] - pd * g.bnorm[ibp][0];
+        double ty = -1.0 * tmp * g.bedge[ibp][1] - pd * g.bnorm[ibp][1];
+        x = g.bnode[ibp][0] + g.bedge[ibp][0] + tx;
+        y = g.bnode[ibp][1] + g.bedge[ibp][1] + ty;
+      }
+      double dt = sc->dt;
+      st.x0[i] = x - st.u[i] * dt; st.y0[i] = y - st.v[i] * dt;
+      st.u0[i] = st.u[i]; st.v0[i] = st.v[i];
+      moved = true;
+    }
+  }
+  if (g.ncirc || moved) { st.x[i] = x; st.y[i] = y; }
+}
+
+// A mirror segment is one reference loop that appends mirrors in candidate order.  The append order
+// is reproduced by an order-preserving compaction: count per block, exclusive scan, emit.
+enum SegKind { SEG_CIRCLE, SEG_WALL, SEG_PERIODIC, SEG_CORNER_WW, SEG_CORNER_PP, SEG_CORNER_MIX };
+struct Seg { int kind, ib, ibp, ncand; double cross; };
+struct MRec { int irel, vrel; double x, y, u, v, th, dP; };
+
+__device__ inline bool near_patch(const Geom& g, double x, double y, int ib, double& bp1, double& bp2) {
+  double pd = bound_dist(g, x, y, ib);                     // create_boundary_boxes :420-433
+  if (fabs(pd) > g.sup + 1.0e-10) return false;
+  double tmp = pos_along(g, x, y, ib, pd);
+  if (tmp < 0.0 || tmp > 1.0) return false;
+  bp1 = tmp; bp2 = pd;
+  return true;
+}
+
+__device__ inline bool near_corner(const Geom& g, double x, double y, int ib) {    // :435-441
+  double ax = x - g.bnode[ib][0], ay = y - g.bnode[ib][1];
+  return !(sqrt(ax * ax + ay * ay) > g.sup + 1.0e-10);
+}
+
+__device__ inline double dPdot(const Geom& g, double dx, double dy) { return g.bf[0] * dx + g.bf[1] * dy; }
+
+__device__ bool seg_eval(const Geom& g, const Seg& s, const State& st, int c, int mlimit, MRec* r) {
+  switch (s.kind) {
+    case SEG_CIRCLE: {                                     // :61-96
+      int i = g.nbS + c / g.ncirc, ib = c % g.ncirc;
+      double R = g.crad[ib];
+      double cdir = R / fabs(R);
+      double xi = st.x[i], yi = st.y[i];
+      double rx = xi - g.ccen[ib][0], ry = yi - g.ccen[ib][1];
+      double rad = sqrt(rx * rx + ry * ry);
+      if (cdir > 0.0) {
+        if (!(rad <= R + g.sup)) return false;
+        if (r) {
+          rx = rx / rad; ry = ry / rad;
+          r->irel = i; r->vrel = 999 + ib + 1;
+          r->x = xi - rx * 2.0 * (rad - R); r->y = yi - ry * 2.0 * (rad - R);
+          double tx = ry, ty = -rx;
+          r->u = 2.0 * g.cvel[ib][0] + 2.0 * tx * R * g.comg[ib] - 1.0 * st.u[i];
+          r->v = 2.0 * g.cvel[ib][1] + 2.0 * ty * R * g.comg[ib] - 1.0 * st.v[i];
+          r->th = 2.0 * g.cth[ib] - 1.0 * st.th[i];
+          r->dP = dPdot(g, r->x - xi, r->y - yi);
+        }
+        return true;
+      } else if (cdir < 0.0) {
+        if (!(rad >= fabs(R) - g.sup)) return false;
+        if (r) {
+          rx = rx / rad; ry = ry / rad;
+          r->irel = i; r->vrel = 999 + ib + 1;
+          r->x = xi + rx * 2.0 * (fabs(R) - rad); r->y = yi + ry * 2.0 * (fabs(R) - rad);
+          double tx = ry, ty = -rx;
+          r->u = 2.0 * g.cvel[ib][0] + 2.0 * tx * fabs(R) * g.comg[ib] - 1.0 * st.u[i];
+          r->v = 2.0 * g.cvel[ib][1] + 2.0 * ty * fabs(R) * g.comg[ib] - 1.0 * st.v[i];
+          r->th = 2.0 * g.cth[ib] - 1.0 * st.th[i];
+          r->dP = dPdot(g, r->x - xi, r->y - yi);
+        }
+        return true;
+      }
+      return false;
+    }
+    case SEG_WALL: {                                       // :106-153
+      int i = c, ib = s.ib, ibp1 = (ib + 1) % g.nbp;
+      double xi = st.x[i], yi = st.y[i], bp1, bp2;
+      if (!near_patch(g, xi, yi, ib, bp1, bp2)) return false;
+      if (bp2 <= 1.0e-10) return false;
+      if (g.bang[ib] < 0.0) {
+        double ta = 0.5 * (3.14159265358979323846 + g.bang[ib]);
+        if (bp2 + 0.1 * g.dx >= bp1 * tan(ta) * g.blen[ib]) return false;
+      }
+      if (g.bang[ibp1] < 0.0) {
+        double ta = 0.5 * (3.14159265358979323846 + g.bang[ibp1]);
+        if (bp2 >= (1.0 - bp1) * tan(ta) * g.blen[ib]) return false;
+      }
+      if (r) {
+        r->irel = i; r->vrel = ib + 1;
+        r->x = xi - 2.0 * bp2 * g.bnorm[ib][0]; r->y = yi - 2.0 * bp2 * g.bnorm[ib][1];
+        double tx = g.bnorm[ib][1], ty = -g.bnorm[ib][0];
+        r->u = 2.0 * g.bvel[ib] * tx - st.u[i];
+        r->v = 2.0 * g.bvel[ib] * ty - st.v[i];
+        r->th = 2.0 * g.bth[ib] - st.th[i];
+        r->dP = dPdot(g, r->x - xi, r->y - yi);
+      }
+      return true;
+    }
+    case SEG_PERIODIC: {                                   // :156-187
+      int i = c, ib = s.ib, ibp = s.ibp;
+      double bp1, bp2;
+      if (!near_patch(g, st.x[i], st.y[i], ibp, bp1, bp2)) return false;
+      if (bp2 <= 0.0) return false;
+      if (r) {
+        r->irel = i; r->vrel = ib + 1;
+        double tx = -1.0 * bp1 * g.bedge[ib][0] - bp2 * g.bnorm[ib][0];
+        double ty = -1.0 * bp1 * g.bedge[ib][1] - bp2 * g.bnorm[ib][1];
+        r->x = g.bnode[ib][0] + g.bedge[ib][0] + tx;
+        r->y = g.bnode[ib][1] + g.bedge[ib][1] + ty;
+        r->u = st.u[i]; r->v = st.v[i]; r->th = st.th[i]; r->dP = 0.0;
+      }
+      return true;
+    }
+    case SEG_CORNER_WW: {                                  // :220-250
+      int i = g.nbS + c, ib = s.ib, ibm1 = (ib + g.nbp - 1) % g.nbp;
+      double xi = st.x[i], yi = st.y[i];
+      if (!near_corner(g, xi, yi, ib)) return false;
+      if (r) {
+        double tbx = xi - g.bnode[ib][0], tby = yi - g.bnode[ib][1];
+        r->irel = i; r->vrel = -(ib + 1);
+        r->x = xi - 2.0 * tbx; r->y = yi - 2.0 * tby;
+        double tmp = s.cross;   // the reference reuses the corner's cross product here
+        double tx = g.bnorm[ib][1], ty = -g.bnorm[ib][0];
+        double uu = 2.0 * tmp * tx - st.u[i], vv = 2.0 * tmp * ty - st.v[i];
+        tx = g.bnorm[ibm1][1]; ty = -g.bnorm[ibm1][0];
+        r->u = uu + 2.0 * tmp * tx; r->v = vv + 2.0 * tmp * ty;
+        r->th = st.th[i] + 2.0 * g.bth[ib];
+        r->dP = dPdot(g, r->x - xi, r->y - yi);
+      }
+      return true;
+    }
+    case SEG_CORNER_PP: {                                  // :253-280
+      int i = g.nbS + c, ib = s.ib, ibp = s.ibp;
+      if (!near_corner(g, st.x[i], st.y[i], ibp)) return false;
+      if (r) {
+        r->irel = i; r->vrel = -(ib + 1);
+        double tx = g.bnode[ib][0] - g.bnode[ibp][0], ty = g.bnode[ib][1] - g.bnode[ibp][1];
+        r->x = st.x[i] + tx; r->y = st.y[i] + ty;
+        r->u = st.u[i]; r->v = st.v[i]; r->th = st.th[i]; r->dP = 0.0;
+      }
+      return true;
+    }
+    case SEG_CORNER_MIX: {                                 // :283-319
+      if (c >= mlimit) return false;
+      int j = g.npfb + c, ib = s.ib, ibb = s.ibp, ibm1 = (ib + g.nbp - 1) % g.nbp;
+      double xj = st.x[j], yj = st.y[j];
+      double pd = bound_dist(g, xj, yj, ibb);
+      if (pd > g.sup + 1.0e-10) return false;
+      if (pd < 1.0e-10) return false;
+      double t = pos_along(g, xj, yj, ibb, pd);
+      if (!((t < 0.0 && ibb == ib) || (t > 1.0 && ibb == ibm1))) return false;
+      if (r) {
+        r->irel = st.irel[j]; r->vrel = -(ib + 1);
+        r->x = xj - 2.0 * pd * g.bnorm[ibb][0]; r->y = yj - 2.0 * pd * g.bnorm[ibb][1];
+        double tx = g.bnorm[ibb][1], ty = -g.bnorm[ibb][0];
+        r->u = 2.0 * g.bvel[ibb] * tx - st.u[j];
+        r->v = 2.0 * g.bvel[ibb] * ty - st.v[j];
+        r->th = 2.0 * g.bth[ibb] - st.th[j];
+        double d = st.dP[j];
+        r->dP = d + dPdot(g, r->x - xj, r->y - yj);
+      }
+      return true;
+    }
+  }
+  return false;
+}
+
+__device__ inline int block_ballot_prefix(bool f, int* total) {
+  __shared__ int wc[kBlock / 64];
+  unsigned long long b = __ballot(f);
+  int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) wc[wid] = __popcll(b);
+  __syncthreads();
+  int pre = 0, tot = 0;
+  for (int w = 0; w < kBlock / 64; w++) { if (w < wid) pre += wc[w]; tot += wc[w]; }
+  pre += __popcll(b & ((1ull << lane) - 1ull));
+  __syncthreads();
+  *total = tot;
+  return pre;
+}
+
+__global__ void k_seg_count(Geom g, Seg s, State st, const Scal* sc, int* counts) {
+  int c = blockIdx.x * kBlock + threadIdx.x;
+  int mlimit = s.kind == SEG_CORNER_MIX ? sc->nmirror : 0;
+  bool f = c < s.ncand && seg_eval(g, s, st, c, mlimit, nullptr);
+  int tot;
+  block_ballot_prefix(f, &tot);
+  if (threadIdx.x == 0) counts[blockIdx.x] = tot;
+}
+
+// exclusive scan of n ints in place by one block of 1024 threads; returns the total to thread 0..
+__device__ int block_scan_1024(int* a, int n) {
+  __shared__ int sh[1024];
+  int per = (n + 1023) / 1024;
+  int b0 = threadIdx.x * per, b1 = min(n, b0 + per);
+  int s = 0;
+  for (int k = b0; k < b1; k++) s += a[k];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 1; o < 1024; o <<= 1) {          // Hillis-Steele inclusive scan of the run sums
+    int v = threadIdx.x >= (unsigned)o ? sh[threadIdx.x - o] : 0;
+    __syncthreads();
+    sh[threadIdx.x] += v;
+    __syncthreads();
+  }
+  int run = threadIdx.x ? sh[threadIdx.x - 1] : 0;
+  int total = sh[1023];
+  for (int k = b0; k < b1; k++) { int t = a[k]; a[k] = run; run += t; }
+  __syncthreads();
+  return total;
+}
+
+__global__ void k_seg_scan(int* counts, int nb, Scal* sc, int mcap) {
+  int total = block_scan_1024(counts, nb);
+  if (threadIdx.x == 0) {
+    int base = sc->nmirror;
+    sc->seg_base = base;
+    if (base + total > mcap) { sc->err |= kErrCapacity; total = mcap - base; }
+    sc->nmirror = base + total;
+  }
+}
+
+__global__ void k_seg_emit(Geom g, Seg s, State st, const Scal* sc, const int* offsets, int mcap) {
+  int c = blockIdx.x * kBlock + threadIdx.x;
+  int base = sc->seg_base;
+  int mlimit = s.kind == SEG_CORNER_MIX ? base : 0;
+  MRec r;
+  bool f = c < s.ncand && seg_eval(g, s, st, c, mlimit, &r);
+  int tot;
+  int pre = block_ballot_prefix(f, &tot);
+  if (f) {
+    int m = base + offsets[blockIdx.x] + pre;
+    if (m < mcap) {
+      int k = g.npfb + m;
+      st.irel[k] = r.irel; st.vrel[k] = r.vrel; st.x[k] = r.x; st.y[k] = r.y;
+      st.u[k] = r.u; st.v[k] = r.v; st.th[k] = r.th; st.dP[k] = r.dP;
+    }
+  }
+}
+
+// mirrors of moving / heated solid particles keep the parent's value, :190-206
+__global__ void k_mirror_fixup(Geom g, State st, const Scal* sc) {
+  int m = blockIdx.x * kBlock + threadIdx.x;
+  if (m >= sc->nmirror) return;
+  int j = g.npfb + m, i = st.irel[j];
+  if (i >= g.nbS) return;
+  double uu = st.u[i] * st.u[i] + st.v[i] * st.v[i];
+  if (uu != 0.0) { st.u[j] = st.u[i]; st.v[j] = st.v[i]; }
+  double tt = st.th[i] * st.th[i];
+  if (tt != 0.0) st.th[j] = st.th[i];
+}
+
+// ------------------------------------------------------------------------------------------------
+// Phase 4: divide (neighbour_finding_mod.F90:56-127) as a counting sort, then
+// neighboring_list_parallel (:261-365).
+// ------------------------------------------------------------------------------------------------
+__global__ void k_fill_int(int* a, int n, int v) {
+  int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+
+__global__ void k_cell_count(Geom g, State st, int np, int* cellkey, int* cellslot, int* nc, Scal* sc) {
+  int k = blockIdx.x * kBlock + threadIdx.x;
+  if (k >= np) return;
+  if (st.inbin[k]) { cellkey[k] = -1; return; }
+  double ddx = st.x[k] - g.xmint, ddy = st.y[k] - g.ymint;
+  int icell = (int)(ddx * g.uno) + 1;
+  int jcell = (int)(ddy * g.uno) + 1;
+  int ii = icell + (jcell - 1) * g.ncx;
+  if (ii > g.nct || ii < 1) { atomicOr(&sc->err, kErrOutOfGrid); cellkey[k] = -1; return; }
+  int slot = atomicAdd(&nc[ii - 1], 1);
+  if (slot + 1 > kNplink) atomicOr(&sc->err, kErrCellOverflow);
+  cellkey[k] = ii;
+  cellslot[k] = slot;
+}
+
+__global__ void k_cell_scan(const int* nc, int* ist, int nct) {
+  // ist[c] = start of cell c (0-based cells), ist[nct] = total
+  for (int c = threadIdx.x; c < nct; c += 1024) ist[c] = nc[c];
+  __syncthreads();
+  int total = block_scan_1024(ist, nct);
+  if (threadIdx.x == 0) ist[nct] = total;
+}
+
+__global__ void k_cell_scatter(int np, const int* cellkey, const int* cellslot, const int* ist, int* ip) {
+  int k = blockIdx.x * kBlock + threadIdx.x;
+  if (k >= np) return;
+  int key = cellkey[k];
+  if (key < 1) return;
+  ip[ist[key - 1] + cellslot[k]] = k;
+}
+
+// atomics give an arbitrary order inside a cell; the reference's scatter is stable (index order)
+__global__ void k_cell_sort(const int* ist, int* ip, int nct) {
+  int c = blockIdx.x * kBlock + threadIdx.x;
+  if (c >= nct) return;
+  int b = ist[c], e = ist[c + 1];
+  for (int a = b + 1; a < e; a++) {
+    int key = ip[a], q = a - 1;
+    while (q >= b && ip[q] > key) { ip[q + 1] = ip[q]; q--; }
+    ip[q + 1] = key;
+  }
+}
+
+__global__ void k_neighbours(Geom g, State st, const int* cellkey, const int* ist, const int* ip,
+                             int* cnt, int* link, Scal* sc) {
+  int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= g.npfb) return;
+  int n = 0;
+  if (!st.inbin[i]) {
+    const int ncx = g.ncx, ncy = g.ncy;
+    int ic = cellkey[i];
+    int icx = ic % ncx, icy = ic / ncx + 1;                // reference's row/column decode
+    int cells[9], nce = 0;
+    cells[nce++] = ic;
+    if (icx + 1 <= ncx) {
+      cells[nce++] = ic + 1;
+      if (icy + 1 <= ncy) cells[nce++] = ic + ncx + 1;
+      if (icy - 1 >= 1) cells[nce++] = ic - ncx + 1;
+    }
+    if (icx - 1 >= 1) {
+      cells[nce++] = ic - 1;
+      if (icy + 1 <= ncy) cells[nce++] = ic + ncx - 1;
+      if (icy - 1 >= 1) cells[nce++] = ic - ncx - 1;
+    }
+    if (icy + 1 <= ncy) cells[nce++] = ic + ncx;
+    if (icy - 1 >= 1) cells[nce++] = ic - ncx;
+    const double xi = st.x[i], yi = st.y[i], sup2 = g.sup2;
+    const int npfb = g.npfb;
+    bool over = false;
+    for (int q = 0; q < nce; q++) {
+      int jc = cells[q] - 1;
+      if (jc < 0 || jc >= g.nct) continue;
+      int e = ist[jc + 1];
+      for (int jj = ist[jc]; jj < e; jj++) {
+        int j = ip[jj];
+        double rx = xi - st.x[j], ry = yi - st.y[j];
+        double rr2 = rx * rx + ry * ry;
+        if (rr2 > 0.0 && rr2 <= sup2) {
+          if (n < kNplink) link[n * npfb + i] = j; else over = true;
+          n++;
+        }
+      }
+    }
+    if (over) { atomicOr(&sc->err, kErrCapacity); n = kNplink; }
+  }
+  cnt[i] = n;
+}
+
+// ------------------------------------------------------------------------------------------------
+// Phase 5: kernel_calculation.F90:24-71 (weights, Morris Laplacian weight, KGC inverse)
+// ------------------------------------------------------------------------------------------------
+__global__ void k_weights(Geom g, State st, const int* cnt, const int* link, double* wg1, double* wg2,
+                          double* wl, double* k11, double* k12, double* k21, double* k22,
+                          float* wg1f, float* wg2f, float* wlf, Scal* sc) {
+  int i = blockIdx.x * kBlock + threadIdx.x;
+  int mx = 0;
+  if (i < g.npfb) {
+    const int npfb = g.npfb, n = cnt[i];
+    mx = n;
+    const double xi = st.x[i], yi = st.y[i], dv = g.dv;
+    double a11 = 0.0, a12 = 0.0, a21 = 0.0, a22 = 0.0;
+    for (int k = 0; k < n; k++) {
+      int s = k * npfb + i;
+      int j = link[s];
+      double rx = xi - st.x[j], ry = yi - st.y[j];
+      double rr2 = rx * rx + ry * ry;
+      double rad = sqrt(rr2);
+      double qq = rad * g.oneoverh;
+      double f = fac(qq, g.ad7h);
+      double den = rad > g.eta2 ? rad : g.eta2;
+      double gx = rx * f / den, gy = ry * f / den;
+      double w1 = gx * dv, w2 = gy * dv;
+      double l = 2.0 * (rx * w1 + ry * w2) / (rr2 + g.eta_small);
+      wg1[s] = w1; wg2[s] = w2; wl[s] = l;
+      if (wg1f) { wg1f[s] = (float)w1; wg2f[s] = (float)w2; wlf[s] = (float)l; }
+      a11 = a11 - dv * gx * rx; a12 = a12 - dv * gy * rx;
+      a21 = a21 - dv * gx * ry; a22 = a22 - dv * gy * ry;
+    }
+    double t = 1.0 / ((a11 * a22 - a12 * a21) + 1.0e-15);
+    k11[i] = t * a22; k12[i] = -t * a12; k21[i] = -t * a21; k22[i] = t * a11;
+  }
+  // maxneighbours = maxval(ij_count)
+  for (int o = 32; o > 0; o >>= 1) { int w = __shfl_down(mx, o, 64); mx = w > mx ? w : mx; }
+  if ((threadIdx.x & 63) == 0 && mx > 0) atomicMax(&sc->maxneigh, mx);
+}
+
+// Phase 6: locate_free_surf_part, closed domain (freesurf_mod.F90:39-45): n_surf = 0 everywhere
+__global__ void k_freesurf(int* nsurf, int np) {
+  int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < np) nsurf[i] = 0;
+}
+
+// ------------------------------------------------------------------------------------------------
+// Phase 7: shifting_forces + new_fick_shift (new_fick_shift.F90:15-92, 117-210), i_shift_coeff 0,
+// closed domain; ushift = dr_fs/dt and -u on walls (div_free.F90:65-78)
+// ------------------------------------------------------------------------------------------------
+__global__ void k_shift(Geom g, State st, const int* cnt, const int* link, const double* wg1,
+                        const double* wg2, double* usx, double* usy, const Scal* sc) {
+  int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= g.npfb) return;
+  const int npfb = g.npfb, n = cnt[i];
+  const double xi = st.x[i], yi = st.y[i], wab1 = sc->wab1, h = g.h;
+  double sfx = 0.0, sfy = 0.0, stx = 0.0, sty = 0.0;
+  for (int k = 0; k < n; k++) {
+    int s = k * npfb + i;
+    int j = link[s];
+    double rx = xi - st.x[j], ry = yi - st.y[j];
+    double rad = sqrt(rx * rx + ry * ry);
+    double qq = rad / h;
+    double fab = 0.25 * pow(wab(qq, g.ad7) / wab1, 4.0);
+    double w1 = wg1[s], w2 = wg2[s];
+    sfx = sfx + w1; sfy = sfy + w2;
+    stx = stx + w1 * fab; sty = sty + w2 * fab;
+  }
+  double dt = sc->dt;
+  if (i < g.nbS) { usx[i] = -st.u[i]; usy[i] = -st.v[i]; return; }
+  double D = 0.25 * h * h;
+  double drx = -D * (sfx + stx), dry = -D * (sfy + sty);
+  double t = sqrt(drx * drx + dry * dry);
+  if (t > 0.1 * h) { drx = 0.1 * h * drx / t; dry = 0.1 * h * dry / t; }
+  usx[i] = drx / dt; usy[i] = dry / dt;
+}
+
+// ------------------------------------------------------------------------------------------------
+// Phase 8: prediction_step (predictor_mod.F90:27-139)
+// ------------------------------------------------------------------------------------------------
+// grad u, grad v (grad_operator, gradient_calculation.F90:61-71) fused with the per-particle
+// log-conformation update (walls included, predictor_mod.F90:166)
+__global__ void k_grad_logconf(Geom g, State st, const int* cnt, const int* link, const double* wg1,
+                               const double* wg2, const double* k11, const double* k12,
+                               const double* k21, const double* k22, const double* kick, int use_kick,
+                               double wi, double eps_ptt, const Scal* sc) {
+  int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= g.npfb) return;
+  const int npfb = g.npfb, n = cnt[i];
+  const double ui = st.u[i], vi = st.v[i];
+  double gux = 0.0, guy = 0.0, gvx = 0.0, gvy = 0.0;
+  for (int k = 0; k < n; k++) {
+    int s = k * npfb + i;
+    int j = link[s];
+    double w1 = wg1[s], w2 = wg2[s];
+    double tu = st.u[j] - ui, tv = st.v[j] - vi;
+    gux = gux + tu * w1; guy = guy + tu * w2;
+    gvx = gvx + tv * w1; gvy = gvy + tv * w2;
+  }
+  double K11 = k11[i], K12 = k12[i], K21 = k21[i], K22 = k22[i];
+  double G[4] = {K11 * gux + K12 * guy, K21 * gux + K22 * guy, K11 * gvx + K12 * gvy, K21 * gvx + K22 * gvy};
+  double tau[4] = {st.t11[i], st.t12[i], st.t21[i], st.t22[i]}, to[4], trc;
+  logconf(tau, st.trc[i], G, sc->dt, wi, eps_ptt, use_kick ? kick[i] : 0.0, use_kick != 0, to, &trc);
+  st.trc[i] = trc;
+  st.t11[i] = to[0]; st.t12[i] = to[1]; st.t21[i] = to[2]; st.t22[i] = to[3];
+}
+
+// mirror_polymeric_stress, mirror_boundaries_mod.F90:586-653
+__device__ inline void reflect_tau(const double* ti, double n1, double n2, double* tj) {
+  double nt1 = -n2, nt2 = n1;
+  double Q[4] = {1.0 * n1 + 0.0 * n2, 1.0 * nt1 + 0.0 * nt2, 0.0 * n1 + 1.0 * n2, 0.0 * nt1 + 1.0 * nt2};
+  double QT[4] = {Q[0], Q[2], Q[1], Q[3]}, a[4], b[4];
+  mm2(ti, Q, a);
+  mm2(QT, a, b);
+  b[0] = -b[0]; b[1] = -b[1];
+  mm2(b, QT, a);
+  mm2(Q, a, tj);
+}
+
+__global__ void k_mirror_stress(Geom g, State st, const Scal* sc) {
+  int m = blockIdx.x * kBlock + threadIdx.x;
+  if (m >= sc->nmirror) return;
+  int j = g.npfb + m, i = st.irel[j], vr = st.vrel[j];
+  double ti[4] = {st.t11[i], st.t12[i], st.t21[i], st.t22[i]}, tj[4];
+  bool set = false;
+  if (vr > 999) {
+    double rx = st.x[i] - st.x[j], ry = st.y[i] - st.y[j];
+    double d = sqrt(rx * rx + ry * ry);
+    reflect_tau(ti, rx / d, ry / d, tj); set = true;
+  }
+  if (vr > 0 && vr < 999) {
+    int t = g.btype[vr - 1];
+    if (t == 1) {
+      double rx = st.x[i] - st.x[j], ry = st.y[i] - st.y[j];
+      double d = sqrt(rx * rx + ry * ry);
+      reflect_tau(ti, rx / d, ry / d, tj); set = true;
+    } else if (t == 2 || t == 4) {
+      for (int q = 0; q < 4; q++) tj[q] = ti[q];
+      set = true;
+    } else if (t == 3) {
+      for (int q = 0; q < 4; q++) tj[q] = 0.0;
+      set = true;
+    }
+  }
+  if (vr < 0 && vr > -g.npfb) {
+    int ib = -vr - 1, ibm1 = (ib + g.nbp - 1) % g.nbp;
+    int a = g.btype[ib], b = g.btype[ibm1];
+    if (a == 1 && b == 1) { for (int q = 0; q < 4; q++) tj[q] = -ti[q]; set = true; }
+    else if (a == 2 && b == 2) { for (int q = 0; q < 4; q++) tj[q] = ti[q]; set = true; }
+    else if ((a == 1 && b == 2) || (a == 2 && b == 1)) {
+      int w = a == 1 ? ib : ibm1;
+      reflect_tau(ti, g.bnorm[w][0], g.bnorm[w][1], tj); set = true;
+    }
+  }
+  if (set) { st.t11[j] = tj[0]; st.t12[j] = tj[1]; st.t21[j] = tj[2]; st.t22[j] = tj[3]; }
+}
+
+// RHS, buoyancy, u* and theta* (predictor_mod.F90:203-282), written to the alternate buffers
+__global__ void k_rhs(Geom g, State st, const int* cnt, const int* link, const double* wg1,
+                      const double* wg2, const double* wl, const double* k11, const double* k12,
+                      const double* k21, const double* k22, double cs, double cp, double ct,
+                      double* un, double* vn, double* thn, const Scal* sc) {
+  int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= g.npfb) return;
+  const int npfb = g.npfb, n = cnt[i];
+  const double ui = st.u[i], vi = st.v[i], thi = st.th[i];
+  if (i < g.nbS) { un[i] = ui; vn[i] = vi; thn[i] = thi; return; }
+  const double T11i = st.t11[i], T12i = st.t12[i], T21i = st.t21[i], T22i = st.t22[i];
+  const double K11 = k11[i], K12 = k12[i], K21 = k21[i], K22 = k22[i];
+  double ax = 0.0, ay = 0.0, at = 0.0;
+  for (int k = 0; k < n; k++) {
+    int s = k * npfb + i;
+    int j = link[s];
+    double w1 = wg1[s], w2 = wg2[s], l = wl[s];
+    double uij = ui - st.u[j], vij = vi - st.v[j];
+    double T11 = st.t11[j] - T11i, T12 = st.t12[j] - T12i, T21 = st.t21[j] - T21i, T22 = st.t22[j] - T22i;
+    double kw1 = K11 * w1 + K12 * w2, kw2 = K21 * w1 + K22 * w2;
+    double tw1 = T11 * kw1 + T12 * kw2, tw2 = T21 * kw1 + T22 * kw2;
+    ax = ax + cp * tw1 + cs * uij * l;
+    ay = ay + cp * tw2 + cs * vij * l;
+    at = at + ct * (thi - st.th[j]) * l;
+  }
+  ay = ay + thi;
+  double dt = sc->dt;
+  un[i] = ui + ax * dt;
+  vn[i] = vi + ay * dt;
+  thn[i] = thi + at * dt;
+}
+
+// Phase 9: mirror_velocities, mirror_boundaries_mod.F90:509-584 (reads u*, writes the mirror rows
+// of the alternate buffers; mirror temperatures are carried over unchanged)
+__global__ void k_mirror_vel(Geom g, State st, double* un, double* vn, double* thn, const Scal* sc) {
+  int m = blockIdx.x * kBlock + threadIdx.x;
+  if (m >= sc->nmirror) return;
+  int j = g.npfb + m, i = st.irel[j], vr = st.vrel[j];
+  double ui = un[i], vi = vn[i];
+  double uj = st.u[j], vj = st.v[j];
+  if (vr > 999) {
+    int ib = vr - 999 - 1;
+    double rx = 0.5 * (st.x[i] - st.x[j]), ry = 0.5 * (st.y[i] - st.y[j]);
+    double t = sqrt(rx * rx + ry * ry);
+    double tx = ry / t, ty = -rx / t;
+    uj = 2.0 * g.cvel[ib][0] + 2.0 * tx * g.crad[ib] * g.comg[ib] - 1.0 * ui;
+    vj = 2.0 * g.cvel[ib][1] + 2.0 * ty * g.crad[ib] * g.comg[ib] - 1.0 * vi;
+  }
+  if (vr > 0 && vr < 999) {
+    int t = g.btype[vr - 1];
+    if (abs(t) == 1) {
+      double rx = 0.5 * (st.x[i] - st.x[j]), ry = 0.5 * (st.y[i] - st.y[j]);
+      double d = sqrt(rx * rx + ry * ry);
+      double tx = ry / d, ty = -rx / d;
+      uj = 2.0 * g.bvel[vr - 1] * tx - ui;
+      vj = 2.0 * g.bvel[vr - 1] * ty - vi;
+    } else if (t == 2) {
+      uj = ui; vj = vi;
+    }
+  }
+  if (vr < 0 && vr > -g.npfb) {
+    int ib = -vr - 1, ibm1 = (ib + g.nbp - 1) % g.nbp;
+    int a = g.btype[ib], b = g.btype[ibm1];
+    if (a == 1 && b == 1) {
+      uj = ui; vj = vi;
+      if (g.bvel[ib] != 0.0) {
+        double tmp = g.bvel[ib];
+        uj = 2.0 * tmp * g.bnorm[ib][1] - ui; vj = 2.0 * tmp * (-g.bnorm[ib][0]) - vi;
+      }
+      if (g.bvel[ibm1] != 0.0) {
+        double tmp = g.bvel[ibm1];
+        uj = uj + 2.0 * tmp * g.bnorm[ibm1][1]; vj = vj + 2.0 * tmp * (-g.bnorm[ibm1][0]);
+      }
+    } else if (a == 2 && b == 2) {
+      uj = ui; vj = vi;
+    } else if ((a == 1 && b == 2) || (a == 2 && b == 1)) {
+      double tmp, tx, ty;
+      if (g.bvel[ib] != 0.0) { tx = g.bnorm[ib][1]; ty = -g.bnorm[ib][0]; tmp = g.bvel[ib]; }
+      else if (g.bvel[ibm1] != 0.0) { tx = g.bnorm[ibm1][1]; ty = -g.bnorm[ibm1][0]; tmp = g.bvel[ibm1]; }
+      else { tmp = 0.0; tx = 0.0; ty = 0.0; }
+      uj = 2.0 * tmp * tx - ui;
+      vj = 2.0 * tmp * ty - vi;
+    }
+  }
+  if (i < g.nbS && (ui * ui + vi * vi) != 0.0) { uj = ui; vj = vi; }
+  un[j] = uj; vn[j] = vj;
+  thn[j] = st.th[j];
+}
+
+// ------------------------------------------------------------------------------------------------
+// Phase 11: calc_pressure_gradient (gradient_calculation.F90:16-48) and the velocity update
+// (div_free.F90:96-101); phase 12: final advection (:111-117), fused in one pass
+// ------------------------------------------------------------------------------------------------
+__global__ void k_mirror_p(Geom g, State st, const Scal* sc) {
+  int m = blockIdx.x * kBlock + threadIdx.x;
+  if (m >= sc->nmirror) return;
+  int j = g.npfb + m;
+  st.p[j] = st.p[st.irel[j]] + st.dP[j];
+}
+
+__global__ void k_pressure_update(Geom g, State st, const int* cnt, const int* link, const double* wg1,
+                                  const double* wg2, const double* k11, const double* k12,
+                                  const double* k21, const double* k22, double* gpx, double* gpy,
+                                  const Scal* sc) {
+  int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= g.npfb) return;
+  const int npfb = g.npfb, n = cnt[i];
+  const double pi = st.p[i];
+  double gx = 0.0, gy = 0.0;
+  for (int k = 0; k < n; k++) {
+    int s = k * npfb + i;
+    double t = st.p[link[s]] - pi;
+    gx = gx + t * wg1[s]; gy = gy + t * wg2[s];
+  }
+  double px = k11[i] * gx + k12[i] * gy, py = k21[i] * gx + k22[i] * gy;
+  gpx[i] = px; gpy[i] = py;
+  if (i < g.nbS) return;
+  double dt = sc->dt;
+  double u = st.u[i] + dt * (g.bf[0] - px), v = st.v[i] + dt * (g.bf[1] - py);
+  if (n <= 3) { u = st.u0[i] + dt * g.bf[0]; v = st.v0[i] + dt * g.bf[1]; }
+  st.u[i] = u; st.v[i] = v;
+}
+
+__global__ void k_final(Geom g, State st, const double* usx, const double* usy, const Scal* sc) {
+  int i = g.nbS + blockIdx.x * kBlock + threadIdx.x;
+  if (i >= g.npfb) return;
+  double dt = sc->dt;
+  st.x[i] = st.x0[i] + 0.5 * (st.u[i] + st.u0[i]) * dt + usx[i] * dt;
+  st.y[i] = st.y0[i] + 0.5 * (st.v[i] + st.v0[i]) * dt + usy[i] * dt;
+}
+
+// ================================================================================================
+// launchers
+// ================================================================================================
+vb_status launch_set_time_step(vb_ctx* c) {
+  int nb = grid_for(c->npfb - c->nbS);
+  VB_LAUNCH(c, k_umax, nb, c->st, c->nbS, c->npfb, c->prm.dt_coef_cfl * c->prm.h, c->dt_visc, c->scal,
+            c->partials);
+  VB_LAUNCH(c, k_advect, grid_1d(c->npfb), c->st, c->nbS, c->npfb, c->scal);
+  c->itime += 1;
+  return VB_OK;
+}
+
+vb_status launch_put_in_bin(vb_ctx* c) {
+  int nf = c->npfb - c->nbS;
+  if (nf > 0) VB_LAUNCH(c, k_put_in_bin, grid_1d(nf), c->g, c->st, c->nbS, c->npfb, c->scal);
+  VB_LAUNCH(c, k_check_all_binned, 1, nf, c->scal);
+  return VB_OK;
+}
+
+static vb_status run_segment(vb_ctx* c, const Seg& s) {
+  if (s.ncand <= 0) return VB_OK;
+  int nb = grid_1d(s.ncand);
+  if (nb > c->seg_blocks) return VB_ERR_CAPACITY;
+  VB_LAUNCH(c, k_seg_count, nb, c->g, s, c->st, c->scal, c->seg_counts);
+  hipLaunchKernelGGL(k_seg_scan, dim3(1), dim3(1024), 0, c->stream, c->seg_counts, nb, c->scal, c->mcap);
+  VB_HIP(c, hipGetLastError());
+  VB_LAUNCH(c, k_seg_emit, nb, c->g, s, c->st, c->scal, c->seg_counts, c->mcap);
+  return VB_OK;
+}
+
+static vb_status read_nmirror(vb_ctx* c) {
+  VB_HIP(c, hipMemcpyAsync(c->scal_host, c->scal, sizeof(Scal), hipMemcpyDeviceToHost, c->stream));
+  VB_HIP(c, hipStreamSynchronize(c->stream));
+  c->nmirror_host = c->scal_host->nmirror;
+  return VB_OK;
+}
+
+vb_status launch_mirrors(vb_ctx* c) {
+  const Geom& g = c->g;
+  int nf = c->npfb - c->nbS;
+  if (nf > 0) VB_LAUNCH(c, k_escapees, grid_1d(nf), c->g, c->st, c->scal);
+  vb_status s;
+  if (g.ncirc) {
+    Seg sg{SEG_CIRCLE, 0, 0, nf * g.ncirc, 0.0};
+    if ((s = run_segment(c, sg)) != VB_OK) return s;
+  }
+  for (int ib = 0; ib < g.nbp; ib++) {
+    if (g.btype[ib] == 1) {
+      Seg sg{SEG_WALL, ib, 0, c->npfb, 0.0};
+      if ((s = run_segment(c, sg)) != VB_OK) return s;
+    } else if (g.btype[ib] == 2) {
+      Seg sg{SEG_PERIODIC, ib, g.bpp[ib], c->npfb, 0.0};
+      if ((s = run_segment(c, sg)) != VB_OK) return s;
+    }
+  }
+  // the fix-up reads the mirrors made so far; launch over the capacity bound, guarded on device
+  VB_LAUNCH(c, k_mirror_fixup, grid_1d(c->mcap), c->g, c->st, c->scal);
+  for (int ib = 0; ib < g.nbp; ib++) {
+    int ibm1 = (ib + g.nbp - 1) % g.nbp;
+    int a = g.btype[ib], b = g.btype[ibm1];
+    if (a == 0 || b == 0) continue;
+    double cross = g.bedge[ibm1][0] * g.bedge[ib][1] - g.bedge[ibm1][1] * g.bedge[ib][0];
+    if (fabs(cross) < 1.0e-10) continue;
+    if (a == 1 && b == 1 && g.bang[ib] > 0.0) {
+      Seg sg{SEG_CORNER_WW, ib, 0, nf, cross};
+      if ((s = run_segment(c, sg)) != VB_OK) return s;
+    }
+    if (a == 2 && b == 2) {
+      Seg sg{SEG_CORNER_PP, ib, g.bpp[ib], nf, cross};
+      if ((s = run_segment(c, sg)) != VB_OK) return s;
+    }
+    if (a != b && (a == 1 || b == 1)) {
+      if ((s = read_nmirror(c)) != VB_OK) return s;
+      Seg sg{SEG_CORNER_MIX, ib, b == 1 ? ibm1 : ib, c->nmirror_host, cross};
+      if ((s = run_segment(c, sg)) != VB_OK) return s;
+    }
+  }
+  if ((s = read_nmirror(c)) != VB_OK) return s;
+  c->np_host = c->npfb + c->nmirror_host;
+  if (c->scal_host->err & kErrCapacity) {
+    c->err = "mirror capacity exceeded";
+    return VB_ERR_CAPACITY;
+  }
+  return VB_OK;
+}
+
+vb_status launch_neighbours(vb_ctx* c) {
+  const Geom& g = c->g;
+  int np = c->np_host;
+  VB_LAUNCH(c, k_fill_int, grid_1d(g.nct), c->nc, g.nct, 0);
+  VB_LAUNCH(c, k_cell_count, grid_1d(np), c->g, c->st, np, c->cellkey, c->cellslot, c->nc, c->scal);
+  hipLaunchKernelGGL(k_cell_scan, dim3(1), dim3(1024), 0, c->stream, c->nc, c->ist, g.nct);
+  VB_HIP(c, hipGetLastError());
+  VB_LAUNCH(c, k_cell_scatter, grid_1d(np), np, c->cellkey, c->cellslot, c->ist, c->ip);
+  VB_LAUNCH(c, k_cell_sort, grid_1d(g.nct), c->ist, c->ip, g.nct);
+  VB_LAUNCH(c, k_neighbours, grid_1d(c->npfb), c->g, c->st, c->cellkey, c->ist, c->ip, c->cnt, c->link,
+            c->scal);
+  return VB_OK;
+}
+
+vb_status launch_weights(vb_ctx* c) {
+  bool f32 = c->prm.precision == VB_PREC_MIXED;
+  VB_LAUNCH(c, k_weights, grid_1d(c->npfb), c->g, c->st, c->cnt, c->link, c->wg1, c->wg2, c->wl, c->k11,
+            c->k12, c->k21, c->k22, f32 ? c->wg1f : nullptr, f32 ? c->wg2f : nullptr,
+            f32 ? c->wlf : nullptr, c->scal);
+  return VB_OK;
+}
+
+vb_status launch_freesurf(vb_ctx* c) {
+  VB_LAUNCH(c, k_freesurf, grid_1d(c->np_host), c->nsurf, c->np_host);
+  return VB_OK;
+}
+
+vb_status launch_shift(vb_ctx* c) {
+  VB_LAUNCH(c, k_shift, grid_1d(c->npfb), c->g, c->st, c->cnt, c->link, c->wg1, c->wg2, c->usx, c->usy,
+            c->scal);
+  return VB_OK;
+}
+
+vb_status launch_predict(vb_ctx* c) {
+  const vb_params& p = c->prm;
+  double wi = p.El * sqrt(p.Ra / p.Pr);
+  int use_kick = p.reference_kick && c->itime != 2;
+  VB_LAUNCH(c, k_grad_logconf, grid_1d(c->npfb), c->g, c->st, c->cnt, c->link, c->wg1, c->wg2, c->k11,
+            c->k12, c->k21, c->k22, c->kick, use_kick, wi, p.ve_nonlinearity, c->scal);
+  VB_LAUNCH(c, k_mirror_stress, grid_1d(c->nmirror_host), c->g, c->st, c->scal);
+  double cs = (p.beta) * sqrt(p.Pr / p.Ra);
+  double cp = (1.0 - p.beta) * p.Pr / (p.Ra * p.El);
+  double ct = 1.0 / sqrt(p.Ra * p.Pr);
+  VB_LAUNCH(c, k_rhs, grid_1d(c->npfb), c->g, c->st, c->cnt, c->link, c->wg1, c->wg2, c->wl, c->k11,
+            c->k12, c->k21, c->k22, cs, cp, ct, c->u_alt, c->v_alt, c->th_alt, c->scal);
+  return VB_OK;
+}
+
+vb_status launch_mirror_vel(vb_ctx* c) {
+  if (c->nmirror_host > 0)
+    VB_LAUNCH(c, k_mirror_vel, grid_1d(c->nmirror_host), c->g, c->st, c->u_alt, c->v_alt, c->th_alt,
+              c->scal);
+  double* t;
+  t = c->st.u; c->st.u = c->u_alt; c->u_alt = t;
+  t = c->st.v; c->st.v = c->v_alt; c->v_alt = t;
+  t = c->st.th; c->st.th = c->th_alt; c->th_alt = t;
+  return VB_OK;
+}
+
+vb_status launch_pressure_update(vb_ctx* c) {
+  if (c->nmirror_host > 0) VB_LAUNCH(c, k_mirror_p, grid_1d(c->nmirror_host), c->g, c->st, c->scal);
+  VB_LAUNCH(c, k_pressure_update, grid_1d(c->npfb), c->g, c->st, c->cnt, c->link, c->wg1, c->wg2, c->k11,
+            c->k12, c->k21, c->k22, c->gpx, c->gpy, c->scal);
+  return VB_OK;
+}
+
+vb_status launch_final(vb_ctx* c) {
+  int nf = c->npfb - c->nbS;
+  if (nf > 0) VB_LAUNCH(c, k_final, grid_1d(nf), c->g, c->st, c->usx, c->usy, c->scal);
+  return VB_OK;
+}
+
+}  // namespace vb

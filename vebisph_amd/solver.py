@@ -1,0 +1,242 @@
+"""Host-side mirror of the reference's step interface over the gfx950 C ABI.
+
+`Solver` keeps the reference's routine names (div_free.F90:22-124 and the subroutines it calls), so a
+driver written against the reference reads the same:
+
+    s = Solver(case)
+    s.div_free()                         # one step (sph2D_incom.F90:97)
+    # or phase by phase:
+    s.set_time_step(); s.put_in_bin(); s.create_mirror_particles(); s.particle_allocation()
+    s.kernel_calculation(); s.locate_free_surf_part(); s.new_fick_shift(); s.prediction_step()
+    s.mirror_velocities(); s.ppe_solve_in_house_simple_dirichlet(); s.calc_pressure_gradient()
+    s.final_advection()
+
+Errors that STOP the reference raise VbError; the BiCGSTAB breakdown / iteration-limit messages of
+the reference become warnings recorded in `last_warning`.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import NPLINK, VbBoundary, VbError, VbParams
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp) if a is not None else None
+
+
+def _i(a):
+    return a.ctypes.data_as(_ip) if a is not None else None
+
+
+class Solver:
+    """One gfx950 context holding a Case's particle state in HBM."""
+
+    def __init__(self, case, precision=_lib.PREC_FP64, reference_kick=True, time=0.0, itime=0,
+                 state=None):
+        L = _lib.lib()
+        self.case = case
+        p = VbParams()
+        p.h, p.dx, p.dv = case.h, case.dx, case.dv
+        p.Ra, p.Pr, p.El, p.beta = case.Ra, case.Pr, case.El, case.beta
+        p.body_force[0], p.body_force[1] = case.body_force
+        p.dt_coef_cfl, p.dt_coef_visc = case.dt_coef_cfl, case.dt_coef_visc
+        p.ve_nonlinearity = case.ve_nonlinearity
+        p.time, p.itime = time, itime
+        p.i_open_domain, p.i_shift_coeff = case.i_open_domain, case.i_shift_coeff
+        p.precision = int(precision)
+        p.reference_kick = 1 if reference_kick else 0
+        self._keep = []
+        f64 = lambda a: self._hold(np.ascontiguousarray(a, dtype=np.float64))
+        i32 = lambda a: self._hold(np.ascontiguousarray(a, dtype=np.int32))
+        b = VbBoundary()
+        b.nb_patches = case.nb_patches
+        b.b_node, b.b_type = _d(f64(case.b_node)), _i(i32(case.b_type))
+        b.b_vel, b.b_thermal = _d(f64(case.b_vel)), _d(f64(case.b_thermal))
+        b.b_periodic_parent = _i(i32(case.b_periodic_parent))
+        b.nb_circles = case.nb_circles
+        if case.nb_circles:
+            b.c_centre, b.c_radius = _d(f64(case.c_centre)), _d(f64(case.c_radius))
+            b.c_omega, b.c_vel = _d(f64(case.c_omega)), _d(f64(case.c_vel))
+            b.c_thermal = _d(f64(case.c_thermal))
+        h = C.c_void_p()
+        st = L.vb_create(C.byref(p), C.byref(b), case.npfb, case.nbS, C.byref(h))
+        if st != 0:
+            raise VbError(st, "vb_create failed")
+        self.h = h
+        self.last_warning = 0
+        n = case.npfb
+        if state is None:
+            state = dict(x=case.x, y=case.y, u=case.u, v=case.v, theta=case.theta,
+                         tau=np.zeros((n, 4)), trace=np.full(n, 2.0), p=np.zeros(n),
+                         inbin=np.zeros(n, np.int32))
+        self.set_state(state)
+
+    def _hold(self, a):
+        self._keep.append(a)
+        return a
+
+    def close(self):
+        if getattr(self, "h", None):
+            _lib.lib().vb_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, st):
+        if st in _lib.WARNINGS:
+            self.last_warning = st
+            return st
+        if st != 0:
+            msg = _lib.lib().vb_last_error(self.h)
+            raise VbError(st, msg.decode() if msg else "")
+        return st
+
+    # ---- state ----
+    def set_state(self, s):
+        n = self.case.npfb
+        c = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+        tau = np.asarray(s["tau"], dtype=np.float64)
+        cols = [c(tau[:, q]) for q in range(4)]
+        arrs = [c(s["x"]), c(s["y"]), c(s["u"]), c(s["v"]), c(s["theta"]), *cols, c(s["trace"]), c(s["p"])]
+        inb = np.ascontiguousarray(s["inbin"], dtype=np.int32)
+        self._chk(_lib.lib().vb_upload_state(self.h, n, *[_d(a) for a in arrs], _i(inb)))
+
+    def get_state(self, n=None):
+        n = self.case.npfb if n is None else n
+        out = {k: np.zeros(n) for k in ("x", "y", "u", "v", "theta", "trace", "p")}
+        taus = [np.zeros(n) for _ in range(4)]
+        inb = np.zeros(n, np.int32)
+        self._chk(_lib.lib().vb_download_state(
+            self.h, n, _d(out["x"]), _d(out["y"]), _d(out["u"]), _d(out["v"]), _d(out["theta"]),
+            *[_d(t) for t in taus], _d(out["trace"]), _d(out["p"]), _i(inb)))
+        out["tau"] = np.stack(taus, axis=1)
+        out["inbin"] = inb
+        return out
+
+    def scalars(self):
+        ints = np.zeros(12, np.int32); dbls = np.zeros(8)
+        self._chk(_lib.lib().vb_get_scalars(self.h, _i(ints), _d(dbls)))
+        keys_i = ("np", "npfb", "nbS", "nmirror", "maxneighbours", "LS_iters", "itime", "n_inbin",
+                  "ncx", "ncy", "nct", "status")
+        keys_d = ("dt", "LS_residual", "time", "umax", "x_mint", "x_maxt", "y_mint", "y_maxt")
+        d = {k: int(v) for k, v in zip(keys_i, ints)}
+        d.update({k: float(v) for k, v in zip(keys_d, dbls)})
+        return d
+
+    # ---- the step (reference names) ----
+    def set_time_step(self):
+        return self._chk(_lib.lib().vb_set_time_step(self.h))
+
+    def put_in_bin(self):
+        return self._chk(_lib.lib().vb_put_in_bin(self.h))
+
+    def create_mirror_particles(self):
+        return self._chk(_lib.lib().vb_create_mirror_particles(self.h))
+
+    def particle_allocation(self):
+        return self._chk(_lib.lib().vb_particle_allocation(self.h))
+
+    def kernel_calculation(self):
+        return self._chk(_lib.lib().vb_kernel_calculation(self.h))
+
+    def locate_free_surf_part(self):
+        return self._chk(_lib.lib().vb_locate_free_surf_part(self.h))
+
+    def new_fick_shift(self):
+        return self._chk(_lib.lib().vb_new_fick_shift(self.h))
+
+    def prediction_step(self):
+        return self._chk(_lib.lib().vb_prediction_step(self.h))
+
+    def mirror_velocities(self):
+        return self._chk(_lib.lib().vb_mirror_velocities(self.h))
+
+    def ppe_solve_in_house_simple_dirichlet(self, tol=1e-8, itermax=1000):
+        it = C.c_int32(0); res = C.c_double(0.0)
+        self._chk(_lib.lib().vb_ppe_solve(self.h, tol, itermax, C.byref(it), C.byref(res)))
+        return it.value, res.value
+
+    def calc_pressure_gradient(self):
+        return self._chk(_lib.lib().vb_pressure_update(self.h))
+
+    def final_advection(self):
+        return self._chk(_lib.lib().vb_final_advection(self.h))
+
+    def div_free(self):
+        return self._chk(_lib.lib().vb_div_free(self.h))
+
+    step = div_free
+
+    def phase(self, k):
+        """Phase k of the same 12-way split the reference harness uses."""
+        fns = [self.set_time_step, self.put_in_bin, self.create_mirror_particles,
+               self.particle_allocation, self.kernel_calculation, self.locate_free_surf_part,
+               self.new_fick_shift, self.prediction_step, self.mirror_velocities,
+               self.ppe_solve_in_house_simple_dirichlet, self.calc_pressure_gradient,
+               self.final_advection]
+        return fns[k - 1]()
+
+    # ---- exports (parity checks) ----
+    def mirrors(self):
+        m = self.scalars()["nmirror"]
+        irel = np.zeros(m, np.int32); vrel = np.zeros(m, np.int32); dp = np.zeros(m)
+        self._chk(_lib.lib().vb_export_mirrors(self.h, m, _i(irel), _i(vrel), _d(dp)))
+        return irel, vrel, dp
+
+    def neighbours(self, w=NPLINK):
+        n = self.case.npfb
+        cnt = np.zeros(n, np.int32); link = np.zeros(n * w, np.int32)
+        self._chk(_lib.lib().vb_export_neighbours(self.h, n, w, _i(cnt), _i(link)))
+        return cnt, link.reshape(w, n).T
+
+    def weights(self, w):
+        n = self.case.npfb
+        wg = np.zeros(2 * w * n); wl = np.zeros(w * n); kg = np.zeros(4 * n)
+        self._chk(_lib.lib().vb_export_weights(self.h, n, w, _d(wg), _d(wl), _d(kg)))
+        return (wg.reshape(2, w, n).transpose(2, 1, 0), wl.reshape(w, n).T, kg.reshape(4, n).T)
+
+    def ushift(self):
+        n = self.case.npfb
+        us = np.zeros(2 * n)
+        self._chk(_lib.lib().vb_export_ushift(self.h, n, _d(us)))
+        return us.reshape(2, n).T
+
+    def grad_p(self):
+        n = self.case.npfb
+        g = np.zeros(2 * n)
+        self._chk(_lib.lib().vb_export_grad_p(self.h, n, _d(g)))
+        return g.reshape(2, n).T
+
+    def ppe_ell(self, w):
+        n = self.case.npfb
+        diag = np.zeros(n); col = np.zeros(w * n, np.int32); val = np.zeros(w * n); rhs = np.zeros(n)
+        self._chk(_lib.lib().vb_export_ppe(self.h, n, w, _d(diag), _i(col), _d(val), _d(rhs)))
+        return diag, col.reshape(w, n).T, val.reshape(w, n).T, rhs
+
+
+def ell_to_nr(diag, col, val, cnt):
+    """Reference Numerical-Recipes row-indexed arrays (1-based ija) from the ELL export."""
+    n = diag.shape[0]
+    nnz = n + 2 + int(cnt.sum())
+    a = np.zeros(nnz); ija = np.zeros(nnz, np.int32)
+    a[:n] = diag
+    pos = n + 2
+    for i in range(n):
+        ija[i] = pos
+        m = int(cnt[i])
+        a[pos - 1:pos - 1 + m] = val[i, :m]
+        ija[pos - 1:pos - 1 + m] = col[i, :m]
+        pos += m
+    ija[n] = pos
+    return a, ija
