@@ -56,6 +56,8 @@ typedef struct {
   int32_t i_shift_coeff;         /* must be 0 (D = 0.25 h^2) */
   int32_t precision;             /* vb_precision */
   int32_t reference_kick;        /* 1: reproduce predictor_mod.F90:369-371 as the reference runs it */
+  int32_t serial_reductions;     /* 1: global sums in index order on one thread, as the reference's
+                                    single-thread loops (bit-for-bit parity runs; slow at scale) */
 } vb_params;
 
 /* Boundary patches and circles: IBOUND, input.F90:51-81.  b_node is (nb_patches,2) row-major;
@@ -142,6 +144,13 @@ vb_status vb_export_grad_p(vb_ctx* ctx, int32_t n, double* grad_p);      /* (n,2
  * beyond ij_count are 0), rhs(n).  In the reference's Numerical-Recipes storage row i holds
  * A(i)=diag and, at ija(i)+k-1, the k-th slot. */
 vb_status vb_export_ppe(vb_ctx* ctx, int32_t n, int32_t w, double* diag, int32_t* col, double* val, double* rhs);
+
+/* ---- measurement (no reference counterpart) ---- */
+/* Times `reps` launches of the BiCGSTAB SpMV kernel on the context's current PPE matrix with HIP
+ * events on the context's stream; returns the mean duration (microseconds) and the algorithmic HBM
+ * bytes of one launch (row data + ELL columns/values of the nnz actually used, gathered vector
+ * counted once).  Leaves the particle state untouched; call after a step. */
+vb_status vb_profile_spmv(vb_ctx* ctx, int32_t reps, double* avg_us, double* alg_bytes);
 
 /* ---- stand-alone device kernels on caller-owned device memory (stream 0) ---- */
 /* nnf_models_mod.F90:8-110 (no call sites in the reference): model 0 newtonian (copies mu_or_sr),

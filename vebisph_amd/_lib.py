@@ -24,8 +24,8 @@ EXPORTS = [
     "vb_kernel_calculation", "vb_locate_free_surf_part", "vb_new_fick_shift", "vb_prediction_step",
     "vb_mirror_velocities", "vb_ppe_solve", "vb_pressure_update", "vb_final_advection", "vb_div_free",
     "vb_get_scalars", "vb_export_mirrors", "vb_export_neighbours", "vb_export_weights",
-    "vb_export_ushift", "vb_export_grad_p", "vb_export_ppe", "vb_nnf", "vb_kernel_functions",
-    "vb_logconf",
+    "vb_export_ushift", "vb_export_grad_p", "vb_export_ppe", "vb_profile_spmv", "vb_nnf",
+    "vb_kernel_functions", "vb_logconf",
 ]
 
 
@@ -35,7 +35,8 @@ class VbParams(C.Structure):
                 ("body_force", C.c_double * 2), ("dt_coef_cfl", C.c_double),
                 ("dt_coef_visc", C.c_double), ("ve_nonlinearity", C.c_double), ("time", C.c_double),
                 ("itime", C.c_int32), ("i_open_domain", C.c_int32), ("i_shift_coeff", C.c_int32),
-                ("precision", C.c_int32), ("reference_kick", C.c_int32)]
+                ("precision", C.c_int32), ("reference_kick", C.c_int32),
+                ("serial_reductions", C.c_int32)]
 
 
 class VbBoundary(C.Structure):
@@ -90,6 +91,7 @@ def lib():
         L.vb_export_ushift.argtypes = [vp, C.c_int32, dp]
         L.vb_export_grad_p.argtypes = [vp, C.c_int32, dp]
         L.vb_export_ppe.argtypes = [vp, C.c_int32, C.c_int32, dp, ip, dp, dp]
+        L.vb_profile_spmv.argtypes = [vp, C.c_int32, dp, dp]
         L.vb_nnf.argtypes = [C.c_int32, C.c_int32, vp, C.c_double, vp]
         L.vb_kernel_functions.argtypes = [C.c_int32, vp, C.c_double, C.c_double, vp, vp]
         L.vb_logconf.argtypes = [C.c_int32, vp, vp, vp, C.c_double, C.c_double, C.c_double,

@@ -67,7 +67,7 @@ static void free_all(vb_ctx* c) {
                   c->wg1, c->wg2, c->wl, c->k11, c->k12, c->k21, c->k22, c->wg1f, c->wg2f, c->wlf,
                   c->usx, c->usy, c->kick, c->gpx, c->gpy, c->a_diag, c->a_val, c->rhs, c->a_col,
                   c->a_valf, c->a_diagf, c->r, c->rb, c->vv, c->pp, c->pn, c->pinv, c->ss, c->s2,
-                  c->sinv, c->tt, c->fbuf, c->partials, c->scal};
+                  c->sinv, c->tt, c->fbuf, c->partials, c->red, c->scal};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (c->scal_host) (void)hipHostFree(c->scal_host);
@@ -185,6 +185,7 @@ vb_status vb_create(const vb_params* prm, const vb_boundary* bnd, int32_t npfb, 
   A(c->r, n); A(c->rb, n); A(c->vv, n); A(c->pp, n); A(c->pn, n); A(c->pinv, n); A(c->ss, n); A(c->s2, n);
   A(c->sinv, n); A(c->tt, n);
   A(c->partials, 4 * kMaxBlocks); A(c->scal, 1);
+  if (prm->serial_reductions) A(c->red, 2 * n);
 #undef A
   if (s == VB_OK) s = hip_check(c, hipHostMalloc((void**)&c->scal_host, sizeof(Scal)), "hipHostMalloc");
   if (s == VB_OK) {
@@ -333,6 +334,11 @@ vb_status vb_get_scalars(vb_ctx* c, int32_t* ints, double* dbls) {
   if (ints) memcpy(ints, iv, sizeof iv);
   if (dbls) memcpy(dbls, dv, sizeof dv);
   return VB_OK;
+}
+
+vb_status vb_profile_spmv(vb_ctx* c, int32_t reps, double* avg_us, double* alg_bytes) {
+  if (!c || reps < 1 || !avg_us || !alg_bytes) return VB_ERR_ARG;
+  return profile_spmv(c, reps, avg_us, alg_bytes);
 }
 
 vb_status vb_export_mirrors(vb_ctx* c, int32_t m, int32_t* irel, int32_t* vrel, double* dP) {

@@ -50,7 +50,6 @@ struct Scal {
   // BiCGSTAB
   double rho, rho0, alpha, omega, beta, n2s0, n2r0, resid, mean_p, mean_s, tol;
   int iter, itermax, done, exit_s, status;
-  unsigned counter;                        // last-block-done ticket
 };
 
 // Error bits accumulated on the device (vb_status codes as bit positions).
@@ -95,9 +94,14 @@ __host__ __device__ inline double wab(double qq, double ad7) {
 }
 
 // ------------------------------------------------------------------------------------------------
-// Deterministic reductions.  Wavefront tree by DPP/shuffle, then the block's 4 waves through LDS,
-// then one partial per block; the last block to finish (acq_rel ticket) folds the partials in
-// index order.  Same inputs and grid => same bits, run after run.
+// Deterministic reductions.  Wavefront tree by shuffles, the block's 4 waves through LDS, one
+// partial per block (plain store); a one-workgroup finalize kernel then folds the partials in a
+// fixed order and computes the next scalar.  The kernel boundary makes the partials visible, so no
+// device-scope fences are needed (an agent-scope release on gfx950 writes back the XCD's L2, which
+// per workgroup costs more than the reduction).  Same inputs and grid => same bits, run after run.
+//
+// Serial mode (parity runs): producers store every row's term instead, and the finalize sums them
+// in index order on one thread, i.e. exactly the reference's single-thread loops.
 // ------------------------------------------------------------------------------------------------
 __device__ inline double wave_sum(double v) {
 #pragma unroll
@@ -133,41 +137,57 @@ __device__ inline void block_sum(double (&v)[NV]) {
   __syncthreads();
 }
 
-// Writes this block's NV partials, takes a ticket, and returns true in ALL threads of the block that
-// finished last.  That block then sees every other block's partials (release/acquire at agent
-// scope, which also orders the per-XCD L2s) and must reset the ticket.
+// Per-row reduction terms: accumulated per thread, or stored per row in serial mode.
 template <int NV>
-__device__ inline bool publish_partials(double (&v)[NV], double* partials, unsigned* ticket) {
-  __shared__ bool last;
-  if (threadIdx.x == 0) {
+struct Red {
+  double acc[NV];
+  double* rows;      // non-null in serial mode: rows[q*n + i]
+  int n;
+  __device__ Red(double* r, int n_) : rows(r), n(n_) {
 #pragma unroll
-    for (int q = 0; q < NV; q++)
-      __hip_atomic_store(&partials[q * gridDim.x + blockIdx.x], v[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __threadfence();
-    unsigned t = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    last = (t == gridDim.x - 1);
-    if (last) __threadfence();
+    for (int q = 0; q < NV; q++) acc[q] = 0.0;
   }
-  __syncthreads();
-  return last;
-}
+  __device__ inline void add(int q, int i, double v) {
+    if (rows) rows[q * n + i] = v; else acc[q] += v;
+  }
+  // block partials (plain stores); partials[q*gridDim.x + blockIdx.x]
+  __device__ inline void store(double* partials) {
+    if (rows) return;
+    block_sum<NV>(acc);
+    if (threadIdx.x == 0) {
+#pragma unroll
+      for (int q = 0; q < NV; q++) partials[q * gridDim.x + blockIdx.x] = acc[q];
+    }
+  }
+};
 
-// In the last block: fold the partials of quantity q in block order; result in all threads.
-__device__ inline double fold_partials(const double* partials, int q, int nblk) {
+// One-workgroup fold (all kBlock threads call it): partials of quantity q in block order, or, in
+// serial mode, rows[q*n + 0..n-1] in index order.  Result in all threads.
+__device__ inline double fold(const double* partials, int q, int nblk, const double* rows, int n) {
   __shared__ double sh[kBlock];
-  double s = 0.0;
-  // each thread sums a contiguous run, then a fixed tree: deterministic for a given nblk
+  double r;
+  if (rows) {
+    if (threadIdx.x == 0) {
+      double s = 0.0;
+      for (int i = 0; i < n; i++) s = s + rows[q * n + i];
+      sh[0] = s;
+    }
+    __syncthreads();
+    r = sh[0];
+    __syncthreads();
+    return r;
+  }
   int per = (nblk + kBlock - 1) / kBlock;
   int b0 = threadIdx.x * per, b1 = min(nblk, b0 + per);
-  for (int b = b0; b < b1; b++)
-    s += __hip_atomic_load(&partials[q * nblk + b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  double s = 0.0;
+  for (int b = b0; b < b1; b++) s += partials[q * nblk + b];
   sh[threadIdx.x] = s;
   __syncthreads();
   for (int w = kBlock / 2; w > 0; w >>= 1) {
     if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
     __syncthreads();
   }
-  double r = sh[0];
+  r = sh[0];
   __syncthreads();
   return r;
 }

@@ -41,6 +41,7 @@ struct vb_ctx {
   float* fbuf;                       // FP32 work vectors of the inner solve (10 * npfb)
 
   double* partials;                  // reduction partials (4 * kMaxBlocks)
+  double* red;                       // per-row reduction terms (2 * npfb), serial mode only
   vb::Scal* scal;                    // device scalars
   vb::Scal* scal_host;               // pinned mirror
   int nmirror_host, np_host;
@@ -61,6 +62,7 @@ vb_status launch_pressure_update(vb_ctx* c);
 vb_status launch_final(vb_ctx* c);
 // PPE (vb_ppe.hip)
 vb_status launch_ppe(vb_ctx* c, double tol, int itermax);
+vb_status profile_spmv(vb_ctx* c, int reps, double* avg_us, double* alg_bytes);
 // helpers (vb_capi.hip)
 vb_status hip_check(vb_ctx* c, hipError_t e, const char* what);
 vb_status sync_scalars(vb_ctx* c);

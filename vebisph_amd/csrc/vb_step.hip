@@ -23,29 +23,25 @@ __global__ void k_umax(State st, int nbS, int npfb, double cfl_h, double dt_visc
   __shared__ double sh[kBlock / 64];
   if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
   __syncthreads();
-  double v[1] = {0.0};
   if (threadIdx.x == 0) {
     double b = sh[0];
     for (int w = 1; w < kBlock / 64; w++) b = sh[w] > b ? sh[w] : b;
-    v[0] = b;
+    partials[blockIdx.x] = b;
   }
-  if (publish_partials<1>(v, partials, &sc->counter)) {
-    if (threadIdx.x == 0) {
-      double um = 1.0e-10;
-      for (int b = 0; b < (int)gridDim.x; b++) {
-        double w = __hip_atomic_load(&partials[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (w > um) um = w;
-      }
-      sc->umax = um;
-      double dt_cfl = cfl_h / um;
-      sc->dt = dt_cfl < dt_visc ? dt_cfl : dt_visc;
-      sc->n_inbin = 0;
-      sc->err = 0;
-      sc->nmirror = 0;
-      sc->maxneigh = 0;
-      sc->counter = 0;
-    }
-  }
+}
+
+// max is order-independent, so one thread folds; also resets the step's counters
+__global__ void k_fin_umax(const double* partials, int nblk, double cfl_h, double dt_visc, Scal* sc) {
+  if (threadIdx.x != 0) return;
+  double um = 1.0e-10;
+  for (int b = 0; b < nblk; b++) um = partials[b] > um ? partials[b] : um;
+  sc->umax = um;
+  double dt_cfl = cfl_h / um;
+  sc->dt = dt_cfl < dt_visc ? dt_cfl : dt_visc;
+  sc->n_inbin = 0;
+  sc->err = 0;
+  sc->nmirror = 0;
+  sc->maxneigh = 0;
 }
 
 __global__ void k_advect(State st, int nbS, int npfb, const Scal* sc) {
@@ -768,6 +764,9 @@ vb_status launch_set_time_step(vb_ctx* c) {
   int nb = grid_for(c->npfb - c->nbS);
   VB_LAUNCH(c, k_umax, nb, c->st, c->nbS, c->npfb, c->prm.dt_coef_cfl * c->prm.h, c->dt_visc, c->scal,
             c->partials);
+  hipLaunchKernelGGL(k_fin_umax, dim3(1), dim3(64), 0, c->stream, c->partials, nb,
+                     c->prm.dt_coef_cfl * c->prm.h, c->dt_visc, c->scal);
+  VB_HIP(c, hipGetLastError());
   VB_LAUNCH(c, k_advect, grid_1d(c->npfb), c->st, c->nbS, c->npfb, c->scal);
   c->itime += 1;
   return VB_OK;

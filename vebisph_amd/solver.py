@@ -39,7 +39,7 @@ class Solver:
     """One gfx950 context holding a Case's particle state in HBM."""
 
     def __init__(self, case, precision=_lib.PREC_FP64, reference_kick=True, time=0.0, itime=0,
-                 state=None):
+                 state=None, serial_reductions=False):
         L = _lib.lib()
         self.case = case
         p = VbParams()
@@ -52,6 +52,7 @@ class Solver:
         p.i_open_domain, p.i_shift_coeff = case.i_open_domain, case.i_shift_coeff
         p.precision = int(precision)
         p.reference_kick = 1 if reference_kick else 0
+        p.serial_reductions = 1 if serial_reductions else 0
         self._keep = []
         f64 = lambda a: self._hold(np.ascontiguousarray(a, dtype=np.float64))
         i32 = lambda a: self._hold(np.ascontiguousarray(a, dtype=np.int32))
@@ -217,6 +218,12 @@ class Solver:
         g = np.zeros(2 * n)
         self._chk(_lib.lib().vb_export_grad_p(self.h, n, _d(g)))
         return g.reshape(2, n).T
+
+    def profile_spmv(self, reps=20):
+        """Mean duration (us) and algorithmic bytes of one BiCGSTAB SpMV launch."""
+        us = C.c_double(0.0); by = C.c_double(0.0)
+        self._chk(_lib.lib().vb_profile_spmv(self.h, reps, C.byref(us), C.byref(by)))
+        return us.value, by.value
 
     def ppe_ell(self, w):
         n = self.case.npfb
