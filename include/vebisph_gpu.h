@@ -121,6 +121,14 @@ vb_status vb_mirror_velocities(vb_ctx* ctx);
  * (linear_solver.F90:275-425).  tol <= 0 / itermax <= 0 select the reference's 1e-8 / 1000.
  * iters and residual may be NULL; when given, the call synchronises to return them. */
 vb_status vb_ppe_solve(vb_ctx* ctx, double tol, int32_t itermax, int32_t* iters, double* residual);
+/* SPARSE_PBICGSTAB_nonull (linear_solver.F90:275-285) on a caller's system, same argument meaning:
+ * A(nmax), ija(nmax) in Numerical-Recipes row-indexed storage (1-based, A(1:n) the diagonal,
+ * ija(1) = n+2), x(n) initial guess in / solution out, B(n).  Rows may hold at most VB_NPLINK
+ * off-diagonal slots.  Uses the context's PPE buffers (n must equal the context's npfb) and its
+ * reduction mode; returns VB_WARN_* like vb_ppe_solve. */
+vb_status vb_sparse_pbicgstab_nonull(vb_ctx* ctx, const double* A, const int32_t* ija, double* x,
+                                     const double* B, int32_t n, int32_t nmax, int32_t itermax,
+                                     double tol, int32_t* iter, double* residual);
 /* calc_pressure_gradient (gradient_calculation.F90:16-48) and div_free.F90:93-102 */
 vb_status vb_pressure_update(vb_ctx* ctx);
 /* div_free.F90:111-117 final advection; advances time */

@@ -99,24 +99,73 @@ def test_step_phases_against_oracle(gpu_lib, name):
 
 
 @pytest.mark.parametrize("name", list(SMALL))
-def test_serial_reductions_match_reference_order(gpu_lib, name):
-    """With the global sums taken in index order (the reference's single-thread loops), the GPU
-    reproduces the oracle's BiCGSTAB iterate for iterate: same iteration counts every step and
-    fields equal to ~1e-10 after several steps (what remains is ulps of the device exp/log/pow
-    against libm)."""
+def test_bicgstab_bit_exact_in_serial_order(gpu_lib, name):
+    """The GPU BiCGSTAB on the oracle's assembled PPE, with its global sums taken in index order (the
+    reference's single-thread loops): the same iterate, bit for bit, and the same iteration count
+    and residual as the reference algorithm (oracle, itself bit-exact against the reference)."""
     case = SMALL[name]()
-    G = solver.Solver(case, serial_reductions=True); O = ow.OracleSolver(case)
-    for _ in range(3):
-        G.div_free(); O.step()
-        sg, so = G.scalars(), O.scalars()
-        assert sg["LS_iters"] == so["LS_iters"]
-        assert sg["LS_residual"] == pytest.approx(so["LS_residual"], rel=1e-6)
-    a, b = G.get_state(), O.get_state()
-    for key in ("x", "y", "u", "v", "theta", "p"):
-        assert _rel(a[key], b[key]) <= 1e-10, key
-    # the conformation A = tau + I is O(1): its ulps are ~1e-16 absolute whatever the size of tau
-    for key in ("tau", "trace"):
-        assert np.max(np.abs(a[key] - b[key])) <= 1e-13, key
+    O = ow.OracleSolver(case)
+    O.step(); O.step()
+    for k in range(1, 11):
+        O.phase(k)
+    a, ija, b = O.ppe()
+    xo, ito, ro, sto = ow.bicgstab_nonull(a, ija, b)
+    G = solver.Solver(case, serial_reductions=True)
+    xg, itg, rg, stg = G.sparse_pbicgstab_nonull(a, ija, b)
+    assert (itg, stg) == (ito, sto)
+    assert rg == ro
+    _eq(xg, xo)
+    # and with a nonzero initial guess (AX of linear_solver.F90:306 is then a real product)
+    x0 = np.linspace(-1e-6, 1e-6, case.npfb)
+    xo2, ito2, ro2, _ = ow.bicgstab_nonull(a, ija, b, x0=x0, itermax=40)
+    xg2, itg2, rg2, _ = G.sparse_pbicgstab_nonull(a, ija, b, x0=x0, itermax=40)
+    assert itg2 == ito2 and rg2 == ro2
+    _eq(xg2, xo2)
+
+
+def test_bicgstab_parallel_order_same_solution(gpu_lib):
+    """The production (parallel-order) reductions on the same system: a different Krylov path,
+    the same stopping rule, and a solution within the tolerance-implied distance."""
+    case = SMALL["poiseuille"]()
+    O = ow.OracleSolver(case)
+    O.step(); O.step()
+    for k in range(1, 11):
+        O.phase(k)
+    a, ija, b = O.ppe()
+    xo, ito, ro, _ = ow.bicgstab_nonull(a, ija, b)
+    G = solver.Solver(case)
+    xg, itg, rg, stg = G.sparse_pbicgstab_nonull(a, ija, b)
+    assert stg == 0 and rg <= 1e-8
+    assert _rel(xg - xg.mean(), xo - xo.mean()) <= 1e-3
+
+
+@pytest.mark.parametrize("name", ["poiseuille", "convection", "cylinder"])
+def test_mixed_precision_fields(gpu_lib, name):
+    """FP32 matvecs inside the FP64 BiCGSTAB: ten steps against the FP64 oracle, held to the same
+    reference-sensitivity scale as the FP64 path, with the FP64 residual of the final solve checked
+    on the assembled system."""
+    from vebisph_amd import _lib
+    case = SMALL[name]()
+    O = ow.OracleSolver(case); O2 = ow.OracleSolver(case, state=_perturbed_state(case))
+    G = solver.Solver(case, precision=_lib.PREC_MIXED)
+    for _ in range(10):
+        O.step(); O2.step(); G.div_free()
+        assert G.scalars()["status"] in (0, 6)
+    a, b, b2 = G.get_state(), O.get_state(), O2.get_state()
+    for key in ("x", "y", "u", "v", "theta", "p", "tau", "trace"):
+        spread = _rel(b2[key], b[key])
+        err = _rel(a[key], b[key])
+        print(f"mixed {key}: vs oracle {err:.2e}, oracle one-ulp spread {spread:.2e}")
+        assert err <= max(10.0 * spread, 1e-12), key
+    # the mixed solve meets the reference criterion in FP64 terms on its own system
+    for k in range(1, 11):
+        G.phase(k)
+    w = G.scalars()["maxneighbours"]
+    diag, col, val, rhs = G.ppe_ell(w)
+    a_nr, ija = solver.ell_to_nr(diag, col, val, G.neighbours()[0])
+    p = G.get_state()["p"]
+    r = rhs - ow.spmv_nr(a_nr, ija, p)
+    assert np.linalg.norm(r - r.mean()) <= 1e-5 * np.linalg.norm(rhs)
 
 
 def test_gpu_pressure_solves_assembled_system(gpu_lib):
@@ -139,17 +188,31 @@ def test_gpu_pressure_solves_assembled_system(gpu_lib):
     assert _rel(p - p.mean(), x - x.mean()) <= 1e-3
 
 
-def test_multi_step_fields(gpu_lib):
-    """Ten steps of the Poiseuille case against the oracle (tolerances from the PPE tolerance)."""
-    case = SMALL["poiseuille"]()
-    G = solver.Solver(case); O = ow.OracleSolver(case)
+def _perturbed_state(case):
+    """The initial state with every coordinate moved by one ulp."""
+    n = case.npfb
+    return dict(x=np.nextafter(case.x, np.inf), y=np.nextafter(case.y, np.inf), u=case.u, v=case.v,
+                theta=case.theta, tau=np.zeros((n, 4)), trace=np.full(n, 2.0), p=np.zeros(n),
+                inbin=np.zeros(n, np.int32))
+
+
+@pytest.mark.parametrize("name", ["poiseuille", "convection"])
+def test_multi_step_fields_within_reference_sensitivity(gpu_lib, name):
+    """Ten steps against the oracle.  The reference's BiCGSTAB stops at a loose relative tolerance on
+    an ill-conditioned system, so its own output moves by a measurable amount when its input moves
+    by one ulp; that spread, measured here with the oracle, is the scale the GPU is held to (any
+    reduction order, thread count or libm differs from the reference's by at least that much)."""
+    case = SMALL[name]()
+    O = ow.OracleSolver(case); O2 = ow.OracleSolver(case, state=_perturbed_state(case))
+    G = solver.Solver(case)
     for _ in range(10):
-        G.div_free(); O.step()
-    a, b = G.get_state(), O.get_state()
-    assert _rel(a["x"], b["x"]) <= 1e-12
-    assert _rel(a["u"], b["u"]) <= 1e-5
-    assert _rel(a["p"], b["p"]) <= 1e-3
-    assert _rel(a["tau"], b["tau"]) <= 1e-4
+        O.step(); O2.step(); G.div_free()
+    a, b, b2 = G.get_state(), O.get_state(), O2.get_state()
+    for key in ("x", "y", "u", "v", "theta", "p", "tau", "trace"):
+        spread = _rel(b2[key], b[key])
+        err = _rel(a[key], b[key])
+        print(f"{key}: gpu-vs-oracle {err:.2e}, oracle one-ulp spread {spread:.2e}")
+        assert err <= max(10.0 * spread, 1e-12), key
     assert G.scalars()["time"] == O.scalars()["time"]
 
 
@@ -160,7 +223,11 @@ def test_large_case_invariants(gpu_lib):
     print(f"convection_1M npfb={case.npfb}", flush=True)
     G.div_free()
     s = G.scalars()
-    assert s["status"] == 0 and s["LS_residual"] <= 1e-8
+    # at 1M particles the unpreconditioned BiCGSTAB, like the reference's, may stop at itermax
+    assert s["status"] in (0, 6) and 1 <= s["LS_iters"] <= 1001
+    for k in range(1, 6):          # next step up to the kernel weights: positions still match them
+        G.phase(k)
+    s = G.scalars()
     w = s["maxneighbours"]
     cnt, link = G.neighbours(w)
     n = case.npfb
@@ -185,29 +252,22 @@ def test_large_case_invariants(gpu_lib):
         assert kg[q].reshape(2, 2) @ g == pytest.approx([-1.25, 0.75], rel=1e-9, abs=1e-9)
 
 
-def _dev(a):
-    import torch
-    return torch.as_tensor(np.ascontiguousarray(a), dtype=torch.float64, device="cuda:0")
-
-
 def test_standalone_kernels(gpu_lib):
-    import ctypes as C
-    import torch  # noqa: F401
+    from tests.hipmem import DeviceArray as D
     from vebisph_amd import _lib
     L = _lib.lib()
     h = 0.013
     ad7 = 7.0 / 478.0 / np.pi / h / h
     q = np.linspace(0.0, 3.2, 10001)
-    dq = _dev(q); df = _dev(np.zeros_like(q)); dw = _dev(np.zeros_like(q))
-    assert L.vb_kernel_functions(q.size, C.c_void_p(dq.data_ptr()), ad7, ad7 / h,
-                                 C.c_void_p(df.data_ptr()), C.c_void_p(dw.data_ptr())) == 0
+    dq = D(q); df = D(np.zeros_like(q)); dw = D(np.zeros_like(q))
+    assert L.vb_kernel_functions(q.size, dq.ptr, ad7, ad7 / h, df.ptr, dw.ptr) == 0
     f, w = ow.fac_wab(q, ad7, ad7 / h)
-    _eq(df.cpu().numpy(), f); _eq(dw.cpu().numpy(), w)
+    _eq(df.numpy(), f); _eq(dw.numpy(), w)
     sr = np.array([0.0, 0.3, 2.0, 50.0, 1e4])
     for model in range(5):
-        ds = _dev(sr); do = _dev(np.zeros_like(sr))
-        assert L.vb_nnf(model, sr.size, C.c_void_p(ds.data_ptr()), 1.7, C.c_void_p(do.data_ptr())) == 0
-        assert np.allclose(do.cpu().numpy(), ow.nnf(model, sr, 1.7), rtol=1e-14, atol=0)
+        ds = D(sr); do = D(np.zeros_like(sr))
+        assert L.vb_nnf(model, sr.size, ds.ptr, 1.7, do.ptr) == 0
+        assert np.allclose(do.numpy(), ow.nnf(model, sr, 1.7), rtol=1e-14, atol=0)
     rng = np.random.default_rng(3)
     n = 4096
     tau = np.zeros((n, 4)); tau[:, 0] = rng.uniform(-0.2, 0.6, n); tau[:, 3] = rng.uniform(-0.2, 0.6, n)
@@ -216,14 +276,12 @@ def test_standalone_kernels(gpu_lib):
     trc = 2.0 + tau[:, 0] + tau[:, 3]
     gv = rng.uniform(-2, 2, (n, 4))
     for kick in (0, 1):
-        dt_ = _dev(tau); dtr = _dev(trc); dg = _dev(gv); do = _dev(np.zeros_like(tau)); dto = _dev(np.zeros(n))
-        assert L.vb_logconf(n, C.c_void_p(dt_.data_ptr()), C.c_void_p(dtr.data_ptr()),
-                            C.c_void_p(dg.data_ptr()), 1e-3, 0.8, 0.0, kick,
-                            C.c_void_p(do.data_ptr()), C.c_void_p(dto.data_ptr())) == 0
+        dt_ = D(tau); dtr = D(trc); dg = D(gv); do = D(np.zeros_like(tau)); dto = D(np.zeros(n))
+        assert L.vb_logconf(n, dt_.ptr, dtr.ptr, dg.ptr, 1e-3, 0.8, 0.0, kick, do.ptr, dto.ptr) == 0
         to, tr = ow.logconf(tau, trc, gv, 1e-3, 0.8, 0.0, kick=bool(kick))
         # exp/log of the device library against libm: a few ulps on O(1) conformation entries
-        assert np.allclose(do.cpu().numpy(), to, rtol=0, atol=1e-13)
-        assert np.allclose(dto.cpu().numpy(), tr, rtol=1e-14, atol=0)
+        assert np.allclose(do.numpy(), to, rtol=0, atol=1e-13)
+        assert np.allclose(dto.numpy(), tr, rtol=1e-14, atol=0)
 
 
 def test_error_all_binned(gpu_lib):

@@ -42,22 +42,25 @@ def _run(cmd, cwd=None):
     return r.stdout
 
 
-def build_hip(jobs: int = 8) -> str:
-    os.makedirs(BUILD, exist_ok=True)
+def build_hip(jobs: int = 8, out: str = LIB, defines: tuple = ()) -> str:
+    """Compile csrc/*.hip into `out`.  `defines` (e.g. "VB_SPMV_BLOCK=256") override the launch
+    constants of vb_common.h, for launch-shape sweeps into libraries outside the package."""
+    bdir = BUILD if out == LIB else out + ".build"
+    os.makedirs(bdir, exist_ok=True)
     srcs = sorted(f for f in os.listdir(CSRC) if f.endswith(".hip"))
-    objs = []
+    dflags = [f"-D{d}" for d in defines]
 
     def one(src):
-        obj = os.path.join(BUILD, src[:-4] + ".o")
-        _run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
+        obj = os.path.join(bdir, src[:-4] + ".o")
+        _run([HIPCC, *HIP_FLAGS, *dflags, "-c", os.path.join(CSRC, src), "-o", obj])
         return obj
 
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
         objs = list(ex.map(one, srcs))
-    tmp = LIB + ".tmp"
+    tmp = out + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, out)
+    return out
 
 
 def build_fortran() -> str | None:

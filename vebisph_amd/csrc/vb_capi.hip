@@ -64,10 +64,11 @@ static void free_all(vb_ctx* c) {
                   c->st.t22, c->st.trc, c->st.p, c->st.inbin, c->st.x0, c->st.y0, c->st.u0, c->st.v0,
                   c->st.irel, c->st.vrel, c->st.dP, c->u_alt, c->v_alt, c->th_alt, c->nsurf,
                   c->seg_counts, c->cellkey, c->cellslot, c->nc, c->ist, c->ip, c->cnt, c->link,
-                  c->wg1, c->wg2, c->wl, c->k11, c->k12, c->k21, c->k22, c->wg1f, c->wg2f, c->wlf,
-                  c->usx, c->usy, c->kick, c->gpx, c->gpy, c->a_diag, c->a_val, c->rhs, c->a_col,
-                  c->a_valf, c->a_diagf, c->r, c->rb, c->vv, c->pp, c->pn, c->pinv, c->ss, c->s2,
-                  c->sinv, c->tt, c->fbuf, c->partials, c->red, c->scal};
+                  c->wg1, c->wg2, c->wl, c->k11, c->k12, c->k21, c->k22,
+                  c->usx, c->usy, c->kick, c->gpx, c->gpy, c->a_diag, c->a_val, c->a_rowsum, c->rhs,
+                  c->a_col, c->a_len, c->rowflag, c->lnk4, c->val4, c->val4f, c->r, c->rb, c->vv,
+                  c->pn, c->pinv, c->ss, c->sinv, c->tt,
+                  c->xsys, c->partials, c->red, c->scal};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (c->scal_host) (void)hipHostFree(c->scal_host);
@@ -167,7 +168,6 @@ vb_status vb_create(const vb_params* prm, const vb_boundary* bnd, int32_t npfb, 
   if (s == VB_OK) s = build_geom(c, prm, bnd);
   if (s == VB_OK) s = hip_check(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "stream");
   const size_t cap = c->cap, n = npfb, nw = (size_t)npfb * kNplink;
-  const bool mixed = prm->precision == VB_PREC_MIXED;
   State& st = c->st;
 #define A(p, cnt) if (s == VB_OK) s = dalloc(c, &(p), (cnt))
   A(st.x, cap); A(st.y, cap); A(st.u, cap); A(st.v, cap); A(st.th, cap); A(st.t11, cap); A(st.t12, cap);
@@ -179,13 +179,16 @@ vb_status vb_create(const vb_params* prm, const vb_boundary* bnd, int32_t npfb, 
   A(c->cellkey, cap); A(c->cellslot, cap); A(c->nc, c->g.nct + 1); A(c->ist, c->g.nct + 1); A(c->ip, cap);
   A(c->cnt, n); A(c->link, nw); A(c->wg1, nw); A(c->wg2, nw); A(c->wl, nw);
   A(c->k11, n); A(c->k12, n); A(c->k21, n); A(c->k22, n);
-  if (mixed) { A(c->wg1f, nw); A(c->wg2f, nw); A(c->wlf, nw); A(c->a_valf, nw); A(c->a_diagf, n); A(c->fbuf, 12 * n); }
   A(c->usx, n); A(c->usy, n); A(c->kick, n); A(c->gpx, n); A(c->gpy, n);
-  A(c->a_diag, n); A(c->a_val, nw); A(c->rhs, n); A(c->a_col, nw);
-  A(c->r, n); A(c->rb, n); A(c->vv, n); A(c->pp, n); A(c->pn, n); A(c->pinv, n); A(c->ss, n); A(c->s2, n);
-  A(c->sinv, n); A(c->tt, n);
-  A(c->partials, 4 * kMaxBlocks); A(c->scal, 1);
-  if (prm->serial_reductions) A(c->red, 2 * n);
+  A(c->a_diag, n); A(c->a_val, nw); A(c->a_rowsum, n); A(c->rhs, n); A(c->a_col, nw); A(c->a_len, n);
+  // compact grouped rows: up to VB_NPLINK slots rounded up to a multiple of 4
+  const size_t nw4 = (size_t)npfb * ((kNplink + 3) & ~3);
+  A(c->rowflag, n); A(c->lnk4, nw4); A(c->val4, nw4);
+  if (prm->precision == VB_PREC_MIXED) A(c->val4f, nw4);
+  A(c->r, n); A(c->rb, n); A(c->vv, n); A(c->pn, n); A(c->pinv, n); A(c->ss, n); A(c->sinv, n);
+  A(c->tt, n); A(c->xsys, n);
+  A(c->partials, (size_t)kSolverStages * 2 * kMaxBlocks); A(c->scal, 1);
+  if (prm->serial_reductions) A(c->red, (size_t)kSolverStages * 2 * n);
 #undef A
   if (s == VB_OK) s = hip_check(c, hipHostMalloc((void**)&c->scal_host, sizeof(Scal)), "hipHostMalloc");
   if (s == VB_OK) {
@@ -206,6 +209,9 @@ vb_status vb_create(const vb_params* prm, const vb_boundary* bnd, int32_t npfb, 
   double a = sqrt(prm->Ra / prm->Pr), b = sqrt(prm->Ra * prm->Pr);
   c->dt_visc = prm->dt_coef_visc * prm->h * prm->h * (a < b ? a : b);
   c->np_host = npfb; c->nmirror_host = 0;
+  // VB_SPMV=stored: every SpMV row from the stored matrix (measurement switch; same results)
+  const char* sp = getenv("VB_SPMV");
+  c->spmv_stored = sp && strcmp(sp, "stored") == 0;
   if (s != VB_OK) {
     free_all(c);
     delete c;
@@ -290,6 +296,19 @@ vb_status vb_ppe_solve(vb_ctx* c, double tol, int32_t itermax, int32_t* iters, d
     if (iters) *iters = c->scal_host->iter;
     if (residual) *residual = c->scal_host->resid;
   }
+  return (vb_status)c->scal_host->status;
+}
+
+vb_status vb_sparse_pbicgstab_nonull(vb_ctx* c, const double* A, const int32_t* ija, double* x,
+                                     const double* B, int32_t n, int32_t nmax, int32_t itermax,
+                                     double tol, int32_t* iter, double* residual) {
+  if (!c || !A || !ija || !x || !B || n < 1 || nmax < n + 1) return VB_ERR_ARG;
+  if (ija[0] != n + 2) { c->err = "Sparse storage mode is not configured properly"; return VB_ERR_ARG; }
+  if (ija[n] > nmax + 1) { c->err = "ija(n+1) beyond nmax"; return VB_ERR_ARG; }
+  vb_status s = solve_nr(c, A, ija, x, B, n, itermax, tol);
+  if (s != VB_OK) return s;
+  if (iter) *iter = c->scal_host->iter;
+  if (residual) *residual = c->scal_host->resid;
   return (vb_status)c->scal_host->status;
 }
 

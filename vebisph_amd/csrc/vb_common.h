@@ -13,8 +13,20 @@
 
 namespace vb {
 
+#ifndef VB_SPMV_BLOCK
+#define VB_SPMV_BLOCK 512
+#endif
+#ifndef VB_MAX_BLOCKS
+#define VB_MAX_BLOCKS 2048
+#endif
 constexpr int kBlock = 256;                // threads per workgroup (4 wavefronts of 64)
-constexpr int kMaxBlocks = 2048;           // grid cap for reduction kernels (8 per CU)
+// SpMV workgroup size (multiple of 64, at most 1024).  Measured on the 1M-row PPE: one row per
+// thread (2048 x 512) beats 2 rows per thread by 8-15%; the gathers need every wave in flight.
+constexpr int kSpmvBlock = VB_SPMV_BLOCK;
+constexpr int kMaxBlocks = VB_MAX_BLOCKS;  // grid cap for the solver's row kernels (multiple of 8)
+static_assert(kSpmvBlock % 64 == 0 && kSpmvBlock <= 1024, "SpMV block");
+static_assert(kMaxBlocks % 8 == 0, "grid cap");
+constexpr int kSolverStages = 6;           // BiCGSTAB reduction stages with their own buffers
 constexpr int kNplink = VB_NPLINK;
 constexpr double kEps = 2.220446049250313e-16;   // epsilon(1d0)
 
@@ -115,11 +127,11 @@ __device__ inline double wave_max(double v) {
   return v;
 }
 
-// Sum of NV values over the block; result valid in thread 0.
+// Sum of NV values over the block (any multiple of 64 threads up to 1024); result in thread 0.
 template <int NV>
 __device__ inline void block_sum(double (&v)[NV]) {
-  __shared__ double sh[NV][kBlock / 64];
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __shared__ double sh[NV][16];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
 #pragma unroll
   for (int q = 0; q < NV; q++) {
     double w = wave_sum(v[q]);
@@ -130,7 +142,7 @@ __device__ inline void block_sum(double (&v)[NV]) {
 #pragma unroll
     for (int q = 0; q < NV; q++) {
       double s = sh[q][0];
-      for (int w = 1; w < kBlock / 64; w++) s += sh[q][w];
+      for (int w = 1; w < nw; w++) s += sh[q][w];
       v[q] = s;
     }
   }
@@ -201,7 +213,7 @@ __device__ inline RowRange block_rows(int n, int nblk) {
   int lb = b;
   if ((nblk & 7) == 0) lb = (b & 7) * (nblk >> 3) + (b >> 3);
   int chunk = (n + nblk - 1) / nblk;
-  chunk = (chunk + kBlock - 1) / kBlock * kBlock;
+  chunk = (chunk + blockDim.x - 1) / blockDim.x * blockDim.x;
   int r0 = lb * chunk;
   int r1 = min(n, r0 + chunk);
   return {r0, r1};

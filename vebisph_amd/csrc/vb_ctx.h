@@ -28,20 +28,22 @@ struct vb_ctx {
   int *cnt, *link;                   // link[k*npfb + i], 0-based
   double *wg1, *wg2, *wl;            // slot-major like link
   double *k11, *k12, *k21, *k22;
-  float *wg1f, *wg2f, *wlf;          // FP32 copies for the mixed-precision sums
   double *usx, *usy;
   double* kick;                      // per-particle B perturbation (reference_kick)
   double *gpx, *gpy;
 
   // PPE (ELL, slot-major) and BiCGSTAB vectors
-  double *a_diag, *a_val, *rhs;
-  int* a_col;
-  float *a_valf, *a_diagf;           // FP32 matrix for the mixed inner solve
-  double *r, *rb, *vv, *pp, *pn, *pinv, *ss, *s2, *sinv, *tt;
-  float* fbuf;                       // FP32 work vectors of the inner solve (10 * npfb)
+  double *a_diag, *a_val, *a_rowsum, *rhs;
+  int *a_col, *a_len;                // a_len: row lengths of a caller-supplied system
+  uint8_t* rowflag;                  // 1: SpMV uses the full stored row (mirror neighbours)
+  int16_t* lnk4;                     // compact rows, grouped by 4 slots: column offsets
+  double* val4;                      //   FP64 values
+  float* val4f;                      //   FP32 values (mixed path)
+  bool spmv_stored;                  // full stored rows everywhere (VB_SPMV=stored)
+  double *r, *rb, *vv, *pn, *pinv, *ss, *sinv, *tt, *xsys;
 
-  double* partials;                  // reduction partials (4 * kMaxBlocks)
-  double* red;                       // per-row reduction terms (2 * npfb), serial mode only
+  double* partials;                  // reduction partials, 2 * kMaxBlocks per solver stage
+  double* red;                       // per-row reduction terms, 2 * npfb per stage (serial mode only)
   vb::Scal* scal;                    // device scalars
   vb::Scal* scal_host;               // pinned mirror
   int nmirror_host, np_host;
@@ -63,6 +65,8 @@ vb_status launch_final(vb_ctx* c);
 // PPE (vb_ppe.hip)
 vb_status launch_ppe(vb_ctx* c, double tol, int itermax);
 vb_status profile_spmv(vb_ctx* c, int reps, double* avg_us, double* alg_bytes);
+vb_status solve_nr(vb_ctx* c, const double* A_nr, const int* ija, double* x, const double* B, int n,
+                   int itermax, double tol);
 // helpers (vb_capi.hip)
 vb_status hip_check(vb_ctx* c, hipError_t e, const char* what);
 vb_status sync_scalars(vb_ctx* c);
@@ -77,5 +81,11 @@ vb_status sync_scalars(vb_ctx* c);
 #define VB_LAUNCH(c, kern, grid, ...)                                       \
   do {                                                                      \
     hipLaunchKernelGGL(kern, dim3(grid), dim3(vb::kBlock), 0, (c)->stream, __VA_ARGS__); \
+    VB_HIP(c, hipGetLastError());                                           \
+  } while (0)
+
+#define VB_LAUNCH_BS(c, kern, grid, bs, ...)                                \
+  do {                                                                      \
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(bs), 0, (c)->stream, __VA_ARGS__); \
     VB_HIP(c, hipGetLastError());                                           \
   } while (0)

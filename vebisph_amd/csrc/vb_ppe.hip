@@ -1,22 +1,34 @@
-// gfx950 pressure Poisson equation: RHS, ELL assembly and the BiCGSTAB of the reference
+// gfx950 pressure Poisson equation: RHS, assembly and the BiCGSTAB of the reference
 // (ppe_solve_in_house_simple_dirichlet.F90:1-158, linear_solver.F90:231-425).
 //
-// Matrix: ELL, slot-major like the neighbour list (col/val[k*npfb + i]) plus a diagonal; slot k of
-// row i is the k-th neighbour of i with mirrors folded onto their parents, which is the
-// reference's Numerical-Recipes row layout slot for slot, including its zero slots (duplicate
-// columns accumulate into the last slot, self-mirrors into the diagonal).  The SpMV therefore
-// performs the reference's sparse_atime operations in the reference's order.
+// Matrix.  The assembly writes the reference's row layout as ELL, slot-major like the neighbour
+// list (col/val[k*npfb + i]) plus a diagonal: slot k of row i is the k-th neighbour of i with
+// mirrors folded onto their parents, zero slots included (duplicate columns accumulate into the
+// last slot, self-mirrors into the diagonal).  The ~98% of rows with no mirror neighbour (and no
+// neighbour further than 32767 rows away) are also stored compact, grouped by four slots so that a
+// thread loads four slots per instruction: 2-byte column offsets and FP64 values (10 bytes per slot,
+// same values in the same order: bit-identical) or FP32 values in the mixed path (6 bytes per slot).
+// (Recomputing w_L from the positions instead of storing it was measured: compute-bound on the FP64
+// divisions and square root, ~1.8x slower.)
 //
-// BiCGSTAB: five row kernels per iteration (two of them SpMVs), each followed by a one-workgroup
-// finalize that folds the block partials in a fixed order and advances the scalars (rho, alpha,
-// omega, norms, means), all kept on the device: no host round trip per iteration.  The remove_dc
-// projections and the inverseM scaling are applied on the fly inside the SpMV gathers, producing
-// bit-identical values to separate passes.
+// Mixed precision: the same BiCGSTAB with FP64 vectors, dot products, scalars and solution update,
+// and FP32 matvecs on the compact rows: FP32 values applied to FP32 differences of the FP64 vector,
+// y_i = sum_k a32_k (x_j - x_i) + rowsum_i x_i (the FP64 row sum carries the diagonal), so smooth
+// vectors do not cancel catastrophically; every kRefresh iterations the recurrence residual is
+// replaced by the FP64 residual B - A x.
+//
+// BiCGSTAB: all scalars (rho, alpha, omega, norms, means) stay on the device; the host only polls the
+// convergence flag every few iterations.  See the section comment below for the kernel sequence.
+#include <vector>
+
 #include "vb_ctx.h"
 
 namespace vb {
 
 constexpr double kOnePlusEps = 1.0 + 1.0e-15;   // M(i) + 1e-15 with M = 1, linear_solver.F90:55,318
+constexpr int kRefresh = 50;                     // mixed path: FP64 residual replacement period
+
+enum SpmvMode { SPMV_FP64 = 0, SPMV_MIXED = 1 };
 
 // ------------------------------------------------------------------------------------------------
 // RHS, ppe_solve_in_house_simple_dirichlet.F90:27-53 (DC removal for the closed domain)
@@ -28,7 +40,7 @@ __global__ void k_ppe_rhs(Geom g, State st, const int* cnt, const int* link, con
   RowRange rr = block_rows(npfb, gridDim.x);
   const double dt = sc->dt;
   Red<1> R(red, npfb);
-  for (int i = rr.begin + threadIdx.x; i < rr.end; i += kBlock) {
+  for (int i = rr.begin + threadIdx.x; i < rr.end; i += blockDim.x) {
     const int n = cnt[i];
     const double ui = st.u[i], vi = st.v[i];
     const double K11 = k11[i], K12 = k12[i], K21 = k21[i], K22 = k22[i];
@@ -53,36 +65,64 @@ __global__ void k_fin_ppe_rhs(const double* partials, int nblk, const double* re
   if (threadIdx.x == 0) sc->mean = s / (double)n;
 }
 
-// Assembly, :60-122.  One thread per row.
+__host__ __device__ inline size_t grouped(int k, int i, int n) { return ((size_t)(k >> 2) * n + i) * 4 + (k & 3); }
+// FP64 compact values: slots 4g,4g+1 of row i in double2 plane 2g, slots 4g+2,4g+3 in plane 2g+1,
+// so that both loads of a group are contiguous across the wavefront's rows.
+__host__ __device__ inline size_t grouped2(int k, int i, int n) {
+  return (((size_t)(2 * (k >> 2) + ((k & 3) >> 1)) * n + i) << 1) + (k & 1);
+}
+
+// Assembly, :60-122.  One thread per row.  Also marks the rows the compact SpMV can take
+// (rowflag 0: no mirror neighbour, every neighbour within 32767 rows) and stores them in the grouped
+// compact arrays (relative column offsets, FP64 values, FP32 values for the mixed path), and the
+// FP64 row sum.
 __global__ void k_assemble(Geom g, State st, const int* cnt, const int* link, const double* wl,
                            const int* nsurf, double* rhs, double* a_diag, int* a_col, double* a_val,
-                           float* a_valf, float* a_diagf, const Scal* sc) {
+                           int16_t* lnk4, double* val4, float* val4f, uint8_t* rowflag, double* rowsum,
+                           const Scal* sc) {
   int i = blockIdx.x * kBlock + threadIdx.x;
   const int npfb = g.npfb;
   if (i >= npfb) return;
   const int n = cnt[i];
   double bi = rhs[i] - sc->mean;
   double adiag = 0.0;
-  bool has_m = false;
+  bool explicit_row = false;
   for (int k = 0; k < n; k++) {
     int s = k * npfb + i;
     int j = link[s];
     double t = -wl[s];
     adiag = adiag - t;
     if (j >= npfb) {
-      has_m = true;
+      explicit_row = true;
       int ij = st.irel[j];
       if (ij == i) adiag = adiag + t;
       bi = bi - st.dP[j] * t;
+    } else {
+      int off = j - i;
+      if (off > 32767 || off < -32767) explicit_row = true;
     }
   }
-  if (!has_m) {
-    for (int k = 0; k < n; k++) {
-      int s = k * npfb + i;
-      double val = 0.0 + (-wl[s]);
-      a_col[s] = link[s];
-      a_val[s] = val;
-      if (a_valf) a_valf[s] = (float)val;
+  if (nsurf[i] != 0) { adiag = 1.0e30; explicit_row = true; }
+  double d = 0.0 + adiag;
+  double rs = d;
+  if (!explicit_row) {
+    const int padded = (n + 3) & ~3;
+    for (int k = 0; k < padded; k++) {
+      size_t e = grouped(k, i, npfb), e2 = grouped2(k, i, npfb);
+      if (k < n) {
+        int s = k * npfb + i;
+        double val = 0.0 + (-wl[s]);
+        a_col[s] = link[s];
+        a_val[s] = val;
+        lnk4[e] = (int16_t)(link[s] - i);
+        val4[e2] = val;
+        if (val4f) val4f[e] = (float)val;
+        rs = rs + val;
+      } else {                           // padding: offset 0 (the row itself), value 0
+        lnk4[e] = 0;
+        val4[e2] = 0.0;
+        if (val4f) val4f[e] = 0.0f;
+      }
     }
   } else {
     for (int k = 0; k < n; k++) {
@@ -107,204 +147,295 @@ __global__ void k_assemble(Geom g, State st, const int* cnt, const int* link, co
       }
       a_col[s] = ck;
       a_val[s] = val;
-      if (a_valf) a_valf[s] = (float)val;
+      rs = rs + val;
     }
   }
-  if (nsurf[i] != 0) adiag = 1.0e30;
-  double d = 0.0 + adiag;
   a_diag[i] = d;
-  if (a_diagf) a_diagf[i] = (float)d;
+  rowflag[i] = explicit_row ? 1 : 0;
+  rowsum[i] = rs;
   rhs[i] = bi;
 }
 
 // ------------------------------------------------------------------------------------------------
-// SPARSE_PBICGSTAB_nonull, linear_solver.F90:275-425, FP64.  Each row kernel is followed by a
-// one-workgroup finalize that folds its partials and advances the scalars.
+// SpMV rows.  x_j = (src_j - mean) / (1 + 1e-15) reproduces remove_dc + inverseM exactly.
 // ------------------------------------------------------------------------------------------------
-struct Vecs {
-  double *x, *r, *rb, *v, *p, *pn, *pinv, *s, *s2, *sinv, *t;
+struct Mat {
+  const double *diag, *val, *rowsum;
+  const int *col, *len;
+  const uint8_t* rowflag;          // null: every row uses the full stored row
+  // compact rows, grouped by 4 slots: element (k, i) at ((k/4)*n + i)*4 + k%4
+  const short4* lnk4;              // column offsets (column = row + offset)
+  const double2* val4;             // FP64 values, two double2 planes per group (grouped2)
+  const float4* val4f;             // FP32 values (mixed path)
+  int n;
 };
 
-__global__ void k_bi_init(int n, Vecs w, const double* rhs, double* partials, double* red) {
-  RowRange rr = block_rows(n, gridDim.x);
-  Red<1> R(red, n);
-  for (int i = rr.begin + threadIdx.x; i < rr.end; i += kBlock) {
-    double b = rhs[i];
-    w.x[i] = 0.0;            // p(1:npfb) = 0 before the solve (ppe_...:145); A*0 = 0, so r = B
-    w.r[i] = b; w.rb[i] = b;
-    w.v[i] = 0.0; w.p[i] = 0.0;
-    R.add(0, i, b * b);      // rho = vtime(r_bar, r) of the first iteration (:330)
+// The full stored row, in the reference's slot order (sparse_atime, linear_solver.F90:264-272).
+__device__ inline double row_full(const Mat& A, int i, int m, double xi, const double* __restrict__ x) {
+  const int n = A.n;
+  double bb = A.diag[i] * xi;
+  int k = 0;
+  for (; k + 4 <= m; k += 4) {
+    int s0 = k * n + i;
+    int c0 = A.col[s0], c1 = A.col[s0 + n], c2 = A.col[s0 + 2 * n], c3 = A.col[s0 + 3 * n];
+    double v0 = A.val[s0], v1 = A.val[s0 + n], v2 = A.val[s0 + 2 * n], v3 = A.val[s0 + 3 * n];
+    double x0 = x[c0], x1 = x[c1], x2 = x[c2], x3 = x[c3];
+    bb = bb + v0 * x0; bb = bb + v1 * x1; bb = bb + v2 * x2; bb = bb + v3 * x3;
   }
-  R.store(partials);
+  for (; k < m; k++) {
+    int s0 = k * n + i;
+    bb = bb + A.val[s0] * x[A.col[s0]];
+  }
+  return bb;
 }
 
-__global__ void k_fin_init(const double* partials, int nblk, const double* red, int n, double tol,
-                           int itermax, Scal* sc) {
-  double rho = fold(partials, 0, nblk, red, n);
+// One SpMV row y_i = A(i) x_i + sum_k a_k x_col(k).  Rows with mirror neighbours use the full stored
+// row; the others (~98%) load four slots per instruction from the grouped compact arrays: a 2-byte
+// column offset and the FP64 value (same operations in the same order: bit-identical), or in the
+// mixed path the FP32 value applied to FP32 differences, y_i = sum_k a32_k (x_j - x_i) + rowsum_i x_i,
+// so that smooth vectors do not cancel catastrophically in FP32.
+template <int MODE>
+__device__ inline double spmv_row(const Mat& A, int i, const double* __restrict__ x) {
+  const int m = A.len[i];
+  const double xi = x[i];
+  if (!A.rowflag || A.rowflag[i]) return row_full(A, i, m, xi, x);
+  const int n = A.n;
+  const int ng = (m + 3) >> 2;
+  // software pipelined: the next group's offsets and values are in flight during this group's
+  // gathers (each row is a chain of ~11 groups; without this every group waits a full HBM latency)
+  size_t e = i;
+  short4 o = A.lnk4[e];
+  if (MODE == SPMV_FP64) {
+    size_t e2 = i;                               // plane 2g: slots 0-1, plane 2g+1: slots 2-3
+    double2 va = A.val4[e2], vb = A.val4[e2 + n];
+    double bb = A.diag[i] * xi;
+    for (int gi = 0; gi < ng; gi++) {
+      short4 on = o;
+      double2 van = va, vbn = vb;
+      if (gi + 1 < ng) {
+        size_t en = e + n;
+        e2 += 2 * (size_t)n;
+        on = A.lnk4[en]; van = A.val4[e2]; vbn = A.val4[e2 + n];
+      }
+      double x0 = x[i + o.x], x1 = x[i + o.y], x2 = x[i + o.z], x3 = x[i + o.w];
+      int k = 4 * gi;
+      bb = bb + va.x * x0;
+      if (k + 1 < m) bb = bb + va.y * x1;
+      if (k + 2 < m) bb = bb + vb.x * x2;
+      if (k + 3 < m) bb = bb + vb.y * x3;
+      o = on; va = van; vb = vbn; e += n;
+    }
+    return bb;
+  } else {
+    float4 a = A.val4f[e];                      // padded slots hold 0
+    float acc = 0.0f;
+    for (int gi = 0; gi < ng; gi++) {
+      short4 on = o;
+      float4 an = a;
+      if (gi + 1 < ng) { size_t en = e + n; on = A.lnk4[en]; an = A.val4f[en]; }
+      double x0 = x[i + o.x], x1 = x[i + o.y], x2 = x[i + o.z], x3 = x[i + o.w];
+      acc = fmaf(a.x, (float)(x0 - xi), acc);
+      acc = fmaf(a.y, (float)(x1 - xi), acc);
+      acc = fmaf(a.z, (float)(x2 - xi), acc);
+      acc = fmaf(a.w, (float)(x3 - xi), acc);
+      o = on; a = an; e += n;
+    }
+    return (double)acc + A.rowsum[i] * xi;
+  }
+}
+
+// ------------------------------------------------------------------------------------------------
+// SPARSE_PBICGSTAB_nonull, linear_solver.F90:275-425.
+//
+// Per iteration: p -> [p_inv] -> SpMV -> s -> [s_inv] -> SpMV -> x,r -> finalize.  Each row kernel
+// that needs a global sum from the kernel before it folds that kernel's block partials itself (every
+// block folds them in the same fixed order, so all agree), and only block 0 records the scalar for
+// later kernels; a kernel never reads a scalar that it writes.  Stage s writes its partials (and, in
+// serial mode, its per-row terms) to its own buffer.  One one-workgroup finalize per iteration
+// (after x,r) advances the iteration counters.
+// ------------------------------------------------------------------------------------------------
+struct Vecs {
+  double *x, *b, *r, *rb, *v, *pn, *pinv, *s, *sinv, *t;   // pn, s: before remove_dc
+};
+
+enum { ST_P = 0, ST_SPMV_P = 1, ST_S = 2, ST_SPMV_S = 3, ST_XR = 4, ST_INIT = 5 };
+static_assert(ST_INIT < kSolverStages, "stage buffers");
+
+struct RedBufs {
+  double* partials;     // kStages * 2 * kMaxBlocks
+  double* rows;         // serial mode: kStages * 2 * n, else null
+  int n;
+  __device__ double* P(int st) const { return partials + (size_t)st * 2 * kMaxBlocks; }
+  __device__ double* R(int st) const { return rows ? rows + (size_t)st * 2 * n : nullptr; }
+};
+
+__device__ inline bool finished(const Scal* sc) { return sc->done || sc->exit_s; }
+
+// :306-321  AX = A x (x = 0 for the PPE, so r = B); r_bar = r; the terms of the first rho (:330)
+__global__ void k_bi_init(Mat A, Vecs w, int x_given, RedBufs rb) {
+  const int n = A.n;
+  RowRange rr = block_rows(n, gridDim.x);
+  Red<1> R(rb.R(ST_INIT), n);
+  for (int i = rr.begin + threadIdx.x; i < rr.end; i += blockDim.x) {
+    double r;
+    if (x_given) r = w.b[i] - spmv_row<SPMV_FP64>(A, i, w.x);
+    else { w.x[i] = 0.0; r = w.b[i]; }
+    w.r[i] = r; w.rb[i] = r;
+    w.v[i] = 0.0; w.pn[i] = 0.0;
+    R.add(0, i, r * r);
+  }
+  R.store(rb.P(ST_INIT));
+}
+
+__global__ void k_fin_init(RedBufs rb, int nblk, double tol, int itermax, Scal* sc) {
+  double rho = fold(rb.P(ST_INIT), 0, nblk, rb.R(ST_INIT), rb.n);
   if (threadIdx.x == 0) {
     sc->rho = rho; sc->rho0 = 1.0; sc->alpha = 1.0; sc->omega = 1.0; sc->beta = 0.0;
     sc->n2s0 = 1.0; sc->n2r0 = 1.0; sc->resid = 0.0; sc->tol = tol;
     sc->iter = 1; sc->itermax = itermax; sc->exit_s = 0; sc->status = VB_OK;
+    sc->mean_p = 0.0; sc->mean_s = 0.0;
     sc->done = 0;
     if (itermax < 1) { sc->done = 1; sc->status = VB_WARN_MAXITER; }
     else if (rho == 0.0) { sc->done = 1; sc->status = VB_WARN_BICGSTAB_BREAKDOWN; }
   }
 }
 
-// :337-349 search direction; the sum for remove_dc(p)
-__global__ void k_bi_p(int n, Vecs w, double* partials, double* red, const Scal* sc) {
-  if (sc->done) return;
+// :337-349 search direction p = r + beta (p - omega v), p = pn - mean_p of the last iteration;
+// the terms of sum(p) for remove_dc
+__global__ void k_bi_p(int n, Vecs w, RedBufs rb, const Scal* sc) {
+  if (finished(sc)) return;
   RowRange rr = block_rows(n, gridDim.x);
   const bool first = sc->iter == 1;
-  const double beta = sc->beta, omega = sc->omega;
-  Red<1> R(red, n);
-  for (int i = rr.begin + threadIdx.x; i < rr.end; i += kBlock) {
-    double pn = first ? w.r[i] : w.r[i] + beta * (w.p[i] - omega * w.v[i]);
+  const double beta = sc->beta, omega = sc->omega, mean = sc->mean_p;
+  Red<1> R(rb.R(ST_P), n);
+  for (int i = rr.begin + threadIdx.x; i < rr.end; i += blockDim.x) {
+    double pn = first ? w.r[i] : w.r[i] + beta * ((w.pn[i] - mean) - omega * w.v[i]);
     w.pn[i] = pn;
     R.add(0, i, pn);
   }
-  R.store(partials);
+  R.store(rb.P(ST_P));
 }
 
-__global__ void k_fin_p(const double* partials, int nblk, const double* red, int n, Scal* sc) {
-  if (sc->done) return;
-  double s = fold(partials, 0, nblk, red, n);
-  if (threadIdx.x == 0) sc->mean_p = s / (double)n;                      // :243
-}
-
-// ELL SpMV y_i = A(i) x_i + sum_k val(i,k) x_col(i,k) (sparse_atime, :264-272) with the vector
-// produced on the fly as x = (src - mean) / (1 + 1e-15) (remove_dc then inverseM, bit-identical to
-// the separate passes).  Four slots per step keep four gathers in flight; the sum keeps slot order.
-__device__ inline double spmv_row(int i, int n, int m, double xi, const double* __restrict__ a_diag,
-                                  const int* __restrict__ a_col, const double* __restrict__ a_val,
-                                  const double* __restrict__ src, double mean) {
-  double bb = a_diag[i] * xi;
-  int k = 0;
-  for (; k + 4 <= m; k += 4) {
-    int s0 = k * n + i;
-    int c0 = a_col[s0], c1 = a_col[s0 + n], c2 = a_col[s0 + 2 * n], c3 = a_col[s0 + 3 * n];
-    double v0 = a_val[s0], v1 = a_val[s0 + n], v2 = a_val[s0 + 2 * n], v3 = a_val[s0 + 3 * n];
-    double x0 = src[c0], x1 = src[c1], x2 = src[c2], x3 = src[c3];
-    bb = bb + v0 * ((x0 - mean) / kOnePlusEps);
-    bb = bb + v1 * ((x1 - mean) / kOnePlusEps);
-    bb = bb + v2 * ((x2 - mean) / kOnePlusEps);
-    bb = bb + v3 * ((x3 - mean) / kOnePlusEps);
-  }
-  for (; k < m; k++) {
-    int s0 = k * n + i;
-    bb = bb + a_val[s0] * ((src[a_col[s0]] - mean) / kOnePlusEps);
-  }
-  return bb;
-}
-
-// :351-356  p = p - mean; p_inv = p/(1+1e-15); v = A p_inv; the terms of r_bar . v
-__global__ void k_bi_spmv_p(int n, Vecs w, const double* a_diag, const int* a_col, const double* a_val,
-                            const int* cnt, double* partials, double* red, const Scal* sc) {
-  if (sc->done) return;
+// :351-352 remove_dc(p), inverseM: p_inv = (pn - mean)/(1+1e-15)
+__global__ void k_bi_pinv(int n, Vecs w, RedBufs rb, Scal* sc) {
+  if (finished(sc)) return;
+  const double mean = fold(rb.P(ST_P), 0, gridDim.x, rb.R(ST_P), n) / (double)n;   // :243
+  if (blockIdx.x == 0 && threadIdx.x == 0) sc->mean_p = mean;
   RowRange rr = block_rows(n, gridDim.x);
-  const double mean = sc->mean_p;
-  Red<1> R(red, n);
-  for (int i = rr.begin + threadIdx.x; i < rr.end; i += kBlock) {
-    double pi = w.pn[i] - mean;
-    double pinv = pi / kOnePlusEps;
-    w.p[i] = pi; w.pinv[i] = pinv;
-    double bb = spmv_row(i, n, cnt[i], pinv, a_diag, a_col, a_val, w.pn, mean);
+  for (int i = rr.begin + threadIdx.x; i < rr.end; i += blockDim.x) w.pinv[i] = (w.pn[i] - mean) / kOnePlusEps;
+}
+
+// :353-354  v = A p_inv; the terms of r_bar . v
+template <int MODE>
+__global__ void k_bi_spmv_p(Mat A, Vecs w, RedBufs rb, const Scal* sc) {
+  if (finished(sc)) return;
+  const int n = A.n;
+  RowRange rr = block_rows(n, gridDim.x);
+  Red<1> R(rb.R(ST_SPMV_P), n);
+  for (int i = rr.begin + threadIdx.x; i < rr.end; i += blockDim.x) {
+    double bb = spmv_row<MODE>(A, i, w.pinv);
     w.v[i] = bb;
     R.add(0, i, w.rb[i] * bb);
   }
-  R.store(partials);
+  R.store(rb.P(ST_SPMV_P));
 }
 
-__global__ void k_fin_spmv_p(const double* partials, int nblk, const double* red, int n, Scal* sc) {
-  if (sc->done) return;
-  double temp = fold(partials, 0, nblk, red, n);
-  if (threadIdx.x == 0) sc->alpha = sc->rho / (temp + 1.0e-15);          // :356
-}
-
-// :358-364  s = r - alpha v; |s|^2 and sum(s)
-__global__ void k_bi_s(int n, Vecs w, double* partials, double* red, const Scal* sc) {
-  if (sc->done) return;
+// :356-364  alpha = rho/(r_bar.v + 1e-15); s = r - alpha v; terms of |s|^2 and sum(s)
+__global__ void k_bi_s(int n, Vecs w, RedBufs rb, Scal* sc) {
+  if (finished(sc)) return;
+  const double temp = fold(rb.P(ST_SPMV_P), 0, gridDim.x, rb.R(ST_SPMV_P), n);
+  const double alpha = sc->rho / (temp + 1.0e-15);
+  if (blockIdx.x == 0 && threadIdx.x == 0) sc->alpha = alpha;
   RowRange rr = block_rows(n, gridDim.x);
-  const double alpha = sc->alpha;
-  Red<2> R(red, n);
-  for (int i = rr.begin + threadIdx.x; i < rr.end; i += kBlock) {
+  Red<2> R(rb.R(ST_S), n);
+  for (int i = rr.begin + threadIdx.x; i < rr.end; i += blockDim.x) {
     double s = w.r[i] - alpha * w.v[i];
     w.s[i] = s;
     R.add(0, i, s * s);
     R.add(1, i, s);
   }
-  R.store(partials);
+  R.store(rb.P(ST_S));
 }
 
-// :366-381 early exit on s
-__global__ void k_fin_s(const double* partials, int nblk, const double* red, int n, Scal* sc) {
-  if (sc->done) return;
-  double n2s = fold(partials, 0, nblk, red, n);
-  double sum = fold(partials, 1, nblk, red, n);
-  if (threadIdx.x == 0) {
-    if (sc->iter == 1) sc->n2s0 = (n2s <= 1.0e-6) ? 1.0 : n2s;
-    if (n2s <= sc->tol * sc->tol * sc->n2s0) {
-      sc->exit_s = 1;                        // x += alpha p_inv happens in the next row kernel
-      sc->resid = sqrt(n2s / sc->n2s0);
-      sc->done = 1;
-    }
-    sc->mean_s = sum / (double)n;
-  }
-}
-
-// :383-387  s = s - mean; s_inv; t = A s_inv; terms of t.s and t.t.  On an early exit: x += alpha p_inv.
-__global__ void k_bi_spmv_s(int n, Vecs w, const double* a_diag, const int* a_col, const double* a_val,
-                            const int* cnt, double* partials, double* red, const Scal* sc) {
+// :366-384  early exit on |s| (x += alpha p_inv), else remove_dc(s), inverseM: s_inv
+__global__ void k_bi_sinv(int n, Vecs w, RedBufs rb, Scal* sc) {
+  if (finished(sc)) return;
+  const double n2s = fold(rb.P(ST_S), 0, gridDim.x, rb.R(ST_S), n);
+  const double sum = fold(rb.P(ST_S), 1, gridDim.x, rb.R(ST_S), n);
+  const bool first = sc->iter == 1;
+  const double n2s0 = first ? ((n2s <= 1.0e-6) ? 1.0 : n2s) : sc->n2s0;
+  const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
   RowRange rr = block_rows(n, gridDim.x);
-  if (sc->exit_s == 1) {
+  if (n2s <= sc->tol * sc->tol * n2s0) {
     const double alpha = sc->alpha;
-    for (int i = rr.begin + threadIdx.x; i < rr.end; i += kBlock) w.x[i] = w.x[i] + alpha * w.pinv[i];
+    for (int i = rr.begin + threadIdx.x; i < rr.end; i += blockDim.x) w.x[i] = w.x[i] + alpha * w.pinv[i];
+    if (lead) { sc->n2s0 = n2s0; sc->resid = sqrt(n2s / n2s0); sc->exit_s = 1; }
     return;
   }
-  if (sc->done) return;
+  const double mean = sum / (double)n;
+  if (lead) { sc->n2s0 = n2s0; sc->mean_s = mean; }
+  for (int i = rr.begin + threadIdx.x; i < rr.end; i += blockDim.x) w.sinv[i] = (w.s[i] - mean) / kOnePlusEps;
+}
+
+// :385-387  t = A s_inv; terms of t.s and t.t with the projected s
+template <int MODE>
+__global__ void k_bi_spmv_s(Mat A, Vecs w, RedBufs rb, const Scal* sc) {
+  if (finished(sc)) return;
+  const int n = A.n;
+  RowRange rr = block_rows(n, gridDim.x);
   const double mean = sc->mean_s;
-  Red<2> R(red, n);
-  for (int i = rr.begin + threadIdx.x; i < rr.end; i += kBlock) {
-    double si = w.s[i] - mean;
-    double sinv = si / kOnePlusEps;
-    w.s2[i] = si; w.sinv[i] = sinv;
-    double bb = spmv_row(i, n, cnt[i], sinv, a_diag, a_col, a_val, w.s, mean);
+  Red<2> R(rb.R(ST_SPMV_S), n);
+  for (int i = rr.begin + threadIdx.x; i < rr.end; i += blockDim.x) {
+    double bb = spmv_row<MODE>(A, i, w.sinv);
     w.t[i] = bb;
+    double si = w.s[i] - mean;
     R.add(0, i, bb * si);
     R.add(1, i, bb * bb);
   }
-  R.store(partials);
+  R.store(rb.P(ST_SPMV_S));
 }
 
-__global__ void k_fin_spmv_s(const double* partials, int nblk, const double* red, int n, Scal* sc) {
-  if (sc->exit_s == 1) { if (threadIdx.x == 0) sc->exit_s = 2; return; }
-  if (sc->done) return;
-  double ts = fold(partials, 0, nblk, red, n);
-  double tt = fold(partials, 1, nblk, red, n);
-  if (threadIdx.x == 0) sc->omega = ts / (tt + 1.0e-15);                 // :389
-}
-
-// :390-401  x, r updates; terms of |r|^2 and of the next r_bar . r
-__global__ void k_bi_xr(int n, Vecs w, double* partials, double* red, const Scal* sc) {
-  if (sc->done) return;
+// :389-401  omega; x = x + alpha p_inv + omega s_inv; r = s - omega t; terms of |r|^2 and the next
+// r_bar . r
+__global__ void k_bi_xr(int n, Vecs w, RedBufs rb, Scal* sc) {
+  if (finished(sc)) return;
+  const double ts = fold(rb.P(ST_SPMV_S), 0, gridDim.x, rb.R(ST_SPMV_S), n);
+  const double tt = fold(rb.P(ST_SPMV_S), 1, gridDim.x, rb.R(ST_SPMV_S), n);
+  const double omega = ts / (tt + 1.0e-15);
+  if (blockIdx.x == 0 && threadIdx.x == 0) sc->omega = omega;
+  const double alpha = sc->alpha, ms = sc->mean_s;
   RowRange rr = block_rows(n, gridDim.x);
-  const double alpha = sc->alpha, omega = sc->omega;
-  Red<2> R(red, n);
-  for (int i = rr.begin + threadIdx.x; i < rr.end; i += kBlock) {
+  Red<2> R(rb.R(ST_XR), n);
+  for (int i = rr.begin + threadIdx.x; i < rr.end; i += blockDim.x) {
     w.x[i] = w.x[i] + alpha * w.pinv[i] + omega * w.sinv[i];
-    double r = w.s2[i] - omega * w.t[i];
+    double r = (w.s[i] - ms) - omega * w.t[i];
     w.r[i] = r;
     R.add(0, i, r * r);
     R.add(1, i, w.rb[i] * r);
   }
-  R.store(partials);
+  R.store(rb.P(ST_XR));
+}
+
+// Mixed path only: replace the recurrence residual by the FP64 residual r = B - A x
+__global__ void k_bi_true_residual(Mat A, Vecs w, RedBufs rb, const Scal* sc) {
+  if (finished(sc)) return;
+  const int n = A.n;
+  RowRange rr = block_rows(n, gridDim.x);
+  Red<2> R(rb.R(ST_XR), n);
+  for (int i = rr.begin + threadIdx.x; i < rr.end; i += blockDim.x) {
+    double r = w.b[i] - spmv_row<SPMV_FP64>(A, i, w.x);
+    w.r[i] = r;
+    R.add(0, i, r * r);
+    R.add(1, i, w.rb[i] * r);
+  }
+  R.store(rb.P(ST_XR));
 }
 
 // :403-419 convergence; next rho (:330) and beta (:342-343)
-__global__ void k_fin_xr(const double* partials, int nblk, const double* red, int n, Scal* sc) {
-  if (sc->done) return;
-  double n2r = fold(partials, 0, nblk, red, n);
-  double rho = fold(partials, 1, nblk, red, n);
+__global__ void k_fin_xr(RedBufs rb, int nblk, Scal* sc) {
+  if (finished(sc)) return;
+  double n2r = fold(rb.P(ST_XR), 0, nblk, rb.R(ST_XR), rb.n);
+  double rho = fold(rb.P(ST_XR), 1, nblk, rb.R(ST_XR), rb.n);
   if (threadIdx.x == 0) {
     if (sc->iter == 1) sc->n2r0 = (n2r <= 1.0e-6) ? 1.0 : n2r;
     sc->resid = sqrt(n2r / sc->n2r0);
@@ -327,78 +458,50 @@ __global__ void k_fin_xr(const double* partials, int nblk, const double* red, in
     VB_HIP(c, hipGetLastError());                                                          \
   } while (0)
 
-static vb_status solve_fp64(vb_ctx* c, double tol, int itermax) {
-  const int n = c->npfb;
+static Mat step_matrix(vb_ctx* c) {
+  Mat A;
+  A.diag = c->a_diag; A.val = c->a_val; A.rowsum = c->a_rowsum; A.col = c->a_col; A.len = c->cnt;
+  A.rowflag = c->rowflag; A.lnk4 = (const short4*)c->lnk4; A.val4 = (const double2*)c->val4;
+  A.val4f = (const float4*)c->val4f; A.n = c->npfb;
+  return A;
+}
+
+static RedBufs red_bufs(vb_ctx* c, int n) { return RedBufs{c->partials, c->red, n}; }
+
+template <int MODE>
+static vb_status solve(vb_ctx* c, const Mat& A, Vecs w, int x_given, double tol, int itermax) {
+  const int n = A.n;
   const int nb = grid_for(n);
-  Vecs w{c->st.p, c->r, c->rb, c->vv, c->pp, c->pn, c->pinv, c->ss, c->s2, c->sinv, c->tt};
-  double* P = c->partials;
-  double* red = c->red;           // null unless serial reductions were requested
+  RedBufs rb = red_bufs(c, n);
   Scal* sc = c->scal;
-  VB_LAUNCH(c, k_bi_init, nb, n, w, c->rhs, P, red);
-  VB_FIN(c, k_fin_init, P, nb, red, n, tol, itermax, sc);
+  VB_LAUNCH(c, k_bi_init, nb, A, w, x_given, rb);
+  VB_FIN(c, k_fin_init, rb, nb, tol, itermax, sc);
   int launched = 0, batch = 8;
   for (;;) {
     for (int b = 0; b < batch; b++) {
-      VB_LAUNCH(c, k_bi_p, nb, n, w, P, red, sc);
-      VB_FIN(c, k_fin_p, P, nb, red, n, sc);
-      VB_LAUNCH(c, k_bi_spmv_p, nb, n, w, c->a_diag, c->a_col, c->a_val, c->cnt, P, red, sc);
-      VB_FIN(c, k_fin_spmv_p, P, nb, red, n, sc);
-      VB_LAUNCH(c, k_bi_s, nb, n, w, P, red, sc);
-      VB_FIN(c, k_fin_s, P, nb, red, n, sc);
-      VB_LAUNCH(c, k_bi_spmv_s, nb, n, w, c->a_diag, c->a_col, c->a_val, c->cnt, P, red, sc);
-      VB_FIN(c, k_fin_spmv_s, P, nb, red, n, sc);
-      VB_LAUNCH(c, k_bi_xr, nb, n, w, P, red, sc);
-      VB_FIN(c, k_fin_xr, P, nb, red, n, sc);
+      VB_LAUNCH(c, k_bi_p, nb, n, w, rb, sc);
+      VB_LAUNCH(c, k_bi_pinv, nb, n, w, rb, sc);
+      VB_LAUNCH_BS(c, k_bi_spmv_p<MODE>, nb, kSpmvBlock, A, w, rb, sc);
+      VB_LAUNCH(c, k_bi_s, nb, n, w, rb, sc);
+      VB_LAUNCH(c, k_bi_sinv, nb, n, w, rb, sc);
+      VB_LAUNCH_BS(c, k_bi_spmv_s<MODE>, nb, kSpmvBlock, A, w, rb, sc);
+      VB_LAUNCH(c, k_bi_xr, nb, n, w, rb, sc);
+      if (MODE == SPMV_MIXED && (launched + b + 1) % kRefresh == 0)
+        VB_LAUNCH_BS(c, k_bi_true_residual, nb, kSpmvBlock, A, w, rb, sc);
+      VB_FIN(c, k_fin_xr, rb, nb, sc);
     }
     launched += batch;
     VB_HIP(c, hipMemcpyAsync(c->scal_host, c->scal, sizeof(Scal), hipMemcpyDeviceToHost, c->stream));
     VB_HIP(c, hipStreamSynchronize(c->stream));
-    if (c->scal_host->done && c->scal_host->exit_s != 1) break;
+    if (c->scal_host->done || c->scal_host->exit_s) break;
     if (launched > itermax + 2) break;
     batch = launched < 32 ? 8 : 32;
   }
   return VB_OK;
 }
 
-__global__ void k_bi_set_live(Scal* sc, int* nnz_out, const int* cnt, int n) {
-  // reopen the solver kernels after a converged solve and count the stored slots
-  __shared__ int sh[kBlock];
-  int s = 0;
-  for (int i = threadIdx.x; i < n; i += kBlock) s += cnt[i];
-  sh[threadIdx.x] = s;
-  __syncthreads();
-  for (int w = kBlock / 2; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) { sc->done = 0; sc->exit_s = 0; *nnz_out = sh[0]; }
-}
-
-vb_status profile_spmv(vb_ctx* c, int reps, double* avg_us, double* alg_bytes) {
-  const int n = c->npfb, nb = grid_for(n);
-  Vecs w{c->st.p, c->r, c->rb, c->vv, c->pp, c->pn, c->pinv, c->ss, c->s2, c->sinv, c->tt};
-  int* d_nnz = c->seg_counts;
-  hipLaunchKernelGGL(k_bi_set_live, dim3(1), dim3(kBlock), 0, c->stream, c->scal, d_nnz, c->cnt, n);
-  VB_HIP(c, hipGetLastError());
-  hipEvent_t e0, e1;
-  VB_HIP(c, hipEventCreate(&e0));
-  VB_HIP(c, hipEventCreate(&e1));
-  VB_LAUNCH(c, k_bi_spmv_p, nb, n, w, c->a_diag, c->a_col, c->a_val, c->cnt, c->partials, nullptr, c->scal);
-  VB_HIP(c, hipEventRecord(e0, c->stream));
-  for (int r = 0; r < reps; r++)
-    VB_LAUNCH(c, k_bi_spmv_p, nb, n, w, c->a_diag, c->a_col, c->a_val, c->cnt, c->partials, nullptr, c->scal);
-  VB_HIP(c, hipEventRecord(e1, c->stream));
-  VB_HIP(c, hipEventSynchronize(e1));
-  float ms = 0.f;
-  VB_HIP(c, hipEventElapsedTime(&ms, e0, e1));
-  int nnz = 0;
-  VB_HIP(c, hipMemcpy(&nnz, d_nnz, 4, hipMemcpyDeviceToHost));
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  *avg_us = 1000.0 * ms / reps;
-  // per row: cnt 4, diag 8, gathered p 8 (once), r_bar 8, writes p, p_inv, v 24; per slot: col 4 + val 8
-  *alg_bytes = 52.0 * n + 12.0 * nnz;
-  return VB_OK;
+static Vecs step_vecs(vb_ctx* c, double* x) {
+  return Vecs{x, c->rhs, c->r, c->rb, c->vv, c->pn, c->pinv, c->ss, c->sinv, c->tt};
 }
 
 vb_status launch_ppe(vb_ctx* c, double tol, int itermax) {
@@ -407,8 +510,105 @@ vb_status launch_ppe(vb_ctx* c, double tol, int itermax) {
             c->k22, c->rhs, c->partials, c->red, c->scal);
   VB_FIN(c, k_fin_ppe_rhs, c->partials, nb, c->red, c->npfb, c->scal);
   VB_LAUNCH(c, k_assemble, grid_1d(c->npfb), c->g, c->st, c->cnt, c->link, c->wl, c->nsurf, c->rhs,
-            c->a_diag, c->a_col, c->a_val, nullptr, nullptr, c->scal);
-  return solve_fp64(c, tol, itermax);
+            c->a_diag, c->a_col, c->a_val, c->lnk4, c->val4,
+            c->prm.precision == VB_PREC_MIXED ? c->val4f : nullptr, c->rowflag, c->a_rowsum, c->scal);
+  Mat A = step_matrix(c);
+  if (c->spmv_stored) A.rowflag = nullptr;       // all rows from the stored matrix
+  Vecs w = step_vecs(c, c->st.p);
+  if (c->prm.precision == VB_PREC_MIXED) return solve<SPMV_MIXED>(c, A, w, 0, tol, itermax);
+  return solve<SPMV_FP64>(c, A, w, 0, tol, itermax);
+}
+
+// SPARSE_PBICGSTAB_nonull on a caller's system in Numerical-Recipes storage (host arrays).
+vb_status solve_nr(vb_ctx* c, const double* A_nr, const int* ija, double* x, const double* B, int n,
+                   int itermax, double tol) {
+  if (n != c->npfb) { c->err = "system size must equal the context's npfb"; return VB_ERR_ARG; }
+  std::vector<int> len(n), col((size_t)n * kNplink, 0);
+  std::vector<double> val((size_t)n * kNplink, 0.0), diag(n);
+  for (int i = 0; i < n; i++) {
+    diag[i] = A_nr[i];
+    int b = ija[i], e = ija[i + 1];
+    int m = e - b;
+    if (m < 0 || m > kNplink) { c->err = "row longer than VB_NPLINK slots"; return VB_ERR_ARG; }
+    len[i] = m;
+    for (int k = 0; k < m; k++) {
+      int col1 = ija[b - 1 + k];
+      if (col1 < 1 || col1 > n) { c->err = "column index out of range"; return VB_ERR_ARG; }
+      col[(size_t)k * n + i] = col1 - 1;
+      val[(size_t)k * n + i] = A_nr[b - 1 + k];
+    }
+  }
+  const size_t nw = (size_t)n * kNplink;
+  VB_HIP(c, hipMemcpyAsync(c->a_len, len.data(), (size_t)n * 4, hipMemcpyHostToDevice, c->stream));
+  VB_HIP(c, hipMemcpyAsync(c->a_col, col.data(), nw * 4, hipMemcpyHostToDevice, c->stream));
+  VB_HIP(c, hipMemcpyAsync(c->a_val, val.data(), nw * 8, hipMemcpyHostToDevice, c->stream));
+  VB_HIP(c, hipMemcpyAsync(c->a_diag, diag.data(), (size_t)n * 8, hipMemcpyHostToDevice, c->stream));
+  VB_HIP(c, hipMemcpyAsync(c->rhs, B, (size_t)n * 8, hipMemcpyHostToDevice, c->stream));
+  VB_HIP(c, hipMemcpyAsync(c->xsys, x, (size_t)n * 8, hipMemcpyHostToDevice, c->stream));
+  Mat A = step_matrix(c);
+  A.len = c->a_len;
+  A.rowflag = nullptr;
+  vb_status s = solve<SPMV_FP64>(c, A, step_vecs(c, c->xsys), 1, tol, itermax);
+  if (s != VB_OK) return s;
+  VB_HIP(c, hipMemcpyAsync(x, c->xsys, (size_t)n * 8, hipMemcpyDeviceToHost, c->stream));
+  VB_HIP(c, hipStreamSynchronize(c->stream));
+  return VB_OK;
+}
+
+// ------------------------------------------------------------------------------------------------
+// measurement hook: time the SpMV kernel of the current solve configuration
+// ------------------------------------------------------------------------------------------------
+__global__ void k_bi_set_live(Scal* sc, int* counts_out, const int* cnt, const uint8_t* rowflag, int n) {
+  __shared__ int sh[2][kBlock];
+  int s = 0, e = 0;
+  for (int i = threadIdx.x; i < n; i += kBlock) {
+    s += cnt[i];
+    if (rowflag && rowflag[i]) e += cnt[i];
+  }
+  sh[0][threadIdx.x] = s; sh[1][threadIdx.x] = e;
+  __syncthreads();
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) { sh[0][threadIdx.x] += sh[0][threadIdx.x + w]; sh[1][threadIdx.x] += sh[1][threadIdx.x + w]; }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) { sc->done = 0; sc->exit_s = 0; counts_out[0] = sh[0][0]; counts_out[1] = sh[1][0]; }
+}
+
+vb_status profile_spmv(vb_ctx* c, int reps, double* avg_us, double* alg_bytes) {
+  const int n = c->npfb, nb = grid_for(n);
+  Mat A = step_matrix(c);
+  if (c->spmv_stored) A.rowflag = nullptr;
+  Vecs w = step_vecs(c, c->st.p);
+  int* d_counts = c->seg_counts;
+  hipLaunchKernelGGL(k_bi_set_live, dim3(1), dim3(kBlock), 0, c->stream, c->scal, d_counts, c->cnt,
+                     A.rowflag, n);
+  VB_HIP(c, hipGetLastError());
+  hipEvent_t e0, e1;
+  VB_HIP(c, hipEventCreate(&e0));
+  VB_HIP(c, hipEventCreate(&e1));
+  const bool mixed = c->prm.precision == VB_PREC_MIXED;
+  for (int r = 0; r <= reps; r++) {
+    if (r == 1) VB_HIP(c, hipEventRecord(e0, c->stream));
+    RedBufs rb{c->partials, nullptr, n};
+    if (mixed) VB_LAUNCH_BS(c, k_bi_spmv_p<SPMV_MIXED>, nb, kSpmvBlock, A, w, rb, c->scal);
+    else VB_LAUNCH_BS(c, k_bi_spmv_p<SPMV_FP64>, nb, kSpmvBlock, A, w, rb, c->scal);
+  }
+  VB_HIP(c, hipEventRecord(e1, c->stream));
+  VB_HIP(c, hipEventSynchronize(e1));
+  float ms = 0.f;
+  VB_HIP(c, hipEventElapsedTime(&ms, e0, e1));
+  int counts[2] = {0, 0};
+  VB_HIP(c, hipMemcpy(counts, d_counts, 8, hipMemcpyDeviceToHost));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *avg_us = 1000.0 * ms / reps;
+  // Algorithmic bytes of one launch.  Per row: length 4, row flag 1, diagonal (or row sum) 8, the
+  // gathered vector 8 (each element once), r_bar 8, v written 8.  Per slot: 2-byte column offset +
+  // 8-byte value (FP64) or 4-byte value (mixed) on compact rows; 4 + 8 on full rows.
+  const double nnz = counts[0], nnz_full = A.rowflag ? counts[1] : counts[0];
+  const double compact_slot = mixed ? 6.0 : 10.0;
+  *alg_bytes = 37.0 * n + compact_slot * (nnz - nnz_full) + 12.0 * nnz_full;
+  return VB_OK;
 }
 
 }  // namespace vb

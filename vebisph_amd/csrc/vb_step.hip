@@ -475,8 +475,7 @@ __global__ void k_neighbours(Geom g, State st, const int* cellkey, const int* is
 // Phase 5: kernel_calculation.F90:24-71 (weights, Morris Laplacian weight, KGC inverse)
 // ------------------------------------------------------------------------------------------------
 __global__ void k_weights(Geom g, State st, const int* cnt, const int* link, double* wg1, double* wg2,
-                          double* wl, double* k11, double* k12, double* k21, double* k22,
-                          float* wg1f, float* wg2f, float* wlf, Scal* sc) {
+                          double* wl, double* k11, double* k12, double* k21, double* k22, Scal* sc) {
   int i = blockIdx.x * kBlock + threadIdx.x;
   int mx = 0;
   if (i < g.npfb) {
@@ -497,7 +496,6 @@ __global__ void k_weights(Geom g, State st, const int* cnt, const int* link, dou
       double w1 = gx * dv, w2 = gy * dv;
       double l = 2.0 * (rx * w1 + ry * w2) / (rr2 + g.eta_small);
       wg1[s] = w1; wg2[s] = w2; wl[s] = l;
-      if (wg1f) { wg1f[s] = (float)w1; wg2f[s] = (float)w2; wlf[s] = (float)l; }
       a11 = a11 - dv * gx * rx; a12 = a12 - dv * gy * rx;
       a21 = a21 - dv * gx * ry; a22 = a22 - dv * gy * ry;
     }
@@ -861,10 +859,8 @@ vb_status launch_neighbours(vb_ctx* c) {
 }
 
 vb_status launch_weights(vb_ctx* c) {
-  bool f32 = c->prm.precision == VB_PREC_MIXED;
   VB_LAUNCH(c, k_weights, grid_1d(c->npfb), c->g, c->st, c->cnt, c->link, c->wg1, c->wg2, c->wl, c->k11,
-            c->k12, c->k21, c->k22, f32 ? c->wg1f : nullptr, f32 ? c->wg2f : nullptr,
-            f32 ? c->wlf : nullptr, c->scal);
+            c->k12, c->k21, c->k22, c->scal);
   return VB_OK;
 }
 

@@ -168,6 +168,18 @@ class Solver:
         self._chk(_lib.lib().vb_ppe_solve(self.h, tol, itermax, C.byref(it), C.byref(res)))
         return it.value, res.value
 
+    def sparse_pbicgstab_nonull(self, a, ija, b, x0=None, itermax=1000, tol=1e-8):
+        """SPARSE_PBICGSTAB_nonull (linear_solver.F90:275) on a caller's NR row-indexed system."""
+        n = b.shape[0]
+        x = np.ascontiguousarray(np.zeros(n) if x0 is None else x0, dtype=np.float64).copy()
+        a = np.ascontiguousarray(a, dtype=np.float64); ija = np.ascontiguousarray(ija, dtype=np.int32)
+        bb = np.ascontiguousarray(b, dtype=np.float64)
+        it = C.c_int32(0); res = C.c_double(0.0)
+        st = self._chk(_lib.lib().vb_sparse_pbicgstab_nonull(self.h, _d(a), _i(ija), _d(x), _d(bb), n,
+                                                              a.shape[0], itermax, tol, C.byref(it),
+                                                              C.byref(res)))
+        return x, it.value, res.value, st
+
     def calc_pressure_gradient(self):
         return self._chk(_lib.lib().vb_pressure_update(self.h))
 
