@@ -1,13 +1,18 @@
 #!/usr/bin/env python3
 """Throughput of one VEBISPH time step on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload cylinder_1M] [--precision mixed|fp64]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload cylinder_1M]
+                    [--precision mixed|fp64] [--no-compare] [--no-cpu-baseline]
 
 A "step" is one full div_free step (binning, mirrors, neighbour lists, kernel weights, shifting,
 prediction with the log-conformation update, PPE assembly and BiCGSTAB solve, pressure update,
-advection) over the whole particle set, with the state resident in HBM.  The workload is the
-flow past a fixed cylinder in an 8x4 periodic channel with ~1.0M particles (synthetic, seeded).
-Prints one JSON line; `value` is particle-steps/s of the requested precision.
+advection) over the whole particle set, with the state resident in HBM.  The workload is the flow
+past a fixed cylinder in an 8x4 periodic channel with ~1.0M particles (synthetic, seeded lattice).
+
+`value` is particle-steps/s of the requested precision (mixed by default: FP32 matvecs inside the
+FP64 BiCGSTAB); the same workload is then run in plain FP64 and reported beside it, with the ratio.
+The roofline entry is the BiCGSTAB SpMV kernel, timed with HIP events on the solver's stream.
+Prints one JSON line on stdout; progress goes to stderr.
 """
 from __future__ import annotations
 
@@ -21,54 +26,84 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0      # MI355X HBM3E
+SPMV_KERNEL = {"mixed": "k_bi_spmv_p<1> (BiCGSTAB SpMV, FP32 values)",
+               "fp64": "k_bi_spmv_p<0> (BiCGSTAB SpMV, FP64 values)"}
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "spmv_traffic_cylinder_1M.json")
 
 
-def cpu_baseline(case, state, time0, itime0, threads):
-    """The reference itself (oracle/_ref, compiled from its Fortran sources) on the host cores, one
-    step of the same workload from the same state; falls back to the C restatement (one thread)."""
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def measured_traffic(workload, precision):
+    """HBM bytes per SpMV launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE runs."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+        if t.get("workload") == workload and precision in t:
+            return t[precision]["traffic_bytes"]
+    except (OSError, ValueError, KeyError):
+        pass
+    return None
+
+
+def cpu_baseline(case, state, time0, itime0, threads, budget_s=20.0):
+    """The reference itself (oracle/_ref, compiled from its Fortran sources) on the host cores,
+    stepping the same workload from the same state for about `budget_s` seconds; falls back to the
+    C restatement (one thread)."""
+    def sample(S, kind, cores, label):
+        dts, its = [], []
+        while True:
+            t = time.perf_counter()
+            S.step()
+            dts.append(time.perf_counter() - t)
+            its.append(S.scalars()["LS_iters"])
+            if sum(dts) + dts[-1] > budget_s or len(dts) >= 10:
+                break
+        tot = sum(dts)
+        return {"value": case.npfb * len(dts) / tot, "unit": "particle-steps/s", "cores": cores,
+                "kind": kind, "sample": f"{len(dts)} step(s) of the same workload from the GPU state "
+                f"after warmup ({label}, BiCGSTAB iterations {its}, {tot:.1f} s)"}
     try:
         from oracle import refwrap
         if refwrap.available():
             R = refwrap.RefSolver(case, time=time0, itime=itime0, state=state)
             R.set_threads(threads)
-            t = time.perf_counter()
-            R.step()
-            dt = time.perf_counter() - t
-            it = R.scalars()["LS_iters"]
-            return {"value": case.npfb / dt, "unit": "particle-steps/s", "cores": threads,
-                    "kind": "reference", "sample": f"1 step of the same workload from the GPU state after "
-                    f"warmup (reference Fortran, OpenMP {threads} threads, {it} BiCGSTAB iterations, {dt:.1f} s)"}
+            return sample(R, "reference", threads, f"reference Fortran, OpenMP {threads} threads")
     except Exception as e:  # noqa: BLE001 - the baseline is reported, never fatal
-        print(f"[bench] reference baseline unavailable: {e}", file=sys.stderr)
+        log(f"reference baseline unavailable: {e}")
     try:
         from oracle import oraclewrap
         O = oraclewrap.OracleSolver(case, time=time0, itime=itime0, state=state)
-        t = time.perf_counter()
-        O.step()
-        dt = time.perf_counter() - t
-        return {"value": case.npfb / dt, "unit": "particle-steps/s", "cores": 1, "kind": "port",
-                "sample": f"1 step of the same workload (C restatement, 1 thread, {dt:.1f} s)"}
+        return sample(O, "port", 1, "C restatement, 1 thread")
     except Exception as e:  # noqa: BLE001
-        print(f"[bench] port baseline unavailable: {e}", file=sys.stderr)
+        log(f"port baseline unavailable: {e}")
         return None
 
 
 def run(case, precision, steps, warmup):
-    from vebisph_amd import solver, _lib
+    """Warmup, then exactly `steps` timed steps; the SpMV roofline figures after the timed region."""
+    from vebisph_amd import _lib, solver
     prec = _lib.PREC_MIXED if precision == "mixed" else _lib.PREC_FP64
     s = solver.Solver(case, precision=prec)
     for _ in range(warmup):
         s.div_free()
     sc0 = s.scalars()                      # synchronises
     state = s.get_state()
-    iters = []
     t0 = time.perf_counter()
     for _ in range(steps):
-        s.div_free()                       # returns after the step's final synchronisation
-        iters.append(s.scalars()["LS_iters"])
+        s.div_free()                       # each step ends with a stream synchronisation
     t1 = time.perf_counter()
+    sc = s.scalars()
     us, by = s.profile_spmv(reps=20)
-    return s, (t1 - t0), iters, (us, by), state, sc0
+    elapsed = t1 - t0
+    res = {"value": case.npfb * steps / elapsed, "ms_per_step": 1e3 * elapsed / steps,
+           "ls_iters_last_step": sc["LS_iters"], "ls_residual_last_step": sc["LS_residual"],
+           "spmv_us": us, "spmv_bytes": by, "spmv_GBs": by / (us * 1e-6) / 1e9}
+    log(f"{precision}: {res['value']:.4g} particle-steps/s, {res['ms_per_step']:.1f} ms/step, "
+        f"BiCGSTAB {sc['LS_iters']} iterations (residual {sc['LS_residual']:.3g}); "
+        f"SpMV {us:.1f} us, {res['spmv_GBs']:.0f} GB/s")
+    return s, res, state, sc0
 
 
 def main():
@@ -77,46 +112,67 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="cylinder_1M")
-    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
+    ap.add_argument("--no-compare", action="store_true", help="skip the run at the other precision")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md on how the "
-              "step would shard); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: the step has no multi-GPU path in this library (the box "
+              "has one MI355X); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     from vebisph_amd import cases
     case = cases.make(a.workload)
-    print(f"[bench] workload {case.name}: {case.npfb} particles ({case.nbS} boundary), "
-          f"precision {a.precision}, warmup {a.warmup}, steps {a.steps}", file=sys.stderr, flush=True)
-    s, elapsed, iters, (us, by), state, sc0 = run(case, a.precision, a.steps, a.warmup)
-    value = case.npfb * a.steps / elapsed
-    achieved = by / (us * 1e-6) / 1e9
-    print(f"[bench] {value:.4g} particle-steps/s, {1e3 * elapsed / a.steps:.1f} ms/step, "
-          f"BiCGSTAB iterations {iters}; SpMV {us:.1f} us, {achieved:.0f} GB/s", file=sys.stderr, flush=True)
+    log(f"workload {case.name}: {case.npfb} particles ({case.nbS} boundary), warmup {a.warmup}, "
+        f"steps {a.steps}")
+
+    s, main_res, state, sc0 = run(case, a.precision, a.steps, a.warmup)
+    copy_bytes = 1 << 30
+    copy_us = s.profile_copy(copy_bytes, reps=10)
+    copy_gbs = 2 * copy_bytes / (copy_us * 1e-6) / 1e9
+    log(f"plain copy of {copy_bytes >> 20} MiB: {copy_us:.1f} us, {copy_gbs:.0f} GB/s")
+    s.close()
+
+    other = None
+    if not a.no_compare:
+        oprec = "fp64" if a.precision == "mixed" else "mixed"
+        so, ores, _, _ = run(case, oprec, a.steps, a.warmup)
+        so.close()
+        other = {"precision": oprec, "value": ores["value"], "ms_per_step": ores["ms_per_step"],
+                 "ls_iters_last_step": ores["ls_iters_last_step"],
+                 "spmv_us": ores["spmv_us"], "spmv_GBs": ores["spmv_GBs"]}
+
     cpu = None
     if not a.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)
         cpu = cpu_baseline(case, state, sc0["time"], sc0["itime"], threads)
+        if cpu:
+            log(f"CPU baseline ({cpu['kind']}, {cpu['cores']} cores): {cpu['value']:.4g} particle-steps/s")
+
+    achieved = main_res["spmv_GBs"]
     line = {
         "metric": "particle-steps/s, FP32+FP64-refined vs FP64, 1M particles; adjoint/forward cost",
-        "value": value,
+        "value": main_res["value"],
         "unit": "particle-steps/s",
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": 1e3 * elapsed / a.steps,
+        "ms_per_step": main_res["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64" if a.precision == "fp64" else "f32+f64",
         "data": "synthetic",
         "config": {"workload": case.name, "particles": case.npfb, "boundary_particles": case.nbS,
-                   "precision": a.precision, "Wi": case.Wi, "beta": case.beta,
-                   "ppe_iters_per_step": iters},
+                   "precision": a.precision, "Wi": case.Wi, "beta": case.beta, "Re": case.Re,
+                   "ppe_iters_last_step": main_res["ls_iters_last_step"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_bi_spmv_p (BiCGSTAB ELL SpMV)", "alg_bytes_per_launch": by,
-                     "avg_us": us},
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(case.name, a.precision),
+                     "kernel": SPMV_KERNEL[a.precision],
+                     "alg_bytes_per_launch": main_res["spmv_bytes"], "avg_us": main_res["spmv_us"],
+                     "copy_GBs": copy_gbs},
+        "compare": other,
+        "ratio_vs_other_precision": (main_res["value"] / other["value"]) if other else None,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -159,6 +159,10 @@ vb_status vb_export_ppe(vb_ctx* ctx, int32_t n, int32_t w, double* diag, int32_t
  * bytes of one launch (row data + ELL columns/values of the nnz actually used, gathered vector
  * counted once).  Leaves the particle state untouched; call after a step. */
 vb_status vb_profile_spmv(vb_ctx* ctx, int32_t reps, double* avg_us, double* alg_bytes);
+/* Times `reps` launches of a plain device copy of `bytes` bytes (kernel vb_copy_calibration, 16-byte
+ * loads and stores) between two temporary buffers on the context's stream: the HBM rate a streaming
+ * kernel reaches, and the known-size kernel that scales hardware byte counters. */
+vb_status vb_profile_copy(vb_ctx* ctx, int64_t bytes, int32_t reps, double* avg_us);
 
 /* ---- stand-alone device kernels on caller-owned device memory (stream 0) ---- */
 /* nnf_models_mod.F90:8-110 (no call sites in the reference): model 0 newtonian (copies mu_or_sr),

@@ -25,7 +25,8 @@ EXPORTS = [
     "vb_mirror_velocities", "vb_ppe_solve", "vb_sparse_pbicgstab_nonull", "vb_pressure_update",
     "vb_final_advection", "vb_div_free",
     "vb_get_scalars", "vb_export_mirrors", "vb_export_neighbours", "vb_export_weights",
-    "vb_export_ushift", "vb_export_grad_p", "vb_export_ppe", "vb_profile_spmv", "vb_nnf",
+    "vb_export_ushift", "vb_export_grad_p", "vb_export_ppe", "vb_profile_spmv", "vb_profile_copy",
+    "vb_nnf",
     "vb_kernel_functions", "vb_logconf",
 ]
 
@@ -95,6 +96,7 @@ def lib():
         L.vb_export_grad_p.argtypes = [vp, C.c_int32, dp]
         L.vb_export_ppe.argtypes = [vp, C.c_int32, C.c_int32, dp, ip, dp, dp]
         L.vb_profile_spmv.argtypes = [vp, C.c_int32, dp, dp]
+        L.vb_profile_copy.argtypes = [vp, C.c_int64, C.c_int32, dp]
         L.vb_nnf.argtypes = [C.c_int32, C.c_int32, vp, C.c_double, vp]
         L.vb_kernel_functions.argtypes = [C.c_int32, vp, C.c_double, C.c_double, vp, vp]
         L.vb_logconf.argtypes = [C.c_int32, vp, vp, vp, C.c_double, C.c_double, C.c_double,

@@ -93,6 +93,10 @@ class Case:
     def Wi(self) -> float:
         return self.El * math.sqrt(self.Ra / self.Pr)
 
+    @property
+    def Re(self) -> float:
+        return math.sqrt(self.Ra / self.Pr)
+
     def params_vector(self) -> np.ndarray:
         return np.array([self.h, self.dx, self.dv, self.Ra, self.Pr, self.El, self.beta,
                          self.body_force[0], self.body_force[1], self.dt_coef_cfl,

@@ -360,6 +360,43 @@ vb_status vb_profile_spmv(vb_ctx* c, int32_t reps, double* avg_us, double* alg_b
   return profile_spmv(c, reps, avg_us, alg_bytes);
 }
 
+__global__ void vb_copy_calibration(const double2* __restrict__ src, double2* __restrict__ dst, size_t n2) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n2; i += stride) dst[i] = src[i];
+}
+
+vb_status vb_profile_copy(vb_ctx* c, int64_t bytes, int32_t reps, double* avg_us) {
+  if (!c || reps < 1 || !avg_us || bytes < 16 || bytes % 16 != 0) return VB_ERR_ARG;
+  void *src = nullptr, *dst = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  vb_status s = VB_OK;
+  const size_t n2 = (size_t)bytes / 16;
+  float ms = 0.f;
+  do {
+    if ((s = hip_check(c, hipMalloc(&src, bytes), "hipMalloc")) != VB_OK) break;
+    if ((s = hip_check(c, hipMalloc(&dst, bytes), "hipMalloc")) != VB_OK) break;
+    if ((s = hip_check(c, hipMemsetAsync(src, 0, bytes, c->stream), "hipMemsetAsync")) != VB_OK) break;
+    if ((s = hip_check(c, hipEventCreate(&e0), "hipEventCreate")) != VB_OK) break;
+    if ((s = hip_check(c, hipEventCreate(&e1), "hipEventCreate")) != VB_OK) break;
+    for (int r = 0; r <= reps && s == VB_OK; r++) {
+      if (r == 1) s = hip_check(c, hipEventRecord(e0, c->stream), "hipEventRecord");
+      hipLaunchKernelGGL(vb_copy_calibration, dim3(4096), dim3(kBlock), 0, c->stream,
+                         (const double2*)src, (double2*)dst, n2);
+      if (s == VB_OK) s = hip_check(c, hipGetLastError(), "vb_copy_calibration");
+    }
+    if (s != VB_OK) break;
+    if ((s = hip_check(c, hipEventRecord(e1, c->stream), "hipEventRecord")) != VB_OK) break;
+    if ((s = hip_check(c, hipEventSynchronize(e1), "hipEventSynchronize")) != VB_OK) break;
+    s = hip_check(c, hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+  } while (0);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (src) (void)hipFree(src);
+  if (dst) (void)hipFree(dst);
+  if (s == VB_OK) *avg_us = 1000.0 * ms / reps;
+  return s;
+}
+
 vb_status vb_export_mirrors(vb_ctx* c, int32_t m, int32_t* irel, int32_t* vrel, double* dP) {
   if (!c || m < 0 || m > c->mcap) return VB_ERR_ARG;
   VB_HIP(c, hipMemcpyAsync(irel, c->st.irel + c->npfb, (size_t)m * 4, hipMemcpyDeviceToHost, c->stream));

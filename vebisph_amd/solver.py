@@ -237,6 +237,12 @@ class Solver:
         self._chk(_lib.lib().vb_profile_spmv(self.h, reps, C.byref(us), C.byref(by)))
         return us.value, by.value
 
+    def profile_copy(self, nbytes=1 << 30, reps=10):
+        """Mean duration (us) of a plain device copy of `nbytes` bytes (HBM streaming rate)."""
+        us = C.c_double(0.0)
+        self._chk(_lib.lib().vb_profile_copy(self.h, nbytes, reps, C.byref(us)))
+        return us.value
+
     def ppe_ell(self, w):
         n = self.case.npfb
         diag = np.zeros(n); col = np.zeros(w * n, np.int32); val = np.zeros(w * n); rhs = np.zeros(n)
