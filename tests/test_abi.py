@@ -1,0 +1,116 @@
+"""The C ABI (include/vebisph_gpu.h) and its Fortran interface, checked without a GPU: the libraries
+load, export every declared entry point, and the record layouts agree between C, ctypes and
+Fortran.  No compute call is made."""
+import ctypes as C
+import os
+import re
+import shutil
+import subprocess
+
+import pytest
+
+from vebisph_amd import _lib, build, fortran_drive
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "vebisph_gpu.h")
+F90 = os.path.join(ROOT, "vebisph_amd", "fortran", "vebisph_gpu.F90")
+
+
+def header_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:vb_status|const char\*)\s+(vb_\w+)\s*\(", text, flags=re.M)))
+
+
+def fortran_bindings():
+    return sorted(set(re.findall(r'bind\(C,\s*name="(vb_\w+)"\)', open(F90).read())))
+
+
+def test_header_declares_the_step():
+    names = header_functions()
+    for n in ("vb_create", "vb_destroy", "vb_div_free", "vb_ppe_solve", "vb_sparse_pbicgstab_nonull",
+              "vb_particle_allocation", "vb_kernel_calculation", "vb_prediction_step", "vb_nnf"):
+        assert n in names
+    assert sorted(_lib.EXPORTS) == names
+
+
+def test_library_exports_every_declared_symbol():
+    L = _lib.lib()
+    missing = [n for n in header_functions() if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_fortran_interface_covers_the_header():
+    assert fortran_bindings() == header_functions()
+
+
+def test_fortran_library_exports_driver():
+    if not os.path.exists(fortran_drive.FLIB_PATH):
+        if not os.path.exists(build.FLANG):
+            pytest.skip("flang absent and libvebisph_f.so not built")
+        build.build_fortran()
+    L = fortran_drive.flib()
+    for n in fortran_drive.FEXPORTS:
+        assert hasattr(L, n)
+
+
+def _c_layout():
+    """sizeof / offsetof of the public records, from a C program compiled against the header."""
+    if shutil.which("gcc") is None:
+        pytest.skip("gcc absent")
+    src = r"""
+#include <stdio.h>
+#include <stddef.h>
+#include "vebisph_gpu.h"
+int main(void) {
+  printf("%zu %zu %zu %zu %zu %zu\n", sizeof(vb_params), offsetof(vb_params, body_force),
+         offsetof(vb_params, itime), offsetof(vb_params, serial_reductions),
+         sizeof(vb_boundary), offsetof(vb_boundary, c_centre));
+  return 0;
+}
+"""
+    d = os.path.join(ROOT, "vebisph_amd", "_build")
+    os.makedirs(d, exist_ok=True)
+    cfile, exe = os.path.join(d, "abi_layout.c"), os.path.join(d, "abi_layout")
+    open(cfile, "w").write(src)
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), cfile, "-o", exe], check=True)
+    return [int(v) for v in subprocess.run([exe], check=True, capture_output=True, text=True).stdout.split()]
+
+
+def test_record_layout_c_vs_ctypes():
+    P, B = _lib.VbParams, _lib.VbBoundary
+    assert _c_layout() == [C.sizeof(P), P.body_force.offset, P.itime.offset, P.serial_reductions.offset,
+                           C.sizeof(B), B.c_centre.offset]
+
+
+def test_record_layout_c_vs_fortran():
+    if not os.path.exists(build.FLANG):
+        pytest.skip("flang absent")
+    if not os.path.exists(os.path.join(build.BUILD, "vebisph_gpu.mod")):
+        build.build_fortran()
+    src = """
+program layout
+  use iso_c_binding
+  use vebisph_gpu
+  type(vb_params), target :: p
+  type(vb_boundary), target :: b
+  print '(6i8)', c_sizeof(p), loc_diff(c_loc(p%body_force), c_loc(p)), loc_diff(c_loc(p%itime), c_loc(p)), &
+       loc_diff(c_loc(p%serial_reductions), c_loc(p)), c_sizeof(b), loc_diff(c_loc(b%c_centre), c_loc(b))
+contains
+  integer function loc_diff(a, o)
+    type(c_ptr), intent(in) :: a, o
+    integer(c_intptr_t) :: ia, io
+    ia = transfer(a, ia); io = transfer(o, io)
+    loc_diff = int(ia - io)
+  end function
+end program
+"""
+    d = build.BUILD
+    f, exe = os.path.join(d, "abi_layout.f90"), os.path.join(d, "abi_layout_f")
+    open(f, "w").write(src)
+    # the record types only: compile the program against the module, link nothing from the GPU side
+    obj = os.path.join(d, "vebisph_gpu_f.o")
+    subprocess.run([build.FLANG, "-I", d, f, obj, "-o", exe, f"-L{build.PKG}", "-lvebisph_hip",
+                    f"-Wl,-rpath,{build.PKG}"], check=True)
+    got = [int(v) for v in subprocess.run([exe], check=True, capture_output=True, text=True).stdout.split()]
+    assert got == _c_layout()

@@ -35,6 +35,53 @@ def _i(a):
     return a.ctypes.data_as(_ip) if a is not None else None
 
 
+def make_params(case, precision=_lib.PREC_FP64, reference_kick=True, time=0.0, itime=0,
+                serial_reductions=False) -> VbParams:
+    """vb_params of a Case (the INDAT values, input.F90:18-37)."""
+    p = VbParams()
+    p.h, p.dx, p.dv = case.h, case.dx, case.dv
+    p.Ra, p.Pr, p.El, p.beta = case.Ra, case.Pr, case.El, case.beta
+    p.body_force[0], p.body_force[1] = case.body_force
+    p.dt_coef_cfl, p.dt_coef_visc = case.dt_coef_cfl, case.dt_coef_visc
+    p.ve_nonlinearity = case.ve_nonlinearity
+    p.time, p.itime = time, itime
+    p.i_open_domain, p.i_shift_coeff = case.i_open_domain, case.i_shift_coeff
+    p.precision = int(precision)
+    p.reference_kick = 1 if reference_kick else 0
+    p.serial_reductions = 1 if serial_reductions else 0
+    return p
+
+
+def make_boundary(case, keep: list) -> VbBoundary:
+    """vb_boundary of a Case (IBOUND, input.F90:51-81); the arrays it points to are appended to
+    `keep`, which must outlive every call that reads the record."""
+    def f64(a):
+        keep.append(np.ascontiguousarray(a, dtype=np.float64))
+        return _d(keep[-1])
+
+    def i32(a):
+        keep.append(np.ascontiguousarray(a, dtype=np.int32))
+        return _i(keep[-1])
+    b = VbBoundary()
+    b.nb_patches = case.nb_patches
+    b.b_node, b.b_type = f64(case.b_node), i32(case.b_type)
+    b.b_vel, b.b_thermal = f64(case.b_vel), f64(case.b_thermal)
+    b.b_periodic_parent = i32(case.b_periodic_parent)
+    b.nb_circles = case.nb_circles
+    if case.nb_circles:
+        b.c_centre, b.c_radius = f64(case.c_centre), f64(case.c_radius)
+        b.c_omega, b.c_vel = f64(case.c_omega), f64(case.c_vel)
+        b.c_thermal = f64(case.c_thermal)
+    return b
+
+
+def initial_state(case):
+    """The state a case starts from: fluid at rest, conformation I (tau = 0, trace 2), p = 0."""
+    n = case.npfb
+    return dict(x=case.x, y=case.y, u=case.u, v=case.v, theta=case.theta, tau=np.zeros((n, 4)),
+                trace=np.full(n, 2.0), p=np.zeros(n), inbin=np.zeros(n, np.int32))
+
+
 class Solver:
     """One gfx950 context holding a Case's particle state in HBM."""
 
@@ -42,46 +89,16 @@ class Solver:
                  state=None, serial_reductions=False):
         L = _lib.lib()
         self.case = case
-        p = VbParams()
-        p.h, p.dx, p.dv = case.h, case.dx, case.dv
-        p.Ra, p.Pr, p.El, p.beta = case.Ra, case.Pr, case.El, case.beta
-        p.body_force[0], p.body_force[1] = case.body_force
-        p.dt_coef_cfl, p.dt_coef_visc = case.dt_coef_cfl, case.dt_coef_visc
-        p.ve_nonlinearity = case.ve_nonlinearity
-        p.time, p.itime = time, itime
-        p.i_open_domain, p.i_shift_coeff = case.i_open_domain, case.i_shift_coeff
-        p.precision = int(precision)
-        p.reference_kick = 1 if reference_kick else 0
-        p.serial_reductions = 1 if serial_reductions else 0
+        p = make_params(case, precision, reference_kick, time, itime, serial_reductions)
         self._keep = []
-        f64 = lambda a: self._hold(np.ascontiguousarray(a, dtype=np.float64))
-        i32 = lambda a: self._hold(np.ascontiguousarray(a, dtype=np.int32))
-        b = VbBoundary()
-        b.nb_patches = case.nb_patches
-        b.b_node, b.b_type = _d(f64(case.b_node)), _i(i32(case.b_type))
-        b.b_vel, b.b_thermal = _d(f64(case.b_vel)), _d(f64(case.b_thermal))
-        b.b_periodic_parent = _i(i32(case.b_periodic_parent))
-        b.nb_circles = case.nb_circles
-        if case.nb_circles:
-            b.c_centre, b.c_radius = _d(f64(case.c_centre)), _d(f64(case.c_radius))
-            b.c_omega, b.c_vel = _d(f64(case.c_omega)), _d(f64(case.c_vel))
-            b.c_thermal = _d(f64(case.c_thermal))
+        b = make_boundary(case, self._keep)
         h = C.c_void_p()
         st = L.vb_create(C.byref(p), C.byref(b), case.npfb, case.nbS, C.byref(h))
         if st != 0:
             raise VbError(st, "vb_create failed")
         self.h = h
         self.last_warning = 0
-        n = case.npfb
-        if state is None:
-            state = dict(x=case.x, y=case.y, u=case.u, v=case.v, theta=case.theta,
-                         tau=np.zeros((n, 4)), trace=np.full(n, 2.0), p=np.zeros(n),
-                         inbin=np.zeros(n, np.int32))
-        self.set_state(state)
-
-    def _hold(self, a):
-        self._keep.append(a)
-        return a
+        self.set_state(initial_state(case) if state is None else state)
 
     def close(self):
         if getattr(self, "h", None):
