@@ -114,3 +114,37 @@ end program
                     f"-Wl,-rpath,{build.PKG}"], check=True)
     got = [int(v) for v in subprocess.run([exe], check=True, capture_output=True, text=True).stdout.split()]
     assert got == _c_layout()
+
+
+def test_reference_glue_compiles_against_reference_modules():
+    """divfree_reference_glue.F90 replaces the reference's module divfree inside the reference's own
+    build: it must compile against the reference's module files (oracle/_ref/obj, built from the
+    reference sources by oracle/Makefile)."""
+    ref_obj = os.path.join(ROOT, "oracle", "_ref", "obj")
+    if not os.path.exists(os.path.join(ref_obj, "common_2d.mod")) or not os.path.exists(build.FLANG):
+        pytest.skip("reference module files not built here")
+    if not os.path.exists(os.path.join(build.BUILD, "divfree_gpu.mod")):
+        build.build_fortran()
+    out = os.path.join(build.BUILD, "glue_check")
+    os.makedirs(out, exist_ok=True)
+    subprocess.run([build.FLANG, "-cpp", "-c", "-fPIC", "-module-dir", out, "-I", build.BUILD, "-I", ref_obj,
+                    os.path.join(ROOT, "vebisph_amd", "fortran", "divfree_reference_glue.F90"),
+                    "-o", os.path.join(out, "glue.o")], check=True)
+
+
+def test_cpp_driver_rejects_bad_input(tmp_path):
+    """vb_drive's own argument and file checks (no device call is reached)."""
+    exe = os.path.join(build.PKG, "vb_drive")
+    if not os.path.exists(exe):
+        build.build_cpp_driver()
+    assert subprocess.run([exe], capture_output=True).returncode == 2
+    bad = tmp_path / "bad.bin"
+    bad.write_bytes(b"XXXX")
+    assert subprocess.run([exe, str(bad), "1", str(tmp_path / "o")], capture_output=True).returncode == 2
+    from vebisph_amd import cases, solver
+    good = tmp_path / "c.bin"
+    solver.write_case_file(str(good), cases.make("poiseuille_2500"))
+    data = good.read_bytes()
+    (tmp_path / "trunc.bin").write_bytes(data[:-100])
+    r = subprocess.run([exe, str(tmp_path / "trunc.bin"), "1", str(tmp_path / "o")], capture_output=True, text=True)
+    assert r.returncode == 2 and "truncated" in r.stderr

@@ -10,6 +10,8 @@ Tolerances, stated where used:
     solution is checked directly against the assembled system;
   * everything downstream of the pressure carries that difference scaled by dt |grad p|.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -350,3 +352,23 @@ def test_error_neighbour_capacity(gpu_lib):
     with pytest.raises(solver.VbError) as e:
         G.phase(4)
     assert e.value.status == 4
+
+
+def test_cpp_driver_matches_python_path(gpu_lib, tmp_path):
+    """vb_drive (stand-alone C++ program over the C ABI) runs the same steps as the ctypes Solver."""
+    import subprocess
+    exe = os.path.join(os.path.dirname(solver.__file__), "vb_drive")
+    case = SMALL["convection"]()
+    cf, of = str(tmp_path / "case.bin"), str(tmp_path / "out.bin")
+    solver.write_case_file(cf, case)
+    print(f"case {case.name} npfb={case.npfb}: {exe}", flush=True)
+    r = subprocess.run([exe, cf, "3", of], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0
+    assert len(r.stdout.strip().splitlines()) == 3
+    G = solver.Solver(case)
+    for _ in range(3):
+        G.div_free()
+    got, want = solver.read_state_file(of, case.npfb), G.get_state()
+    for k in want:
+        _eq(got[k], want[k])

@@ -79,6 +79,16 @@ def build_fortran() -> str | None:
     return FLIB
 
 
+def build_cpp_driver() -> str:
+    """vb_drive: the stand-alone C++ driver of the C ABI (vebisph_amd/cpp/vb_drive.cpp)."""
+    out = os.path.join(PKG, "vb_drive")
+    _run(["g++", "-O2", "-std=c++17", "-Wall", f"-I{os.path.join(ROOT, 'include')}",
+          os.path.join(PKG, "cpp", "vb_drive.cpp"), "-o", out + ".tmp", f"-L{PKG}", "-lvebisph_hip",
+          "-Wl,-rpath,$ORIGIN"])
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def build_checkers() -> None:
     """oracle/liboracle.so always; oracle/_ref when the reference sources are present."""
     _run(["make", "-C", os.path.join(ROOT, "oracle"), "oracle"])
@@ -89,6 +99,7 @@ def build_checkers() -> None:
 def build_all() -> None:
     build_hip()
     build_fortran()
+    build_cpp_driver()
     build_checkers()
 
 

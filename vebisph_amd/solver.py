@@ -82,6 +82,48 @@ def initial_state(case):
                 trace=np.full(n, 2.0), p=np.zeros(n), inbin=np.zeros(n, np.int32))
 
 
+STATE_COLUMNS = ("x", "y", "u", "v", "theta", "tau11", "tau12", "tau21", "tau22", "trace", "p")
+
+
+def _state_columns(s):
+    tau = np.asarray(s["tau"], dtype=np.float64)
+    cols = {"x": s["x"], "y": s["y"], "u": s["u"], "v": s["v"], "theta": s["theta"],
+            "tau11": tau[:, 0], "tau12": tau[:, 1], "tau21": tau[:, 2], "tau22": tau[:, 3],
+            "trace": s["trace"], "p": s["p"]}
+    return [np.ascontiguousarray(cols[k], dtype=np.float64) for k in STATE_COLUMNS]
+
+
+def write_case_file(path, case, precision=_lib.PREC_FP64, state=None):
+    """The case file read by the C++ driver vb_drive (layout in vebisph_amd/cpp/vb_drive.cpp)."""
+    p = make_params(case, precision)
+    s = initial_state(case) if state is None else state
+    f64 = lambda a: np.ascontiguousarray(a, dtype=np.float64).tobytes()
+    i32 = lambda a: np.ascontiguousarray(a, dtype=np.int32).tobytes()
+    with open(path, "wb") as f:
+        f.write(b"VBC1")
+        f.write(i32([case.npfb, case.nbS, case.nb_patches, case.nb_circles]))
+        f.write(bytes(p))
+        f.write(f64(case.b_node) + i32(case.b_type) + f64(case.b_vel) + f64(case.b_thermal)
+                + i32(case.b_periodic_parent))
+        if case.nb_circles:
+            f.write(f64(case.c_centre) + f64(case.c_radius) + f64(case.c_omega) + f64(case.c_vel)
+                    + f64(case.c_thermal))
+        for c in _state_columns(s):
+            f.write(c.tobytes())
+        f.write(i32(s["inbin"]))
+
+
+def read_state_file(path, n):
+    """The state file written by vb_drive, as a dict like Solver.get_state()."""
+    raw = open(path, "rb").read()
+    cols = np.frombuffer(raw[:88 * n], dtype=np.float64).reshape(11, n)
+    d = dict(zip(STATE_COLUMNS, cols))
+    out = {k: d[k].copy() for k in ("x", "y", "u", "v", "theta", "trace", "p")}
+    out["tau"] = np.stack([d["tau11"], d["tau12"], d["tau21"], d["tau22"]], axis=1)
+    out["inbin"] = np.frombuffer(raw[88 * n:88 * n + 4 * n], dtype=np.int32).copy()
+    return out
+
+
 class Solver:
     """One gfx950 context holding a Case's particle state in HBM."""
 
@@ -123,10 +165,7 @@ class Solver:
     # ---- state ----
     def set_state(self, s):
         n = self.case.npfb
-        c = lambda a: np.ascontiguousarray(a, dtype=np.float64)
-        tau = np.asarray(s["tau"], dtype=np.float64)
-        cols = [c(tau[:, q]) for q in range(4)]
-        arrs = [c(s["x"]), c(s["y"]), c(s["u"]), c(s["v"]), c(s["theta"]), *cols, c(s["trace"]), c(s["p"])]
+        arrs = _state_columns(s)
         inb = np.ascontiguousarray(s["inbin"], dtype=np.int32)
         self._chk(_lib.lib().vb_upload_state(self.h, n, *[_d(a) for a in arrs], _i(inb)))
 
