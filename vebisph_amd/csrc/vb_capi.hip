@@ -63,7 +63,7 @@ static void free_all(vb_ctx* c) {
   void* ptrs[] = {c->st.x, c->st.y, c->st.u, c->st.v, c->st.th, c->st.t11, c->st.t12, c->st.t21,
                   c->st.t22, c->st.trc, c->st.p, c->st.inbin, c->st.x0, c->st.y0, c->st.u0, c->st.v0,
                   c->st.irel, c->st.vrel, c->st.dP, c->u_alt, c->v_alt, c->th_alt, c->nsurf,
-                  c->seg_counts, c->cellkey, c->cellslot, c->nc, c->ist, c->ip, c->cnt, c->link,
+                  c->seg_counts, c->cellkey, c->cellslot, c->nc, c->ist, c->ip, c->cell_bsum, c->cnt, c->link,
                   c->wg1, c->wg2, c->wl, c->k11, c->k12, c->k21, c->k22,
                   c->usx, c->usy, c->kick, c->gpx, c->gpy, c->a_diag, c->a_val, c->a_rowsum, c->rhs,
                   c->a_col, c->a_len, c->rowflag, c->lnk4, c->val4, c->val4f, c->r, c->rb, c->vv,
@@ -177,6 +177,7 @@ vb_status vb_create(const vb_params* prm, const vb_boundary* bnd, int32_t npfb, 
   c->seg_blocks = grid_1d(std::max<long>((long)cap, (long)npfb * std::max(1, c->g.ncirc)));
   A(c->seg_counts, c->seg_blocks);
   A(c->cellkey, cap); A(c->cellslot, cap); A(c->nc, c->g.nct + 1); A(c->ist, c->g.nct + 1); A(c->ip, cap);
+  A(c->cell_bsum, (c->g.nct + 1023) / 1024 + 1);
   A(c->cnt, n); A(c->link, nw); A(c->wg1, nw); A(c->wg2, nw); A(c->wl, nw);
   A(c->k11, n); A(c->k12, n); A(c->k21, n); A(c->k22, n);
   A(c->usx, n); A(c->usy, n); A(c->kick, n); A(c->gpx, n); A(c->gpy, n);
@@ -187,7 +188,7 @@ vb_status vb_create(const vb_params* prm, const vb_boundary* bnd, int32_t npfb, 
   if (prm->precision == VB_PREC_MIXED) A(c->val4f, nw4);
   A(c->r, n); A(c->rb, n); A(c->vv, n); A(c->pn, n); A(c->pinv, n); A(c->ss, n); A(c->sinv, n);
   A(c->tt, n); A(c->xsys, n);
-  A(c->partials, (size_t)kSolverStages * 2 * kMaxBlocks); A(c->scal, 1);
+  A(c->partials, (size_t)kSolverStages * 2 * kPartialStride); A(c->scal, 1);
   if (prm->serial_reductions) A(c->red, (size_t)kSolverStages * 2 * n);
 #undef A
   if (s == VB_OK) s = hip_check(c, hipHostMalloc((void**)&c->scal_host, sizeof(Scal)), "hipHostMalloc");

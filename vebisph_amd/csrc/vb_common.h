@@ -14,19 +14,25 @@
 namespace vb {
 
 #ifndef VB_SPMV_BLOCK
-#define VB_SPMV_BLOCK 512
+#define VB_SPMV_BLOCK 256
 #endif
 #ifndef VB_MAX_BLOCKS
-#define VB_MAX_BLOCKS 2048
+#define VB_MAX_BLOCKS 1024
 #endif
 constexpr int kBlock = 256;                // threads per workgroup (4 wavefronts of 64)
-// SpMV workgroup size (multiple of 64, at most 1024).  Measured on the 1M-row PPE: one row per
-// thread (2048 x 512) beats 2 rows per thread by 8-15%; the gathers need every wave in flight.
+// SpMV launch: one row per thread (up to kSpmvMaxBlocks workgroups), so that every row's chain of
+// dependent gathers has its own wavefront slot.  Measured on the 1M-row PPE: one row per thread
+// beats two by 8-15%, and 256-thread workgroups beat 512.
 constexpr int kSpmvBlock = VB_SPMV_BLOCK;
-constexpr int kMaxBlocks = VB_MAX_BLOCKS;  // grid cap for the solver's row kernels (multiple of 8)
+constexpr int kSpmvMaxBlocks = 8192;
+// Grid cap of the solver's streaming vector kernels (several rows per thread).  The kernel after
+// each reduction folds the previous kernel's per-workgroup partials in every workgroup, so this
+// cap also bounds the fold traffic (workgroups x partials).
+constexpr int kMaxBlocks = VB_MAX_BLOCKS;
+constexpr int kPartialStride = kSpmvMaxBlocks > kMaxBlocks ? kSpmvMaxBlocks : kMaxBlocks;
 static_assert(kSpmvBlock % 64 == 0 && kSpmvBlock <= 1024, "SpMV block");
-static_assert(kMaxBlocks % 8 == 0, "grid cap");
-constexpr int kSolverStages = 6;           // BiCGSTAB reduction stages with their own buffers
+static_assert(kMaxBlocks % 8 == 0 && kSpmvMaxBlocks % 8 == 0, "grid caps");
+constexpr int kSolverStages = 7;           // BiCGSTAB reduction stages with their own buffers
 constexpr int kNplink = VB_NPLINK;
 constexpr double kEps = 2.220446049250313e-16;   // epsilon(1d0)
 
@@ -173,35 +179,52 @@ struct Red {
   }
 };
 
-// One-workgroup fold (all kBlock threads call it): partials of quantity q in block order, or, in
-// serial mode, rows[q*n + 0..n-1] in index order.  Result in all threads.
-__device__ inline double fold(const double* partials, int q, int nblk, const double* rows, int n) {
-  __shared__ double sh[kBlock];
-  double r;
+// Fold of NV quantities inside one workgroup (every thread of the workgroup calls it; any multiple
+// of 64 threads): the partials of quantity q, partials[q*nblk + b], are read coalesced (thread t
+// takes b = t, t + blockDim, ...), summed per thread, per wavefront by shuffles, then the wavefront
+// sums in wavefront order by every thread.  The order depends only on nblk and blockDim, so all
+// workgroups of a kernel get the same bits.  In serial mode rows[q*n + 0..n-1] are summed in index
+// order by one thread instead.  Result in all threads.
+template <int NV>
+__device__ inline void fold_n(const double* partials, int nblk, const double* rows, int n, double (&out)[NV]) {
+  __shared__ double sh[NV][16];
   if (rows) {
     if (threadIdx.x == 0) {
-      double s = 0.0;
-      for (int i = 0; i < n; i++) s = s + rows[q * n + i];
-      sh[0] = s;
+#pragma unroll
+      for (int q = 0; q < NV; q++) {
+        double s = 0.0;
+        for (int i = 0; i < n; i++) s = s + rows[(size_t)q * n + i];
+        sh[q][0] = s;
+      }
     }
     __syncthreads();
-    r = sh[0];
+#pragma unroll
+    for (int q = 0; q < NV; q++) out[q] = sh[q][0];
     __syncthreads();
-    return r;
+    return;
   }
-  int per = (nblk + kBlock - 1) / kBlock;
-  int b0 = threadIdx.x * per, b1 = min(nblk, b0 + per);
-  double s = 0.0;
-  for (int b = b0; b < b1; b++) s += partials[q * nblk + b];
-  sh[threadIdx.x] = s;
-  __syncthreads();
-  for (int w = kBlock / 2; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
-    __syncthreads();
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+#pragma unroll
+  for (int q = 0; q < NV; q++) {
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nblk; b += blockDim.x) s += partials[q * nblk + b];
+    s = wave_sum(s);
+    if (lane == 0) sh[q][wid] = s;
   }
-  r = sh[0];
   __syncthreads();
-  return r;
+#pragma unroll
+  for (int q = 0; q < NV; q++) {
+    double s = sh[q][0];
+    for (int w = 1; w < nw; w++) s += sh[q][w];
+    out[q] = s;
+  }
+  __syncthreads();
+}
+
+__device__ inline double fold(const double* partials, int q, int nblk, const double* rows, int n) {
+  double r[1];
+  fold_n<1>(partials + (size_t)q * nblk, nblk, rows ? rows + (size_t)q * n : nullptr, n, r);
+  return r[0];
 }
 
 // Rows owned by a block: contiguous chunks, with consecutive chunks placed on the same XCD
@@ -223,6 +246,14 @@ inline int grid_for(int n) {
   int b = (n + kBlock - 1) / kBlock;
   if (b > kMaxBlocks) b = kMaxBlocks;
   if (b > 8) b = (b + 7) / 8 * 8;          // multiple of 8 for the XCD mapping
+  if (b < 1) b = 1;
+  return b;
+}
+
+inline int spmv_grid(int n) {
+  int b = (n + kSpmvBlock - 1) / kSpmvBlock;
+  if (b > kSpmvMaxBlocks) b = kSpmvMaxBlocks;
+  if (b > 8) b = (b + 7) / 8 * 8;
   if (b < 1) b = 1;
   return b;
 }

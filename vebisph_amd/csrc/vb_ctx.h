@@ -25,6 +25,7 @@ struct vb_ctx {
 
   // cells and neighbours
   int *cellkey, *cellslot, *nc, *ist, *ip;
+  int* cell_bsum;                    // block sums of the cell-start scan (one per 1024 cells)
   int *cnt, *link;                   // link[k*npfb + i], 0-based
   double *wg1, *wg2, *wl;            // slot-major like link
   double *k11, *k12, *k21, *k22;
@@ -42,7 +43,7 @@ struct vb_ctx {
   bool spmv_stored;                  // full stored rows everywhere (VB_SPMV=stored)
   double *r, *rb, *vv, *pn, *pinv, *ss, *sinv, *tt, *xsys;
 
-  double* partials;                  // reduction partials, 2 * kMaxBlocks per solver stage
+  double* partials;                  // reduction partials, 2 * kPartialStride per solver stage
   double* red;                       // per-row reduction terms, 2 * npfb per stage (serial mode only)
   vb::Scal* scal;                    // device scalars
   vb::Scal* scal_host;               // pinned mirror

@@ -199,46 +199,57 @@ __device__ inline double spmv_row(const Mat& A, int i, const double* __restrict_
   const int m = A.len[i];
   const double xi = x[i];
   if (!A.rowflag || A.rowflag[i]) return row_full(A, i, m, xi, x);
-  const int n = A.n;
+  const size_t n = (size_t)A.n;
   const int ng = (m + 3) >> 2;
-  // software pipelined: the next group's offsets and values are in flight during this group's
-  // gathers (each row is a chain of ~11 groups; without this every group waits a full HBM latency)
-  size_t e = i;
-  short4 o = A.lnk4[e];
+  // Three-stage software pipeline over the row's groups g (a row is a dependent chain of ~11
+  // groups: offsets -> gathers -> accumulate): while group g is accumulated, the gathers of g+1 and
+  // the offset/value loads of g+2 are in flight, so each step overlaps one HBM round trip and one
+  // L2 gather round trip instead of waiting for both.
+  const short4 zo = make_short4(0, 0, 0, 0);
+  short4 o1 = zo, o2 = zo;
+  double g0[4], g1[4] = {0.0, 0.0, 0.0, 0.0};
+  const short4 o0 = A.lnk4[i];
+  if (ng > 1) o1 = A.lnk4[n + i];
   if (MODE == SPMV_FP64) {
-    size_t e2 = i;                               // plane 2g: slots 0-1, plane 2g+1: slots 2-3
-    double2 va = A.val4[e2], vb = A.val4[e2 + n];
+    // FP64 values: double2 plane 2g holds slots 4g, 4g+1; plane 2g+1 slots 4g+2, 4g+3
+    const double2 z2 = make_double2(0.0, 0.0);
+    double2 a0 = A.val4[i], b0 = A.val4[n + i], a1 = z2, b1 = z2, a2 = z2, b2 = z2;
+    if (ng > 1) { a1 = A.val4[2 * n + i]; b1 = A.val4[3 * n + i]; }
+    g0[0] = x[i + o0.x]; g0[1] = x[i + o0.y]; g0[2] = x[i + o0.z]; g0[3] = x[i + o0.w];
     double bb = A.diag[i] * xi;
     for (int gi = 0; gi < ng; gi++) {
-      short4 on = o;
-      double2 van = va, vbn = vb;
-      if (gi + 1 < ng) {
-        size_t en = e + n;
-        e2 += 2 * (size_t)n;
-        on = A.lnk4[en]; van = A.val4[e2]; vbn = A.val4[e2 + n];
+      if (gi + 2 < ng) {
+        const size_t g2 = (size_t)(gi + 2);
+        o2 = A.lnk4[g2 * n + i]; a2 = A.val4[2 * g2 * n + i]; b2 = A.val4[(2 * g2 + 1) * n + i];
       }
-      double x0 = x[i + o.x], x1 = x[i + o.y], x2 = x[i + o.z], x3 = x[i + o.w];
-      int k = 4 * gi;
-      bb = bb + va.x * x0;
-      if (k + 1 < m) bb = bb + va.y * x1;
-      if (k + 2 < m) bb = bb + vb.x * x2;
-      if (k + 3 < m) bb = bb + vb.y * x3;
-      o = on; va = van; vb = vbn; e += n;
+      if (gi + 1 < ng) { g1[0] = x[i + o1.x]; g1[1] = x[i + o1.y]; g1[2] = x[i + o1.z]; g1[3] = x[i + o1.w]; }
+      const int k = 4 * gi;
+      bb = bb + a0.x * g0[0];
+      if (k + 1 < m) bb = bb + a0.y * g0[1];
+      if (k + 2 < m) bb = bb + b0.x * g0[2];
+      if (k + 3 < m) bb = bb + b0.y * g0[3];
+      a0 = a1; b0 = b1; a1 = a2; b1 = b2; o1 = o2;
+      g0[0] = g1[0]; g0[1] = g1[1]; g0[2] = g1[2]; g0[3] = g1[3];
     }
     return bb;
   } else {
-    float4 a = A.val4f[e];                      // padded slots hold 0
+    const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+    float4 a0 = A.val4f[i], a1 = z4, a2 = z4;             // padded slots hold 0
+    if (ng > 1) a1 = A.val4f[n + i];
+    g0[0] = x[i + o0.x]; g0[1] = x[i + o0.y]; g0[2] = x[i + o0.z]; g0[3] = x[i + o0.w];
     float acc = 0.0f;
     for (int gi = 0; gi < ng; gi++) {
-      short4 on = o;
-      float4 an = a;
-      if (gi + 1 < ng) { size_t en = e + n; on = A.lnk4[en]; an = A.val4f[en]; }
-      double x0 = x[i + o.x], x1 = x[i + o.y], x2 = x[i + o.z], x3 = x[i + o.w];
-      acc = fmaf(a.x, (float)(x0 - xi), acc);
-      acc = fmaf(a.y, (float)(x1 - xi), acc);
-      acc = fmaf(a.z, (float)(x2 - xi), acc);
-      acc = fmaf(a.w, (float)(x3 - xi), acc);
-      o = on; a = an; e += n;
+      if (gi + 2 < ng) {
+        const size_t g2 = (size_t)(gi + 2);
+        o2 = A.lnk4[g2 * n + i]; a2 = A.val4f[g2 * n + i];
+      }
+      if (gi + 1 < ng) { g1[0] = x[i + o1.x]; g1[1] = x[i + o1.y]; g1[2] = x[i + o1.z]; g1[3] = x[i + o1.w]; }
+      acc = fmaf(a0.x, (float)(g0[0] - xi), acc);
+      acc = fmaf(a0.y, (float)(g0[1] - xi), acc);
+      acc = fmaf(a0.z, (float)(g0[2] - xi), acc);
+      acc = fmaf(a0.w, (float)(g0[3] - xi), acc);
+      a0 = a1; a1 = a2; o1 = o2;
+      g0[0] = g1[0]; g0[1] = g1[1]; g0[2] = g1[2]; g0[3] = g1[3];
     }
     return (double)acc + A.rowsum[i] * xi;
   }
@@ -258,14 +269,14 @@ struct Vecs {
   double *x, *b, *r, *rb, *v, *pn, *pinv, *s, *sinv, *t;   // pn, s: before remove_dc
 };
 
-enum { ST_P = 0, ST_SPMV_P = 1, ST_S = 2, ST_SPMV_S = 3, ST_XR = 4, ST_INIT = 5 };
-static_assert(ST_INIT < kSolverStages, "stage buffers");
+enum { ST_P = 0, ST_SPMV_P = 1, ST_S = 2, ST_SPMV_S = 3, ST_XR = 4, ST_INIT = 5, ST_TRUE = 6 };
+static_assert(ST_TRUE < kSolverStages, "stage buffers");
 
 struct RedBufs {
-  double* partials;     // kStages * 2 * kMaxBlocks
+  double* partials;     // kStages * 2 * kPartialStride
   double* rows;         // serial mode: kStages * 2 * n, else null
   int n;
-  __device__ double* P(int st) const { return partials + (size_t)st * 2 * kMaxBlocks; }
+  __device__ double* P(int st) const { return partials + (size_t)st * 2 * kPartialStride; }
   __device__ double* R(int st) const { return rows ? rows + (size_t)st * 2 * n : nullptr; }
 };
 
@@ -341,9 +352,10 @@ __global__ void k_bi_spmv_p(Mat A, Vecs w, RedBufs rb, const Scal* sc) {
 }
 
 // :356-364  alpha = rho/(r_bar.v + 1e-15); s = r - alpha v; terms of |s|^2 and sum(s)
-__global__ void k_bi_s(int n, Vecs w, RedBufs rb, Scal* sc) {
+// (nspmv: workgroups of the SpMV launch whose partials are folded)
+__global__ void k_bi_s(int n, Vecs w, RedBufs rb, Scal* sc, int nspmv) {
   if (finished(sc)) return;
-  const double temp = fold(rb.P(ST_SPMV_P), 0, gridDim.x, rb.R(ST_SPMV_P), n);
+  const double temp = fold(rb.P(ST_SPMV_P), 0, nspmv, rb.R(ST_SPMV_P), n);
   const double alpha = sc->rho / (temp + 1.0e-15);
   if (blockIdx.x == 0 && threadIdx.x == 0) sc->alpha = alpha;
   RowRange rr = block_rows(n, gridDim.x);
@@ -360,8 +372,9 @@ __global__ void k_bi_s(int n, Vecs w, RedBufs rb, Scal* sc) {
 // :366-384  early exit on |s| (x += alpha p_inv), else remove_dc(s), inverseM: s_inv
 __global__ void k_bi_sinv(int n, Vecs w, RedBufs rb, Scal* sc) {
   if (finished(sc)) return;
-  const double n2s = fold(rb.P(ST_S), 0, gridDim.x, rb.R(ST_S), n);
-  const double sum = fold(rb.P(ST_S), 1, gridDim.x, rb.R(ST_S), n);
+  double f[2];
+  fold_n<2>(rb.P(ST_S), gridDim.x, rb.R(ST_S), n, f);
+  const double n2s = f[0], sum = f[1];
   const bool first = sc->iter == 1;
   const double n2s0 = first ? ((n2s <= 1.0e-6) ? 1.0 : n2s) : sc->n2s0;
   const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
@@ -397,10 +410,11 @@ __global__ void k_bi_spmv_s(Mat A, Vecs w, RedBufs rb, const Scal* sc) {
 
 // :389-401  omega; x = x + alpha p_inv + omega s_inv; r = s - omega t; terms of |r|^2 and the next
 // r_bar . r
-__global__ void k_bi_xr(int n, Vecs w, RedBufs rb, Scal* sc) {
+__global__ void k_bi_xr(int n, Vecs w, RedBufs rb, Scal* sc, int nspmv) {
   if (finished(sc)) return;
-  const double ts = fold(rb.P(ST_SPMV_S), 0, gridDim.x, rb.R(ST_SPMV_S), n);
-  const double tt = fold(rb.P(ST_SPMV_S), 1, gridDim.x, rb.R(ST_SPMV_S), n);
+  double f[2];
+  fold_n<2>(rb.P(ST_SPMV_S), nspmv, rb.R(ST_SPMV_S), n, f);
+  const double ts = f[0], tt = f[1];
   const double omega = ts / (tt + 1.0e-15);
   if (blockIdx.x == 0 && threadIdx.x == 0) sc->omega = omega;
   const double alpha = sc->alpha, ms = sc->mean_s;
@@ -416,14 +430,35 @@ __global__ void k_bi_xr(int n, Vecs w, RedBufs rb, Scal* sc) {
   R.store(rb.P(ST_XR));
 }
 
-// Mixed path only: replace the recurrence residual by the FP64 residual r = B - A x
+// Mixed path only: replace the recurrence residual by the FP64 residual B - A x.  The recurrence
+// residual differs from B - A x by a multiple of the constant vector (the DC projections of p and s
+// remove constants the recurrence never adds back), and A's columns need not sum to zero, so B - A x
+// carries a constant component the projected iteration cannot reduce.  The replacement therefore
+// keeps the recurrence residual's mean: r = (B - A x) - mean(B - A x) + mean(r).  Pass 1 writes
+// B - A x to t and the terms of sum(r), sum(B - A x); pass 2 shifts and forms |r|^2 and r_bar . r.
 __global__ void k_bi_true_residual(Mat A, Vecs w, RedBufs rb, const Scal* sc) {
   if (finished(sc)) return;
   const int n = A.n;
   RowRange rr = block_rows(n, gridDim.x);
+  Red<2> R(rb.R(ST_TRUE), n);
+  for (int i = rr.begin + threadIdx.x; i < rr.end; i += blockDim.x) {
+    const double rt = w.b[i] - spmv_row<SPMV_FP64>(A, i, w.x);
+    R.add(0, i, w.r[i]);
+    R.add(1, i, rt);
+    w.t[i] = rt;
+  }
+  R.store(rb.P(ST_TRUE));
+}
+
+__global__ void k_bi_true_shift(int n, Vecs w, RedBufs rb, const Scal* sc) {
+  if (finished(sc)) return;
+  double f[2];
+  fold_n<2>(rb.P(ST_TRUE), gridDim.x, rb.R(ST_TRUE), n, f);
+  const double d = (f[0] - f[1]) / (double)n;
+  RowRange rr = block_rows(n, gridDim.x);
   Red<2> R(rb.R(ST_XR), n);
   for (int i = rr.begin + threadIdx.x; i < rr.end; i += blockDim.x) {
-    double r = w.b[i] - spmv_row<SPMV_FP64>(A, i, w.x);
+    const double r = w.t[i] + d;
     w.r[i] = r;
     R.add(0, i, r * r);
     R.add(1, i, w.rb[i] * r);
@@ -434,8 +469,9 @@ __global__ void k_bi_true_residual(Mat A, Vecs w, RedBufs rb, const Scal* sc) {
 // :403-419 convergence; next rho (:330) and beta (:342-343)
 __global__ void k_fin_xr(RedBufs rb, int nblk, Scal* sc) {
   if (finished(sc)) return;
-  double n2r = fold(rb.P(ST_XR), 0, nblk, rb.R(ST_XR), rb.n);
-  double rho = fold(rb.P(ST_XR), 1, nblk, rb.R(ST_XR), rb.n);
+  double f[2];
+  fold_n<2>(rb.P(ST_XR), nblk, rb.R(ST_XR), rb.n, f);
+  const double n2r = f[0], rho = f[1];
   if (threadIdx.x == 0) {
     if (sc->iter == 1) sc->n2r0 = (n2r <= 1.0e-6) ? 1.0 : n2r;
     sc->resid = sqrt(n2r / sc->n2r0);
@@ -471,7 +507,8 @@ static RedBufs red_bufs(vb_ctx* c, int n) { return RedBufs{c->partials, c->red, 
 template <int MODE>
 static vb_status solve(vb_ctx* c, const Mat& A, Vecs w, int x_given, double tol, int itermax) {
   const int n = A.n;
-  const int nb = grid_for(n);
+  const int nb = grid_for(n);          // streaming vector kernels
+  const int ns = spmv_grid(n);         // SpMV kernels, one row per thread
   RedBufs rb = red_bufs(c, n);
   Scal* sc = c->scal;
   VB_LAUNCH(c, k_bi_init, nb, A, w, x_given, rb);
@@ -481,13 +518,15 @@ static vb_status solve(vb_ctx* c, const Mat& A, Vecs w, int x_given, double tol,
     for (int b = 0; b < batch; b++) {
       VB_LAUNCH(c, k_bi_p, nb, n, w, rb, sc);
       VB_LAUNCH(c, k_bi_pinv, nb, n, w, rb, sc);
-      VB_LAUNCH_BS(c, k_bi_spmv_p<MODE>, nb, kSpmvBlock, A, w, rb, sc);
-      VB_LAUNCH(c, k_bi_s, nb, n, w, rb, sc);
+      VB_LAUNCH_BS(c, k_bi_spmv_p<MODE>, ns, kSpmvBlock, A, w, rb, sc);
+      VB_LAUNCH(c, k_bi_s, nb, n, w, rb, sc, ns);
       VB_LAUNCH(c, k_bi_sinv, nb, n, w, rb, sc);
-      VB_LAUNCH_BS(c, k_bi_spmv_s<MODE>, nb, kSpmvBlock, A, w, rb, sc);
-      VB_LAUNCH(c, k_bi_xr, nb, n, w, rb, sc);
-      if (MODE == SPMV_MIXED && (launched + b + 1) % kRefresh == 0)
-        VB_LAUNCH_BS(c, k_bi_true_residual, nb, kSpmvBlock, A, w, rb, sc);
+      VB_LAUNCH_BS(c, k_bi_spmv_s<MODE>, ns, kSpmvBlock, A, w, rb, sc);
+      VB_LAUNCH(c, k_bi_xr, nb, n, w, rb, sc, ns);
+      if (MODE == SPMV_MIXED && (launched + b + 1) % kRefresh == 0) {
+        VB_LAUNCH(c, k_bi_true_residual, nb, A, w, rb, sc);
+        VB_LAUNCH(c, k_bi_true_shift, nb, n, w, rb, sc);
+      }
       VB_FIN(c, k_fin_xr, rb, nb, sc);
     }
     launched += batch;
@@ -575,7 +614,7 @@ __global__ void k_bi_set_live(Scal* sc, int* counts_out, const int* cnt, const u
 }
 
 vb_status profile_spmv(vb_ctx* c, int reps, double* avg_us, double* alg_bytes) {
-  const int n = c->npfb, nb = grid_for(n);
+  const int n = c->npfb, nb = spmv_grid(n);
   Mat A = step_matrix(c);
   if (c->spmv_stored) A.rowflag = nullptr;
   Vecs w = step_vecs(c, c->st.p);

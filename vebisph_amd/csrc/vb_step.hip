@@ -30,11 +30,13 @@ __global__ void k_umax(State st, int nbS, int npfb, double cfl_h, double dt_visc
   }
 }
 
-// max is order-independent, so one thread folds; also resets the step's counters
+// max is order-independent: one wavefront folds the partials (strided loads, all in flight at
+// once, then a shuffle max); also resets the step's counters
 __global__ void k_fin_umax(const double* partials, int nblk, double cfl_h, double dt_visc, Scal* sc) {
-  if (threadIdx.x != 0) return;
   double um = 1.0e-10;
-  for (int b = 0; b < nblk; b++) um = partials[b] > um ? partials[b] : um;
+  for (int b = threadIdx.x; b < nblk; b += 64) um = partials[b] > um ? partials[b] : um;
+  um = wave_max(um);
+  if (threadIdx.x != 0) return;
   sc->umax = um;
   double dt_cfl = cfl_h / um;
   sc->dt = dt_cfl < dt_visc ? dt_cfl : dt_visc;
@@ -398,12 +400,54 @@ __global__ void k_cell_count(Geom g, State st, int np, int* cellkey, int* cellsl
   cellslot[k] = slot;
 }
 
-__global__ void k_cell_scan(const int* nc, int* ist, int nct) {
-  // ist[c] = start of cell c (0-based cells), ist[nct] = total
-  for (int c = threadIdx.x; c < nct; c += 1024) ist[c] = nc[c];
+// Inclusive scan of v over a 1024-thread block (wavefront shuffles, then the 16 wave totals).
+__device__ inline int block_inclusive_scan_1024(int v, int* total) {
+  __shared__ int wsum[16];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    int t = __shfl_up(v, o, 64);
+    if (lane >= o) v += t;
+  }
+  if (lane == 63) wsum[wid] = v;
   __syncthreads();
-  int total = block_scan_1024(ist, nct);
-  if (threadIdx.x == 0) ist[nct] = total;
+  int off = 0, tot = 0;
+  for (int w = 0; w < 16; w++) {
+    if (w < wid) off += wsum[w];
+    tot += wsum[w];
+  }
+  __syncthreads();
+  *total = tot;
+  return v + off;
+}
+
+// Cell starts ist[c] = sum of nc[0..c-1], ist[nct] = total: a three-pass scan over 1024-cell
+// blocks (block sums, one-block scan of the sums, block offsets added).
+__global__ void k_cell_scan_blocks(const int* nc, int* ist, int* bsum, int nct) {
+  int c = blockIdx.x * 1024 + threadIdx.x;
+  int v = c < nct ? nc[c] : 0, tot;
+  int inc = block_inclusive_scan_1024(v, &tot);
+  if (c < nct) ist[c] = inc - v;                   // exclusive within the block
+  if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+
+__global__ void k_cell_scan_top(int* bsum, int nblk, int* ist, int nct) {
+  // nblk <= 1024 * 1024 / 1024: one thread per block sum, in passes of 1024
+  int carry = 0;
+  for (int b0 = 0; b0 < nblk; b0 += 1024) {
+    int b = b0 + threadIdx.x;
+    int v = b < nblk ? bsum[b] : 0, tot;
+    int inc = block_inclusive_scan_1024(v, &tot);
+    if (b < nblk) bsum[b] = carry + inc - v;
+    carry += tot;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) ist[nct] = carry;
+}
+
+__global__ void k_cell_scan_add(int* ist, const int* bsum, int nct) {
+  int c = blockIdx.x * 1024 + threadIdx.x;
+  if (c < nct) ist[c] += bsum[blockIdx.x];
 }
 
 __global__ void k_cell_scatter(int np, const int* cellkey, const int* cellslot, const int* ist, int* ip) {
@@ -849,7 +893,12 @@ vb_status launch_neighbours(vb_ctx* c) {
   int np = c->np_host;
   VB_LAUNCH(c, k_fill_int, grid_1d(g.nct), c->nc, g.nct, 0);
   VB_LAUNCH(c, k_cell_count, grid_1d(np), c->g, c->st, np, c->cellkey, c->cellslot, c->nc, c->scal);
-  hipLaunchKernelGGL(k_cell_scan, dim3(1), dim3(1024), 0, c->stream, c->nc, c->ist, g.nct);
+  const int nsb = (g.nct + 1023) / 1024;
+  hipLaunchKernelGGL(k_cell_scan_blocks, dim3(nsb), dim3(1024), 0, c->stream, c->nc, c->ist, c->cell_bsum, g.nct);
+  VB_HIP(c, hipGetLastError());
+  hipLaunchKernelGGL(k_cell_scan_top, dim3(1), dim3(1024), 0, c->stream, c->cell_bsum, nsb, c->ist, g.nct);
+  VB_HIP(c, hipGetLastError());
+  hipLaunchKernelGGL(k_cell_scan_add, dim3(nsb), dim3(1024), 0, c->stream, c->ist, c->cell_bsum, g.nct);
   VB_HIP(c, hipGetLastError());
   VB_LAUNCH(c, k_cell_scatter, grid_1d(np), np, c->cellkey, c->cellslot, c->ist, c->ip);
   VB_LAUNCH(c, k_cell_sort, grid_1d(g.nct), c->ist, c->ip, g.nct);
