@@ -40,7 +40,7 @@ struct vb_ctx {
   int16_t* lnk4;                     // compact rows, grouped by 4 slots: column offsets
   double* val4;                      //   FP64 values
   float* val4f;                      //   FP32 values (mixed path)
-  bool spmv_stored;                  // full stored rows everywhere (VB_SPMV=stored)
+  bool spmv_stored;                  // FP64 path: full stored rows everywhere (VB_SPMV=stored)
   double *r, *rb, *vv, *pn, *pinv, *ss, *sinv, *tt, *xsys;
 
   double* partials;                  // reduction partials, 2 * kPartialStride per solver stage

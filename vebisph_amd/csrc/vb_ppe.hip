@@ -166,7 +166,7 @@ struct Mat {
   // compact rows, grouped by 4 slots: element (k, i) at ((k/4)*n + i)*4 + k%4
   const short4* lnk4;              // column offsets (column = row + offset)
   const double2* val4;             // FP64 values, two double2 planes per group (grouped2)
-  const float4* val4f;             // FP32 values (mixed path)
+  const float4* val4f;             // FP32 values (mixed path), one float4 per group (grouped)
   int n;
 };
 
@@ -189,13 +189,10 @@ __device__ inline double row_full(const Mat& A, int i, int m, double xi, const d
   return bb;
 }
 
-// One SpMV row y_i = A(i) x_i + sum_k a_k x_col(k).  Rows with mirror neighbours use the full stored
-// row; the others (~98%) load four slots per instruction from the grouped compact arrays: a 2-byte
-// column offset and the FP64 value (same operations in the same order: bit-identical), or in the
-// mixed path the FP32 value applied to FP32 differences, y_i = sum_k a32_k (x_j - x_i) + rowsum_i x_i,
-// so that smooth vectors do not cancel catastrophically in FP32.
-template <int MODE>
-__device__ inline double spmv_row(const Mat& A, int i, const double* __restrict__ x) {
+// One FP64 SpMV row y_i = A(i) x_i + sum_k a_k x_col(k).  Rows with mirror neighbours use the full
+// stored row; the others (~98%) load four slots per instruction from the grouped compact arrays: a
+// 2-byte column offset and the FP64 value (same operations in the same order: bit-identical).
+__device__ inline double spmv_row_fp64(const Mat& A, int i, const double* __restrict__ x) {
   const int m = A.len[i];
   const double xi = x[i];
   if (!A.rowflag || A.rowflag[i]) return row_full(A, i, m, xi, x);
@@ -210,7 +207,7 @@ __device__ inline double spmv_row(const Mat& A, int i, const double* __restrict_
   double g0[4], g1[4] = {0.0, 0.0, 0.0, 0.0};
   const short4 o0 = A.lnk4[i];
   if (ng > 1) o1 = A.lnk4[n + i];
-  if (MODE == SPMV_FP64) {
+  {
     // FP64 values: double2 plane 2g holds slots 4g, 4g+1; plane 2g+1 slots 4g+2, 4g+3
     const double2 z2 = make_double2(0.0, 0.0);
     double2 a0 = A.val4[i], b0 = A.val4[n + i], a1 = z2, b1 = z2, a2 = z2, b2 = z2;
@@ -232,27 +229,58 @@ __device__ inline double spmv_row(const Mat& A, int i, const double* __restrict_
       g0[0] = g1[0]; g0[1] = g1[1]; g0[2] = g1[2]; g0[3] = g1[3];
     }
     return bb;
-  } else {
-    const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
-    float4 a0 = A.val4f[i], a1 = z4, a2 = z4;             // padded slots hold 0
-    if (ng > 1) a1 = A.val4f[n + i];
-    g0[0] = x[i + o0.x]; g0[1] = x[i + o0.y]; g0[2] = x[i + o0.z]; g0[3] = x[i + o0.w];
-    float acc = 0.0f;
-    for (int gi = 0; gi < ng; gi++) {
-      if (gi + 2 < ng) {
-        const size_t g2 = (size_t)(gi + 2);
-        o2 = A.lnk4[g2 * n + i]; a2 = A.val4f[g2 * n + i];
-      }
-      if (gi + 1 < ng) { g1[0] = x[i + o1.x]; g1[1] = x[i + o1.y]; g1[2] = x[i + o1.z]; g1[3] = x[i + o1.w]; }
-      acc = fmaf(a0.x, (float)(g0[0] - xi), acc);
-      acc = fmaf(a0.y, (float)(g0[1] - xi), acc);
-      acc = fmaf(a0.z, (float)(g0[2] - xi), acc);
-      acc = fmaf(a0.w, (float)(g0[3] - xi), acc);
-      a0 = a1; a1 = a2; o1 = o2;
-      g0[0] = g1[0]; g0[1] = g1[1]; g0[2] = g1[2]; g0[3] = g1[3];
-    }
-    return (double)acc + A.rowsum[i] * xi;
   }
+}
+
+// ------------------------------------------------------------------------------------------------
+// Mixed path SpMV rows.  Compact rows: the FP32 values applied to FP32 differences of the FP64
+// vector, y_i = sum_k a32_k (x_j - x_i) + rowsum_i x_i with the FP64 row sum carrying the diagonal,
+// so the vector's DC part and smooth components do not cancel in FP32.  Same three-stage pipeline
+// as the FP64 row; padded slots hold offset 0 and value 0.  Rows with mirror neighbours use the
+// full FP64 row.  (Measured alternatives: recomputing the FP32 entries from the pair distance,
+// three FP64 gathers per slot, was 1.5x slower than reading 4 bytes per slot; folding the p and s
+// projections into the SpMV, which then folds the partial sums in every one of its workgroups,
+// made the SpMV 1.5x slower than the separate projection kernels.)
+// ------------------------------------------------------------------------------------------------
+__device__ inline double row_f32(const Mat& A, int i, int m, const double* __restrict__ v, double xhat_i) {
+  const size_t n = (size_t)A.n;
+  const int ng = (m + 3) >> 2;
+  const double vi = v[i];
+  const short4 zo = make_short4(0, 0, 0, 0);
+  const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+  const short4 o0 = A.lnk4[i];
+  short4 o1 = zo, o2 = zo;
+  float4 a0 = A.val4f[i], a1 = z4, a2 = z4;
+  if (ng > 1) { o1 = A.lnk4[n + i]; a1 = A.val4f[n + i]; }
+  double g0[4], g1[4] = {0.0, 0.0, 0.0, 0.0};
+  g0[0] = v[i + o0.x]; g0[1] = v[i + o0.y]; g0[2] = v[i + o0.z]; g0[3] = v[i + o0.w];
+  float acc = 0.0f;
+  for (int gi = 0; gi < ng; gi++) {
+    if (gi + 2 < ng) {
+      const size_t g2 = (size_t)(gi + 2);
+      o2 = A.lnk4[g2 * n + i]; a2 = A.val4f[g2 * n + i];
+    }
+    if (gi + 1 < ng) { g1[0] = v[i + o1.x]; g1[1] = v[i + o1.y]; g1[2] = v[i + o1.z]; g1[3] = v[i + o1.w]; }
+    acc = fmaf(a0.x, (float)(g0[0] - vi), acc);
+    acc = fmaf(a0.y, (float)(g0[1] - vi), acc);
+    acc = fmaf(a0.z, (float)(g0[2] - vi), acc);
+    acc = fmaf(a0.w, (float)(g0[3] - vi), acc);
+    a0 = a1; a1 = a2; o1 = o2;
+    g0[0] = g1[0]; g0[1] = g1[1]; g0[2] = g1[2]; g0[3] = g1[3];
+  }
+  return (double)acc + A.rowsum[i] * xhat_i;
+}
+
+__device__ inline double spmv_row_f32(const Mat& A, int i, const double* __restrict__ x) {
+  const int m = A.len[i];
+  const double xi = x[i];
+  if (A.rowflag[i]) return row_full(A, i, m, xi, x);
+  return row_f32(A, i, m, x, xi);
+}
+
+template <int MODE>
+__device__ inline double spmv_row(const Mat& A, int i, const double* __restrict__ x) {
+  return MODE == SPMV_MIXED ? spmv_row_f32(A, i, x) : spmv_row_fp64(A, i, x);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -289,7 +317,7 @@ __global__ void k_bi_init(Mat A, Vecs w, int x_given, RedBufs rb) {
   Red<1> R(rb.R(ST_INIT), n);
   for (int i = rr.begin + threadIdx.x; i < rr.end; i += blockDim.x) {
     double r;
-    if (x_given) r = w.b[i] - spmv_row<SPMV_FP64>(A, i, w.x);
+    if (x_given) r = w.b[i] - spmv_row_fp64(A, i, w.x);
     else { w.x[i] = 0.0; r = w.b[i]; }
     w.r[i] = r; w.rb[i] = r;
     w.v[i] = 0.0; w.pn[i] = 0.0;
@@ -442,7 +470,7 @@ __global__ void k_bi_true_residual(Mat A, Vecs w, RedBufs rb, const Scal* sc) {
   RowRange rr = block_rows(n, gridDim.x);
   Red<2> R(rb.R(ST_TRUE), n);
   for (int i = rr.begin + threadIdx.x; i < rr.end; i += blockDim.x) {
-    const double rt = w.b[i] - spmv_row<SPMV_FP64>(A, i, w.x);
+    const double rt = w.b[i] - spmv_row_fp64(A, i, w.x);
     R.add(0, i, w.r[i]);
     R.add(1, i, rt);
     w.t[i] = rt;
@@ -552,9 +580,9 @@ vb_status launch_ppe(vb_ctx* c, double tol, int itermax) {
             c->a_diag, c->a_col, c->a_val, c->lnk4, c->val4,
             c->prm.precision == VB_PREC_MIXED ? c->val4f : nullptr, c->rowflag, c->a_rowsum, c->scal);
   Mat A = step_matrix(c);
-  if (c->spmv_stored) A.rowflag = nullptr;       // all rows from the stored matrix
   Vecs w = step_vecs(c, c->st.p);
   if (c->prm.precision == VB_PREC_MIXED) return solve<SPMV_MIXED>(c, A, w, 0, tol, itermax);
+  if (c->spmv_stored) A.rowflag = nullptr;       // FP64: all rows from the stored matrix
   return solve<SPMV_FP64>(c, A, w, 0, tol, itermax);
 }
 
@@ -615,8 +643,9 @@ __global__ void k_bi_set_live(Scal* sc, int* counts_out, const int* cnt, const u
 
 vb_status profile_spmv(vb_ctx* c, int reps, double* avg_us, double* alg_bytes) {
   const int n = c->npfb, nb = spmv_grid(n);
+  const bool mixed = c->prm.precision == VB_PREC_MIXED;
   Mat A = step_matrix(c);
-  if (c->spmv_stored) A.rowflag = nullptr;
+  if (c->spmv_stored && !mixed) A.rowflag = nullptr;
   Vecs w = step_vecs(c, c->st.p);
   int* d_counts = c->seg_counts;
   hipLaunchKernelGGL(k_bi_set_live, dim3(1), dim3(kBlock), 0, c->stream, c->scal, d_counts, c->cnt,
@@ -625,7 +654,6 @@ vb_status profile_spmv(vb_ctx* c, int reps, double* avg_us, double* alg_bytes) {
   hipEvent_t e0, e1;
   VB_HIP(c, hipEventCreate(&e0));
   VB_HIP(c, hipEventCreate(&e1));
-  const bool mixed = c->prm.precision == VB_PREC_MIXED;
   for (int r = 0; r <= reps; r++) {
     if (r == 1) VB_HIP(c, hipEventRecord(e0, c->stream));
     RedBufs rb{c->partials, nullptr, n};
@@ -641,12 +669,12 @@ vb_status profile_spmv(vb_ctx* c, int reps, double* avg_us, double* alg_bytes) {
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   *avg_us = 1000.0 * ms / reps;
-  // Algorithmic bytes of one launch.  Per row: length 4, row flag 1, diagonal (or row sum) 8, the
-  // gathered vector 8 (each element once), r_bar 8, v written 8.  Per slot: 2-byte column offset +
-  // 8-byte value (FP64) or 4-byte value (mixed) on compact rows; 4 + 8 on full rows.
+  // Algorithmic bytes of one launch.  FP64: per row length 4, row flag 1, diagonal 8, the gathered
+  // vector 8 (each element once), r_bar 8, v written 8; per slot the 2-byte column offset and the
+  // 8-byte value on compact rows, 4 + 8 on full rows.  Mixed: the same per row (row sum for the
+  // diagonal) and 2 + 4 bytes per compact slot.
   const double nnz = counts[0], nnz_full = A.rowflag ? counts[1] : counts[0];
-  const double compact_slot = mixed ? 6.0 : 10.0;
-  *alg_bytes = 37.0 * n + compact_slot * (nnz - nnz_full) + 12.0 * nnz_full;
+  *alg_bytes = 37.0 * n + (mixed ? 6.0 : 10.0) * (nnz - nnz_full) + 12.0 * nnz_full;
   return VB_OK;
 }
 
