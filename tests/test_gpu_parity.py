@@ -141,13 +141,17 @@ def test_bicgstab_parallel_order_same_solution(gpu_lib):
     assert _rel(xg - xg.mean(), xo - xo.mean()) <= 1e-3
 
 
-@pytest.mark.parametrize("name", ["poiseuille", "convection", "cylinder"])
-def test_mixed_precision_fields(gpu_lib, name):
+@pytest.mark.parametrize("name,spmv", [("poiseuille", "compact"), ("convection", "compact"),
+                                       ("cylinder", "compact"), ("cylinder", "tiled")])
+def test_mixed_precision_fields(gpu_lib, name, spmv, monkeypatch):
     """FP32 matvecs inside the FP64 BiCGSTAB: ten steps against the FP64 oracle, held to the same
     reference-sensitivity scale as the FP64 path, with the FP64 residual of the final solve checked
-    on the assembled system."""
+    on the assembled system.  spmv: stored FP32 rows (default) or the cell-tile SpMV (VB_SPMV=tiled)."""
     from vebisph_amd import _lib
     case = SMALL[name]()
+    print(f"case {case.name} spmv {spmv}", flush=True)
+    if spmv == "tiled":
+        monkeypatch.setenv("VB_SPMV", "tiled")
     O = ow.OracleSolver(case); O2 = ow.OracleSolver(case, state=_perturbed_state(case))
     G = solver.Solver(case, precision=_lib.PREC_MIXED)
     for _ in range(10):

@@ -63,7 +63,9 @@ static void free_all(vb_ctx* c) {
   void* ptrs[] = {c->st.x, c->st.y, c->st.u, c->st.v, c->st.th, c->st.t11, c->st.t12, c->st.t21,
                   c->st.t22, c->st.trc, c->st.p, c->st.inbin, c->st.x0, c->st.y0, c->st.u0, c->st.v0,
                   c->st.irel, c->st.vrel, c->st.dP, c->u_alt, c->v_alt, c->th_alt, c->nsurf,
-                  c->seg_counts, c->cellkey, c->cellslot, c->nc, c->ist, c->ip, c->cell_bsum, c->cnt, c->link,
+                  c->seg_counts, c->cellkey, c->cellslot, c->nc, c->ist, c->ip, c->scan_bsum, c->cnt, c->link,
+                  c->perm, c->rank, c->pflag, c->pscan, c->ist_real, c->posP, c->idx16, c->lenP,
+                  c->rowflagP, c->rowsumP, c->bperm, c->xperm, c->atab,
                   c->wg1, c->wg2, c->wl, c->k11, c->k12, c->k21, c->k22,
                   c->usx, c->usy, c->kick, c->gpx, c->gpy, c->a_diag, c->a_val, c->a_rowsum, c->rhs,
                   c->a_col, c->a_len, c->rowflag, c->lnk4, c->val4, c->val4f, c->r, c->rb, c->vv,
@@ -177,7 +179,7 @@ vb_status vb_create(const vb_params* prm, const vb_boundary* bnd, int32_t npfb, 
   c->seg_blocks = grid_1d(std::max<long>((long)cap, (long)npfb * std::max(1, c->g.ncirc)));
   A(c->seg_counts, c->seg_blocks);
   A(c->cellkey, cap); A(c->cellslot, cap); A(c->nc, c->g.nct + 1); A(c->ist, c->g.nct + 1); A(c->ip, cap);
-  A(c->cell_bsum, (c->g.nct + 1023) / 1024 + 1);
+  A(c->scan_bsum, (std::max<size_t>(cap, (size_t)c->g.nct) + 1023) / 1024 + 1);
   A(c->cnt, n); A(c->link, nw); A(c->wg1, nw); A(c->wg2, nw); A(c->wl, nw);
   A(c->k11, n); A(c->k12, n); A(c->k21, n); A(c->k22, n);
   A(c->usx, n); A(c->usy, n); A(c->kick, n); A(c->gpx, n); A(c->gpy, n);
@@ -185,7 +187,17 @@ vb_status vb_create(const vb_params* prm, const vb_boundary* bnd, int32_t npfb, 
   // compact grouped rows: up to VB_NPLINK slots rounded up to a multiple of 4
   const size_t nw4 = (size_t)npfb * ((kNplink + 3) & ~3);
   A(c->rowflag, n); A(c->lnk4, nw4); A(c->val4, nw4);
+  // VB_SPMV=stored: every FP64 SpMV row from the stored matrix (measurement switch; same results).
+  // VB_SPMV=tiled: the mixed path's SpMV on LDS-staged cell tiles with recomputed FP32 entries.
+  const char* sp = getenv("VB_SPMV");
+  c->spmv_stored = sp && strcmp(sp, "stored") == 0;
+  c->spmv_tiled = sp && strcmp(sp, "tiled") == 0 && prm->precision == VB_PREC_MIXED;
   if (prm->precision == VB_PREC_MIXED) A(c->val4f, nw4);
+  if (c->spmv_tiled) {
+    A(c->perm, n); A(c->rank, n); A(c->pflag, cap + 1); A(c->pscan, cap + 1); A(c->ist_real, c->g.nct + 1);
+    A(c->posP, 2 * n); A(c->idx16, nw4); A(c->lenP, n); A(c->rowflagP, n); A(c->rowsumP, n);
+    A(c->bperm, n); A(c->xperm, n); A(c->atab, 4 * kTabN);
+  }
   A(c->r, n); A(c->rb, n); A(c->vv, n); A(c->pn, n); A(c->pinv, n); A(c->ss, n); A(c->sinv, n);
   A(c->tt, n); A(c->xsys, n);
   A(c->partials, (size_t)kSolverStages * 2 * kPartialStride); A(c->scal, 1);
@@ -210,9 +222,16 @@ vb_status vb_create(const vb_params* prm, const vb_boundary* bnd, int32_t npfb, 
   double a = sqrt(prm->Ra / prm->Pr), b = sqrt(prm->Ra * prm->Pr);
   c->dt_visc = prm->dt_coef_visc * prm->h * prm->h * (a < b ? a : b);
   c->np_host = npfb; c->nmirror_host = 0;
-  // VB_SPMV=stored: every SpMV row from the stored matrix (measurement switch; same results)
-  const char* sp = getenv("VB_SPMV");
-  c->spmv_stored = sp && strcmp(sp, "stored") == 0;
+  // mixed-path SpMV tiles: 16 cells along x (about 240 rows, a 3 x 18-cell window), coarser when
+  // the tile count would exceed the partial-sum buffers
+  c->tile_k = kTileK;
+  for (;;) {
+    c->ntx = (c->g.ncx + c->tile_k - 1) / c->tile_k;
+    c->ntiles = c->ntx * c->g.ncy;
+    if (c->ntiles + 1 <= kSpmvMaxBlocks) break;
+    c->tile_k *= 2;
+  }
+  if (s == VB_OK && c->spmv_tiled) s = upload_entry_table(c);
   if (s != VB_OK) {
     free_all(c);
     delete c;

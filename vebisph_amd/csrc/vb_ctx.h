@@ -25,7 +25,7 @@ struct vb_ctx {
 
   // cells and neighbours
   int *cellkey, *cellslot, *nc, *ist, *ip;
-  int* cell_bsum;                    // block sums of the cell-start scan (one per 1024 cells)
+  int* scan_bsum;                    // block sums of launch_exclusive_scan (one per 1024 elements)
   int *cnt, *link;                   // link[k*npfb + i], 0-based
   double *wg1, *wg2, *wl;            // slot-major like link
   double *k11, *k12, *k21, *k22;
@@ -39,8 +39,20 @@ struct vb_ctx {
   uint8_t* rowflag;                  // 1: SpMV uses the full stored row (mirror neighbours)
   int16_t* lnk4;                     // compact rows, grouped by 4 slots: column offsets
   double* val4;                      //   FP64 values
-  float* val4f;                      //   FP32 values (mixed path)
+  float* val4f;                      //   FP32 values (mixed path, stored-value SpMV)
+
+  // mixed path: the solve runs on the real particles in cell order (rows k = rank of particle
+  // perm[k]) with the SpMV on tiles of cells staged in LDS
+  int *perm, *rank, *pflag, *pscan, *ist_real;
+  double* posP;                      // (x, y) of perm[k], interleaved
+  uint16_t* idx16;                   // slot-major [s*npfb + k]: neighbour's index in its tile window
+  uint8_t *lenP, *rowflagP;          // neighbour count, 1 = full stored row
+  double *rowsumP, *bperm, *xperm;
+  int tile_k, ntx, ntiles;           // cells per tile along x, tiles per cell row, tiles
+  int wlds;                          // staged window rows the SpMV is launched with
+  float* atab;                       // entry table of the tiled SpMV (4 floats per interval)
   bool spmv_stored;                  // FP64 path: full stored rows everywhere (VB_SPMV=stored)
+  bool spmv_tiled;                   // mixed path: SpMV on cell tiles (VB_SPMV=tiled)
   double *r, *rb, *vv, *pn, *pinv, *ss, *sinv, *tt, *xsys;
 
   double* partials;                  // reduction partials, 2 * kPartialStride per solver stage
@@ -65,9 +77,12 @@ vb_status launch_pressure_update(vb_ctx* c);
 vb_status launch_final(vb_ctx* c);
 // PPE (vb_ppe.hip)
 vb_status launch_ppe(vb_ctx* c, double tol, int itermax);
+vb_status upload_entry_table(vb_ctx* c);
 vb_status profile_spmv(vb_ctx* c, int reps, double* avg_us, double* alg_bytes);
 vb_status solve_nr(vb_ctx* c, const double* A_nr, const int* ija, double* x, const double* B, int n,
                    int itermax, double tol);
+// out[0..n-1] = exclusive prefix sums of in[0..n-1], out[n] = total (vb_step.hip)
+vb_status launch_exclusive_scan(vb_ctx* c, const int* in, int* out, int n);
 // helpers (vb_capi.hip)
 vb_status hip_check(vb_ctx* c, hipError_t e, const char* what);
 vb_status sync_scalars(vb_ctx* c);

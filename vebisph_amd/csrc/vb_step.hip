@@ -421,8 +421,9 @@ __device__ inline int block_inclusive_scan_1024(int v, int* total) {
   return v + off;
 }
 
-// Cell starts ist[c] = sum of nc[0..c-1], ist[nct] = total: a three-pass scan over 1024-cell
-// blocks (block sums, one-block scan of the sums, block offsets added).
+// Exclusive scan out[c] = sum of in[0..c-1], out[n] = total (the cell starts ist from the counts
+// nc, and the cell-order ranks of the mixed solve): three passes over 1024-element blocks (block
+// sums, one-block scan of the sums, block offsets added).
 __global__ void k_cell_scan_blocks(const int* nc, int* ist, int* bsum, int nct) {
   int c = blockIdx.x * 1024 + threadIdx.x;
   int v = c < nct ? nc[c] : 0, tot;
@@ -432,7 +433,7 @@ __global__ void k_cell_scan_blocks(const int* nc, int* ist, int* bsum, int nct) 
 }
 
 __global__ void k_cell_scan_top(int* bsum, int nblk, int* ist, int nct) {
-  // nblk <= 1024 * 1024 / 1024: one thread per block sum, in passes of 1024
+  // one thread per block sum, in passes of 1024 block sums
   int carry = 0;
   for (int b0 = 0; b0 < nblk; b0 += 1024) {
     int b = b0 + threadIdx.x;
@@ -888,18 +889,25 @@ vb_status launch_mirrors(vb_ctx* c) {
   return VB_OK;
 }
 
+vb_status launch_exclusive_scan(vb_ctx* c, const int* in, int* out, int n) {
+  const int nsb = (n + 1023) / 1024;
+  if (nsb < 1) return VB_OK;
+  hipLaunchKernelGGL(k_cell_scan_blocks, dim3(nsb), dim3(1024), 0, c->stream, in, out, c->scan_bsum, n);
+  VB_HIP(c, hipGetLastError());
+  hipLaunchKernelGGL(k_cell_scan_top, dim3(1), dim3(1024), 0, c->stream, c->scan_bsum, nsb, out, n);
+  VB_HIP(c, hipGetLastError());
+  hipLaunchKernelGGL(k_cell_scan_add, dim3(nsb), dim3(1024), 0, c->stream, out, c->scan_bsum, n);
+  VB_HIP(c, hipGetLastError());
+  return VB_OK;
+}
+
 vb_status launch_neighbours(vb_ctx* c) {
   const Geom& g = c->g;
   int np = c->np_host;
   VB_LAUNCH(c, k_fill_int, grid_1d(g.nct), c->nc, g.nct, 0);
   VB_LAUNCH(c, k_cell_count, grid_1d(np), c->g, c->st, np, c->cellkey, c->cellslot, c->nc, c->scal);
-  const int nsb = (g.nct + 1023) / 1024;
-  hipLaunchKernelGGL(k_cell_scan_blocks, dim3(nsb), dim3(1024), 0, c->stream, c->nc, c->ist, c->cell_bsum, g.nct);
-  VB_HIP(c, hipGetLastError());
-  hipLaunchKernelGGL(k_cell_scan_top, dim3(1), dim3(1024), 0, c->stream, c->cell_bsum, nsb, c->ist, g.nct);
-  VB_HIP(c, hipGetLastError());
-  hipLaunchKernelGGL(k_cell_scan_add, dim3(nsb), dim3(1024), 0, c->stream, c->ist, c->cell_bsum, g.nct);
-  VB_HIP(c, hipGetLastError());
+  vb_status s = launch_exclusive_scan(c, c->nc, c->ist, g.nct);
+  if (s != VB_OK) return s;
   VB_LAUNCH(c, k_cell_scatter, grid_1d(np), np, c->cellkey, c->cellslot, c->ist, c->ip);
   VB_LAUNCH(c, k_cell_sort, grid_1d(g.nct), c->ist, c->ip, g.nct);
   VB_LAUNCH(c, k_neighbours, grid_1d(c->npfb), c->g, c->st, c->cellkey, c->ist, c->ip, c->cnt, c->link,
