@@ -189,6 +189,25 @@ __device__ inline double row_full(const Mat& A, int i, int m, double xi, const d
   return bb;
 }
 
+// The compact matrix arrays are read once per SpMV: non-temporal loads keep them from displacing the
+// gathered vector in L2 (measured on MI355X: a copy kernel with non-temporal accesses reaches 6.2
+// TB/s, a plain one 4.6).
+typedef short nt_s4 __attribute__((ext_vector_type(4)));
+typedef double nt_d2 __attribute__((ext_vector_type(2)));
+typedef float nt_f4 __attribute__((ext_vector_type(4)));
+__device__ inline short4 ld_nt(const short4* p) {
+  const nt_s4 v = __builtin_nontemporal_load(reinterpret_cast<const nt_s4*>(p));
+  return make_short4(v.x, v.y, v.z, v.w);
+}
+__device__ inline double2 ld_nt(const double2* p) {
+  const nt_d2 v = __builtin_nontemporal_load(reinterpret_cast<const nt_d2*>(p));
+  return make_double2(v.x, v.y);
+}
+__device__ inline float4 ld_nt(const float4* p) {
+  const nt_f4 v = __builtin_nontemporal_load(reinterpret_cast<const nt_f4*>(p));
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+
 // One FP64 SpMV row y_i = A(i) x_i + sum_k a_k x_col(k).  Rows with mirror neighbours use the full
 // stored row; the others (~98%) load four slots per instruction from the grouped compact arrays: a
 // 2-byte column offset and the FP64 value (same operations in the same order: bit-identical).
@@ -205,19 +224,21 @@ __device__ inline double spmv_row_fp64(const Mat& A, int i, const double* __rest
   const short4 zo = make_short4(0, 0, 0, 0);
   short4 o1 = zo, o2 = zo;
   double g0[4], g1[4] = {0.0, 0.0, 0.0, 0.0};
-  const short4 o0 = A.lnk4[i];
-  if (ng > 1) o1 = A.lnk4[n + i];
+  const short4 o0 = ld_nt(A.lnk4 + i);
+  if (ng > 1) o1 = ld_nt(A.lnk4 + n + i);
   {
     // FP64 values: double2 plane 2g holds slots 4g, 4g+1; plane 2g+1 slots 4g+2, 4g+3
     const double2 z2 = make_double2(0.0, 0.0);
-    double2 a0 = A.val4[i], b0 = A.val4[n + i], a1 = z2, b1 = z2, a2 = z2, b2 = z2;
-    if (ng > 1) { a1 = A.val4[2 * n + i]; b1 = A.val4[3 * n + i]; }
+    double2 a0 = ld_nt(A.val4 + i), b0 = ld_nt(A.val4 + n + i), a1 = z2, b1 = z2, a2 = z2, b2 = z2;
+    if (ng > 1) { a1 = ld_nt(A.val4 + 2 * n + i); b1 = ld_nt(A.val4 + 3 * n + i); }
     g0[0] = x[i + o0.x]; g0[1] = x[i + o0.y]; g0[2] = x[i + o0.z]; g0[3] = x[i + o0.w];
     double bb = A.diag[i] * xi;
     for (int gi = 0; gi < ng; gi++) {
       if (gi + 2 < ng) {
         const size_t g2 = (size_t)(gi + 2);
-        o2 = A.lnk4[g2 * n + i]; a2 = A.val4[2 * g2 * n + i]; b2 = A.val4[(2 * g2 + 1) * n + i];
+        o2 = ld_nt(A.lnk4 + g2 * n + i);
+        a2 = ld_nt(A.val4 + 2 * g2 * n + i);
+        b2 = ld_nt(A.val4 + (2 * g2 + 1) * n + i);
       }
       if (gi + 1 < ng) { g1[0] = x[i + o1.x]; g1[1] = x[i + o1.y]; g1[2] = x[i + o1.z]; g1[3] = x[i + o1.w]; }
       const int k = 4 * gi;
@@ -248,17 +269,17 @@ __device__ inline double row_f32(const Mat& A, int i, int m, const double* __res
   const double vi = v[i];
   const short4 zo = make_short4(0, 0, 0, 0);
   const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
-  const short4 o0 = A.lnk4[i];
+  const short4 o0 = ld_nt(A.lnk4 + i);
   short4 o1 = zo, o2 = zo;
-  float4 a0 = A.val4f[i], a1 = z4, a2 = z4;
-  if (ng > 1) { o1 = A.lnk4[n + i]; a1 = A.val4f[n + i]; }
+  float4 a0 = ld_nt(A.val4f + i), a1 = z4, a2 = z4;
+  if (ng > 1) { o1 = ld_nt(A.lnk4 + n + i); a1 = ld_nt(A.val4f + n + i); }
   double g0[4], g1[4] = {0.0, 0.0, 0.0, 0.0};
   g0[0] = v[i + o0.x]; g0[1] = v[i + o0.y]; g0[2] = v[i + o0.z]; g0[3] = v[i + o0.w];
   float acc = 0.0f;
   for (int gi = 0; gi < ng; gi++) {
     if (gi + 2 < ng) {
       const size_t g2 = (size_t)(gi + 2);
-      o2 = A.lnk4[g2 * n + i]; a2 = A.val4f[g2 * n + i];
+      o2 = ld_nt(A.lnk4 + g2 * n + i); a2 = ld_nt(A.val4f + g2 * n + i);
     }
     if (gi + 1 < ng) { g1[0] = v[i + o1.x]; g1[1] = v[i + o1.y]; g1[2] = v[i + o1.z]; g1[3] = v[i + o1.w]; }
     acc = fmaf(a0.x, (float)(g0[0] - vi), acc);
