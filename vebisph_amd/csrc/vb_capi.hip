@@ -68,7 +68,8 @@ static void free_all(vb_ctx* c) {
                   c->rowflagP, c->rowsumP, c->bperm, c->xperm, c->atab,
                   c->wg1, c->wg2, c->wl, c->k11, c->k12, c->k21, c->k22,
                   c->usx, c->usy, c->kick, c->gpx, c->gpy, c->a_diag, c->a_val, c->a_rowsum, c->rhs,
-                  c->a_col, c->a_len, c->rowflag, c->lnk4, c->val4, c->val4f, c->r, c->rb, c->vv,
+                  c->a_col, c->a_len, c->rowflag, c->chunkg, c->choff, c->lnk4, c->val4, c->val4f,
+                  c->r, c->rb, c->vv,
                   c->pn, c->pinv, c->ss, c->sinv, c->tt,
                   c->xsys, c->partials, c->red, c->scal};
   for (void* p : ptrs)
@@ -185,7 +186,10 @@ vb_status vb_create(const vb_params* prm, const vb_boundary* bnd, int32_t npfb, 
   A(c->usx, n); A(c->usy, n); A(c->kick, n); A(c->gpx, n); A(c->gpy, n);
   A(c->a_diag, n); A(c->a_val, nw); A(c->a_rowsum, n); A(c->rhs, n); A(c->a_col, nw); A(c->a_len, n);
   // compact grouped rows: up to VB_NPLINK slots rounded up to a multiple of 4
-  const size_t nw4 = (size_t)npfb * ((kNplink + 3) & ~3);
+  // compact rows: at most VB_NPLINK slots rounded up to groups of 4, for every row of every chunk
+  const size_t nch = ((size_t)npfb + 63) / 64;
+  const size_t nw4 = nch * 64 * ((kNplink + 3) & ~3);
+  A(c->chunkg, nch + 1); A(c->choff, nch + 1);
   A(c->rowflag, n); A(c->lnk4, nw4); A(c->val4, nw4);
   // VB_SPMV=stored: every FP64 SpMV row from the stored matrix (measurement switch; same results).
   // VB_SPMV=tiled: the mixed path's SpMV on LDS-staged cell tiles with recomputed FP32 entries.

@@ -37,7 +37,8 @@ struct vb_ctx {
   double *a_diag, *a_val, *a_rowsum, *rhs;
   int *a_col, *a_len;                // a_len: row lengths of a caller-supplied system
   uint8_t* rowflag;                  // 1: SpMV uses the full stored row (mirror neighbours)
-  int16_t* lnk4;                     // compact rows, grouped by 4 slots: column offsets
+  int *chunkg, *choff;               // compact rows: groups per 64-row chunk, first group slot
+  int16_t* lnk4;                     // compact rows, sliced groups of 4 slots: column offsets
   double* val4;                      //   FP64 values
   float* val4f;                      //   FP32 values (mixed path, stored-value SpMV)
 

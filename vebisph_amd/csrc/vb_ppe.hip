@@ -66,23 +66,28 @@ __global__ void k_fin_ppe_rhs(const double* partials, int nblk, const double* re
 }
 
 __host__ __device__ inline size_t grouped(int k, int i, int n) { return ((size_t)(k >> 2) * n + i) * 4 + (k & 3); }
-// FP64 compact values: slots 4g,4g+1 of row i in double2 plane 2g, slots 4g+2,4g+3 in plane 2g+1,
-// so that both loads of a group are contiguous across the wavefront's rows.
-__host__ __device__ inline size_t grouped2(int k, int i, int n) {
-  return (((size_t)(2 * (k >> 2) + ((k & 3) >> 1)) * n + i) << 1) + (k & 1);
-}
 
-// Assembly, :60-122.  One thread per row.  Also marks the rows the compact SpMV can take
-// (rowflag 0: no mirror neighbour, every neighbour within 32767 rows) and stores them in the grouped
-// compact arrays (relative column offsets, FP64 values, FP32 values for the mixed path), and the
-// FP64 row sum.
+// Compact rows are stored sliced: the 64 rows of a wavefront (a chunk) keep each group of four slots
+// contiguous, chunk after chunk, so that a wavefront streams one contiguous region instead of one
+// region per group.  Chunk c holds G_c groups (the most any of its compact rows needs), starting at
+// group slot P = choff[c]; group g of row i (lane = i % 64) is group slot (P + g) * 64 + lane:
+//   column offsets   short4 lnk4[(P + g) * 64 + lane]
+//   FP64 values      double2 val4[(2 (P + g)) * 64 + lane] (slots 4g, 4g+1),
+//                    double2 val4[(2 (P + g) + 1) * 64 + lane] (slots 4g+2, 4g+3)
+//   FP32 values      float4 val4f[(P + g) * 64 + lane]
+constexpr int kChunk = 64;
+
+// Assembly, :60-122.  One thread per row (a wavefront = one chunk of 64 rows).  Writes the full
+// stored rows, the diagonal, row sum and right-hand side, marks the rows the compact SpMV can take
+// (rowflag 0: no mirror neighbour, every neighbour within 32767 rows) and records per chunk the
+// groups its compact rows need (chunkg); k_assemble_sliced then writes the compact rows.
 __global__ void k_assemble(Geom g, State st, const int* cnt, const int* link, const double* wl,
                            const int* nsurf, double* rhs, double* a_diag, int* a_col, double* a_val,
-                           int16_t* lnk4, double* val4, float* val4f, uint8_t* rowflag, double* rowsum,
-                           const Scal* sc) {
-  int i = blockIdx.x * kBlock + threadIdx.x;
+                           uint8_t* rowflag, double* rowsum, int* chunkg, const Scal* sc) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
   const int npfb = g.npfb;
-  if (i >= npfb) return;
+  int ngroups = 0;
+  if (i < npfb) {
   const int n = cnt[i];
   double bi = rhs[i] - sc->mean;
   double adiag = 0.0;
@@ -106,24 +111,14 @@ __global__ void k_assemble(Geom g, State st, const int* cnt, const int* link, co
   double d = 0.0 + adiag;
   double rs = d;
   if (!explicit_row) {
-    const int padded = (n + 3) & ~3;
-    for (int k = 0; k < padded; k++) {
-      size_t e = grouped(k, i, npfb), e2 = grouped2(k, i, npfb);
-      if (k < n) {
-        int s = k * npfb + i;
-        double val = 0.0 + (-wl[s]);
-        a_col[s] = link[s];
-        a_val[s] = val;
-        lnk4[e] = (int16_t)(link[s] - i);
-        val4[e2] = val;
-        if (val4f) val4f[e] = (float)val;
-        rs = rs + val;
-      } else {                           // padding: offset 0 (the row itself), value 0
-        lnk4[e] = 0;
-        val4[e2] = 0.0;
-        if (val4f) val4f[e] = 0.0f;
-      }
+    for (int k = 0; k < n; k++) {
+      int s = k * npfb + i;
+      double val = 0.0 + (-wl[s]);
+      a_col[s] = link[s];
+      a_val[s] = val;
+      rs = rs + val;
     }
+    ngroups = (n + 3) >> 2;
   } else {
     for (int k = 0; k < n; k++) {
       int s = k * npfb + i;
@@ -154,6 +149,38 @@ __global__ void k_assemble(Geom g, State st, const int* cnt, const int* link, co
   rowflag[i] = explicit_row ? 1 : 0;
   rowsum[i] = rs;
   rhs[i] = bi;
+  }
+  // groups of the chunk: the wavefront's maximum
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) { const int t = __shfl_down(ngroups, o, 64); ngroups = t > ngroups ? t : ngroups; }
+  if ((threadIdx.x & 63) == 0 && i < npfb) chunkg[i / kChunk] = ngroups;
+}
+
+// The compact rows in the sliced layout (choff = exclusive scan of chunkg); padding slots of a row's
+// last group hold offset 0 (the row itself) and value 0.
+__global__ void k_assemble_sliced(int npfb, const int* cnt, const int* link, const double* a_val,
+                                  const uint8_t* rowflag, const int* choff, int16_t* lnk4, double* val4,
+                                  float* val4f) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= npfb || rowflag[i]) return;
+  const int n = cnt[i], padded = (n + 3) & ~3;
+  const size_t P = (size_t)choff[i / kChunk], lane = i % kChunk;
+  for (int k = 0; k < padded; k++) {
+    const size_t gs = (P + (k >> 2)) * kChunk + lane;                 // group slot
+    const size_t e = gs * 4 + (k & 3);
+    const size_t e2 = ((2 * (P + (k >> 2)) + ((k & 3) >> 1)) * kChunk + lane) * 2 + (k & 1);
+    if (k < n) {
+      const int s = k * npfb + i;
+      const double val = a_val[s];
+      lnk4[e] = (int16_t)(link[s] - i);
+      val4[e2] = val;
+      if (val4f) val4f[e] = (float)val;
+    } else {
+      lnk4[e] = 0;
+      val4[e2] = 0.0;
+      if (val4f) val4f[e] = 0.0f;
+    }
+  }
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -163,10 +190,11 @@ struct Mat {
   const double *diag, *val, *rowsum;
   const int *col, *len;
   const uint8_t* rowflag;          // null: every row uses the full stored row
-  // compact rows, grouped by 4 slots: element (k, i) at ((k/4)*n + i)*4 + k%4
+  // compact rows, sliced by chunks of 64 rows (layout at kChunk)
+  const int* choff;                // first group slot of each chunk
   const short4* lnk4;              // column offsets (column = row + offset)
-  const double2* val4;             // FP64 values, two double2 planes per group (grouped2)
-  const float4* val4f;             // FP32 values (mixed path), one float4 per group (grouped)
+  const double2* val4;             // FP64 values, two double2 planes per group
+  const float4* val4f;             // FP32 values (mixed path), one float4 per group
   int n;
 };
 
@@ -215,8 +243,11 @@ __device__ inline double spmv_row_fp64(const Mat& A, int i, const double* __rest
   const int m = A.len[i];
   const double xi = x[i];
   if (!A.rowflag || A.rowflag[i]) return row_full(A, i, m, xi, x);
-  const size_t n = (size_t)A.n;
   const int ng = (m + 3) >> 2;
+  // group g of this row: group slot (P + g) * 64 + lane (sliced layout, see kChunk)
+  const size_t P = (size_t)A.choff[i / kChunk], lane = (size_t)(i % kChunk);
+  auto gs = [&](size_t g) { return (P + g) * kChunk + lane; };
+  auto vs = [&](size_t g, size_t h) { return (2 * (P + g) + h) * kChunk + lane; };
   // Three-stage software pipeline over the row's groups g (a row is a dependent chain of ~11
   // groups: offsets -> gathers -> accumulate): while group g is accumulated, the gathers of g+1 and
   // the offset/value loads of g+2 are in flight, so each step overlaps one HBM round trip and one
@@ -224,21 +255,21 @@ __device__ inline double spmv_row_fp64(const Mat& A, int i, const double* __rest
   const short4 zo = make_short4(0, 0, 0, 0);
   short4 o1 = zo, o2 = zo;
   double g0[4], g1[4] = {0.0, 0.0, 0.0, 0.0};
-  const short4 o0 = ld_nt(A.lnk4 + i);
-  if (ng > 1) o1 = ld_nt(A.lnk4 + n + i);
+  const short4 o0 = ld_nt(A.lnk4 + gs(0));
+  if (ng > 1) o1 = ld_nt(A.lnk4 + gs(1));
   {
     // FP64 values: double2 plane 2g holds slots 4g, 4g+1; plane 2g+1 slots 4g+2, 4g+3
     const double2 z2 = make_double2(0.0, 0.0);
-    double2 a0 = ld_nt(A.val4 + i), b0 = ld_nt(A.val4 + n + i), a1 = z2, b1 = z2, a2 = z2, b2 = z2;
-    if (ng > 1) { a1 = ld_nt(A.val4 + 2 * n + i); b1 = ld_nt(A.val4 + 3 * n + i); }
+    double2 a0 = ld_nt(A.val4 + vs(0, 0)), b0 = ld_nt(A.val4 + vs(0, 1)), a1 = z2, b1 = z2, a2 = z2, b2 = z2;
+    if (ng > 1) { a1 = ld_nt(A.val4 + vs(1, 0)); b1 = ld_nt(A.val4 + vs(1, 1)); }
     g0[0] = x[i + o0.x]; g0[1] = x[i + o0.y]; g0[2] = x[i + o0.z]; g0[3] = x[i + o0.w];
     double bb = A.diag[i] * xi;
     for (int gi = 0; gi < ng; gi++) {
       if (gi + 2 < ng) {
         const size_t g2 = (size_t)(gi + 2);
-        o2 = ld_nt(A.lnk4 + g2 * n + i);
-        a2 = ld_nt(A.val4 + 2 * g2 * n + i);
-        b2 = ld_nt(A.val4 + (2 * g2 + 1) * n + i);
+        o2 = ld_nt(A.lnk4 + gs(g2));
+        a2 = ld_nt(A.val4 + vs(g2, 0));
+        b2 = ld_nt(A.val4 + vs(g2, 1));
       }
       if (gi + 1 < ng) { g1[0] = x[i + o1.x]; g1[1] = x[i + o1.y]; g1[2] = x[i + o1.z]; g1[3] = x[i + o1.w]; }
       const int k = 4 * gi;
@@ -264,22 +295,23 @@ __device__ inline double spmv_row_fp64(const Mat& A, int i, const double* __rest
 // made the SpMV 1.5x slower than the separate projection kernels.)
 // ------------------------------------------------------------------------------------------------
 __device__ inline double row_f32(const Mat& A, int i, int m, const double* __restrict__ v, double xhat_i) {
-  const size_t n = (size_t)A.n;
   const int ng = (m + 3) >> 2;
+  const size_t P = (size_t)A.choff[i / kChunk], lane = (size_t)(i % kChunk);
+  auto gs = [&](size_t g) { return (P + g) * kChunk + lane; };
   const double vi = v[i];
   const short4 zo = make_short4(0, 0, 0, 0);
   const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
-  const short4 o0 = ld_nt(A.lnk4 + i);
+  const short4 o0 = ld_nt(A.lnk4 + gs(0));
   short4 o1 = zo, o2 = zo;
-  float4 a0 = ld_nt(A.val4f + i), a1 = z4, a2 = z4;
-  if (ng > 1) { o1 = ld_nt(A.lnk4 + n + i); a1 = ld_nt(A.val4f + n + i); }
+  float4 a0 = ld_nt(A.val4f + gs(0)), a1 = z4, a2 = z4;
+  if (ng > 1) { o1 = ld_nt(A.lnk4 + gs(1)); a1 = ld_nt(A.val4f + gs(1)); }
   double g0[4], g1[4] = {0.0, 0.0, 0.0, 0.0};
   g0[0] = v[i + o0.x]; g0[1] = v[i + o0.y]; g0[2] = v[i + o0.z]; g0[3] = v[i + o0.w];
   float acc = 0.0f;
   for (int gi = 0; gi < ng; gi++) {
     if (gi + 2 < ng) {
       const size_t g2 = (size_t)(gi + 2);
-      o2 = ld_nt(A.lnk4 + g2 * n + i); a2 = ld_nt(A.val4f + g2 * n + i);
+      o2 = ld_nt(A.lnk4 + gs(g2)); a2 = ld_nt(A.val4f + gs(g2));
     }
     if (gi + 1 < ng) { g1[0] = v[i + o1.x]; g1[1] = v[i + o1.y]; g1[2] = v[i + o1.z]; g1[3] = v[i + o1.w]; }
     acc = fmaf(a0.x, (float)(g0[0] - vi), acc);
@@ -331,6 +363,11 @@ struct RedBufs {
 
 __device__ inline bool finished(const Scal* sc) { return sc->done || sc->exit_s; }
 
+// Rows a vector kernel loads before it folds the previous kernel's partials (1M rows on 1024 x 256
+// threads: 4 per thread), so that the fold's latency overlaps the vector loads instead of preceding
+// them; rows beyond that are loaded after the fold.
+constexpr int kPre = 4;
+
 // :306-321  AX = A x (x = 0 for the PPE, so r = B); r_bar = r; the terms of the first rho (:330)
 __global__ void k_bi_init(Mat A, Vecs w, int x_given, RedBufs rb) {
   const int n = A.n;
@@ -379,10 +416,16 @@ __global__ void k_bi_p(int n, Vecs w, RedBufs rb, const Scal* sc) {
 // :351-352 remove_dc(p), inverseM: p_inv = (pn - mean)/(1+1e-15)
 __global__ void k_bi_pinv(int n, Vecs w, RedBufs rb, Scal* sc) {
   if (finished(sc)) return;
+  RowRange rr = block_rows(n, gridDim.x);
+  const int i0 = rr.begin + threadIdx.x, bd = blockDim.x;
+  double pv[kPre];
+#pragma unroll
+  for (int u = 0; u < kPre; u++) if (i0 + u * bd < rr.end) pv[u] = w.pn[i0 + u * bd];
   const double mean = fold(rb.P(ST_P), 0, gridDim.x, rb.R(ST_P), n) / (double)n;   // :243
   if (blockIdx.x == 0 && threadIdx.x == 0) sc->mean_p = mean;
-  RowRange rr = block_rows(n, gridDim.x);
-  for (int i = rr.begin + threadIdx.x; i < rr.end; i += blockDim.x) w.pinv[i] = (w.pn[i] - mean) / kOnePlusEps;
+#pragma unroll
+  for (int u = 0; u < kPre; u++) if (i0 + u * bd < rr.end) w.pinv[i0 + u * bd] = (pv[u] - mean) / kOnePlusEps;
+  for (int i = i0 + kPre * bd; i < rr.end; i += bd) w.pinv[i] = (w.pn[i] - mean) / kOnePlusEps;
 }
 
 // :353-354  v = A p_inv; the terms of r_bar . v
@@ -404,39 +447,53 @@ __global__ void k_bi_spmv_p(Mat A, Vecs w, RedBufs rb, const Scal* sc) {
 // (nspmv: workgroups of the SpMV launch whose partials are folded)
 __global__ void k_bi_s(int n, Vecs w, RedBufs rb, Scal* sc, int nspmv) {
   if (finished(sc)) return;
+  RowRange rr = block_rows(n, gridDim.x);
+  const int i0 = rr.begin + threadIdx.x, bd = blockDim.x;
+  double rv[kPre], vv[kPre];
+#pragma unroll
+  for (int u = 0; u < kPre; u++)
+    if (i0 + u * bd < rr.end) { rv[u] = w.r[i0 + u * bd]; vv[u] = w.v[i0 + u * bd]; }
   const double temp = fold(rb.P(ST_SPMV_P), 0, nspmv, rb.R(ST_SPMV_P), n);
   const double alpha = sc->rho / (temp + 1.0e-15);
   if (blockIdx.x == 0 && threadIdx.x == 0) sc->alpha = alpha;
-  RowRange rr = block_rows(n, gridDim.x);
   Red<2> R(rb.R(ST_S), n);
-  for (int i = rr.begin + threadIdx.x; i < rr.end; i += blockDim.x) {
-    double s = w.r[i] - alpha * w.v[i];
+  auto row = [&](int i, double r, double v) {
+    double s = r - alpha * v;
     w.s[i] = s;
     R.add(0, i, s * s);
     R.add(1, i, s);
-  }
+  };
+#pragma unroll
+  for (int u = 0; u < kPre; u++) if (i0 + u * bd < rr.end) row(i0 + u * bd, rv[u], vv[u]);
+  for (int i = i0 + kPre * bd; i < rr.end; i += bd) row(i, w.r[i], w.v[i]);
   R.store(rb.P(ST_S));
 }
 
 // :366-384  early exit on |s| (x += alpha p_inv), else remove_dc(s), inverseM: s_inv
 __global__ void k_bi_sinv(int n, Vecs w, RedBufs rb, Scal* sc) {
   if (finished(sc)) return;
+  RowRange rr = block_rows(n, gridDim.x);
+  const int i0 = rr.begin + threadIdx.x, bd = blockDim.x;
+  double sv[kPre];
+#pragma unroll
+  for (int u = 0; u < kPre; u++) if (i0 + u * bd < rr.end) sv[u] = w.s[i0 + u * bd];
   double f[2];
   fold_n<2>(rb.P(ST_S), gridDim.x, rb.R(ST_S), n, f);
   const double n2s = f[0], sum = f[1];
   const bool first = sc->iter == 1;
   const double n2s0 = first ? ((n2s <= 1.0e-6) ? 1.0 : n2s) : sc->n2s0;
   const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
-  RowRange rr = block_rows(n, gridDim.x);
   if (n2s <= sc->tol * sc->tol * n2s0) {
     const double alpha = sc->alpha;
-    for (int i = rr.begin + threadIdx.x; i < rr.end; i += blockDim.x) w.x[i] = w.x[i] + alpha * w.pinv[i];
+    for (int i = i0; i < rr.end; i += bd) w.x[i] = w.x[i] + alpha * w.pinv[i];
     if (lead) { sc->n2s0 = n2s0; sc->resid = sqrt(n2s / n2s0); sc->exit_s = 1; }
     return;
   }
   const double mean = sum / (double)n;
   if (lead) { sc->n2s0 = n2s0; sc->mean_s = mean; }
-  for (int i = rr.begin + threadIdx.x; i < rr.end; i += blockDim.x) w.sinv[i] = (w.s[i] - mean) / kOnePlusEps;
+#pragma unroll
+  for (int u = 0; u < kPre; u++) if (i0 + u * bd < rr.end) w.sinv[i0 + u * bd] = (sv[u] - mean) / kOnePlusEps;
+  for (int i = i0 + kPre * bd; i < rr.end; i += bd) w.sinv[i] = (w.s[i] - mean) / kOnePlusEps;
 }
 
 // :385-387  t = A s_inv; terms of t.s and t.t with the projected s
@@ -461,21 +518,32 @@ __global__ void k_bi_spmv_s(Mat A, Vecs w, RedBufs rb, const Scal* sc) {
 // r_bar . r
 __global__ void k_bi_xr(int n, Vecs w, RedBufs rb, Scal* sc, int nspmv) {
   if (finished(sc)) return;
+  RowRange rr = block_rows(n, gridDim.x);
+  const int i0 = rr.begin + threadIdx.x, bd = blockDim.x;
+  double xv[kPre], pv[kPre], qv[kPre], sv[kPre], tv[kPre], bv[kPre];
+#pragma unroll
+  for (int u = 0; u < kPre; u++) {
+    const int i = i0 + u * bd;
+    if (i < rr.end) { xv[u] = w.x[i]; pv[u] = w.pinv[i]; qv[u] = w.sinv[i]; sv[u] = w.s[i]; tv[u] = w.t[i]; bv[u] = w.rb[i]; }
+  }
   double f[2];
   fold_n<2>(rb.P(ST_SPMV_S), nspmv, rb.R(ST_SPMV_S), n, f);
   const double ts = f[0], tt = f[1];
   const double omega = ts / (tt + 1.0e-15);
   if (blockIdx.x == 0 && threadIdx.x == 0) sc->omega = omega;
   const double alpha = sc->alpha, ms = sc->mean_s;
-  RowRange rr = block_rows(n, gridDim.x);
   Red<2> R(rb.R(ST_XR), n);
-  for (int i = rr.begin + threadIdx.x; i < rr.end; i += blockDim.x) {
-    w.x[i] = w.x[i] + alpha * w.pinv[i] + omega * w.sinv[i];
-    double r = (w.s[i] - ms) - omega * w.t[i];
+  auto row = [&](int i, double x, double pinv, double sinv, double s, double t, double rbi) {
+    w.x[i] = x + alpha * pinv + omega * sinv;
+    double r = (s - ms) - omega * t;
     w.r[i] = r;
     R.add(0, i, r * r);
-    R.add(1, i, w.rb[i] * r);
-  }
+    R.add(1, i, rbi * r);
+  };
+#pragma unroll
+  for (int u = 0; u < kPre; u++)
+    if (i0 + u * bd < rr.end) row(i0 + u * bd, xv[u], pv[u], qv[u], sv[u], tv[u], bv[u]);
+  for (int i = i0 + kPre * bd; i < rr.end; i += bd) row(i, w.x[i], w.pinv[i], w.sinv[i], w.s[i], w.t[i], w.rb[i]);
   R.store(rb.P(ST_XR));
 }
 
@@ -808,7 +876,7 @@ static Mat step_matrix(vb_ctx* c) {
   Mat A;
   A.diag = c->a_diag; A.val = c->a_val; A.rowsum = c->a_rowsum; A.col = c->a_col; A.len = c->cnt;
   A.rowflag = c->rowflag; A.lnk4 = (const short4*)c->lnk4; A.val4 = (const double2*)c->val4;
-  A.val4f = (const float4*)c->val4f; A.n = c->npfb;
+  A.val4f = (const float4*)c->val4f; A.choff = c->choff; A.n = c->npfb;
   return A;
 }
 
@@ -961,8 +1029,12 @@ vb_status launch_ppe(vb_ctx* c, double tol, int itermax) {
             c->k22, c->rhs, c->partials, c->red, c->scal);
   VB_FIN(c, k_fin_ppe_rhs, c->partials, nb, c->red, c->npfb, c->scal);
   VB_LAUNCH(c, k_assemble, grid_1d(c->npfb), c->g, c->st, c->cnt, c->link, c->wl, c->nsurf, c->rhs,
-            c->a_diag, c->a_col, c->a_val, c->lnk4, c->val4,
-            c->prm.precision == VB_PREC_MIXED ? c->val4f : nullptr, c->rowflag, c->a_rowsum, c->scal);
+            c->a_diag, c->a_col, c->a_val, c->rowflag, c->a_rowsum, c->chunkg, c->scal);
+  const int nch = (c->npfb + kChunk - 1) / kChunk;
+  vb_status ss = launch_exclusive_scan(c, c->chunkg, c->choff, nch);
+  if (ss != VB_OK) return ss;
+  VB_LAUNCH(c, k_assemble_sliced, grid_1d(c->npfb), c->npfb, c->cnt, c->link, c->a_val, c->rowflag,
+            c->choff, c->lnk4, c->val4, c->prm.precision == VB_PREC_MIXED ? c->val4f : nullptr);
   Mat A = step_matrix(c);
   Vecs w = step_vecs(c, c->st.p);
   if (c->prm.precision == VB_PREC_MIXED) {
