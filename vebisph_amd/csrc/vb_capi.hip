@@ -68,7 +68,8 @@ static void free_all(vb_ctx* c) {
                   c->rowflagP, c->rowsumP, c->bperm, c->xperm, c->atab,
                   c->wg1, c->wg2, c->wl, c->k11, c->k12, c->k21, c->k22,
                   c->usx, c->usy, c->kick, c->gpx, c->gpy, c->a_diag, c->a_val, c->a_rowsum, c->rhs,
-                  c->a_col, c->a_len, c->rowflag, c->chunkg, c->choff, c->lnk4, c->val4, c->val4f,
+                  c->a_col, c->a_len, c->rowflag, c->chunkg, c->choff, c->xflag, c->xscan, c->xrows,
+                  c->lnk4, c->val4, c->val4f,
                   c->r, c->rb, c->vv,
                   c->pn, c->pinv, c->ss, c->sinv, c->tt,
                   c->xsys, c->partials, c->red, c->scal};
@@ -190,6 +191,7 @@ vb_status vb_create(const vb_params* prm, const vb_boundary* bnd, int32_t npfb, 
   const size_t nch = ((size_t)npfb + 63) / 64;
   const size_t nw4 = nch * 64 * ((kNplink + 3) & ~3);
   A(c->chunkg, nch + 1); A(c->choff, nch + 1);
+  A(c->xflag, n + 1); A(c->xscan, n + 1); A(c->xrows, n);
   A(c->rowflag, n); A(c->lnk4, nw4); A(c->val4, nw4);
   // VB_SPMV=stored: every FP64 SpMV row from the stored matrix (measurement switch; same results).
   // VB_SPMV=tiled: the mixed path's SpMV on LDS-staged cell tiles with recomputed FP32 entries.

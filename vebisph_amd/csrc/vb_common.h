@@ -81,6 +81,7 @@ struct Scal {
   int iter, itermax, done, exit_s, status;
   // mixed path: real particles binned in cells (the cell-ordered rows), largest tile window
   int nreal_cells, wmax;
+  int nexplicit;                           // PPE rows with mirror neighbours (full stored rows)
 };
 
 // Error bits accumulated on the device (vb_status codes as bit positions).
@@ -122,6 +123,28 @@ __host__ __device__ inline double wab(double qq, double ad7) {
     return ad7 * p5(q3);
   }
   return 0.0;
+}
+
+// Visits the neighbour slots k = 0..n-1 of row i in order: body(s, j, g) with s = k*npfb + i (the
+// slot-major index), j = link[s] and g = gather(j), the neighbour's operands.  Four slots at a time
+// load their indices, then their operands, before any body runs, so a row's chain of dependent loads
+// is a quarter as long; the bodies still run in slot order (same arithmetic, same bits).
+template <class Gather, class Body>
+__device__ inline void for_slots(const int* __restrict__ link, int i, int n, int npfb, Gather gather, Body body) {
+  int k = 0;
+  for (; k + 4 <= n; k += 4) {
+    const int s0 = k * npfb + i, s1 = s0 + npfb, s2 = s1 + npfb, s3 = s2 + npfb;
+    const int j0 = link[s0], j1 = link[s1], j2 = link[s2], j3 = link[s3];
+    const auto g0 = gather(j0);
+    const auto g1 = gather(j1);
+    const auto g2 = gather(j2);
+    const auto g3 = gather(j3);
+    body(s0, j0, g0); body(s1, j1, g1); body(s2, j2, g2); body(s3, j3, g3);
+  }
+  for (; k < n; k++) {
+    const int s = k * npfb + i, j = link[s];
+    body(s, j, gather(j));
+  }
 }
 
 // ------------------------------------------------------------------------------------------------

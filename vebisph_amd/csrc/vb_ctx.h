@@ -37,6 +37,7 @@ struct vb_ctx {
   double *a_diag, *a_val, *a_rowsum, *rhs;
   int *a_col, *a_len;                // a_len: row lengths of a caller-supplied system
   uint8_t* rowflag;                  // 1: SpMV uses the full stored row (mirror neighbours)
+  int *xflag, *xscan, *xrows;        // the rows with rowflag 1, listed for their assembly
   int *chunkg, *choff;               // compact rows: groups per 64-row chunk, first group slot
   int16_t* lnk4;                     // compact rows, sliced groups of 4 slots: column offsets
   double* val4;                      //   FP64 values

@@ -45,14 +45,13 @@ __global__ void k_ppe_rhs(Geom g, State st, const int* cnt, const int* link, con
     const double ui = st.u[i], vi = st.v[i];
     const double K11 = k11[i], K12 = k12[i], K21 = k21[i], K22 = k22[i];
     double fb = 0.0;
-    for (int k = 0; k < n; k++) {
-      int s = k * npfb + i;
-      int j = link[s];
+    for_slots(link, i, n, npfb, [&](int j) { return make_double2(st.u[j], st.v[j]); },
+              [&](int s, int, double2 uj) {
       double w1 = wg1[s], w2 = wg2[s];
-      double uij = ui - st.u[j], vij = vi - st.v[j];
+      double uij = ui - uj.x, vij = vi - uj.y;
       double kw1 = K11 * w1 + K12 * w2, kw2 = K21 * w1 + K22 * w2;
       fb = fb - (uij * kw1 + vij * kw2);
-    }
+    });
     double b = fb / dt;
     rhs[i] = b;
     R.add(0, i, b);
@@ -119,41 +118,75 @@ __global__ void k_assemble(Geom g, State st, const int* cnt, const int* link, co
       rs = rs + val;
     }
     ngroups = (n + 3) >> 2;
-  } else {
-    for (int k = 0; k < n; k++) {
-      int s = k * npfb + i;
-      int j = link[s];
-      int ck = j < npfb ? j : st.irel[j];
-      double val = 0.0;
-      if (ck != i) {
-        bool last = true;
-        for (int k2 = k + 1; k2 < n && last; k2++) {
-          int j2 = link[k2 * npfb + i];
-          if ((j2 < npfb ? j2 : st.irel[j2]) == ck) last = false;
-        }
-        if (last) {
-          double acc = 0.0;
-          for (int k2 = 0; k2 <= k; k2++) {
-            int j2 = link[k2 * npfb + i];
-            if ((j2 < npfb ? j2 : st.irel[j2]) == ck) acc = acc + (-wl[k2 * npfb + i]);
-          }
-          val = acc;
-        }
-      }
-      a_col[s] = ck;
-      a_val[s] = val;
-      rs = rs + val;
-    }
+    rowsum[i] = rs;
   }
+  // rows with mirror neighbours: k_assemble_explicit writes their slots and row sum
   a_diag[i] = d;
   rowflag[i] = explicit_row ? 1 : 0;
-  rowsum[i] = rs;
   rhs[i] = bi;
   }
   // groups of the chunk: the wavefront's maximum
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) { const int t = __shfl_down(ngroups, o, 64); ngroups = t > ngroups ? t : ngroups; }
   if ((threadIdx.x & 63) == 0 && i < npfb) chunkg[i / kChunk] = ngroups;
+}
+
+// Rows with mirror neighbours (:76-115), one workgroup per row (rows = the explicit rows listed by
+// k_list_explicit), one thread per slot.  Slot k's column is the neighbour or the mirror's parent;
+// a column's value goes to its last slot and is the sum of -w_L over its slots in slot order
+// (earlier slots of the column hold 0); a self-mirror's column is the row itself and its value 0
+// (it went into the diagonal).  The columns and -w_L sit in LDS, so each thread's scans read LDS.
+constexpr int kExplicitThreads = 128;
+static_assert(kExplicitThreads >= kNplink, "one thread per slot");
+__global__ void __launch_bounds__(kExplicitThreads)
+k_assemble_explicit(int npfb, const int* rows, int nrows, const int* cnt, const int* link,
+                    const double* wl, const int* irel, const double* a_diag, int* a_col, double* a_val,
+                    double* rowsum) {
+  if ((int)blockIdx.x >= nrows) return;
+  __shared__ int cols[kExplicitThreads];
+  __shared__ double mw[kExplicitThreads], vals[kExplicitThreads];
+  const int i = rows[blockIdx.x], n = cnt[i], k = threadIdx.x;
+  int ck = -1;
+  if (k < n) {
+    const int j = link[k * npfb + i];
+    ck = j < npfb ? j : irel[j];
+    cols[k] = ck;
+    mw[k] = -wl[k * npfb + i];
+  }
+  __syncthreads();
+  if (k < n) {
+    double val = 0.0;
+    if (ck != i) {
+      bool last = true;
+      for (int k2 = k + 1; k2 < n && last; k2++) if (cols[k2] == ck) last = false;
+      if (last) {
+        double acc = 0.0;
+        for (int k2 = 0; k2 <= k; k2++) if (cols[k2] == ck) acc = acc + mw[k2];
+        val = acc;
+      }
+    }
+    a_col[k * npfb + i] = ck;
+    a_val[k * npfb + i] = val;
+    vals[k] = val;
+  }
+  __syncthreads();
+  if (k == 0) {
+    double rs = a_diag[i];
+    for (int k2 = 0; k2 < n; k2++) rs = rs + vals[k2];
+    rowsum[i] = rs;
+  }
+}
+
+// The explicit rows in index order: rows[0..nexplicit) from the exclusive scan of rowflag.
+__global__ void k_list_explicit(int npfb, const uint8_t* rowflag, const int* excl, int* rows, Scal* sc) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < npfb && rowflag[i]) rows[excl[i]] = i;
+  if (i == 0) sc->nexplicit = excl[npfb];
+}
+
+__global__ void k_flag_to_int(int n, const uint8_t* f, int* out) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) out[i] = f[i];
 }
 
 // The compact rows in the sliced layout (choff = exclusive scan of chunkg); padding slots of a row's
@@ -1035,6 +1068,17 @@ vb_status launch_ppe(vb_ctx* c, double tol, int itermax) {
   if (ss != VB_OK) return ss;
   VB_LAUNCH(c, k_assemble_sliced, grid_1d(c->npfb), c->npfb, c->cnt, c->link, c->a_val, c->rowflag,
             c->choff, c->lnk4, c->val4, c->prm.precision == VB_PREC_MIXED ? c->val4f : nullptr);
+  // rows with mirror neighbours: listed, counted (one synchronisation), one workgroup each
+  VB_LAUNCH(c, k_flag_to_int, grid_1d(c->npfb), c->npfb, c->rowflag, c->xflag);
+  if ((ss = launch_exclusive_scan(c, c->xflag, c->xscan, c->npfb)) != VB_OK) return ss;
+  VB_LAUNCH(c, k_list_explicit, grid_1d(c->npfb), c->npfb, c->rowflag, c->xscan, c->xrows, c->scal);
+  if ((ss = sync_scalars(c)) != VB_OK) return ss;
+  const int nx = c->scal_host->nexplicit;
+  if (nx > 0) {
+    hipLaunchKernelGGL(k_assemble_explicit, dim3(nx), dim3(kExplicitThreads), 0, c->stream, c->npfb, c->xrows,
+                       nx, c->cnt, c->link, c->wl, c->st.irel, c->a_diag, c->a_col, c->a_val, c->a_rowsum);
+    VB_HIP(c, hipGetLastError());
+  }
   Mat A = step_matrix(c);
   Vecs w = step_vecs(c, c->st.p);
   if (c->prm.precision == VB_PREC_MIXED) {

@@ -528,10 +528,9 @@ __global__ void k_weights(Geom g, State st, const int* cnt, const int* link, dou
     mx = n;
     const double xi = st.x[i], yi = st.y[i], dv = g.dv;
     double a11 = 0.0, a12 = 0.0, a21 = 0.0, a22 = 0.0;
-    for (int k = 0; k < n; k++) {
-      int s = k * npfb + i;
-      int j = link[s];
-      double rx = xi - st.x[j], ry = yi - st.y[j];
+    for_slots(link, i, n, npfb, [&](int j) { return make_double2(st.x[j], st.y[j]); },
+              [&](int s, int, double2 pj) {
+      double rx = xi - pj.x, ry = yi - pj.y;
       double rr2 = rx * rx + ry * ry;
       double rad = sqrt(rr2);
       double qq = rad * g.oneoverh;
@@ -543,7 +542,7 @@ __global__ void k_weights(Geom g, State st, const int* cnt, const int* link, dou
       wg1[s] = w1; wg2[s] = w2; wl[s] = l;
       a11 = a11 - dv * gx * rx; a12 = a12 - dv * gy * rx;
       a21 = a21 - dv * gx * ry; a22 = a22 - dv * gy * ry;
-    }
+    });
     double t = 1.0 / ((a11 * a22 - a12 * a21) + 1.0e-15);
     k11[i] = t * a22; k12[i] = -t * a12; k21[i] = -t * a21; k22[i] = t * a11;
   }
@@ -569,17 +568,16 @@ __global__ void k_shift(Geom g, State st, const int* cnt, const int* link, const
   const int npfb = g.npfb, n = cnt[i];
   const double xi = st.x[i], yi = st.y[i], wab1 = sc->wab1, h = g.h;
   double sfx = 0.0, sfy = 0.0, stx = 0.0, sty = 0.0;
-  for (int k = 0; k < n; k++) {
-    int s = k * npfb + i;
-    int j = link[s];
-    double rx = xi - st.x[j], ry = yi - st.y[j];
+  for_slots(link, i, n, npfb, [&](int j) { return make_double2(st.x[j], st.y[j]); },
+            [&](int s, int, double2 pj) {
+    double rx = xi - pj.x, ry = yi - pj.y;
     double rad = sqrt(rx * rx + ry * ry);
     double qq = rad / h;
     double fab = 0.25 * pow(wab(qq, g.ad7) / wab1, 4.0);
     double w1 = wg1[s], w2 = wg2[s];
     sfx = sfx + w1; sfy = sfy + w2;
     stx = stx + w1 * fab; sty = sty + w2 * fab;
-  }
+  });
   double dt = sc->dt;
   if (i < g.nbS) { usx[i] = -st.u[i]; usy[i] = -st.v[i]; return; }
   double D = 0.25 * h * h;
@@ -603,14 +601,13 @@ __global__ void k_grad_logconf(Geom g, State st, const int* cnt, const int* link
   const int npfb = g.npfb, n = cnt[i];
   const double ui = st.u[i], vi = st.v[i];
   double gux = 0.0, guy = 0.0, gvx = 0.0, gvy = 0.0;
-  for (int k = 0; k < n; k++) {
-    int s = k * npfb + i;
-    int j = link[s];
+  for_slots(link, i, n, npfb, [&](int j) { return make_double2(st.u[j], st.v[j]); },
+            [&](int s, int, double2 uj) {
     double w1 = wg1[s], w2 = wg2[s];
-    double tu = st.u[j] - ui, tv = st.v[j] - vi;
+    double tu = uj.x - ui, tv = uj.y - vi;
     gux = gux + tu * w1; guy = guy + tu * w2;
     gvx = gvx + tv * w1; gvy = gvy + tv * w2;
-  }
+  });
   double K11 = k11[i], K12 = k12[i], K21 = k21[i], K22 = k22[i];
   double G[4] = {K11 * gux + K12 * guy, K21 * gux + K22 * guy, K11 * gvx + K12 * gvy, K21 * gvx + K22 * gvy};
   double tau[4] = {st.t11[i], st.t12[i], st.t21[i], st.t22[i]}, to[4], trc;
@@ -682,18 +679,19 @@ __global__ void k_rhs(Geom g, State st, const int* cnt, const int* link, const d
   const double T11i = st.t11[i], T12i = st.t12[i], T21i = st.t21[i], T22i = st.t22[i];
   const double K11 = k11[i], K12 = k12[i], K21 = k21[i], K22 = k22[i];
   double ax = 0.0, ay = 0.0, at = 0.0;
-  for (int k = 0; k < n; k++) {
-    int s = k * npfb + i;
-    int j = link[s];
+  struct Nb { double u, v, t11, t12, t21, t22, th; };
+  for_slots(link, i, n, npfb,
+            [&](int j) { return Nb{st.u[j], st.v[j], st.t11[j], st.t12[j], st.t21[j], st.t22[j], st.th[j]}; },
+            [&](int s, int, const Nb& q) {
     double w1 = wg1[s], w2 = wg2[s], l = wl[s];
-    double uij = ui - st.u[j], vij = vi - st.v[j];
-    double T11 = st.t11[j] - T11i, T12 = st.t12[j] - T12i, T21 = st.t21[j] - T21i, T22 = st.t22[j] - T22i;
+    double uij = ui - q.u, vij = vi - q.v;
+    double T11 = q.t11 - T11i, T12 = q.t12 - T12i, T21 = q.t21 - T21i, T22 = q.t22 - T22i;
     double kw1 = K11 * w1 + K12 * w2, kw2 = K21 * w1 + K22 * w2;
     double tw1 = T11 * kw1 + T12 * kw2, tw2 = T21 * kw1 + T22 * kw2;
     ax = ax + cp * tw1 + cs * uij * l;
     ay = ay + cp * tw2 + cs * vij * l;
-    at = at + ct * (thi - st.th[j]) * l;
-  }
+    at = at + ct * (thi - q.th) * l;
+  });
   ay = ay + thi;
   double dt = sc->dt;
   un[i] = ui + ax * dt;
@@ -778,11 +776,10 @@ __global__ void k_pressure_update(Geom g, State st, const int* cnt, const int* l
   const int npfb = g.npfb, n = cnt[i];
   const double pi = st.p[i];
   double gx = 0.0, gy = 0.0;
-  for (int k = 0; k < n; k++) {
-    int s = k * npfb + i;
-    double t = st.p[link[s]] - pi;
+  for_slots(link, i, n, npfb, [&](int j) { return st.p[j]; }, [&](int s, int, double pj) {
+    double t = pj - pi;
     gx = gx + t * wg1[s]; gy = gy + t * wg2[s];
-  }
+  });
   double px = k11[i] * gx + k12[i] * gy, py = k21[i] * gx + k22[i] * gy;
   gpx[i] = px; gpy[i] = py;
   if (i < g.nbS) return;
