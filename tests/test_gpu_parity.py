@@ -96,7 +96,10 @@ def test_step_phases_against_oracle(gpu_lib, name):
                 assert _rel(G.get_state()["p"], O.get_state()["p"]) <= 1e-3
             if k == 12:
                 a, b = G.get_state(), O.get_state()
-                assert _rel(a["x"], b["x"]) <= 1e-12 and _rel(a["y"], b["y"]) <= 1e-12
+                # positions carry the pressure difference allowed above (a different Krylov path
+                # on an ill-conditioned system) through the velocity update: measured up to 3e-12;
+                # held to the FP64 field bar of 1e-10
+                assert _rel(a["x"], b["x"]) <= 1e-10 and _rel(a["y"], b["y"]) <= 1e-10
                 assert _rel(a["u"], b["u"]) <= 1e-5 and _rel(a["v"], b["v"]) <= 1e-5
 
 
@@ -139,6 +142,41 @@ def test_bicgstab_parallel_order_same_solution(gpu_lib):
     xg, itg, rg, stg = G.sparse_pbicgstab_nonull(a, ija, b)
     assert stg == 0 and rg <= 1e-8
     assert _rel(xg - xg.mean(), xo - xo.mean()) <= 1e-3
+
+
+def test_bicgstab_parallel_iterates_follow_reference(gpu_lib):
+    """The parallel-order solve (five kernels per iteration: the convergence test folded into the
+    next search-direction kernel, the remove_dc means taken from sums already folded) stopped after
+    k iterations lands where the reference algorithm does after k iterations, to rounding.  The
+    counts cross the host's polling batches (8, 8, 32, ...), where the finishing step moves
+    between kernels, and the same holds with a nonzero initial guess."""
+    case = SMALL["poiseuille"]()
+    O = ow.OracleSolver(case)
+    O.step(); O.step()
+    for k in range(1, 11):
+        O.phase(k)
+    a, ija, b = O.ppe()
+    G = solver.Solver(case)
+    x0 = np.linspace(-1e-6, 1e-6, case.npfb)
+    b_ulp = np.nextafter(b, np.inf)
+    for itermax in (1, 2, 7, 8, 9, 17, 33):
+        for guess in (None, x0):
+            xo, ito, ro, sto = ow.bicgstab_nonull(a, ija, b, x0=guess, itermax=itermax)
+            xg, itg, rg, stg = G.sparse_pbicgstab_nonull(a, ija, b, x0=guess, itermax=itermax)
+            # the reference algorithm's own sensitivity at this iteration: its iterate when every
+            # component of B moves by one ulp (rounding differences grow along the Krylov path)
+            xs, _, rs, _ = ow.bicgstab_nonull(a, ija, b_ulp, x0=guess, itermax=itermax)
+            sx = _rel(xs, xo)
+            sr = abs(rs - ro) / ro
+            print(f"itermax {itermax} guess {guess is not None}: iters {itg}/{ito} status {stg}/{sto} "
+                  f"resid {rg:.6e}/{ro:.6e} dx {_rel(xg, xo):.2e} (one-ulp spread {sx:.2e}), "
+                  f"dresid {abs(rg - ro) / ro:.2e} (spread {sr:.2e})", flush=True)
+            assert itg == ito
+            # the oracle returns 0 at the limit (the reference just leaves the loop); the C ABI
+            # reports VB_WARN_MAXITER (6)
+            assert stg == (6 if itg > itermax else sto)
+            assert _rel(xg, xo) <= max(100.0 * sx, 1e-13)
+            assert abs(rg - ro) / ro <= max(100.0 * sr, 1e-13)
 
 
 @pytest.mark.parametrize("name,spmv", [("poiseuille", "compact"), ("convection", "compact"),

@@ -79,6 +79,9 @@ struct Scal {
   // BiCGSTAB
   double rho, rho0, alpha, omega, beta, n2s0, n2r0, resid, mean_p, mean_s, tol;
   int iter, itermax, done, exit_s, status;
+  // BiCGSTAB in five kernels per iteration: rho of iteration k in rho_pp[k & 1]; sums of r, v and
+  // of the projected p, each recorded by the kernel that folds it
+  double rho_pp[2], sum_r, sum_v, sum_p;
   // mixed path: real particles binned in cells (the cell-ordered rows), largest tile window
   int nreal_cells, wmax;
   int nexplicit;                           // PPE rows with mirror neighbours (full stored rows)
@@ -240,12 +243,29 @@ __device__ inline void fold_n(const double* partials, int nblk, const double* ro
     return;
   }
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  // Loads go out kFoldBatch per quantity at a time and are summed afterwards in b order (a
+  // missing b adds an exact 0.0), so a thread waits on one round trip per batch, not per partial.
+  constexpr int kFoldBatch = 8;
+  double s[NV];
+#pragma unroll
+  for (int q = 0; q < NV; q++) s[q] = 0.0;
+  for (int b0 = threadIdx.x; b0 < nblk; b0 += kFoldBatch * blockDim.x) {
+    double v[NV][kFoldBatch];
+#pragma unroll
+    for (int u = 0; u < kFoldBatch; u++) {
+      const int b = b0 + u * blockDim.x;
+#pragma unroll
+      for (int q = 0; q < NV; q++) v[q][u] = b < nblk ? partials[q * nblk + b] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < kFoldBatch; u++)
+#pragma unroll
+      for (int q = 0; q < NV; q++) s[q] += v[q][u];
+  }
 #pragma unroll
   for (int q = 0; q < NV; q++) {
-    double s = 0.0;
-    for (int b = threadIdx.x; b < nblk; b += blockDim.x) s += partials[q * nblk + b];
-    s = wave_sum(s);
-    if (lane == 0) sh[q][wid] = s;
+    const double t = wave_sum(s[q]);
+    if (lane == 0) sh[q][wid] = t;
   }
   __syncthreads();
 #pragma unroll

@@ -6,8 +6,10 @@
 // mirrors folded onto their parents, zero slots included (duplicate columns accumulate into the
 // last slot, self-mirrors into the diagonal).  The ~98% of rows with no mirror neighbour (and no
 // neighbour further than 32767 rows away) are also stored compact, grouped by four slots so that a
-// thread loads four slots per instruction: 2-byte column offsets and FP64 values (10 bytes per slot,
-// same values in the same order: bit-identical) or FP32 values in the mixed path (6 bytes per slot).
+// thread loads four slots per instruction: 2-byte column offsets and FP64 values (10 bytes per slot)
+// or FP32 values in the mixed path (6 bytes per slot), in increasing column order (the gathers of a
+// wavefront then touch fewer cache lines; the row sum differs from the reference's slot order by
+// rounding).  Serial mode, whose solve reproduces the reference bit for bit, uses the full rows.
 // (Recomputing w_L from the positions instead of storing it was measured: compute-bound on the FP64
 // divisions and square root, ~1.8x slower.)
 //
@@ -190,30 +192,38 @@ __global__ void k_flag_to_int(int n, const uint8_t* f, int* out) {
 }
 
 // The compact rows in the sliced layout (choff = exclusive scan of chunkg); padding slots of a row's
-// last group hold offset 0 (the row itself) and value 0.
+// last group hold offset 0 (the row itself) and value 0.  The slots are placed in increasing column
+// order (slot k goes to its rank among the row's column offsets, kept in LDS): then slot k of 64
+// consecutive rows gathers from nearly consecutive columns, 5.3 instead of 9.1 distinct 128-byte
+// lines per wavefront gather on the 1M-particle cylinder.  The row sum is then taken in column
+// order, not in the reference's slot order; the full stored rows keep the reference's order.
 __global__ void k_assemble_sliced(int npfb, const int* cnt, const int* link, const double* a_val,
                                   const uint8_t* rowflag, const int* choff, int16_t* lnk4, double* val4,
                                   float* val4f) {
-  const int i = blockIdx.x * kBlock + threadIdx.x;
+  __shared__ int16_t off[kNplink][kBlock];     // each thread uses only its own column
+  const int t = threadIdx.x, i = blockIdx.x * kBlock + t;
   if (i >= npfb || rowflag[i]) return;
   const int n = cnt[i], padded = (n + 3) & ~3;
   const size_t P = (size_t)choff[i / kChunk], lane = i % kChunk;
-  for (int k = 0; k < padded; k++) {
+  for (int k = 0; k < n; k++) off[k][t] = (int16_t)(link[k * npfb + i] - i);
+  auto put = [&](int k, int16_t o, double val) {
     const size_t gs = (P + (k >> 2)) * kChunk + lane;                 // group slot
     const size_t e = gs * 4 + (k & 3);
     const size_t e2 = ((2 * (P + (k >> 2)) + ((k & 3) >> 1)) * kChunk + lane) * 2 + (k & 1);
-    if (k < n) {
-      const int s = k * npfb + i;
-      const double val = a_val[s];
-      lnk4[e] = (int16_t)(link[s] - i);
-      val4[e2] = val;
-      if (val4f) val4f[e] = (float)val;
-    } else {
-      lnk4[e] = 0;
-      val4[e2] = 0.0;
-      if (val4f) val4f[e] = 0.0f;
+    lnk4[e] = o;
+    val4[e2] = val;
+    if (val4f) val4f[e] = (float)val;
+  };
+  for (int k = 0; k < n; k++) {
+    const int16_t ok = off[k][t];
+    int pos = 0;
+    for (int j = 0; j < n; j++) {
+      const int16_t oj = off[j][t];
+      pos += (oj < ok) || (oj == ok && j < k);
     }
+    put(pos, ok, a_val[k * npfb + i]);
   }
+  for (int k = n; k < padded; k++) put(k, 0, 0.0);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -271,7 +281,7 @@ __device__ inline float4 ld_nt(const float4* p) {
 
 // One FP64 SpMV row y_i = A(i) x_i + sum_k a_k x_col(k).  Rows with mirror neighbours use the full
 // stored row; the others (~98%) load four slots per instruction from the grouped compact arrays: a
-// 2-byte column offset and the FP64 value (same operations in the same order: bit-identical).
+// 2-byte column offset and the FP64 value, slots in increasing column order.
 __device__ inline double spmv_row_fp64(const Mat& A, int i, const double* __restrict__ x) {
   const int m = A.len[i];
   const double xi = x[i];
@@ -377,7 +387,8 @@ __device__ inline double spmv_row(const Mat& A, int i, const double* __restrict_
 // block folds them in the same fixed order, so all agree), and only block 0 records the scalar for
 // later kernels; a kernel never reads a scalar that it writes.  Stage s writes its partials (and, in
 // serial mode, its per-row terms) to its own buffer.  One one-workgroup finalize per iteration
-// (after x,r) advances the iteration counters.
+// (after x,r) advances the iteration counters.  These eight kernels are the serial-mode iteration;
+// with parallel reductions the solve runs the five-kernel form further down (k_fu_*).
 // ------------------------------------------------------------------------------------------------
 struct Vecs {
   double *x, *b, *r, *rb, *v, *pn, *pinv, *s, *sinv, *t;   // pn, s: before remove_dc
@@ -402,10 +413,11 @@ __device__ inline bool finished(const Scal* sc) { return sc->done || sc->exit_s;
 constexpr int kPre = 4;
 
 // :306-321  AX = A x (x = 0 for the PPE, so r = B); r_bar = r; the terms of the first rho (:330)
+// and of sum(r)
 __global__ void k_bi_init(Mat A, Vecs w, int x_given, RedBufs rb) {
   const int n = A.n;
   RowRange rr = block_rows(n, gridDim.x);
-  Red<1> R(rb.R(ST_INIT), n);
+  Red<2> R(rb.R(ST_INIT), n);
   for (int i = rr.begin + threadIdx.x; i < rr.end; i += blockDim.x) {
     double r;
     if (x_given) r = w.b[i] - spmv_row_fp64(A, i, w.x);
@@ -413,17 +425,20 @@ __global__ void k_bi_init(Mat A, Vecs w, int x_given, RedBufs rb) {
     w.r[i] = r; w.rb[i] = r;
     w.v[i] = 0.0; w.pn[i] = 0.0;
     R.add(0, i, r * r);
+    R.add(1, i, r);
   }
   R.store(rb.P(ST_INIT));
 }
 
 __global__ void k_fin_init(RedBufs rb, int nblk, double tol, int itermax, Scal* sc) {
-  double rho = fold(rb.P(ST_INIT), 0, nblk, rb.R(ST_INIT), rb.n);
+  double f[2];
+  fold_n<2>(rb.P(ST_INIT), nblk, rb.R(ST_INIT), rb.n, f);
+  const double rho = f[0];
   if (threadIdx.x == 0) {
-    sc->rho = rho; sc->rho0 = 1.0; sc->alpha = 1.0; sc->omega = 1.0; sc->beta = 0.0;
+    sc->rho = rho; sc->rho_pp[1] = rho; sc->sum_r = f[1]; sc->rho0 = 1.0; sc->alpha = 1.0; sc->omega = 1.0; sc->beta = 0.0;
     sc->n2s0 = 1.0; sc->n2r0 = 1.0; sc->resid = 0.0; sc->tol = tol;
     sc->iter = 1; sc->itermax = itermax; sc->exit_s = 0; sc->status = VB_OK;
-    sc->mean_p = 0.0; sc->mean_s = 0.0;
+    sc->mean_p = 0.0; sc->mean_s = 0.0; sc->sum_v = 0.0; sc->sum_p = 0.0;
     sc->done = 0;
     if (itermax < 1) { sc->done = 1; sc->status = VB_WARN_MAXITER; }
     else if (rho == 0.0) { sc->done = 1; sc->status = VB_WARN_BICGSTAB_BREAKDOWN; }
@@ -462,16 +477,18 @@ __global__ void k_bi_pinv(int n, Vecs w, RedBufs rb, Scal* sc) {
 }
 
 // :353-354  v = A p_inv; the terms of r_bar . v
-template <int MODE>
+// (FUSED: the five-kernel iteration, which also needs the terms of sum(v))
+template <int MODE, bool FUSED = false>
 __global__ void k_bi_spmv_p(Mat A, Vecs w, RedBufs rb, const Scal* sc) {
   if (finished(sc)) return;
   const int n = A.n;
   RowRange rr = block_rows(n, gridDim.x);
-  Red<1> R(rb.R(ST_SPMV_P), n);
+  Red<FUSED ? 2 : 1> R(rb.R(ST_SPMV_P), n);
   for (int i = rr.begin + threadIdx.x; i < rr.end; i += blockDim.x) {
     double bb = spmv_row<MODE>(A, i, w.pinv);
     w.v[i] = bb;
     R.add(0, i, w.rb[i] * bb);
+    if (FUSED) R.add(FUSED ? 1 : 0, i, bb);
   }
   R.store(rb.P(ST_SPMV_P));
 }
@@ -530,12 +547,13 @@ __global__ void k_bi_sinv(int n, Vecs w, RedBufs rb, Scal* sc) {
 }
 
 // :385-387  t = A s_inv; terms of t.s and t.t with the projected s
-template <int MODE>
+// (FUSED: the five-kernel iteration stores s already projected)
+template <int MODE, bool FUSED = false>
 __global__ void k_bi_spmv_s(Mat A, Vecs w, RedBufs rb, const Scal* sc) {
   if (finished(sc)) return;
   const int n = A.n;
   RowRange rr = block_rows(n, gridDim.x);
-  const double mean = sc->mean_s;
+  const double mean = FUSED ? 0.0 : sc->mean_s;
   Red<2> R(rb.R(ST_SPMV_S), n);
   for (int i = rr.begin + threadIdx.x; i < rr.end; i += blockDim.x) {
     double bb = spmv_row<MODE>(A, i, w.sinv);
@@ -600,18 +618,20 @@ __global__ void k_bi_true_residual(Mat A, Vecs w, RedBufs rb, const Scal* sc) {
   R.store(rb.P(ST_TRUE));
 }
 
-__global__ void k_bi_true_shift(int n, Vecs w, RedBufs rb, const Scal* sc) {
+// (with_sum: also the terms of sum(r), which the five-kernel iteration folds after x,r)
+__global__ void k_bi_true_shift(int n, Vecs w, RedBufs rb, const Scal* sc, int with_sum) {
   if (finished(sc)) return;
   double f[2];
   fold_n<2>(rb.P(ST_TRUE), gridDim.x, rb.R(ST_TRUE), n, f);
   const double d = (f[0] - f[1]) / (double)n;
   RowRange rr = block_rows(n, gridDim.x);
-  Red<2> R(rb.R(ST_XR), n);
+  Red<3> R(rb.R(ST_XR), n);
   for (int i = rr.begin + threadIdx.x; i < rr.end; i += blockDim.x) {
     const double r = w.t[i] + d;
     w.r[i] = r;
     R.add(0, i, r * r);
     R.add(1, i, w.rb[i] * r);
+    if (with_sum) R.add(2, i, r);
   }
   R.store(rb.P(ST_XR));
 }
@@ -636,6 +656,177 @@ __global__ void k_fin_xr(RedBufs rb, int nblk, Scal* sc) {
       else sc->beta = rho / (sc->rho0 + 1.0e-15) * sc->alpha / (sc->omega + 1.0e-15);
     }
   }
+}
+
+// ------------------------------------------------------------------------------------------------
+// The same iteration in five kernels (parallel reductions only; serial mode keeps the kernels above,
+// which reproduce the reference's operation order):
+//
+//   k_fu_p      [finish iteration k-1: convergence, rho, beta]  p = r + beta (p - omega v),
+//               p -= mean(p), p_inv = p/(1+1e-15)                              -> sum(p) terms
+//   k_bi_spmv_p v = A p_inv                                                    -> r_bar.v, sum(v)
+//   k_fu_s      alpha; s = r - alpha v, s -= mean(s), s_inv = s/(1+1e-15)     -> |s|^2 (before
+//               remove_dc, as the reference's early-exit test)
+//   k_bi_spmv_s t = A s_inv                                                    -> t.s, t.t
+//   k_fu_xr     early exit on |s|, omega; x += alpha p_inv + omega s_inv; r = s - omega t
+//                                                                              -> |r|^2, r_bar.r, sum(r)
+//
+// The two remove_dc means are not folded from the new vectors (that takes a kernel boundary each)
+// but follow from sums already folded: sum(p_k) = sum(r) + beta (sum(p_{k-1}) - omega sum(v)) and
+// sum(s) = sum(r) - alpha sum(v), with sum(p_{k-1}) the terms of the projected p written one
+// iteration earlier.  They equal the folded means up to rounding of the sums, so the projections
+// remove the constant component as the reference's do, and the iterates agree with the
+// eight-kernel ones to rounding (not bit for bit).
+//
+// Scalars: a kernel still never reads a scalar that it writes.  k_fu_p both finishes iteration k-1
+// (writing rho_k) and uses rho_{k-1}, so rho is kept by iteration parity in rho_pp; the iteration
+// number comes from the host (while the solve runs, the device's counter equals it).  The last
+// iteration of each polling batch is finished by the one-workgroup k_fu_fin, and the first k_fu_p
+// of the next batch then reads beta and sum(r) from the scalars.  The sum(p) terms go to the second
+// quantity of the s stage, so k_fu_xr folds them with |s|^2 in one pass.
+// ------------------------------------------------------------------------------------------------
+static_assert(3 * kMaxBlocks <= 2 * kPartialStride, "three partial quantities per vector-kernel stage");
+
+struct FinState { bool done; double beta, sum_r; };
+
+// :403-419, :329-343 for iteration k from its folded |r|^2, r_bar.r and sum(r); lead records them
+__device__ inline FinState finish_iteration(const double (&f)[3], int k, Scal* sc, bool lead) {
+  const double n2r = f[0], rho = f[1];
+  const double n2r0 = k == 1 ? ((n2r <= 1.0e-6) ? 1.0 : n2r) : sc->n2r0;
+  FinState o{false, 0.0, f[2]};
+  int status = VB_OK;
+  const bool converged = n2r <= sc->tol * sc->tol * n2r0;
+  const double rho0 = sc->rho_pp[k & 1];
+  if (converged) o.done = true;
+  else if (k + 1 > sc->itermax) { o.done = true; status = VB_WARN_MAXITER; }
+  else if (rho == 0.0) { o.done = true; status = VB_WARN_BICGSTAB_BREAKDOWN; }
+  else o.beta = rho / (rho0 + 1.0e-15) * sc->alpha / (sc->omega + 1.0e-15);
+  if (lead) {
+    if (k == 1) sc->n2r0 = n2r0;
+    sc->resid = sqrt(n2r / n2r0);
+    if (!converged) { sc->iter = k + 1; sc->rho0 = rho0; sc->rho = rho; sc->rho_pp[(k + 1) & 1] = rho; }
+    sc->beta = o.beta; sc->sum_r = o.sum_r;
+    if (o.done) { sc->done = 1; sc->status = status; }
+  }
+  return o;
+}
+
+// Iteration k; fin = 1: finish iteration k-1 here (its x,r partials), 0: k_fu_fin or k_fin_init did
+__global__ void k_fu_p(int n, Vecs w, RedBufs rb, Scal* sc, int k, int fin) {
+  if (finished(sc)) return;
+  RowRange rr = block_rows(n, gridDim.x);
+  const int i0 = rr.begin + threadIdx.x, bd = blockDim.x;
+  double rv[kPre], pv[kPre], vv[kPre];
+#pragma unroll
+  for (int u = 0; u < kPre; u++) {
+    const int i = i0 + u * bd;
+    if (i < rr.end) { rv[u] = w.r[i]; pv[u] = w.pn[i]; vv[u] = w.v[i]; }
+  }
+  double beta, sum_r;
+  if (fin) {
+    double f[3];
+    fold_n<3>(rb.P(ST_XR), gridDim.x, nullptr, n, f);
+    const FinState o = finish_iteration(f, k - 1, sc, blockIdx.x == 0 && threadIdx.x == 0);
+    if (o.done) return;
+    beta = o.beta; sum_r = o.sum_r;
+  } else {
+    beta = sc->beta; sum_r = sc->sum_r;
+  }
+  const bool first = k == 1;
+  const double omega = sc->omega;
+  const double sum_p = first ? sum_r : sum_r + beta * (sc->sum_p - omega * sc->sum_v);
+  const double mean = sum_p / (double)n;                                  // remove_dc, :243
+  Red<1> R(nullptr, n);
+  auto row = [&](int i, double r, double p, double v) {
+    const double pn = first ? r : r + beta * (p - omega * v);
+    const double pp = pn - mean;
+    w.pn[i] = pp;
+    w.pinv[i] = pp / kOnePlusEps;
+    R.add(0, i, pp);
+  };
+#pragma unroll
+  for (int u = 0; u < kPre; u++) if (i0 + u * bd < rr.end) row(i0 + u * bd, rv[u], pv[u], vv[u]);
+  for (int i = i0 + kPre * bd; i < rr.end; i += bd) row(i, w.r[i], w.pn[i], w.v[i]);
+  R.store(rb.P(ST_S) + gridDim.x);
+}
+
+__global__ void k_fu_s(int n, Vecs w, RedBufs rb, Scal* sc, int nspmv, int k) {
+  if (finished(sc)) return;
+  RowRange rr = block_rows(n, gridDim.x);
+  const int i0 = rr.begin + threadIdx.x, bd = blockDim.x;
+  double rv[kPre], vv[kPre];
+#pragma unroll
+  for (int u = 0; u < kPre; u++)
+    if (i0 + u * bd < rr.end) { rv[u] = w.r[i0 + u * bd]; vv[u] = w.v[i0 + u * bd]; }
+  double f[2];
+  fold_n<2>(rb.P(ST_SPMV_P), nspmv, nullptr, n, f);
+  const double alpha = sc->rho_pp[k & 1] / (f[0] + 1.0e-15);
+  const double mean = (sc->sum_r - alpha * f[1]) / (double)n;
+  if (blockIdx.x == 0 && threadIdx.x == 0) { sc->alpha = alpha; sc->sum_v = f[1]; sc->mean_s = mean; }
+  Red<1> R(nullptr, n);
+  auto row = [&](int i, double r, double v) {
+    const double s = r - alpha * v;
+    R.add(0, i, s * s);
+    const double sp = s - mean;
+    w.s[i] = sp;
+    w.sinv[i] = sp / kOnePlusEps;
+  };
+#pragma unroll
+  for (int u = 0; u < kPre; u++) if (i0 + u * bd < rr.end) row(i0 + u * bd, rv[u], vv[u]);
+  for (int i = i0 + kPre * bd; i < rr.end; i += bd) row(i, w.r[i], w.v[i]);
+  R.store(rb.P(ST_S));
+}
+
+__global__ void k_fu_xr(int n, Vecs w, RedBufs rb, Scal* sc, int nspmv, int k) {
+  if (finished(sc)) return;
+  RowRange rr = block_rows(n, gridDim.x);
+  const int i0 = rr.begin + threadIdx.x, bd = blockDim.x;
+  double xv[kPre], pv[kPre], qv[kPre], sv[kPre], tv[kPre], bv[kPre];
+#pragma unroll
+  for (int u = 0; u < kPre; u++) {
+    const int i = i0 + u * bd;
+    if (i < rr.end) { xv[u] = w.x[i]; pv[u] = w.pinv[i]; qv[u] = w.sinv[i]; sv[u] = w.s[i]; tv[u] = w.t[i]; bv[u] = w.rb[i]; }
+  }
+  double fs[2], ft[2];
+  fold_n<2>(rb.P(ST_S), gridDim.x, nullptr, n, fs);          // |s|^2, sum(p)
+  fold_n<2>(rb.P(ST_SPMV_S), nspmv, nullptr, n, ft);         // t.s, t.t
+  const double n2s = fs[0];
+  const double n2s0 = k == 1 ? ((n2s <= 1.0e-6) ? 1.0 : n2s) : sc->n2s0;
+  const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+  const double alpha = sc->alpha;
+  if (lead) {
+    if (k == 1) sc->n2s0 = n2s0;
+    sc->sum_p = fs[1];
+  }
+  if (n2s <= sc->tol * sc->tol * n2s0) {                     // :374-381
+    for (int i = i0; i < rr.end; i += bd) w.x[i] = w.x[i] + alpha * w.pinv[i];
+    if (lead) { sc->resid = sqrt(n2s / n2s0); sc->exit_s = 1; }
+    return;
+  }
+  const double omega = ft[0] / (ft[1] + 1.0e-15);
+  if (lead) sc->omega = omega;
+  Red<3> R(nullptr, n);
+  auto row = [&](int i, double x, double pinv, double sinv, double s, double t, double rbi) {
+    w.x[i] = x + alpha * pinv + omega * sinv;
+    const double r = s - omega * t;
+    w.r[i] = r;
+    R.add(0, i, r * r);
+    R.add(1, i, rbi * r);
+    R.add(2, i, r);
+  };
+#pragma unroll
+  for (int u = 0; u < kPre; u++)
+    if (i0 + u * bd < rr.end) row(i0 + u * bd, xv[u], pv[u], qv[u], sv[u], tv[u], bv[u]);
+  for (int i = i0 + kPre * bd; i < rr.end; i += bd) row(i, w.x[i], w.pinv[i], w.sinv[i], w.s[i], w.t[i], w.rb[i]);
+  R.store(rb.P(ST_XR));
+}
+
+// finishes iteration k at the end of a polling batch (one workgroup)
+__global__ void k_fu_fin(RedBufs rb, int nblk, Scal* sc, int k) {
+  if (finished(sc)) return;
+  double f[3];
+  fold_n<3>(rb.P(ST_XR), nblk, nullptr, rb.n, f);
+  finish_iteration(f, k, sc, threadIdx.x == 0);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -922,11 +1113,26 @@ static vb_status solve(vb_ctx* c, const Mat& A, Vecs w, int x_given, double tol,
   const int ns = spmv_grid(n);         // SpMV kernels, one row per thread
   RedBufs rb = red_bufs(c, n);
   Scal* sc = c->scal;
+  const bool fused = rb.rows == nullptr;   // serial mode: the reference's eight-kernel order
   VB_LAUNCH(c, k_bi_init, nb, A, w, x_given, rb);
   VB_FIN(c, k_fin_init, rb, nb, tol, itermax, sc);
   int launched = 0, batch = 8;
   for (;;) {
     for (int b = 0; b < batch; b++) {
+      const int k = launched + b + 1;
+      const bool refresh = MODE == SPMV_MIXED && k % kRefresh == 0;
+      if (fused) {
+        VB_LAUNCH(c, k_fu_p, nb, n, w, rb, sc, k, b > 0 ? 1 : 0);
+        VB_LAUNCH_BS(c, (k_bi_spmv_p<MODE, true>), ns, kSpmvBlock, A, w, rb, sc);
+        VB_LAUNCH(c, k_fu_s, nb, n, w, rb, sc, ns, k);
+        VB_LAUNCH_BS(c, (k_bi_spmv_s<MODE, true>), ns, kSpmvBlock, A, w, rb, sc);
+        VB_LAUNCH(c, k_fu_xr, nb, n, w, rb, sc, ns, k);
+        if (refresh) {
+          VB_LAUNCH(c, k_bi_true_residual, nb, A, w, rb, sc);
+          VB_LAUNCH(c, k_bi_true_shift, nb, n, w, rb, sc, 1);
+        }
+        continue;
+      }
       VB_LAUNCH(c, k_bi_p, nb, n, w, rb, sc);
       VB_LAUNCH(c, k_bi_pinv, nb, n, w, rb, sc);
       VB_LAUNCH_BS(c, k_bi_spmv_p<MODE>, ns, kSpmvBlock, A, w, rb, sc);
@@ -934,12 +1140,13 @@ static vb_status solve(vb_ctx* c, const Mat& A, Vecs w, int x_given, double tol,
       VB_LAUNCH(c, k_bi_sinv, nb, n, w, rb, sc);
       VB_LAUNCH_BS(c, k_bi_spmv_s<MODE>, ns, kSpmvBlock, A, w, rb, sc);
       VB_LAUNCH(c, k_bi_xr, nb, n, w, rb, sc, ns);
-      if (MODE == SPMV_MIXED && (launched + b + 1) % kRefresh == 0) {
+      if (refresh) {
         VB_LAUNCH(c, k_bi_true_residual, nb, A, w, rb, sc);
-        VB_LAUNCH(c, k_bi_true_shift, nb, n, w, rb, sc);
+        VB_LAUNCH(c, k_bi_true_shift, nb, n, w, rb, sc, 0);
       }
       VB_FIN(c, k_fin_xr, rb, nb, sc);
     }
+    if (fused) VB_FIN(c, k_fu_fin, rb, nb, sc, launched + batch);
     launched += batch;
     VB_HIP(c, hipMemcpyAsync(c->scal_host, c->scal, sizeof(Scal), hipMemcpyDeviceToHost, c->stream));
     VB_HIP(c, hipStreamSynchronize(c->stream));
@@ -993,7 +1200,7 @@ static vb_status solve_tiled(vb_ctx* c, const TileMat& T, Vecs w, double tol, in
       VB_LAUNCH(c, k_bi_xr, nb, n, w, rb, sc, ns);
       if ((launched + b + 1) % kRefresh == 0) {
         VB_LAUNCH(c, k_tile_true_residual, nb, T, w, rb, sc);
-        VB_LAUNCH(c, k_bi_true_shift, nb, n, w, rb, sc);
+        VB_LAUNCH(c, k_bi_true_shift, nb, n, w, rb, sc, 0);
       }
       VB_FIN(c, k_fin_xr, rb, nb, sc);
     }
@@ -1090,7 +1297,8 @@ vb_status launch_ppe(vb_ctx* c, double tol, int itermax) {
     VB_LAUNCH(c, k_perm_scatter, grid_1d(c->npfb), c->perm, c->xperm, c->st.p, c->npfb);
     return VB_OK;
   }
-  if (c->spmv_stored) A.rowflag = nullptr;       // FP64: all rows from the stored matrix
+  // FP64 from the stored rows in the reference's slot order: on request, and in serial mode
+  if (c->spmv_stored || c->red) A.rowflag = nullptr;
   return solve<SPMV_FP64>(c, A, w, 0, tol, itermax);
 }
 
@@ -1153,7 +1361,7 @@ vb_status profile_spmv(vb_ctx* c, int reps, double* avg_us, double* alg_bytes) {
   const int n = c->npfb, nb = spmv_grid(n);
   const bool mixed = c->prm.precision == VB_PREC_MIXED;
   Mat A = step_matrix(c);
-  if (c->spmv_stored && !mixed) A.rowflag = nullptr;
+  if ((c->spmv_stored || c->red) && !mixed) A.rowflag = nullptr;
   Vecs w = step_vecs(c, c->st.p);
   int* d_counts = c->seg_counts;
   hipLaunchKernelGGL(k_bi_set_live, dim3(1), dim3(kBlock), 0, c->stream, c->scal, d_counts, c->cnt,
@@ -1162,7 +1370,7 @@ vb_status profile_spmv(vb_ctx* c, int reps, double* avg_us, double* alg_bytes) {
   hipEvent_t e0, e1;
   VB_HIP(c, hipEventCreate(&e0));
   VB_HIP(c, hipEventCreate(&e1));
-  const bool tiled = mixed && c->spmv_tiled;
+  const bool tiled = mixed && c->spmv_tiled, fused = c->red == nullptr;
   const TileMat T = tile_mat(c);
   const size_t lds = (size_t)3 * sizeof(double) * (c->wlds > 0 ? c->wlds : 1);
   for (int r = 0; r <= reps; r++) {
@@ -1172,10 +1380,12 @@ vb_status profile_spmv(vb_ctx* c, int reps, double* avg_us, double* alg_bytes) {
       hipLaunchKernelGGL(k_tile_spmv<0>, dim3(c->ntiles + 1), dim3(kTileThreads), lds, c->stream, T, w, rb,
                          c->scal);
       VB_HIP(c, hipGetLastError());
-    } else if (mixed) {
-      VB_LAUNCH_BS(c, k_bi_spmv_p<SPMV_MIXED>, nb, kSpmvBlock, A, w, rb, c->scal);
+    } else if (fused) {                     // the kernel the solve launches (five-kernel iteration)
+      if (mixed) VB_LAUNCH_BS(c, (k_bi_spmv_p<SPMV_MIXED, true>), nb, kSpmvBlock, A, w, rb, c->scal);
+      else VB_LAUNCH_BS(c, (k_bi_spmv_p<SPMV_FP64, true>), nb, kSpmvBlock, A, w, rb, c->scal);
     } else {
-      VB_LAUNCH_BS(c, k_bi_spmv_p<SPMV_FP64>, nb, kSpmvBlock, A, w, rb, c->scal);
+      if (mixed) VB_LAUNCH_BS(c, k_bi_spmv_p<SPMV_MIXED>, nb, kSpmvBlock, A, w, rb, c->scal);
+      else VB_LAUNCH_BS(c, k_bi_spmv_p<SPMV_FP64>, nb, kSpmvBlock, A, w, rb, c->scal);
     }
   }
   VB_HIP(c, hipEventRecord(e1, c->stream));
