@@ -214,9 +214,11 @@ __global__ void k_assemble_sliced(int npfb, const int* cnt, const int* link, con
     val4[e2] = val;
     if (val4f) val4f[e] = (float)val;
   };
+  // (~0.7 ms per step at 1M, against 0.28 ms unsorted: the stores of a wavefront now scatter over
+  // the row's groups; the SpMVs gain ~5 ms per step)
   for (int k = 0; k < n; k++) {
     const int16_t ok = off[k][t];
-    int pos = 0;
+    int pos = 0;                                  // rank: smaller offsets, ties broken by slot
     for (int j = 0; j < n; j++) {
       const int16_t oj = off[j][t];
       pos += (oj < ok) || (oj == ok && j < k);
