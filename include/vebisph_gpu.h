@@ -136,6 +136,17 @@ vb_status vb_final_advection(vb_ctx* ctx);
 /* the whole step: the calls above in order (one synchronisation for the step's scalars) */
 vb_status vb_div_free(vb_ctx* ctx);
 
+/* ---- output quantities of the reference's driver (these synchronise) ---- */
+/* secondary_properties (div_free.F90:148-192, STEP 11 of an output step): the vorticity
+ * vort(1..n), n <= npfb, from the last step's velocities, neighbour lists and w_G, and
+ * u_rms = sqrt(sum_i u_i.u_i / npfb) (the value the reference writes to unit 192).  The reference
+ * computes them between the velocity update and the final advection, which changes neither, so
+ * call this after vb_pressure_update or vb_div_free.  vort and u_rms may be NULL. */
+vb_status vb_secondary_properties(vb_ctx* ctx, int32_t n, double* vort, double* u_rms);
+/* maxval(abs(u(1:npfb,1))), maxval(abs(u(1:npfb,2))), maxval(abs(P(1:npfb))): the per-step line of
+ * data_directory/TIME (output.F90:86), from the current state */
+vb_status vb_field_maxima(vb_ctx* ctx, double* maxima3);
+
 /* ---- scalars and exports (for the host driver and parity checks; these synchronise) ---- */
 /* ints: np, npfb, nbS, nmirror, maxneighbours, LS_iters, itime, n_inbin, ncx, ncy, nct, status
  * dbls: dt, LS_residual, time, umax, x_mint, x_maxt, y_mint, y_maxt */

@@ -46,6 +46,7 @@ def lib():
         L.vbo_get_ppe_nnz.argtypes = [C.c_void_p]
         L.vbo_get_ppe.argtypes = [C.c_void_p, C.c_int, _d, _i, _d]
         L.vbo_get_grad_p.argtypes = [C.c_void_p, C.c_int, _d]
+        L.vbo_secondary_properties.argtypes = [C.c_void_p, C.c_int, _d, C.POINTER(C.c_double)]
         L.vbo_fac_wab.argtypes = [C.c_int, _d, C.c_double, C.c_double, _d, _d]
         L.vbo_calc_eigens.argtypes = [C.c_int, _d, _d, _d]
         L.vbo_logconf.argtypes = [C.c_int, _d, _d, _d, C.c_double, C.c_double, C.c_double, C.c_int, _d, _d]
@@ -168,6 +169,13 @@ class OracleSolver:
         g = np.zeros(2 * n)
         lib().vbo_get_grad_p(self.h, n, g)
         return g.reshape(2, n).T
+
+    def secondary_properties(self):
+        """Vorticity of rows 1..npfb and RMS velocity (after phase 11, before phase 12)."""
+        n = self.case.npfb
+        vort = np.zeros(n); u_rms = C.c_double(0.0)
+        lib().vbo_secondary_properties(self.h, n, vort, C.byref(u_rms))
+        return vort, u_rms.value
 
 
 def fac_wab(q, ad_7, ad_7h):

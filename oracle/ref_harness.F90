@@ -378,4 +378,121 @@ contains
     deallocate(aa, bb, xx, jj)
   end subroutine vbref_bicgstab_nonull
 
+  !! ---- the reference's own file reader and writer (input.F90, output.F90) ----
+
+  !! getdata (input.F90:12-180) on INDAT, IBOUND and IPART (TEMP_REC.dat when INDAT says restart)
+  !! in the current directory, then the padded box and cell grid of sph2D_incom.F90:64-75.  Arrays
+  !! the reference allocates without setting on the closed-domain path (conc, a_out, vort) and
+  !! never-set scalars (av_conc, LS_iters, LS_residual, n_inbin) are zeroed, so written files are
+  !! reproducible.
+  subroutine vbref_getdata() bind(C, name="vbref_getdata")
+    use input, only: getdata
+    call vbref_free()
+    call vbref_set_threads(1)
+    call getdata()
+    conc = 0.0d0; a_out = 0.0d0; vort = 0.0d0
+    av_conc = 0.0d0; LS_iters = 0; LS_residual = 0.0d0; n_inbin = 0
+    cap = npar
+    hx_mint = xb_min - sup_size*1.5d0
+    hx_maxt = xb_max + sup_size*1.5d0
+    hy_mint = yb_min - sup_size*1.5d0
+    hy_maxt = yb_max + sup_size*1.5d0
+    call cell_generator(hx_mint,hx_maxt,hy_mint,hy_maxt,uno_sup_size,ncx,ncy,nct)
+    ncell_m = nct
+    nsheet_m = nct
+    next_dump = time; next_dump2 = time
+    n_dump = 0
+  end subroutine vbref_getdata
+
+  !! INDAT values and counts as getdata left them.
+  !! ints: 1 i_RESTART, 2 i_open_domain, 3 i_shift_coeff, 4 np, 5 npfb, 6 nbS, 7 itime, 8 nb_patches,
+  !!       9 nb_circles, 10 n_dump
+  !! dbls: 1 tmax, 2 dt_out, 3 h, 4 dx, 5 dt, 6-7 body_force, 8 Ra, 9 Pr, 10 El, 11 beta,
+  !!       12 dt_coef_cfl, 13 dt_coef_visc, 14 ve_nonlinearity, 15 alpha_evss, 16 dv, 17 time
+  subroutine vbref_get_run_params(ints, dbls) bind(C, name="vbref_get_run_params")
+    integer(c_int), intent(out) :: ints(10)
+    real(c_double), intent(out) :: dbls(17)
+    ints = (/ merge(1, 0, i_RESTART), i_open_domain, i_shift_coeff, np, npfb, nbS, itime, nb_patches, &
+              nb_circles, n_dump /)
+    dbls = (/ tmax, dt_out, h, dx, dt, body_force(1), body_force(2), Ra, Pr, El, beta, dt_coef_cfl, &
+              dt_coef_visc, ve_nonlinearity, alpha_evss, dv, time /)
+  end subroutine vbref_get_run_params
+
+  !! Boundary arrays as read (patch arrays (nbp,2) column-major; periodic parents only meaningful
+  !! for type-2 patches; circles (ncirc[,2]))
+  subroutine vbref_get_boundary(nbp, ncirc, bnode, bedge, btype, bvel, btherm, bpp, ccen, crad, comg, cvel, &
+       ctherm) bind(C, name="vbref_get_boundary")
+    integer(c_int), value :: nbp, ncirc
+    real(c_double), intent(out) :: bnode(nbp,2), bedge(nbp,2), bvel(nbp), btherm(nbp)
+    integer(c_int), intent(out) :: btype(nbp), bpp(nbp)
+    real(c_double), intent(out) :: ccen(max(ncirc,1),2), crad(max(ncirc,1)), comg(max(ncirc,1))
+    real(c_double), intent(out) :: cvel(max(ncirc,1),2), ctherm(max(ncirc,1))
+    integer(ikind) :: i
+    do i = 1, nbp
+       bnode(i,:) = b_node(i,:); bedge(i,:) = b_edge(i,:); btype(i) = b_type(i)
+       bvel(i) = b_vel(i); btherm(i) = b_thermal(i)
+       bpp(i) = 0
+       if(b_type(i).eq.2) bpp(i) = b_periodic_parent(i)
+    end do
+    do i = 1, ncirc
+       ccen(i,:) = c_centre(i,:); crad(i) = c_radius(i); comg(i) = c_omega(i)
+       cvel(i,:) = c_vel(i,:); ctherm(i) = c_thermal(i)
+    end do
+  end subroutine vbref_get_boundary
+
+  !! next_dump and next_dump2 of the driver: output writes files when time > next_dump
+  !! (output.F90:18), div_free computes the vorticity when time > next_dump2 (div_free.F90:154)
+  subroutine vbref_set_dumps(nd, nd2, dtout) bind(C, name="vbref_set_dumps")
+    real(c_double), value :: nd, nd2, dtout
+    next_dump = nd; next_dump2 = nd2; dt_out = dtout
+  end subroutine vbref_set_dumps
+
+  subroutine vbref_get_vort(n, vo, a_o) bind(C, name="vbref_get_vort")
+    integer(c_int), value :: n
+    real(c_double), intent(out) :: vo(n), a_o(n)
+    vo = vort(1:n)
+    a_o = a_out(1:n)
+  end subroutine vbref_get_vort
+
+  !! The units of sph2D_incom.F90:52-61 (data_directory/ must exist in the current directory)
+  subroutine vbref_open_units(restart) bind(C, name="vbref_open_units")
+    integer(c_int), value :: restart
+    if(restart.ne.0) then
+       open(unit=92,file='./data_directory/TIME',status='unknown',access='append')
+       open(unit=93,file='./data_directory/LS',status='unknown',access='append')
+       open(unit=94,file='./data_directory/TIME_OUT',status='unknown',access='append')
+    else
+       open(unit=92,file='./data_directory/TIME')
+       open(unit=93,file='./data_directory/LS')
+       open(unit=94,file='./data_directory/TIME_OUT')
+    end if
+  end subroutine vbref_open_units
+
+  subroutine vbref_close_units() bind(C, name="vbref_close_units")
+    close(92); close(93); close(94); close(192)
+  end subroutine vbref_close_units
+
+  !! the reference's output (output.F90)
+  subroutine vbref_output() bind(C, name="vbref_output")
+    external :: output
+    call output
+  end subroutine vbref_output
+
+  !! list-directed records of this compiler, for the writer's format tests: one value per record
+  subroutine vbref_ld_write(path, plen, n, vals) bind(C, name="vbref_ld_write")
+    integer(c_int), value :: plen, n
+    character(kind=c_char), intent(in) :: path(plen)
+    real(c_double), intent(in) :: vals(n)
+    character(len=plen) :: fname
+    integer(ikind) :: i
+    do i = 1, plen
+       fname(i:i) = path(i)
+    end do
+    open(unit=77, file=fname, status='replace')
+    do i = 1, n
+       write(77,*) vals(i)
+    end do
+    close(77)
+  end subroutine vbref_ld_write
+
 end module vbref_harness

@@ -42,6 +42,12 @@ def lib():
         L.vbref_get_grad_p.argtypes = [C.c_int, _dp]
         L.vbref_bicgstab_nonull.argtypes = [C.c_int, C.c_int, _dp, _ip, _dp, _dp, C.c_int, C.c_double,
                                             C.POINTER(C.c_int), C.POINTER(C.c_double)]
+        L.vbref_get_run_params.argtypes = [_ip, _dp]
+        L.vbref_get_boundary.argtypes = [C.c_int, C.c_int, _dp, _dp, _ip, _dp, _dp, _ip, _dp, _dp, _dp, _dp, _dp]
+        L.vbref_set_dumps.argtypes = [C.c_double, C.c_double, C.c_double]
+        L.vbref_get_vort.argtypes = [C.c_int, _dp, _dp]
+        L.vbref_open_units.argtypes = [C.c_int]
+        L.vbref_ld_write.argtypes = [C.c_char_p, C.c_int, C.c_int, _dp]
         _lib = L
     return _lib
 
@@ -157,3 +163,92 @@ def bicgstab_nonull(a, ija, b, x0=None, itermax=1000, tol=1e-8):
     lib().vbref_bicgstab_nonull(n, a.shape[0], _f(a), _f(ija, np.int32), x, _f(b), itermax, tol,
                                 C.byref(it), C.byref(res))
     return x, it.value, res.value
+
+
+# ---- the reference's own run-file reader and writer (input.F90 getdata, output.F90) ----
+class RefRun:
+    """The reference's module state as getdata (input.F90:12-180) leaves it after reading the run
+    directory `path` (INDAT, IBOUND, IPART); steps and output() run in that directory, where the
+    reference writes its files (data_directory/ is created).  One per process at a time."""
+
+    def __init__(self, path):
+        self.path = os.path.abspath(path)
+        os.makedirs(os.path.join(self.path, "data_directory"), exist_ok=True)
+        with _cwd(self.path):
+            lib().vbref_getdata()
+        self.params = self.run_params()
+
+    def run_params(self):
+        ints = np.zeros(10, np.int32, order="F"); dbls = np.zeros(17, order="F")
+        lib().vbref_get_run_params(ints, dbls)
+        ki = ("i_restart", "i_open_domain", "i_shift_coeff", "np", "npfb", "nbS", "itime", "nb_patches",
+              "nb_circles", "n_dump")
+        kd = ("tmax", "dt_out", "h", "dx", "dt", "bf_x", "bf_y", "Ra", "Pr", "El", "beta", "dt_coef_cfl",
+              "dt_coef_visc", "ve_nonlinearity", "alpha_evss", "dv", "time")
+        d = {k: int(v) for k, v in zip(ki, ints)}
+        d.update({k: float(v) for k, v in zip(kd, dbls)})
+        return d
+
+    def boundary(self):
+        nb, nc = self.params["nb_patches"], self.params["nb_circles"]
+        z = lambda *s: np.zeros(s, order="F")
+        bn, be, bt, bv, bth, bp = z(nb, 2), z(nb, 2), np.zeros(nb, np.int32, order="F"), z(nb), z(nb), \
+            np.zeros(nb, np.int32, order="F")
+        m = max(nc, 1)
+        cc, cr, co, cv, ct = z(m, 2), z(m), z(m), z(m, 2), z(m)
+        lib().vbref_get_boundary(nb, nc, bn, be, bt, bv, bth, bp, cc, cr, co, cv, ct)
+        return dict(b_node=bn, b_edge=be, b_type=bt, b_vel=bv, b_thermal=bth, b_periodic_parent=bp,
+                    c_centre=cc[:nc], c_radius=cr[:nc], c_omega=co[:nc], c_vel=cv[:nc], c_thermal=ct[:nc])
+
+    def get_state(self, n=None):
+        n = self.params["npfb"] if n is None else n
+        out = {k: np.zeros(n, order="F") for k in ("x", "y", "u", "v", "theta", "trace", "p")}
+        tau = np.zeros((n, 4), order="F")
+        inb = np.zeros(n, np.int32, order="F")
+        lib().vbref_get_particles(n, out["x"], out["y"], out["u"], out["v"], out["theta"], tau,
+                                  out["trace"], out["p"], inb)
+        out["tau"] = tau
+        out["inbin"] = inb
+        return out
+
+    def set_dumps(self, next_dump, next_dump2, dt_out):
+        lib().vbref_set_dumps(next_dump, next_dump2, dt_out)
+
+    def step(self):
+        with _cwd(self.path):
+            lib().vbref_step()
+
+    def vort_a_out(self, n=None):
+        n = self.params["npfb"] if n is None else n
+        vo = np.zeros(n, order="F"); ao = np.zeros(n, order="F")
+        lib().vbref_get_vort(n, vo, ao)
+        return vo, ao
+
+    def open_units(self, restart=False):
+        with _cwd(self.path):
+            lib().vbref_open_units(int(restart))
+
+    def output(self):
+        with _cwd(self.path):
+            lib().vbref_output()
+
+    def close_units(self):
+        lib().vbref_close_units()
+
+
+def ld_write(path, vals):
+    """One list-directed record per value, written by the reference's compiler."""
+    p = os.fsencode(path)
+    lib().vbref_ld_write(p, len(p), len(vals), _f(vals))
+
+
+class _cwd:
+    def __init__(self, path):
+        self.path = path
+
+    def __enter__(self):
+        self.old = os.getcwd()
+        os.chdir(self.path)
+
+    def __exit__(self, *a):
+        os.chdir(self.old)

@@ -1245,6 +1245,26 @@ void vbo_get_grad_p(const vbo_ctx* c, int n, double* gp) {
   for (int i = 0; i < n; i++) { gp[i] = c->gpx[i]; gp[n + i] = c->gpy[i]; }
 }
 
+/* secondary_properties, div_free.F90:148-192 (STEP 11: after phase 11, before phase 12): the
+ * vorticity of rows 1..npfb from the neighbour lists, w_G and the current velocities, and the RMS
+ * velocity sqrt(sum u.u / npfb) that the reference writes to unit 192 (its reduction in index order,
+ * as with one OpenMP thread). */
+void vbo_secondary_properties(const vbo_ctx* c, int n, double* vort, double* u_rms) {
+  const int npfb = c->npfb;
+  for (int i = 0; i < n && i < npfb; i++) {                   /* :158-167 */
+    double w = 0.0;
+    for (int k = 0; k < c->cnt[i]; k++) {
+      const int s = k * npfb + i, j = c->link[s];
+      const double tmp = (c->v[i] - c->v[j]) * c->wg1[s] - (c->u[i] - c->u[j]) * c->wg2[s];
+      w = w - tmp;
+    }
+    vort[i] = w;
+  }
+  double a = 0.0;                                             /* :171-181 */
+  for (int i = 0; i < npfb; i++) a = a + (c->u[i] * c->u[i] + c->v[i] * c->v[i]);
+  *u_rms = sqrt(a / (double)npfb);
+}
+
 /* nnf_models_mod.F90:8-110 (no callers in the reference; restated for completeness) */
 void vbo_nnf(int model, int n, const double* sr, double mu, double* out) {
   for (int i = 0; i < n; i++) {

@@ -60,6 +60,9 @@ void vbo_get_grad_vel(const vbo_ctx* c, int n, double* g_n4);
 int  vbo_get_ppe_nnz(const vbo_ctx* c);
 void vbo_get_ppe(const vbo_ctx* c, int nnz, double* a, int* ija, double* b);
 void vbo_get_grad_p(const vbo_ctx* c, int n, double* gp_n2);
+/* secondary_properties (div_free.F90:148-192), valid after phase 11: vorticity of rows 1..n and
+ * the RMS velocity */
+void vbo_secondary_properties(const vbo_ctx* c, int n, double* vort, double* u_rms);
 
 /* Stand-alone pieces. */
 void vbo_fac_wab(int n, const double* q, double ad_7, double ad_7h, double* fac, double* wab);

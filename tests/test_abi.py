@@ -148,3 +148,48 @@ def test_cpp_driver_rejects_bad_input(tmp_path):
     (tmp_path / "trunc.bin").write_bytes(data[:-100])
     r = subprocess.run([exe, str(tmp_path / "trunc.bin"), "1", str(tmp_path / "o")], capture_output=True, text=True)
     assert r.returncode == 2 and "truncated" in r.stderr
+
+
+# ---- the run-file library (include/vebisph_refio.h) ----
+REFIO_HEADER = os.path.join(ROOT, "include", "vebisph_refio.h")
+
+
+def refio_functions():
+    text = re.sub(r"/\*.*?\*/", "", open(REFIO_HEADER).read(), flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:int32_t|const char\*)\s+(vbio_\w+)\s*\(", text, flags=re.M)))
+
+
+def test_refio_library_exports_every_declared_symbol():
+    from vebisph_amd import refio
+    names = refio_functions()
+    assert "vbio_read_indat" in names and "vbio_write_part" in names and "vbio_read_temp_rec" in names
+    L = refio.lib()
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_refio_record_layout_c_vs_ctypes():
+    from vebisph_amd import refio
+    if shutil.which("gcc") is None:
+        pytest.skip("gcc absent")
+    src = r"""
+#include <stdio.h>
+#include <stddef.h>
+#include "vebisph_refio.h"
+int main(void) {
+  printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(vbio_indat), offsetof(vbio_indat, body_force),
+         offsetof(vbio_indat, alpha_evss), sizeof(vbio_ibound), offsetof(vbio_ibound, u_inflow),
+         offsetof(vbio_ibound, c_thermal), sizeof(vbio_restart), offsetof(vbio_restart, xb_min),
+         offsetof(vbio_restart, av_conc));
+  return 0;
+}
+"""
+    d = os.path.join(ROOT, "vebisph_amd", "_build")
+    os.makedirs(d, exist_ok=True)
+    cfile, exe = os.path.join(d, "refio_layout.c"), os.path.join(d, "refio_layout")
+    open(cfile, "w").write(src)
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), cfile, "-o", exe], check=True)
+    got = [int(v) for v in subprocess.run([exe], check=True, capture_output=True, text=True).stdout.split()]
+    I, B, R = refio.Indat, refio.Ibound, refio.Restart
+    assert got == [C.sizeof(I), I.body_force.offset, I.alpha_evss.offset, C.sizeof(B), B.u_inflow.offset,
+                   B.c_thermal.offset, C.sizeof(R), R.xb_min.offset, R.av_conc.offset]

@@ -414,3 +414,147 @@ def test_cpp_driver_matches_python_path(gpu_lib, tmp_path):
     got, want = solver.read_state_file(of, case.npfb), G.get_state()
     for k in want:
         _eq(got[k], want[k])
+
+
+def test_secondary_properties_and_field_maxima(gpu_lib):
+    """vb_secondary_properties: the vorticity kernel on the GPU's own velocities, lists and w_G
+    equals the reference's loop (div_free.F90:158-167) evaluated on the host in slot order, bit for
+    bit, and the RMS velocity its sum to rounding; against the oracle (itself bit-exact against the
+    reference's secondary_properties) after two steps within the velocity tolerance of the step
+    tests.  vb_field_maxima equals the host maxima of the downloaded state."""
+    case = SMALL["convection"]()
+    print(f"case {case.name} npfb={case.npfb}", flush=True)
+    G = solver.Solver(case); O = ow.OracleSolver(case)
+    G.div_free(); O.step()
+    G.div_free()
+    for k in range(1, 12):
+        O.phase(k)
+    vort_o, urms_o = O.secondary_properties()
+    O.phase(12)
+    vort, urms = G.secondary_properties()
+    sc = G.scalars()
+    n, w = case.npfb, sc["maxneighbours"]
+    st = G.get_state(sc["np"])
+    cnt, link = G.neighbours(w)
+    wg, _, _ = G.weights(w)
+    u, v = st["u"], st["v"]
+    want = np.zeros(n)
+    for k in range(w):
+        m = k < cnt
+        j = np.where(m, link[:, k] - 1, 0)
+        t = (v[:n] - v[j]) * wg[:, k, 0] - (u[:n] - u[j]) * wg[:, k, 1]
+        want = np.where(m, want - t, want)
+    _eq(vort, want)
+    assert abs(urms - np.sqrt(np.sum(u[:n] * u[:n] + v[:n] * v[:n]) / n)) <= 1e-14 * urms
+    print(f"vorticity vs oracle {_rel(vort, vort_o):.2e}, u_rms {abs(urms - urms_o) / urms_o:.2e}")
+    assert _rel(vort, vort_o) <= 1e-4 and abs(urms - urms_o) <= 1e-5 * urms_o
+    m = G.field_maxima()
+    assert tuple(m) == (np.abs(u[:n]).max(), np.abs(v[:n]).max(), np.abs(st["p"][:n]).max())
+
+
+def _drive(run_dir, *args):
+    import subprocess
+    exe = os.path.join(os.path.dirname(solver.__file__), "vb_drive")
+    print(f"vb_drive run {run_dir} {' '.join(args)}", flush=True)
+    r = subprocess.run([exe, "run", str(run_dir), *args], capture_output=True, text=True, timeout=120)
+    print(r.stdout[-2000:], r.stderr[-2000:], flush=True)
+    assert r.returncode == 0
+
+
+def _python_driver(case, tmax, dt_out, max_steps):
+    """The reference's driver loop (sph2D_incom.F90:51-129, output.F90, div_free.F90:152-185) on the
+    ctypes Solver: what vb_drive must write."""
+    from vebisph_amd import refio  # noqa: F401  (same formats; the values are compared)
+    G = solver.Solver(case)
+    n = case.npfb
+    out = dict(parts=[], tout=[], time=[], ls=[], r192=[], temp=None)
+    s = dict(time=0.0, next_dump=0.0, next_dump2=0.0, n_dump=0, dt=0.0, iters=0, res=0.0, n_inbin=0, itime=0)
+    vort = np.zeros(n)
+
+    def output():
+        if s["time"] > s["next_dump"]:
+            s["next_dump"] += dt_out
+            s["n_dump"] += 1
+            st = G.get_state()
+            out["parts"].append((st, vort.copy()))
+            out["tout"].append([s["time"], n, s["itime"], s["dt"]])
+            out["temp"] = (s["itime"], s["time"], s["n_dump"], st)
+        m = G.field_maxima()
+        out["time"].append([s["time"], s["dt"], n - s["n_inbin"], *m])
+        out["ls"].append([s["iters"], s["res"]])
+
+    output()
+    steps = 0
+    while s["time"] <= tmax and steps < max_steps:
+        t_before = s["time"]
+        G.div_free()
+        if t_before > s["next_dump2"]:
+            s["next_dump2"] += dt_out
+            vort, urms = G.secondary_properties()
+            out["r192"].append([t_before, urms])
+        sc = G.scalars()
+        s.update(time=sc["time"], dt=sc["dt"], iters=sc["LS_iters"], res=sc["LS_residual"], n_inbin=sc["n_inbin"],
+                 itime=sc["itime"])
+        steps += 1
+        output()
+    output()
+    return out
+
+
+def test_vb_drive_runs_reference_run_directory(gpu_lib, tmp_path):
+    """vb_drive run DIR: reads the reference's INDAT/IBOUND/IPART, steps while time <= tmax and
+    writes PART<n>, TEMP_REC.dat, TIME, LS, TIME_OUT and fort.192 as the reference's driver does,
+    with the values of the same loop run through the ctypes Solver, bit for bit."""
+    from vebisph_amd import refio
+    case = SMALL["poiseuille"]()
+    dt = 1.352e-5                                  # the Poiseuille case's viscous step
+    tmax, dt_out = 8.5 * dt, 2.5 * dt
+    refio.write_case_dir(str(tmp_path), case, tmax=tmax, dt_out=dt_out)
+    _drive(tmp_path, "fp64")
+    want = _python_driver(case, tmax, dt_out, 10 ** 9)
+    dd = tmp_path / "data_directory"
+    assert len(want["parts"]) >= 3
+    for k, (st, vort) in enumerate(want["parts"]):
+        part = refio.read_part(str(dd / f"PART{k}"), case.npfb)
+        keep = st["inbin"] == 0
+        for key, ref in (("x", st["x"]), ("y", st["y"]), ("u", st["u"]), ("v", st["v"]), ("theta", st["theta"]),
+                         ("p", st["p"]), ("a_out", st["trace"]), ("vort", vort), ("tau11", st["tau"][:, 0]),
+                         ("tau12", st["tau"][:, 1]), ("tau22", st["tau"][:, 3])):
+            _eq(part[key], ref[keep])
+        assert not np.any(part["conc"]) and not np.any(part["n_surf"])
+    assert not (dd / f"PART{len(want['parts'])}").exists()
+    _eq(refio.read_table(str(dd / "TIME"), 6), np.array(want["time"]))
+    _eq(refio.read_table(str(dd / "LS"), 2), np.array(want["ls"]))
+    _eq(refio.read_table(str(dd / "TIME_OUT"), 4), np.array(want["tout"]))
+    _eq(refio.read_table(str(tmp_path / "fort.192"), 2), np.array(want["r192"]))
+    hd, cols = refio.read_temp_rec(str(tmp_path / "TEMP_REC.dat"))
+    itime, time, n_dump, st = want["temp"]
+    assert (hd.itime, hd.time, hd.n_dump, hd.npfb, hd.nbS) == (itime, time, n_dump, case.npfb, case.nbS)
+    for key in ("x", "y", "u", "v", "theta", "p"):
+        _eq(cols[key], st[key])
+
+
+def test_vb_drive_restart_continues_the_run(gpu_lib, tmp_path):
+    """A run stopped after three steps and restarted from its TEMP_REC.dat (INDAT i_RESTART = T)
+    ends where the uninterrupted six-step run ends, bit for bit (Oldroyd-B: the trace, which the file
+    does not hold, does not enter the step; the reference's own restart reader would take theta for
+    the pressure, input.F90:126)."""
+    from vebisph_amd import refio
+    case = SMALL["convection"]()
+    a, b = tmp_path / "a", tmp_path / "b"
+    for d in (a, b):
+        refio.write_case_dir(str(d), case, tmax=1.0, dt_out=1e-12)
+    _drive(a, "fp64", "6")
+    _drive(b, "fp64", "3")
+    refio.write_case_dir(str(b), case, tmax=1.0, dt_out=1e-12, restart=True)
+    _drive(b, "fp64", "3")
+    ha, ca = refio.read_temp_rec(str(a / "TEMP_REC.dat"))
+    hb, cb = refio.read_temp_rec(str(b / "TEMP_REC.dat"))
+    assert (ha.itime, ha.npfb) == (hb.itime, hb.npfb) == (6, case.npfb)
+    # a dump at every step and at the driver's closing output call (sph2D_incom.F90:129): 6 + 1 in
+    # one run, 3 + 1 and 3 + 1 in the other
+    assert (ha.n_dump, hb.n_dump) == (7, 8)
+    assert ha.time == hb.time
+    for key in ("x", "y", "u", "v", "theta", "p", "tau11", "tau12", "tau21", "tau22"):
+        _eq(ca[key], cb[key])
+    assert refio.read_table(str(b / "data_directory" / "LS"), 2).shape[0] == 2 * 5

@@ -102,3 +102,30 @@ def test_reference_bicgstab_on_oracle_matrix():
     xo, ito, ro, st = oraclewrap.bicgstab_nonull(a, ija, b)
     assert itr == ito and rr == ro
     _same(xr, xo)
+
+
+@pytest.mark.parametrize("name", ["poiseuille", "convection"])
+def test_secondary_properties_bit_exact(tmp_path, monkeypatch, name):
+    """secondary_properties (div_free.F90:148-192): the oracle's vorticity and RMS velocity, taken
+    after phase 11 of the second step, against the reference's (its div_free computes them when
+    time > next_dump2 and writes the RMS velocity to unit 192, i.e. fort.192 here)."""
+    from oracle import refwrap
+    from vebisph_amd import refio
+    monkeypatch.chdir(tmp_path)
+    case = cases.poiseuille() if name == "poiseuille" else cases.convection(nx=100, ny=50, name="convection_5k")
+    R, O = _pair(case)
+    refwrap.lib().vbref_set_dumps(1.0e300, -1.0, 1.0)   # vorticity at both steps (then next_dump2 = 1)
+    R.step(); R.step()
+    refwrap.lib().vbref_close_units()                    # flushes unit 192
+    O.step()
+    for k in range(1, 12):
+        O.phase(k)
+    vort_o, urms_o = O.secondary_properties()
+    O.phase(12)
+    vort_r = np.zeros(case.npfb, order="F"); a_out = np.zeros(case.npfb, order="F")
+    refwrap.lib().vbref_get_vort(case.npfb, vort_r, a_out)
+    _same(vort_o, vort_r)
+    rows = refio.read_table(str(tmp_path / "fort.192"), 2)
+    assert rows.shape == (2, 2)
+    assert urms_o == rows[1, 1] and rows[1, 0] == R.scalars()["time"] - R.scalars()["dt"]
+    assert np.abs(vort_o).max() > 0

@@ -79,12 +79,24 @@ def build_fortran() -> str | None:
     return FLIB
 
 
+REFIO_LIB = os.path.join(PKG, "libvebisph_refio.so")
+
+
+def build_refio() -> str:
+    """libvebisph_refio.so: readers/writers of the reference's run files (host C++, no GPU)."""
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-Wextra", "-fPIC", "-shared",
+          f"-I{os.path.join(ROOT, 'include')}", os.path.join(PKG, "cpp", "vb_refio.cpp"),
+          "-o", REFIO_LIB + ".tmp"])
+    os.replace(REFIO_LIB + ".tmp", REFIO_LIB)
+    return REFIO_LIB
+
+
 def build_cpp_driver() -> str:
     """vb_drive: the stand-alone C++ driver of the C ABI (vebisph_amd/cpp/vb_drive.cpp)."""
     out = os.path.join(PKG, "vb_drive")
     _run(["g++", "-O2", "-std=c++17", "-Wall", f"-I{os.path.join(ROOT, 'include')}",
           os.path.join(PKG, "cpp", "vb_drive.cpp"), "-o", out + ".tmp", f"-L{PKG}", "-lvebisph_hip",
-          "-Wl,-rpath,$ORIGIN"])
+          "-lvebisph_refio", "-Wl,-rpath,$ORIGIN"])
     os.replace(out + ".tmp", out)
     return out
 
@@ -99,6 +111,7 @@ def build_checkers() -> None:
 def build_all() -> None:
     build_hip()
     build_fortran()
+    build_refio()
     build_cpp_driver()
     build_checkers()
 

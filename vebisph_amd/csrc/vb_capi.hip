@@ -67,7 +67,7 @@ static void free_all(vb_ctx* c) {
                   c->perm, c->rank, c->pflag, c->pscan, c->ist_real, c->posP, c->idx16, c->lenP,
                   c->rowflagP, c->rowsumP, c->bperm, c->xperm, c->atab,
                   c->wg1, c->wg2, c->wl, c->k11, c->k12, c->k21, c->k22,
-                  c->usx, c->usy, c->kick, c->gpx, c->gpy, c->a_diag, c->a_val, c->a_rowsum, c->rhs,
+                  c->usx, c->usy, c->kick, c->gpx, c->gpy, c->vort, c->a_diag, c->a_val, c->a_rowsum, c->rhs,
                   c->a_col, c->a_len, c->rowflag, c->chunkg, c->choff, c->xflag, c->xscan, c->xrows,
                   c->lnk4, c->val4, c->val4f,
                   c->r, c->rb, c->vv,
@@ -184,7 +184,7 @@ vb_status vb_create(const vb_params* prm, const vb_boundary* bnd, int32_t npfb, 
   A(c->scan_bsum, (std::max<size_t>(cap, (size_t)c->g.nct) + 1023) / 1024 + 1);
   A(c->cnt, n); A(c->link, nw); A(c->wg1, nw); A(c->wg2, nw); A(c->wl, nw);
   A(c->k11, n); A(c->k12, n); A(c->k21, n); A(c->k22, n);
-  A(c->usx, n); A(c->usy, n); A(c->kick, n); A(c->gpx, n); A(c->gpy, n);
+  A(c->usx, n); A(c->usy, n); A(c->kick, n); A(c->gpx, n); A(c->gpy, n); A(c->vort, n);
   A(c->a_diag, n); A(c->a_val, nw); A(c->a_rowsum, n); A(c->rhs, n); A(c->a_col, nw); A(c->a_len, n);
   // compact grouped rows: up to VB_NPLINK slots rounded up to a multiple of 4
   // compact rows: at most VB_NPLINK slots rounded up to groups of 4, for every row of every chunk
@@ -488,6 +488,26 @@ vb_status vb_export_ushift(vb_ctx* c, int32_t n, double* us) {
   VB_HIP(c, hipMemcpyAsync(us, c->usx, (size_t)n * 8, hipMemcpyDeviceToHost, c->stream));
   VB_HIP(c, hipMemcpyAsync(us + n, c->usy, (size_t)n * 8, hipMemcpyDeviceToHost, c->stream));
   VB_HIP(c, hipStreamSynchronize(c->stream));
+  return VB_OK;
+}
+
+vb_status vb_secondary_properties(vb_ctx* c, int32_t n, double* vort, double* u_rms) {
+  if (!c || n < 0 || n > c->npfb) return VB_ERR_ARG;
+  vb_status s = launch_secondary_properties(c);
+  if (s != VB_OK) return s;
+  if (vort && n > 0)
+    VB_HIP(c, hipMemcpyAsync(vort, c->vort, (size_t)n * 8, hipMemcpyDeviceToHost, c->stream));
+  if ((s = sync_scalars(c)) != VB_OK) return s;
+  if (u_rms) *u_rms = c->scal_host->u_rms;
+  return VB_OK;
+}
+
+vb_status vb_field_maxima(vb_ctx* c, double* maxima) {
+  if (!c || !maxima) return VB_ERR_ARG;
+  vb_status s = launch_field_maxima(c);
+  if (s != VB_OK) return s;
+  if ((s = sync_scalars(c)) != VB_OK) return s;
+  for (int q = 0; q < 3; q++) maxima[q] = c->scal_host->fmax[q];
   return VB_OK;
 }
 

@@ -85,6 +85,8 @@ struct Scal {
   // mixed path: real particles binned in cells (the cell-ordered rows), largest tile window
   int nreal_cells, wmax;
   int nexplicit;                           // PPE rows with mirror neighbours (full stored rows)
+  // output quantities: RMS velocity (secondary_properties), max |u|, |v|, |p| over rows 1..npfb
+  double u_rms, fmax[3];
 };
 
 // Error bits accumulated on the device (vb_status codes as bit positions).

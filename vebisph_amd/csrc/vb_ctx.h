@@ -32,6 +32,7 @@ struct vb_ctx {
   double *usx, *usy;
   double* kick;                      // per-particle B perturbation (reference_kick)
   double *gpx, *gpy;
+  double* vort;                      // secondary_properties: vorticity of rows 0..npfb-1
 
   // PPE (ELL, slot-major) and BiCGSTAB vectors
   double *a_diag, *a_val, *a_rowsum, *rhs;
@@ -77,6 +78,8 @@ vb_status launch_predict(vb_ctx* c);
 vb_status launch_mirror_vel(vb_ctx* c);
 vb_status launch_pressure_update(vb_ctx* c);
 vb_status launch_final(vb_ctx* c);
+vb_status launch_secondary_properties(vb_ctx* c);   // vort, Scal::u_rms
+vb_status launch_field_maxima(vb_ctx* c);           // Scal::fmax
 // PPE (vb_ppe.hip)
 vb_status launch_ppe(vb_ctx* c, double tol, int itermax);
 vb_status upload_entry_table(vb_ctx* c);

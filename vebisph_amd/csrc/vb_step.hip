@@ -797,6 +797,66 @@ __global__ void k_final(Geom g, State st, const double* usx, const double* usy, 
   st.y[i] = st.y0[i] + 0.5 * (st.v[i] + st.v0[i]) * dt + usy[i] * dt;
 }
 
+// secondary_properties, div_free.F90:148-192: the vorticity of every row i <= npfb from the step's
+// velocities, neighbour list and w_G (no KGC), vort_i = -sum_k [(v_i - v_j) w_G1 - (u_i - u_j) w_G2]
+// in slot order (binned rows have no neighbours: 0), and the terms of sum_i u_i.u_i for the RMS
+// velocity (:171-181).  The reference runs it between the velocity update and the final advection,
+// which changes neither u nor the lists and weights, so it may run after the step.
+__global__ void k_vorticity(Geom g, State st, const int* cnt, const int* link, const double* wg1,
+                            const double* wg2, double* vort, double* partials, double* red) {
+  const int npfb = g.npfb;
+  RowRange rr = block_rows(npfb, gridDim.x);
+  Red<1> R(red, npfb);
+  for (int i = rr.begin + threadIdx.x; i < rr.end; i += blockDim.x) {
+    const double ui = st.u[i], vi = st.v[i];
+    double w = 0.0;
+    for_slots(link, i, cnt[i], npfb, [&](int j) { return make_double2(st.u[j], st.v[j]); },
+              [&](int s, int, double2 uj) {
+      const double t = (vi - uj.y) * wg1[s] - (ui - uj.x) * wg2[s];
+      w = w - t;
+    });
+    vort[i] = w;
+    R.add(0, i, ui * ui + vi * vi);                  // dot_product(u(i,:),u(i,:))
+  }
+  R.store(partials);
+}
+
+__global__ void k_fin_vorticity(const double* partials, int nblk, const double* red, int n, Scal* sc) {
+  const double s = fold(partials, 0, nblk, red, n);
+  if (threadIdx.x == 0) sc->u_rms = sqrt(s / (double)n);
+}
+
+// maxval(abs(u(1:npfb,1))), maxval(abs(u(1:npfb,2))), maxval(abs(P(1:npfb))) of output.F90:86
+// (max is order-independent: per-block maxima, then one workgroup)
+__global__ void k_field_max(State st, int npfb, double* partials) {
+  double m[3] = {0.0, 0.0, 0.0};
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < npfb; i += gridDim.x * kBlock) {
+    m[0] = fmax(m[0], fabs(st.u[i])); m[1] = fmax(m[1], fabs(st.v[i])); m[2] = fmax(m[2], fabs(st.p[i]));
+  }
+  __shared__ double sh[3][kBlock / 64];
+#pragma unroll
+  for (int q = 0; q < 3; q++) {
+    const double w = wave_max(m[q]);
+    if ((threadIdx.x & 63) == 0) sh[q][threadIdx.x >> 6] = w;
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    double b = sh[threadIdx.x][0];
+    for (int w = 1; w < kBlock / 64; w++) b = fmax(b, sh[threadIdx.x][w]);
+    partials[threadIdx.x * gridDim.x + blockIdx.x] = b;
+  }
+}
+
+__global__ void k_fin_field_max(const double* partials, int nblk, Scal* sc) {
+  double m[3] = {0.0, 0.0, 0.0};
+  for (int b = threadIdx.x; b < nblk; b += 64)
+    for (int q = 0; q < 3; q++) m[q] = fmax(m[q], partials[q * nblk + b]);
+  for (int q = 0; q < 3; q++) {
+    const double w = wave_max(m[q]);
+    if (threadIdx.x == 0) sc->fmax[q] = w;
+  }
+}
+
 // ================================================================================================
 // launchers
 // ================================================================================================
@@ -965,6 +1025,23 @@ vb_status launch_pressure_update(vb_ctx* c) {
 vb_status launch_final(vb_ctx* c) {
   int nf = c->npfb - c->nbS;
   if (nf > 0) VB_LAUNCH(c, k_final, grid_1d(nf), c->g, c->st, c->usx, c->usy, c->scal);
+  return VB_OK;
+}
+
+vb_status launch_secondary_properties(vb_ctx* c) {
+  const int nb = grid_for(c->npfb);
+  VB_LAUNCH(c, k_vorticity, nb, c->g, c->st, c->cnt, c->link, c->wg1, c->wg2, c->vort, c->partials, c->red);
+  hipLaunchKernelGGL(k_fin_vorticity, dim3(1), dim3(kBlock), 0, c->stream, c->partials, nb, c->red, c->npfb,
+                     c->scal);
+  VB_HIP(c, hipGetLastError());
+  return VB_OK;
+}
+
+vb_status launch_field_maxima(vb_ctx* c) {
+  const int nb = grid_for(c->npfb);
+  VB_LAUNCH(c, k_field_max, nb, c->st, c->npfb, c->partials);
+  hipLaunchKernelGGL(k_fin_field_max, dim3(1), dim3(64), 0, c->stream, c->partials, nb, c->scal);
+  VB_HIP(c, hipGetLastError());
   return VB_OK;
 }
 

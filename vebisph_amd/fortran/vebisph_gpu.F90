@@ -223,6 +223,21 @@ module vebisph_gpu
        integer(c_int) :: s
      end function vb_export_ppe
 
+     !! ---- output quantities of the driver ----
+     function vb_secondary_properties(ctx, n, vort, u_rms) bind(C, name="vb_secondary_properties") result(s)
+       import :: c_int, c_int32_t, c_double, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: n
+       real(c_double), intent(out) :: vort(*), u_rms
+       integer(c_int) :: s
+     end function vb_secondary_properties
+     function vb_field_maxima(ctx, maxima) bind(C, name="vb_field_maxima") result(s)
+       import :: c_int, c_double, c_ptr
+       type(c_ptr), value :: ctx
+       real(c_double), intent(out) :: maxima(3)
+       integer(c_int) :: s
+     end function vb_field_maxima
+
      !! ---- measurement ----
      function vb_profile_spmv(ctx, reps, avg_us, alg_bytes) bind(C, name="vb_profile_spmv") result(s)
        import :: c_int, c_int32_t, c_double, c_ptr

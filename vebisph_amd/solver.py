@@ -287,6 +287,20 @@ class Solver:
         self._chk(_lib.lib().vb_export_grad_p(self.h, n, _d(g)))
         return g.reshape(2, n).T
 
+    def secondary_properties(self):
+        """secondary_properties (div_free.F90:148-192): the vorticity of rows 1..npfb and the RMS
+        velocity, from the last step's velocities and weights."""
+        n = self.case.npfb
+        vort = np.zeros(n); u_rms = C.c_double(0.0)
+        self._chk(_lib.lib().vb_secondary_properties(self.h, n, _d(vort), C.byref(u_rms)))
+        return vort, u_rms.value
+
+    def field_maxima(self):
+        """max |u|, |v|, |p| over rows 1..npfb (the per-step line of data_directory/TIME)."""
+        m = np.zeros(3)
+        self._chk(_lib.lib().vb_field_maxima(self.h, _d(m)))
+        return m
+
     def profile_spmv(self, reps=20):
         """Mean duration (us) and algorithmic bytes of one BiCGSTAB SpMV launch."""
         us = C.c_double(0.0); by = C.c_double(0.0)
