@@ -8,9 +8,9 @@
 !!   every call      one step on the device, then dt, np, nmirror, n_inbin, LS_iters, LS_residual;
 !!   output steps    the particle state is copied back into common_2d when the driver will write
 !!                   it (time + dt > next_dump, output.F90:17, or the run ends), so the state
-!!                   crosses PCIe only for output.
-!!
-!! vort (secondary_properties, div_free.F90:148-192) is not computed on the device.
+!!                   crosses PCIe only for output; when time > next_dump2 the vorticity and RMS
+!!                   velocity of secondary_properties (div_free.F90:152-185) come from the device
+!!                   into common_2d's vort and unit 192, as the reference's step computes them.
 !! Build: add vebisph_amd/fortran/vebisph_gpu.F90 and this file to the reference's source list in
 !! place of div_free.F90 and link with -L<lib> -lvebisph_hip (see INTEGRATION.md of this package).
 module divfree
@@ -18,7 +18,8 @@ module divfree
   use kind_parameters
   use common_parameter
   use common_2d
-  use vebisph_gpu, only: vb_params, vb_boundary, vb_get_scalars, VB_PREC_FP64, VB_PREC_MIXED
+  use vebisph_gpu, only: vb_params, vb_boundary, vb_get_scalars, vb_secondary_properties, VB_PREC_FP64, &
+       VB_PREC_MIXED
   use divfree_gpu, only: gpu_div_free => div_free, vbf_init, vbf_upload, vbf_download, vbf_ctx, &
        vbf_status
   implicit none
@@ -38,7 +39,7 @@ contains
   subroutine div_free(x_mint, y_mint, x_maxt, y_maxt)
     real(rkind) :: x_mint, x_maxt, y_mint, y_maxt
     integer(c_int32_t) :: ivals(12)
-    real(c_double) :: dvals(8)
+    real(c_double) :: dvals(8), u_rms
     integer(c_int) :: s
 
     !! the flags div_free.F90:31-37 sets
@@ -54,6 +55,13 @@ contains
     s = vb_get_scalars(vbf_ctx, ivals, dvals)
     np = ivals(1); nmirror = ivals(4); maxneighbours = ivals(5); LS_iters = ivals(6); n_inbin = ivals(8)
     dt = dvals(1); LS_residual = dvals(2); umax = dvals(4)
+    !! secondary_properties, div_free.F90:152-185 (time is still the step's start time here)
+    if (time .gt. next_dump2 .or. output_everytime) then
+       next_dump2 = next_dump2 + dt_out
+       s = vb_secondary_properties(vbf_ctx, int(npfb, c_int32_t), vort, u_rms)
+       write(192,*) time, u_rms
+       flush(192)
+    end if
     if (time + dt > next_dump .or. time + dt > tmax) &
          call vbf_download(int(npfb, c_int32_t), r, u, theta, tau_p, trace_conf, p, inbin)
   end subroutine div_free
