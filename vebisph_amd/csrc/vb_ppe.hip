@@ -409,10 +409,13 @@ struct RedBufs {
 
 __device__ inline bool finished(const Scal* sc) { return sc->done || sc->exit_s; }
 
-// Rows a vector kernel loads before it folds the previous kernel's partials (1M rows on 1024 x 256
-// threads: 4 per thread), so that the fold's latency overlaps the vector loads instead of preceding
+// Rows a vector kernel loads before it folds the previous kernel's partials (1M rows on 512 x 256
+// threads: 8 per thread), so that the fold's latency overlaps the vector loads instead of preceding
 // them; rows beyond that are loaded after the fold.
-constexpr int kPre = 4;
+#ifndef VB_PRE
+#define VB_PRE 4
+#endif
+constexpr int kPre = VB_PRE;
 
 // :306-321  AX = A x (x = 0 for the PPE, so r = B); r_bar = r; the terms of the first rho (:330)
 // and of sum(r)
