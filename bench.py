@@ -81,6 +81,15 @@ def cpu_baseline(case, state, time0, itime0, threads, budget_s=20.0):
         return None
 
 
+def force_diff(force, other):
+    """max |F - F_other| / |F_other| over circles and components (None without both)."""
+    if not force or not other or not other.get("force_on_cylinder"):
+        return None
+    import numpy as np
+    a, b = np.array(force), np.array(other["force_on_cylinder"])
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
 def run(case, precision, steps, warmup):
     """Warmup, then exactly `steps` timed steps; the SpMV roofline figures after the timed region."""
     from vebisph_amd import _lib, solver
@@ -95,11 +104,12 @@ def run(case, precision, steps, warmup):
         s.div_free()                       # each step ends with a stream synchronisation
     t1 = time.perf_counter()
     sc = s.scalars()
+    force = s.circle_forces()[0].tolist() if case.nb_circles else None
     us, by = s.profile_spmv(reps=20)
     elapsed = t1 - t0
     res = {"value": case.npfb * steps / elapsed, "ms_per_step": 1e3 * elapsed / steps,
            "ls_iters_last_step": sc["LS_iters"], "ls_residual_last_step": sc["LS_residual"],
-           "spmv_us": us, "spmv_bytes": by, "spmv_GBs": by / (us * 1e-6) / 1e9}
+           "spmv_us": us, "spmv_bytes": by, "spmv_GBs": by / (us * 1e-6) / 1e9, "force": force}
     log(f"{precision}: {res['value']:.4g} particle-steps/s, {res['ms_per_step']:.1f} ms/step, "
         f"BiCGSTAB {sc['LS_iters']} iterations (residual {sc['LS_residual']:.3g}); "
         f"SpMV {us:.1f} us, {res['spmv_GBs']:.0f} GB/s")
@@ -139,7 +149,7 @@ def main():
         so.close()
         other = {"precision": oprec, "value": ores["value"], "ms_per_step": ores["ms_per_step"],
                  "ls_iters_last_step": ores["ls_iters_last_step"],
-                 "spmv_us": ores["spmv_us"], "spmv_GBs": ores["spmv_GBs"]}
+                 "spmv_us": ores["spmv_us"], "spmv_GBs": ores["spmv_GBs"], "force_on_cylinder": ores["force"]}
 
     cpu = None
     if not a.no_cpu_baseline:
@@ -172,6 +182,9 @@ def main():
                      "alg_bytes_per_launch": main_res["spmv_bytes"], "avg_us": main_res["spmv_us"],
                      "copy_GBs": copy_gbs},
         "compare": other,
+        # force of the fluid on the cylinder after the same steps in both precisions (vb_circle_forces)
+        "force_on_cylinder": main_res["force"],
+        "force_rel_diff_vs_other_precision": force_diff(main_res["force"], other),
         "ratio_vs_other_precision": (main_res["value"] / other["value"]) if other else None,
         "cpu_baseline": cpu,
     }

@@ -147,6 +147,22 @@ vb_status vb_secondary_properties(vb_ctx* ctx, int32_t n, double* vort, double* 
  * data_directory/TIME (output.F90:86), from the current state */
 vb_status vb_field_maxima(vb_ctx* ctx, double* maxima3);
 
+/* ---- run diagnostics (no counterpart in the reference, which computes neither; defined here and
+ * identically in the CPU restatement) ---- */
+/* Force of the fluid on each circle (the cylinder of the flow-past-a-cylinder runs):
+ * F = sum over the circle's boundary particles b of ds (sigma_b . n_b), sigma = -p I +
+ * c_s (G + G^T) + c_p tau, n_b the circle's outward unit normal at b, ds = 2 pi |R| / N_c, G the
+ * corrected velocity gradient of gradient_calculation.F90:62-71 over the step's neighbour list, and
+ * c_s, c_p the viscous and polymeric coefficients of predictor_mod.F90:54-56.  Boundary particles of
+ * circle ib are the wall rows at distance |R| (1e-8 relative) from its centre.  From the state after
+ * the velocity update: call after vb_pressure_update or vb_div_free.  force is (nc,2) row-major,
+ * counts(nc) the boundary particles found; nc must equal the context's circle count. */
+vb_status vb_circle_forces(vb_ctx* ctx, int32_t nc, double* force, int32_t* counts);
+/* Nusselt number of the heated-from-below runs, Nu = 1 + sqrt(Ra Pr) <v theta> H / dT, with <.> the
+ * mean over the fluid rows not in the bin, H the y extent of the boundary nodes and dT the largest
+ * minus the smallest wall-patch temperature (VB_ERR_ARG when that is 0). */
+vb_status vb_nusselt(vb_ctx* ctx, double* nu);
+
 /* ---- scalars and exports (for the host driver and parity checks; these synchronise) ---- */
 /* ints: np, npfb, nbS, nmirror, maxneighbours, LS_iters, itime, n_inbin, ncx, ncy, nct, status
  * dbls: dt, LS_residual, time, umax, x_mint, x_maxt, y_mint, y_maxt */

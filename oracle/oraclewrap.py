@@ -47,6 +47,9 @@ def lib():
         L.vbo_get_ppe.argtypes = [C.c_void_p, C.c_int, _d, _i, _d]
         L.vbo_get_grad_p.argtypes = [C.c_void_p, C.c_int, _d]
         L.vbo_secondary_properties.argtypes = [C.c_void_p, C.c_int, _d, C.POINTER(C.c_double)]
+        L.vbo_circle_forces.argtypes = [C.c_void_p, C.c_int, _d, _i]
+        L.vbo_nusselt.argtypes = [C.c_void_p]
+        L.vbo_nusselt.restype = C.c_double
         L.vbo_fac_wab.argtypes = [C.c_int, _d, C.c_double, C.c_double, _d, _d]
         L.vbo_calc_eigens.argtypes = [C.c_int, _d, _d, _d]
         L.vbo_logconf.argtypes = [C.c_int, _d, _d, _d, C.c_double, C.c_double, C.c_double, C.c_int, _d, _d]
@@ -176,6 +179,22 @@ class OracleSolver:
         vort = np.zeros(n); u_rms = C.c_double(0.0)
         lib().vbo_secondary_properties(self.h, n, vort, C.byref(u_rms))
         return vort, u_rms.value
+
+    def circle_forces(self):
+        """Force of the fluid on each circle (after phase 11), (nc, 2), and particles per circle."""
+        nc = self.case.nb_circles
+        f = np.zeros(2 * max(nc, 1)); cnt = np.zeros(max(nc, 1), np.int32)
+        lib().vbo_circle_forces(self.h, nc, f, cnt)
+        return f[:2 * nc].reshape(nc, 2), cnt[:nc]
+
+    def nusselt(self):
+        return lib().vbo_nusselt(self.h)
+
+    def set_rows(self, s, n):
+        """Rows 1..n of the state (n up to np: the mirror rows of the current step too)."""
+        lib().vbo_set_particles(self.h, n, _c(s["x"][:n]), _c(s["y"][:n]), _c(s["u"][:n]), _c(s["v"][:n]),
+                                _c(s["theta"][:n]), _c(s["tau"][:n]), _c(s["trace"][:n]), _c(s["p"][:n]),
+                                _c(s["inbin"][:n], np.int32))
 
 
 def fac_wab(q, ad_7, ad_7h):

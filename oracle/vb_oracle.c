@@ -1265,6 +1265,72 @@ void vbo_secondary_properties(const vbo_ctx* c, int n, double* vort, double* u_r
   *u_rms = sqrt(a / (double)npfb);
 }
 
+/* Run diagnostics the reference does not compute (the GPU library defines them identically).
+ * Force of the fluid on circle ib: sum over its boundary particles b (wall rows at distance |R| from
+ * the centre, 1e-8 relative) of ds (sigma_b . n_b), sigma = -p I + c_s (G + G^T) + c_p tau,
+ * n_b = (r_b - c)/|r_b - c|, ds = 2 pi |R| / N_c, G the corrected velocity gradient of
+ * gradient_calculation.F90:62-71 over the step's neighbour list, c_s and c_p as predictor_mod.F90:54-56.
+ * Valid after phase 11 (needs the lists and weights).  force is (nc,2) row-major. */
+void vbo_circle_forces(const vbo_ctx* c, int nc, double* force, int* counts) {
+  const int npfb = c->npfb;
+  const double cs = c->beta * sqrt(c->Pr / c->Ra);
+  const double cp = (1.0 - c->beta) * c->Pr / (c->Ra * c->El);
+  for (int ib = 0; ib < nc && ib < c->ncirc; ib++) {
+    const double cx = c->ccen[2 * ib], cy = c->ccen[2 * ib + 1], R = fabs(c->crad[ib]);
+    int n = 0;
+    for (int i = 0; i < c->nbS; i++) {
+      const double dx = c->x[i] - cx, dy = c->y[i] - cy;
+      if (fabs(sqrt(dx * dx + dy * dy) - R) <= 1.0e-8 * R) n++;
+    }
+    const double ds = n > 0 ? 2.0 * 3.14159265358979323846 * R / (double)n : 0.0;
+    double fx = 0.0, fy = 0.0;
+    for (int i = 0; i < c->nbS; i++) {
+      const double rx = c->x[i] - cx, ry = c->y[i] - cy;
+      if (!(fabs(sqrt(rx * rx + ry * ry) - R) <= 1.0e-8 * R)) continue;
+      double gux = 0.0, guy = 0.0, gvx = 0.0, gvy = 0.0;
+      for (int k = 0; k < c->cnt[i]; k++) {
+        const int s = k * npfb + i, j = c->link[s];
+        const double tu = c->u[j] - c->u[i], tv = c->v[j] - c->v[i];
+        gux = gux + tu * c->wg1[s]; guy = guy + tu * c->wg2[s];
+        gvx = gvx + tv * c->wg1[s]; gvy = gvy + tv * c->wg2[s];
+      }
+      const double dudx = c->k11[i] * gux + c->k12[i] * guy, dudy = c->k21[i] * gux + c->k22[i] * guy;
+      const double dvdx = c->k11[i] * gvx + c->k12[i] * gvy, dvdy = c->k21[i] * gvx + c->k22[i] * gvy;
+      const double rn = sqrt(rx * rx + ry * ry), nx = rx / rn, ny = ry / rn;
+      const double p = c->p[i];
+      const double sxx = -p + cs * (2.0 * dudx) + cp * c->t11[i];
+      const double sxy = cs * (dudy + dvdx) + cp * c->t12[i];
+      const double syx = cs * (dvdx + dudy) + cp * c->t21[i];
+      const double syy = -p + cs * (2.0 * dvdy) + cp * c->t22[i];
+      fx = fx + ds * (sxx * nx + sxy * ny);
+      fy = fy + ds * (syx * nx + syy * ny);
+    }
+    force[2 * ib] = fx; force[2 * ib + 1] = fy; counts[ib] = n;
+  }
+}
+
+/* Nusselt number, Nu = 1 + sqrt(Ra Pr) <v theta> H / dT over the fluid rows not in the bin, H the y
+ * extent of the boundary nodes, dT the largest minus the smallest wall-patch temperature. */
+double vbo_nusselt(const vbo_ctx* c) {
+  double ylo = c->bnode[1], yhi = ylo, tlo = 0.0, thi = 0.0;
+  int wall = 0;
+  for (int i = 0; i < c->nbp; i++) {
+    const double y = c->bnode[2 * i + 1];
+    ylo = y < ylo ? y : ylo; yhi = y > yhi ? y : yhi;
+    if (c->btype[i] != 1) continue;
+    if (!wall) { tlo = thi = c->bth[i]; wall = 1; }
+    tlo = c->bth[i] < tlo ? c->bth[i] : tlo; thi = c->bth[i] > thi ? c->bth[i] : thi;
+  }
+  if (!wall || thi - tlo <= 0.0) return 0.0;
+  double s = 0.0, n = 0.0;
+  for (int i = c->nbS; i < c->npfb; i++) {
+    if (c->inbin[i]) continue;
+    s = s + c->v[i] * c->th[i];
+    n = n + 1.0;
+  }
+  return n > 0.0 ? 1.0 + sqrt(c->Ra * c->Pr) * (yhi - ylo) / (thi - tlo) * (s / n) : 0.0;
+}
+
 /* nnf_models_mod.F90:8-110 (no callers in the reference; restated for completeness) */
 void vbo_nnf(int model, int n, const double* sr, double mu, double* out) {
   for (int i = 0; i < n; i++) {

@@ -63,6 +63,10 @@ void vbo_get_grad_p(const vbo_ctx* c, int n, double* gp_n2);
 /* secondary_properties (div_free.F90:148-192), valid after phase 11: vorticity of rows 1..n and
  * the RMS velocity */
 void vbo_secondary_properties(const vbo_ctx* c, int n, double* vort, double* u_rms);
+/* Run diagnostics (no reference counterpart; the GPU library defines them identically): force of the
+ * fluid on each circle (valid after phase 11; force (nc,2) row-major) and the Nusselt number. */
+void vbo_circle_forces(const vbo_ctx* c, int nc, double* force, int* counts);
+double vbo_nusselt(const vbo_ctx* c);
 
 /* Stand-alone pieces. */
 void vbo_fac_wab(int n, const double* q, double ad_7, double ad_7h, double* fac, double* wab);

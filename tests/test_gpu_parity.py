@@ -558,3 +558,33 @@ def test_vb_drive_restart_continues_the_run(gpu_lib, tmp_path):
     for key in ("x", "y", "u", "v", "theta", "p", "tau11", "tau12", "tau21", "tau22"):
         _eq(ca[key], cb[key])
     assert refio.read_table(str(b / "data_directory" / "LS"), 2).shape[0] == 2 * 5
+
+
+def test_run_diagnostics_match_oracle(gpu_lib):
+    """vb_circle_forces and vb_nusselt (no reference counterpart; the oracle's restatement is pinned
+    by closed forms in tests/test_oracle_math.py) against the oracle on the same state: after the
+    first step's phase 11 the neighbour lists and weights are identical, the GPU state (mirror rows
+    included) is copied into the oracle, and the two differ only in summation order.  The final
+    advection changes neither the force nor its inputs."""
+    case = SMALL["cylinder"]()
+    print(f"case {case.name} npfb={case.npfb}", flush=True)
+    G = solver.Solver(case); O = ow.OracleSolver(case)
+    for k in range(1, 12):
+        G.phase(k); O.phase(k)
+    n = G.scalars()["np"]
+    assert n == O.scalars()["np"]
+    O.set_rows(G.get_state(n), n)
+    fg, cg = G.circle_forces()
+    fo, co = O.circle_forces()
+    print(f"force on the cylinder: GPU {fg[0]}, oracle {fo[0]}, {cg[0]} boundary particles", flush=True)
+    _eq(cg, co)
+    assert np.abs(fg).max() > 0 and _rel(fg, fo) <= 1e-12
+    G.phase(12)
+    _eq(G.circle_forces()[0], fg)
+    case = SMALL["convection"]()
+    G = solver.Solver(case); O = ow.OracleSolver(case)
+    G.div_free(); G.div_free()
+    O.set_rows(G.get_state(case.npfb), case.npfb)
+    nug, nuo = G.nusselt(), O.nusselt()
+    print(f"Nusselt: GPU {nug!r}, oracle {nuo!r}", flush=True)
+    assert abs(nug - nuo) <= 1e-12 * abs(nuo)

@@ -236,3 +236,69 @@ def test_nnf_closed_forms():
     assert ow.nnf(3, sr, mu)[0] == mu and ow.nnf(4, sr, mu)[0] == pytest.approx(mu, rel=1e-15)
     cr = ow.nnf(3, sr, mu)
     assert cr[3] == pytest.approx(0.05 * mu + 0.95 * mu / (1.0 + 1000.0), rel=1e-14)
+
+
+# ---- run diagnostics (no reference counterpart): circle force and Nusselt number ----------------
+@pytest.fixture(scope="module")
+def cyl_after_update():
+    """A small cylinder case after one step's phases 1..11 (lists and weights in place)."""
+    c = cases.cylinder(nx=160, ny=80, name="cylinder_13k")
+    O = ow.OracleSolver(c)
+    for k in range(1, 12):
+        O.phase(k)
+    return c, O
+
+
+def _with_fields(c, O, p=None, u=None, v=None, tau=None):
+    n = O.scalars()["np"]
+    s = O.get_state(n)
+    x, y = s["x"], s["y"]
+    z = np.zeros(n)
+    s["p"] = z.copy() if p is None else p(x, y)
+    s["u"] = z.copy() if u is None else u(x, y)
+    s["v"] = z.copy() if v is None else v(x, y)
+    s["tau"] = np.zeros((n, 4)) if tau is None else tau(x, y)
+    O.set_rows(s, n)
+    return O.circle_forces()
+
+
+def test_circle_force_closed_forms(cyl_after_update):
+    """On the circle's N equally spaced boundary particles (angles 2 pi k / N), sum(n ds) = 0 and
+    sum(cos^2 ds) = pi R: p = a + b (x - cx) gives F = (-b pi R^2, 0); tau11 = b (x - cx) gives
+    +c_p b pi R^2 in x, tau12 = tau21 = b (y - cy) the same in x and in y; a uniform tau and a
+    velocity field linear everywhere (corrected gradient exact, so uniform viscous stress) give 0."""
+    c, O = cyl_after_update
+    (cx, cy), R = c.c_centre[0], abs(c.c_radius[0])
+    cp = (1.0 - c.beta) * c.Pr / (c.Ra * c.El)
+    area = math.pi * R * R
+    f, cnt = _with_fields(c, O, p=lambda x, y: 3.0 + 2.0 * (x - cx))
+    assert cnt[0] == math.ceil(2.0 * math.pi * R / c.dx)
+    assert abs(f[0, 0] - (-2.0 * area)) <= 1e-13 * 2.0 * area and abs(f[0, 1]) <= 1e-13 * area
+    def t11(x, y):
+        t = np.zeros((x.shape[0], 4)); t[:, 0] = 5.0 * (x - cx); return t
+    f, _ = _with_fields(c, O, tau=t11)
+    assert abs(f[0, 0] - cp * 5.0 * area) <= 1e-13 * cp * 5.0 * area and abs(f[0, 1]) <= 1e-13 * cp * area
+    def t12(x, y):
+        t = np.zeros((x.shape[0], 4)); t[:, 1] = t[:, 2] = 5.0 * (y - cy); return t
+    f, _ = _with_fields(c, O, tau=t12)
+    assert abs(f[0, 0] - cp * 5.0 * area) <= 1e-13 * cp * 5.0 * area
+    def tu(x, y):
+        t = np.zeros((x.shape[0], 4)); t[:, 0] = 1.5; t[:, 1] = t[:, 2] = -0.5; t[:, 3] = 2.0; return t
+    f, _ = _with_fields(c, O, tau=tu, p=lambda x, y: np.full_like(x, 7.0))
+    assert np.abs(f).max() <= 1e-13 * (7.0 + cp * 2.0) * 2.0 * math.pi * R
+    f, _ = _with_fields(c, O, u=lambda x, y: 0.3 * x - 1.1 * y, v=lambda x, y: 0.7 * x - 0.3 * y)
+    cs = c.beta * math.sqrt(c.Pr / c.Ra)
+    assert np.abs(f).max() <= 1e-9 * cs * 2.0 * math.pi * R
+
+
+def test_nusselt_closed_form():
+    """v theta = c on every fluid particle: Nu = 1 + sqrt(Ra Pr) c H / dT (H = 1, dT = 2 here; the
+    values are exact in binary, so the sum and mean are exact and so is Nu); the wall rows do not
+    count."""
+    c = cases.convection(nx=100, ny=50, name="convection_5k")
+    O = ow.OracleSolver(c)
+    s = O.get_state()
+    s["v"] = np.full(c.npfb, 0.25); s["theta"] = np.full(c.npfb, -0.5)
+    s["v"][:c.nbS] = 1.0e3
+    O.set_state(s)
+    assert O.nusselt() == 1.0 + math.sqrt(c.Ra * c.Pr) * (0.25 * -0.5) * 1.0 / 2.0

@@ -502,6 +502,46 @@ vb_status vb_secondary_properties(vb_ctx* c, int32_t n, double* vort, double* u_
   return VB_OK;
 }
 
+vb_status vb_circle_forces(vb_ctx* c, int32_t nc, double* force, int32_t* counts) {
+  if (!c || nc != c->g.ncirc || (nc > 0 && (!force || !counts))) {
+    if (c) c->err = "vb_circle_forces: nc must equal the context's circle count";
+    return VB_ERR_ARG;
+  }
+  if (nc == 0) return VB_OK;
+  vb_status s = launch_circle_forces(c);
+  if (s != VB_OK) return s;
+  if ((s = sync_scalars(c)) != VB_OK) return s;
+  for (int ib = 0; ib < nc; ib++) {
+    force[2 * ib] = c->scal_host->circ[3 * ib];
+    force[2 * ib + 1] = c->scal_host->circ[3 * ib + 1];
+    counts[ib] = (int32_t)c->scal_host->circ[3 * ib + 2];
+  }
+  return VB_OK;
+}
+
+vb_status vb_nusselt(vb_ctx* c, double* nu) {
+  if (!c || !nu) return VB_ERR_ARG;
+  const Geom& g = c->g;
+  double ylo = g.bnode[0][1], yhi = ylo, tlo = 0.0, thi = 0.0;
+  bool wall = false;
+  for (int i = 0; i < g.nbp; i++) {
+    ylo = std::min(ylo, g.bnode[i][1]); yhi = std::max(yhi, g.bnode[i][1]);
+    if (g.btype[i] != 1) continue;
+    if (!wall) { tlo = thi = g.bth[i]; wall = true; }
+    tlo = std::min(tlo, g.bth[i]); thi = std::max(thi, g.bth[i]);
+  }
+  if (!wall || thi - tlo <= 0.0) {
+    c->err = "vb_nusselt: the walls carry no temperature difference";
+    return VB_ERR_ARG;
+  }
+  const double scale = sqrt(c->prm.Ra * c->prm.Pr) * (yhi - ylo) / (thi - tlo);
+  vb_status s = launch_nusselt(c, scale);
+  if (s != VB_OK) return s;
+  if ((s = sync_scalars(c)) != VB_OK) return s;
+  *nu = c->scal_host->nusselt;
+  return VB_OK;
+}
+
 vb_status vb_field_maxima(vb_ctx* c, double* maxima) {
   if (!c || !maxima) return VB_ERR_ARG;
   vb_status s = launch_field_maxima(c);

@@ -88,6 +88,8 @@ struct Scal {
   int nexplicit;                           // PPE rows with mirror neighbours (full stored rows)
   // output quantities: RMS velocity (secondary_properties), max |u|, |v|, |p| over rows 1..npfb
   double u_rms, fmax[3];
+  // diagnostics: per circle Fx, Fy, boundary-particle count; the Nusselt number
+  double circ[3 * VB_MAX_CIRCLES], nusselt;
 };
 
 // Error bits accumulated on the device (vb_status codes as bit positions).

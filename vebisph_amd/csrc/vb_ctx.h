@@ -80,6 +80,8 @@ vb_status launch_pressure_update(vb_ctx* c);
 vb_status launch_final(vb_ctx* c);
 vb_status launch_secondary_properties(vb_ctx* c);   // vort, Scal::u_rms
 vb_status launch_field_maxima(vb_ctx* c);           // Scal::fmax
+vb_status launch_circle_forces(vb_ctx* c);          // Scal::circ
+vb_status launch_nusselt(vb_ctx* c, double scale);  // Scal::nusselt
 // PPE (vb_ppe.hip)
 vb_status launch_ppe(vb_ctx* c, double tol, int itermax);
 vb_status upload_entry_table(vb_ctx* c);

@@ -826,6 +826,94 @@ __global__ void k_fin_vorticity(const double* partials, int nblk, const double* 
   if (threadIdx.x == 0) sc->u_rms = sqrt(s / (double)n);
 }
 
+// Run diagnostics the reference does not compute (defined here and identically in the oracle).
+//
+// Force of the fluid on circle ib: F = sum over its boundary particles b (wall rows whose distance
+// from the centre is |R| to 1e-8 relative) of ds (sigma_b . n_b), with n_b = (r_b - c)/|r_b - c|,
+// ds = 2 pi |R| / N_c, sigma = -p I + c_s (G + G^T) + c_p tau, and G the corrected velocity gradient
+// K_b sum_k (u_j - u_b) w_G(b,k) (gradient_calculation.F90:62-71) over the step's neighbour list.
+// One workgroup, one circle at a time: each thread sums its particles in index order, then the
+// wavefronts and the workgroup fold in a fixed order.  out[3*ib + 0..2] = Fx, Fy, N_c.
+constexpr int kDiagThreads = 1024;
+
+__device__ inline bool on_circle(const Geom& g, const State& st, int i, int ib) {
+  const double dx = st.x[i] - g.ccen[ib][0], dy = st.y[i] - g.ccen[ib][1];
+  const double R = fabs(g.crad[ib]);
+  return fabs(sqrt(dx * dx + dy * dy) - R) <= 1.0e-8 * R;
+}
+
+__device__ inline double block_sum_fixed(double v, double* sh) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  v = wave_sum(v);
+  if (lane == 0) sh[wid] = v;
+  __syncthreads();
+  double s = sh[0];
+  for (int w = 1; w < nw; w++) s += sh[w];
+  __syncthreads();
+  return s;
+}
+
+__global__ void __launch_bounds__(kDiagThreads)
+k_circle_forces(Geom g, State st, const int* cnt, const int* link, const double* wg1, const double* wg2,
+                const double* k11, const double* k12, const double* k21, const double* k22, double cs,
+                double cp, double* out) {
+  __shared__ double sh[kDiagThreads / 64];
+  const int npfb = g.npfb;
+  for (int ib = 0; ib < g.ncirc; ib++) {
+    double nloc = 0.0;
+    for (int i = threadIdx.x; i < g.nbS; i += blockDim.x) nloc += on_circle(g, st, i, ib) ? 1.0 : 0.0;
+    const double nc = block_sum_fixed(nloc, sh);
+    const double ds = nc > 0.0 ? 2.0 * 3.14159265358979323846 * fabs(g.crad[ib]) / nc : 0.0;
+    double fx = 0.0, fy = 0.0;
+    for (int i = threadIdx.x; i < g.nbS; i += blockDim.x) {
+      if (!on_circle(g, st, i, ib)) continue;
+      const double ui = st.u[i], vi = st.v[i];
+      double gux = 0.0, guy = 0.0, gvx = 0.0, gvy = 0.0;
+      for (int k = 0; k < cnt[i]; k++) {
+        const int s = k * npfb + i, j = link[s];
+        const double tu = st.u[j] - ui, tv = st.v[j] - vi;
+        gux = gux + tu * wg1[s]; guy = guy + tu * wg2[s];
+        gvx = gvx + tv * wg1[s]; gvy = gvy + tv * wg2[s];
+      }
+      const double dudx = k11[i] * gux + k12[i] * guy, dudy = k21[i] * gux + k22[i] * guy;
+      const double dvdx = k11[i] * gvx + k12[i] * gvy, dvdy = k21[i] * gvx + k22[i] * gvy;
+      const double rx = st.x[i] - g.ccen[ib][0], ry = st.y[i] - g.ccen[ib][1];
+      const double rn = sqrt(rx * rx + ry * ry), nx = rx / rn, ny = ry / rn;
+      const double p = st.p[i];
+      const double sxx = -p + cs * (2.0 * dudx) + cp * st.t11[i];
+      const double sxy = cs * (dudy + dvdx) + cp * st.t12[i];
+      const double syx = cs * (dvdx + dudy) + cp * st.t21[i];
+      const double syy = -p + cs * (2.0 * dvdy) + cp * st.t22[i];
+      fx = fx + ds * (sxx * nx + sxy * ny);
+      fy = fy + ds * (syx * nx + syy * ny);
+    }
+    const double Fx = block_sum_fixed(fx, sh);
+    const double Fy = block_sum_fixed(fy, sh);
+    if (threadIdx.x == 0) { out[3 * ib] = Fx; out[3 * ib + 1] = Fy; out[3 * ib + 2] = nc; }
+  }
+}
+
+// Nusselt number of the heated-from-below runs: the terms of sum v theta and of the particle count
+// over the fluid rows not in the bin; Nu = 1 + scale * sum(v theta) / N with scale = sqrt(Ra Pr) H / dT.
+__global__ void k_nusselt_terms(Geom g, State st, double* partials, double* red) {
+  const int npfb = g.npfb;
+  RowRange rr = block_rows(npfb, gridDim.x);
+  Red<2> R(red, npfb);
+  for (int i = rr.begin + threadIdx.x; i < rr.end; i += blockDim.x) {
+    const bool fluid = i >= g.nbS && !st.inbin[i];
+    R.add(0, i, fluid ? st.v[i] * st.th[i] : 0.0);
+    R.add(1, i, fluid ? 1.0 : 0.0);
+  }
+  R.store(partials);
+}
+
+__global__ void k_fin_nusselt(const double* partials, int nblk, const double* red, int n, double scale,
+                              double* out) {
+  double f[2];
+  fold_n<2>(partials, nblk, red, n, f);
+  if (threadIdx.x == 0) out[0] = f[1] > 0.0 ? 1.0 + scale * (f[0] / f[1]) : 0.0;
+}
+
 // maxval(abs(u(1:npfb,1))), maxval(abs(u(1:npfb,2))), maxval(abs(P(1:npfb))) of output.F90:86
 // (max is order-independent: per-block maxima, then one workgroup)
 __global__ void k_field_max(State st, int npfb, double* partials) {
@@ -1033,6 +1121,28 @@ vb_status launch_secondary_properties(vb_ctx* c) {
   VB_LAUNCH(c, k_vorticity, nb, c->g, c->st, c->cnt, c->link, c->wg1, c->wg2, c->vort, c->partials, c->red);
   hipLaunchKernelGGL(k_fin_vorticity, dim3(1), dim3(kBlock), 0, c->stream, c->partials, nb, c->red, c->npfb,
                      c->scal);
+  VB_HIP(c, hipGetLastError());
+  return VB_OK;
+}
+
+// device address of a double field of the context's device scalars
+static double* scal_field(vb_ctx* c, size_t off) { return reinterpret_cast<double*>(reinterpret_cast<char*>(c->scal) + off); }
+
+vb_status launch_circle_forces(vb_ctx* c) {
+  const vb_params& p = c->prm;
+  const double cs = (p.beta) * sqrt(p.Pr / p.Ra);            // the coefficients of launch_predict
+  const double cp = (1.0 - p.beta) * p.Pr / (p.Ra * p.El);
+  hipLaunchKernelGGL(k_circle_forces, dim3(1), dim3(kDiagThreads), 0, c->stream, c->g, c->st, c->cnt, c->link,
+                     c->wg1, c->wg2, c->k11, c->k12, c->k21, c->k22, cs, cp, scal_field(c, offsetof(Scal, circ)));
+  VB_HIP(c, hipGetLastError());
+  return VB_OK;
+}
+
+vb_status launch_nusselt(vb_ctx* c, double scale) {
+  const int nb = grid_for(c->npfb);
+  VB_LAUNCH(c, k_nusselt_terms, nb, c->g, c->st, c->partials, c->red);
+  hipLaunchKernelGGL(k_fin_nusselt, dim3(1), dim3(kBlock), 0, c->stream, c->partials, nb, c->red, c->npfb, scale,
+                     scal_field(c, offsetof(Scal, nusselt)));
   VB_HIP(c, hipGetLastError());
   return VB_OK;
 }

@@ -238,6 +238,22 @@ module vebisph_gpu
        integer(c_int) :: s
      end function vb_field_maxima
 
+     !! ---- run diagnostics (no reference counterpart) ----
+     function vb_circle_forces(ctx, nc, force, counts) bind(C, name="vb_circle_forces") result(s)
+       import :: c_int, c_int32_t, c_double, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: nc
+       real(c_double), intent(out) :: force(*)
+       integer(c_int32_t), intent(out) :: counts(*)
+       integer(c_int) :: s
+     end function vb_circle_forces
+     function vb_nusselt(ctx, nu) bind(C, name="vb_nusselt") result(s)
+       import :: c_int, c_double, c_ptr
+       type(c_ptr), value :: ctx
+       real(c_double), intent(out) :: nu
+       integer(c_int) :: s
+     end function vb_nusselt
+
      !! ---- measurement ----
      function vb_profile_spmv(ctx, reps, avg_us, alg_bytes) bind(C, name="vb_profile_spmv") result(s)
        import :: c_int, c_int32_t, c_double, c_ptr

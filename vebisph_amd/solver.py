@@ -301,6 +301,18 @@ class Solver:
         self._chk(_lib.lib().vb_field_maxima(self.h, _d(m)))
         return m
 
+    def circle_forces(self):
+        """Force of the fluid on each circle, (nc, 2), and the boundary particles found on each."""
+        nc = self.case.nb_circles
+        f = np.zeros(2 * max(nc, 1)); cnt = np.zeros(max(nc, 1), np.int32)
+        self._chk(_lib.lib().vb_circle_forces(self.h, nc, _d(f), _i(cnt)))
+        return f[:2 * nc].reshape(nc, 2), cnt[:nc]
+
+    def nusselt(self):
+        nu = C.c_double(0.0)
+        self._chk(_lib.lib().vb_nusselt(self.h, C.byref(nu)))
+        return nu.value
+
     def profile_spmv(self, reps=20):
         """Mean duration (us) and algorithmic bytes of one BiCGSTAB SpMV launch."""
         us = C.c_double(0.0); by = C.c_double(0.0)
