@@ -624,10 +624,11 @@ __global__ void k_bi_true_residual(Mat A, Vecs w, RedBufs rb, const Scal* sc) {
 }
 
 // (with_sum: also the terms of sum(r), which the five-kernel iteration folds after x,r)
-__global__ void k_bi_true_shift(int n, Vecs w, RedBufs rb, const Scal* sc, int with_sum) {
+// (ntrue: workgroups of the pass-1 launch whose partials are folded)
+__global__ void k_bi_true_shift(int n, Vecs w, RedBufs rb, const Scal* sc, int with_sum, int ntrue) {
   if (finished(sc)) return;
   double f[2];
-  fold_n<2>(rb.P(ST_TRUE), gridDim.x, rb.R(ST_TRUE), n, f);
+  fold_n<2>(rb.P(ST_TRUE), ntrue, rb.R(ST_TRUE), n, f);
   const double d = (f[0] - f[1]) / (double)n;
   RowRange rr = block_rows(n, gridDim.x);
   Red<3> R(rb.R(ST_XR), n);
@@ -1133,8 +1134,8 @@ static vb_status solve(vb_ctx* c, const Mat& A, Vecs w, int x_given, double tol,
         VB_LAUNCH_BS(c, (k_bi_spmv_s<MODE, true>), ns, kSpmvBlock, A, w, rb, sc);
         VB_LAUNCH(c, k_fu_xr, nb, n, w, rb, sc, ns, k);
         if (refresh) {
-          VB_LAUNCH(c, k_bi_true_residual, nb, A, w, rb, sc);
-          VB_LAUNCH(c, k_bi_true_shift, nb, n, w, rb, sc, 1);
+          VB_LAUNCH_BS(c, k_bi_true_residual, ns, kSpmvBlock, A, w, rb, sc);   // one row per thread
+          VB_LAUNCH(c, k_bi_true_shift, nb, n, w, rb, sc, 1, ns);
         }
         continue;
       }
@@ -1146,8 +1147,8 @@ static vb_status solve(vb_ctx* c, const Mat& A, Vecs w, int x_given, double tol,
       VB_LAUNCH_BS(c, k_bi_spmv_s<MODE>, ns, kSpmvBlock, A, w, rb, sc);
       VB_LAUNCH(c, k_bi_xr, nb, n, w, rb, sc, ns);
       if (refresh) {
-        VB_LAUNCH(c, k_bi_true_residual, nb, A, w, rb, sc);
-        VB_LAUNCH(c, k_bi_true_shift, nb, n, w, rb, sc, 0);
+        VB_LAUNCH_BS(c, k_bi_true_residual, ns, kSpmvBlock, A, w, rb, sc);
+        VB_LAUNCH(c, k_bi_true_shift, nb, n, w, rb, sc, 0, ns);
       }
       VB_FIN(c, k_fin_xr, rb, nb, sc);
     }
@@ -1205,7 +1206,7 @@ static vb_status solve_tiled(vb_ctx* c, const TileMat& T, Vecs w, double tol, in
       VB_LAUNCH(c, k_bi_xr, nb, n, w, rb, sc, ns);
       if ((launched + b + 1) % kRefresh == 0) {
         VB_LAUNCH(c, k_tile_true_residual, nb, T, w, rb, sc);
-        VB_LAUNCH(c, k_bi_true_shift, nb, n, w, rb, sc, 0);
+        VB_LAUNCH(c, k_bi_true_shift, nb, n, w, rb, sc, 0, nb);
       }
       VB_FIN(c, k_fin_xr, rb, nb, sc);
     }
