@@ -55,6 +55,8 @@ def lib():
         L.vbo_logconf.argtypes = [C.c_int, _d, _d, _d, C.c_double, C.c_double, C.c_double, C.c_int, _d, _d]
         L.vbo_bicgstab_nonull.argtypes = [C.c_int, C.c_int, _d, _i, _d, _d, C.c_int, C.c_double,
                                           C.POINTER(C.c_int), C.POINTER(C.c_double)]
+        L.vbo_bicgstab_open.argtypes = L.vbo_bicgstab_nonull.argtypes
+        L.vbo_get_surface.argtypes = [C.c_void_p, C.c_int, _i, _d, _d, _d, _d]
         L.vbo_spmv_nr.argtypes = [C.c_int, _d, _i, _d, _d]
         L.vbo_nnf.argtypes = [C.c_int, C.c_int, _d, C.c_double, _d]
         _lib = L
@@ -190,6 +192,13 @@ class OracleSolver:
     def nusselt(self):
         return lib().vbo_nusselt(self.h)
 
+    def surface(self, n=None):
+        """n_surf, conc, div_r, smoothed surface normal (n, 2), ppe_smooth of rows 1..n (from phase 6)."""
+        n = self.case.npfb if n is None else n
+        ns = np.zeros(n, np.int32); co = np.zeros(n); dr = np.zeros(n); sn = np.zeros(2 * n); ps = np.zeros(n)
+        lib().vbo_get_surface(self.h, n, ns, co, dr, sn, ps)
+        return {"nsurf": ns, "conc": co, "divr": dr, "sn": sn.reshape(2, n).T, "psmooth": ps}
+
     def set_rows(self, s, n):
         """Rows 1..n of the state (n up to np: the mirror rows of the current step too)."""
         lib().vbo_set_particles(self.h, n, _c(s["x"][:n]), _c(s["y"][:n]), _c(s["u"][:n]), _c(s["v"][:n]),
@@ -223,6 +232,15 @@ def bicgstab_nonull(a, ija, b, x0=None, itermax=1000, tol=1e-8):
     it = C.c_int(0); res = C.c_double(0.0)
     st = lib().vbo_bicgstab_nonull(n, a.shape[0], _c(a), _c(ija, np.int32), x, _c(b), itermax, tol,
                                    C.byref(it), C.byref(res))
+    return x, it.value, res.value, st
+
+
+def bicgstab_open(a, ija, b, x0=None, itermax=1000, tol=1e-8):
+    n = b.shape[0]
+    x = _c(np.zeros(n) if x0 is None else x0).copy()
+    it = C.c_int(0); res = C.c_double(0.0)
+    st = lib().vbo_bicgstab_open(n, a.shape[0], _c(a), _c(ija, np.int32), x, _c(b), itermax, tol,
+                                 C.byref(it), C.byref(res))
     return x, it.value, res.value, st
 
 

@@ -119,6 +119,7 @@ contains
     allocate(n_surf(cap), div_r(cap))
     allocate(inbin(cap))
     r = 0.0d0; u = 0.0d0; p = 0.0d0; theta = 0.0d0; tau_p = 0.0d0; a_out = 0.0d0
+    conc = 0.0d0; div_r = 0.0d0
     n_surf(:) = 0_ikind
     n_free = 0
     allocate(p_free(cap))
@@ -191,6 +192,8 @@ contains
     if(allocated(r0)) deallocate(r0)
     if(allocated(u0)) deallocate(u0)
     if(allocated(h_grad_p)) deallocate(h_grad_p)
+    if(allocated(surf_norm)) deallocate(surf_norm)
+    if(allocated(ppe_smooth)) deallocate(ppe_smooth)
   end subroutine vbref_free
 
   !! Particle state in and out (1..n rows). tau is (n,4) = tau11, tau12, tau21, tau22.
@@ -377,6 +380,41 @@ contains
     x = xx; iters = it; resid = rs
     deallocate(aa, bb, xx, jj)
   end subroutine vbref_bicgstab_nonull
+
+  !! The open-domain BiCGSTAB (linear_solver.F90:82-227) on a caller-supplied row-indexed matrix.
+  subroutine vbref_bicgstab_open(n, nnz, a, ija_in, x, b, itermax, tol, iters, resid) &
+       bind(C, name="vbref_bicgstab_open")
+    integer(c_int), value :: n, nnz, itermax
+    real(c_double), value :: tol
+    real(c_double), intent(in) :: a(nnz), b(n)
+    integer(c_int), intent(in) :: ija_in(nnz)
+    real(c_double), intent(inout) :: x(n)
+    integer(c_int), intent(out) :: iters
+    real(c_double), intent(out) :: resid
+    real(rkind), allocatable :: aa(:), bb(:), xx(:)
+    integer :: jj(nnz), nn, nm, itm, it
+    real(rkind) :: rs, tl
+    allocate(aa(nnz), bb(n), xx(n))
+    aa = a; bb = b; xx = x; jj = ija_in
+    nn = n; nm = nnz; itm = itermax; tl = tol
+    rs = 0.0d0; it = 0
+    call SPARSE_PBICGSTAB(aa, jj, xx, bb, nn, nm, itm, tl, it, rs)
+    x = xx; iters = it; resid = rs
+    deallocate(aa, bb, xx)
+  end subroutine vbref_bicgstab_open
+
+  !! Free-surface quantities of rows 1..n after locate_free_surf_part (freesurf_mod.F90:17-68):
+  !! n_surf, conc, div_r, and in an open domain the smoothed surf_norm (n,2) and ppe_smooth while
+  !! they are allocated (zeros otherwise).
+  subroutine vbref_get_surface(n, ns, co, dr, sn, ps) bind(C, name="vbref_get_surface")
+    integer(c_int), value :: n
+    integer(c_int), intent(out) :: ns(n)
+    real(c_double), intent(out) :: co(n), dr(n), sn(n,2), ps(n)
+    ns(:) = n_surf(1:n); co(:) = conc(1:n); dr(:) = div_r(1:n)
+    sn = 0.0d0; ps = 0.0d0
+    if(allocated(surf_norm)) sn(:,:) = surf_norm(1:n,:)
+    if(allocated(ppe_smooth)) ps(:) = ppe_smooth(1:n)
+  end subroutine vbref_get_surface
 
   !! ---- the reference's own file reader and writer (input.F90, output.F90) ----
 

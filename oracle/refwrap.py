@@ -42,6 +42,8 @@ def lib():
         L.vbref_get_grad_p.argtypes = [C.c_int, _dp]
         L.vbref_bicgstab_nonull.argtypes = [C.c_int, C.c_int, _dp, _ip, _dp, _dp, C.c_int, C.c_double,
                                             C.POINTER(C.c_int), C.POINTER(C.c_double)]
+        L.vbref_bicgstab_open.argtypes = L.vbref_bicgstab_nonull.argtypes
+        L.vbref_get_surface.argtypes = [C.c_int, _ip, _dp, _dp, _dp, _dp]
         L.vbref_get_run_params.argtypes = [_ip, _dp]
         L.vbref_get_boundary.argtypes = [C.c_int, C.c_int, _dp, _dp, _ip, _dp, _dp, _ip, _dp, _dp, _dp, _dp, _dp]
         L.vbref_set_dumps.argtypes = [C.c_double, C.c_double, C.c_double]
@@ -154,6 +156,14 @@ class RefSolver:
         lib().vbref_get_grad_p(n, gp)
         return gp
 
+    def surface(self, n=None):
+        """n_surf, conc, div_r, smoothed surface normal (n, 2), ppe_smooth of rows 1..n (from phase 6)."""
+        n = self.case.npfb if n is None else n
+        ns = np.zeros(n, np.int32); co = np.zeros(n); dr = np.zeros(n)
+        sn = np.zeros((n, 2), order="F"); ps = np.zeros(n)
+        lib().vbref_get_surface(n, ns, co, dr, sn, ps)
+        return {"nsurf": ns, "conc": co, "divr": dr, "sn": sn, "psmooth": ps}
+
 
 def bicgstab_nonull(a, ija, b, x0=None, itermax=1000, tol=1e-8):
     """Reference SPARSE_PBICGSTAB_nonull on a 1-based row-indexed (Numerical Recipes) matrix."""
@@ -162,6 +172,16 @@ def bicgstab_nonull(a, ija, b, x0=None, itermax=1000, tol=1e-8):
     it = C.c_int(0); res = C.c_double(0.0)
     lib().vbref_bicgstab_nonull(n, a.shape[0], _f(a), _f(ija, np.int32), x, _f(b), itermax, tol,
                                 C.byref(it), C.byref(res))
+    return x, it.value, res.value
+
+
+def bicgstab_open(a, ija, b, x0=None, itermax=1000, tol=1e-8):
+    """Reference SPARSE_PBICGSTAB (the open-domain solve) on a 1-based row-indexed matrix."""
+    n = b.shape[0]
+    x = _f(np.zeros(n) if x0 is None else x0)
+    it = C.c_int(0); res = C.c_double(0.0)
+    lib().vbref_bicgstab_open(n, a.shape[0], _f(a), _f(ija, np.int32), x, _f(b), itermax, tol,
+                              C.byref(it), C.byref(res))
     return x, it.value, res.value
 
 

@@ -37,6 +37,7 @@ struct vbo_ctx {
   double *x, *y, *u, *v, *th, *t11, *t12, *t21, *t22, *trc, *p;
   unsigned char* inbin;
   int* nsurf;
+  double *conc, *divr;    /* open domain: calc_part_dist_vars (freesurf_mod.F90:70-103) */
   /* per-step data */
   double *x0, *y0, *u0, *v0;
   int np, nmirror, nmirror_esti, n_inbin, maxneigh, ls_iters, status;
@@ -49,6 +50,7 @@ struct vbo_ctx {
   double *wg1, *wg2, *wl; /* slot-major like link */
   double *k11, *k12, *k21, *k22;
   double *usx, *usy;
+  double *snx, *sny, *psmooth;   /* open domain: surf_norm (smoothed), ppe_smooth */
   double* gvel;           /* (npfb,4): du/dx du/dy dv/dx dv/dy */
   double *ppe_a, *ppe_b; int* ppe_ija; int ppe_nnz;
   double *gpx, *gpy;
@@ -170,6 +172,7 @@ vbo_ctx* vbo_create(const int* iprm, const double* prm, int nbp, const double* b
   c->t22 = (double*)xcalloc(n, 8); c->trc = (double*)xcalloc(n, 8);
   for (size_t i = 0; i < n; i++) c->trc[i] = 2.0;
   c->inbin = (unsigned char*)xcalloc(n, 1); c->nsurf = (int*)xcalloc(n, 4);
+  c->conc = (double*)xcalloc(n, 8); c->divr = (double*)xcalloc(n, 8);
   c->irel = (int*)xcalloc(n, 4); c->vrel = (int*)xcalloc(n, 4); c->dP = (double*)xcalloc(n, 8);
   c->kick = make_kick(c->npfb);
   return c;
@@ -182,6 +185,7 @@ static void free_step(vbo_ctx* c) {
   free(c->wg1); free(c->wg2); free(c->wl); c->wg1 = c->wg2 = c->wl = NULL;
   free(c->k11); free(c->k12); free(c->k21); free(c->k22); c->k11 = c->k12 = c->k21 = c->k22 = NULL;
   free(c->usx); free(c->usy); c->usx = c->usy = NULL;
+  free(c->snx); free(c->sny); free(c->psmooth); c->snx = c->sny = c->psmooth = NULL;
   free(c->gvel); c->gvel = NULL;
   free(c->ppe_a); free(c->ppe_b); free(c->ppe_ija); c->ppe_a = c->ppe_b = NULL; c->ppe_ija = NULL;
   free(c->gpx); free(c->gpy); c->gpx = c->gpy = NULL;
@@ -194,7 +198,7 @@ void vbo_destroy(vbo_ctx* c) {
   free(c->ccen); free(c->crad); free(c->comg); free(c->cvel); free(c->cth);
   free(c->x); free(c->y); free(c->u); free(c->v); free(c->th); free(c->p);
   free(c->t11); free(c->t12); free(c->t21); free(c->t22); free(c->trc);
-  free(c->inbin); free(c->nsurf); free(c->irel); free(c->vrel); free(c->dP); free(c->kick);
+  free(c->inbin); free(c->nsurf); free(c->conc); free(c->divr); free(c->irel); free(c->vrel); free(c->dP); free(c->kick);
   free(c);
 }
 
@@ -668,20 +672,98 @@ static void phase_weights(vbo_ctx* c) {
   }
 }
 
-/* Phase 6: locate_free_surf_part, closed-domain branch, freesurf_mod.F90:39-45 */
+/* Phase 6: locate_free_surf_part (freesurf_mod.F90:17-68).  Closed domain: n_surf = 0.  Open
+ * domain: calc_part_dist_vars (:70-103) gives the concentration, div r and the surface normal
+ * -h sum w_G; a particle is on the surface when div r <= 1.4 + 0.5 max(conc - 1, -0.2) (:27-38);
+ * mirrors take their parent's flag (:43-45); smooth_normals (:105-137) Shepard-filters the normals
+ * over the real neighbours; ppe_smooth (:54-64) ramps the PPE rows of the particles just inside the
+ * surface. */
 static void phase_freesurf(vbo_ctx* c) {
-  for (int i = 0; i < c->npfb; i++) c->nsurf[i] = 0;
-  for (int j = c->npfb; j < c->np; j++) c->nsurf[j] = c->nsurf[c->irel[j]];
+  const int npfb = c->npfb;
+  const int open = c->open_domain == 1;
+  double* bdr = NULL;
+  if (open) {
+    c->snx = (double*)xcalloc(npfb, 8); c->sny = (double*)xcalloc(npfb, 8);
+    c->psmooth = (double*)xcalloc(npfb, 8);
+    bdr = (double*)xcalloc(npfb, 8);
+    for (int i = 0; i < npfb; i++) {                             /* :85-97 */
+      double ct = 0.0, dr = 0.0, sx = 0.0, sy = 0.0;
+      for (int k = 0; k < c->cnt[i]; k++) {
+        const size_t s = (size_t)k * npfb + i;
+        const int j = c->link[s];
+        const double rx = c->x[i] - c->x[j], ry = c->y[i] - c->y[j];
+        const double rad = sqrt(rx * rx + ry * ry);
+        const double qq = rad / c->h;
+        ct = ct + wab(qq, c->ad7);
+        dr = dr - (rx * c->wg1[s] + ry * c->wg2[s]);
+        sx = sx - c->wg1[s]; sy = sy - c->wg2[s];
+      }
+      c->conc[i] = ct * c->dv; c->divr[i] = dr; c->snx[i] = sx; c->sny[i] = sy;
+    }
+    const double w0 = wab(0.0, c->ad7);                          /* :99-101 */
+    for (int i = 0; i < npfb; i++) c->conc[i] = c->conc[i] + c->dv * w0;
+    for (int i = 0; i < npfb; i++) { c->snx[i] = c->snx[i] * c->h; c->sny[i] = c->sny[i] * c->h; }
+    for (int i = 0; i < npfb; i++) {                             /* :29-37 */
+      bdr[i] = 1.4 + 0.5 * fmax(c->conc[i] - 1.0, -0.2);
+      c->nsurf[i] = (c->divr[i] > bdr[i]) ? 0 : 1;
+    }
+  } else {
+    for (int i = 0; i < npfb; i++) c->nsurf[i] = 0;
+  }
+  for (int j = npfb; j < c->np; j++) c->nsurf[j] = c->nsurf[c->irel[j]];
+  if (!open) return;
+  {                                                              /* smooth_normals, :114-135 */
+    const double w0 = wab(0.0, c->ad7);
+    double* tx = (double*)xcalloc(npfb, 8);
+    double* ty = (double*)xcalloc(npfb, 8);
+    for (int i = 0; i < npfb; i++) {
+      double sx = 0.0, sy = 0.0;
+      for (int k = 0; k < c->cnt[i]; k++) {
+        const int j = c->link[(size_t)k * npfb + i];
+        if (j >= npfb) continue;
+        const double rx = c->x[i] - c->x[j], ry = c->y[i] - c->y[j];
+        const double rad = sqrt(rx * rx + ry * ry);
+        const double w = wab(rad / c->h, c->ad7);
+        sx = sx + w * c->snx[j]; sy = sy + w * c->sny[j];
+      }
+      tx[i] = sx * c->dv + c->snx[i] * w0 * c->dv;
+      ty[i] = sy * c->dv + c->sny[i] * w0 * c->dv;
+    }
+    memcpy(c->snx, tx, 8 * (size_t)npfb); memcpy(c->sny, ty, 8 * (size_t)npfb);
+    free(tx); free(ty);
+  }
+  for (int i = 0; i < npfb; i++) {                               /* ppe_smooth, :55-64 */
+    double ps = 1.0;
+    if (!(c->divr[i] > 1.2 * bdr[i])) {
+      if (c->nsurf[i] == 1) ps = 0.0;
+      else ps = 0.5 * (1.0 - cos((c->divr[i] - bdr[i]) * PI / (0.2 * bdr[i])));
+    }
+    c->psmooth[i] = ps;
+  }
+  free(bdr);
 }
 
-/* Phase 7: new_fick_shift + shifting_forces (new_fick_shift.F90:15-92, 117-210) with
- * i_shift_coeff = 0, closed domain; ushift as div_free.F90:65-78 */
+/* Phase 7: new_fick_shift + shifting_forces (new_fick_shift.F90:15-92, 117-210; akinci off) and
+ * ushift as div_free.F90:65-78.  Shifting coefficient D: i_shift_coeff 0, 0.25 h^2 (:44);
+ * i_shift_coeff 1, 4 h dt (|u| + 0.2 exp(-|u|/(0.2 maxU)) maxU) (:46-50), whose REAL(4) literals 0.2
+ * are the float 0.2 widened.  maxU (:31-37) assigns rather than maximises inside an OpenMP max
+ * reduction, so it is max(1e-10, the speed of each thread's last particle): with one thread
+ * max(1e-10, |u(npfb)|), which is what is computed here.  Open-domain surface particles shift along
+ * the surface only, dr = -D (t . grad C) t with t the rotated unit smoothed normal
+ * n = sn / max(|sn|, 1e-8) (REAL(4) 1e-8 widened) (:60-73). */
 static void phase_shift(vbo_ctx* c) {
   const int npfb = c->npfb;
   const double nt = 4.0, Rt = 0.25;
   const double wab1 = wab(c->dx / c->h, c->ad7);
+  const double f02 = (double)0.2f, small_n = (double)1.0e-8f;
   c->usx = (double*)xcalloc(npfb, 8); c->usy = (double*)xcalloc(npfb, 8);
-  const double D = 0.25 * c->h * c->h;
+  double maxU = 1.0e-10;
+  if (npfb > 0) {
+    const int l = npfb - 1;
+    const double t = sqrt(c->u[l] * c->u[l] + c->v[l] * c->v[l]);
+    if (t > maxU) maxU = t;
+  }
+  const int open = c->open_domain == 1;
   for (int i = 0; i < npfb; i++) {
     double sfx = 0.0, sfy = 0.0, stx = 0.0, sty = 0.0;
     for (int k = 0; k < c->cnt[i]; k++) {
@@ -697,7 +779,24 @@ static void phase_shift(vbo_ctx* c) {
     double gcx = sfx + stx, gcy = sfy + sty;
     double drx = 0.0, dry = 0.0;
     if (i >= c->nbS) {
-      drx = -D * gcx; dry = -D * gcy;
+      double D;
+      if (c->shift_coeff == 1) {
+        const double absU = sqrt(c->u[i] * c->u[i] + c->v[i] * c->v[i]);
+        const double temp = f02 * exp(-absU / (f02 * maxU));
+        D = 4.0 * c->h * c->dt * (absU + temp * maxU);
+      } else {
+        D = 0.25 * c->h * c->h;
+      }
+      if (open && c->nsurf[i] == 1) {
+        const double tn = sqrt(c->snx[i] * c->snx[i] + c->sny[i] * c->sny[i]);
+        const double den = tn > small_n ? tn : small_n;
+        const double nx = c->snx[i] / den, ny = c->sny[i] / den;
+        const double tx = -ny, ty = nx;
+        const double dCds = tx * gcx + ty * gcy;
+        drx = -D * dCds * tx; dry = -D * dCds * ty;
+      } else {
+        drx = -D * gcx; dry = -D * gcy;
+      }
       double t = sqrt(drx * drx + dry * dry);
       if (t > 0.1 * c->h) { drx = 0.1 * c->h * drx / t; dry = 0.1 * c->h * dry / t; }
     }
@@ -1014,12 +1113,19 @@ static void remove_dc(int n, double* a) {                 /* :231-250 */
   for (int i = 0; i < n; i++) a[i] = a[i] - t;
 }
 
-int vbo_bicgstab_nonull(int n, int nnz, const double* A, const int* ija, double* x,
-                        const double* B, int itermax, double tol, int* iters, double* resid) {
-  (void)nnz;                                                /* :275-425 */
+/* The reference's two BiCGSTABs, linear_solver.F90:82-227 (open) and :275-425 (nonull).  They differ
+ * in three places: SPARSE_PBICGSTAB preconditions with Jacobi, M = A(i) (:63-75), and removes no
+ * mean; SPARSE_PBICGSTAB_nonull uses M = 1 (:318) and projects the constant out of p and s
+ * (remove_dc, :351, :383); and the first-iteration norm thresholds are 10-6 -- an integer expression
+ * equal to 4 -- in the former (:171, :207) and 1.0d-6 in the latter (:367, :404). */
+static int bicgstab(int n, const double* A, const int* ija, double* x, const double* B, int itermax,
+                    double tol, int* iters, double* resid, int open) {
   double *r = (double*)xcalloc(n, 8), *rb = (double*)xcalloc(n, 8), *v = (double*)xcalloc(n, 8);
   double *ax = (double*)xcalloc(n, 8), *p = (double*)xcalloc(n, 8), *pinv = (double*)xcalloc(n, 8);
   double *s = (double*)xcalloc(n, 8), *sinv = (double*)xcalloc(n, 8), *t = (double*)xcalloc(n, 8);
+  double* M = (double*)xcalloc(n, 8);
+  for (int i = 0; i < n; i++) M[i] = open ? A[i] : 1.0;
+  const double small = open ? (double)(10 - 6) : 1.0e-6;
   double temp = 0.0, rho0 = 1.0, alpha = 1.0, omega = 1.0, rho = 0.0, beta;
   int status = VBO_OK;
   vbo_spmv_nr(n, A, ija, x, ax);
@@ -1035,21 +1141,21 @@ int vbo_bicgstab_nonull(int n, int nnz, const double* A, const int* ija, double*
       beta = rho / (rho0 + 1.0e-15) * alpha / (omega + 1.0e-15);
       for (int i = 0; i < n; i++) p[i] = r[i] + beta * (p[i] - omega * v[i]);
     }
-    remove_dc(n, p);
-    for (int i = 0; i < n; i++) pinv[i] = p[i] / (1.0 + 1.0e-15);
+    if (!open) remove_dc(n, p);
+    for (int i = 0; i < n; i++) pinv[i] = p[i] / (M[i] + 1.0e-15);
     vbo_spmv_nr(n, A, ija, pinv, v);
     temp = dotp(n, rb, v);
     alpha = rho / (temp + 1.0e-15);
     for (int i = 0; i < n; i++) s[i] = r[i] - alpha * v[i];
     n2s = dotp(n, s, s);
-    if (iter == 1) n2s0 = (n2s <= 1.0e-6) ? 1.0 : n2s;
+    if (iter == 1) n2s0 = (n2s <= small) ? 1.0 : n2s;
     if (n2s <= tol * tol * n2s0) {
       for (int i = 0; i < n; i++) x[i] = x[i] + alpha * pinv[i];
       *resid = sqrt(n2s / n2s0);
       break;
     }
-    remove_dc(n, s);
-    for (int i = 0; i < n; i++) sinv[i] = s[i] / (1.0 + 1.0e-15);
+    if (!open) remove_dc(n, s);
+    for (int i = 0; i < n; i++) sinv[i] = s[i] / (M[i] + 1.0e-15);
     vbo_spmv_nr(n, A, ija, sinv, t);
     temp = dotp(n, t, s);
     n2t = dotp(n, t, t);
@@ -1057,18 +1163,30 @@ int vbo_bicgstab_nonull(int n, int nnz, const double* A, const int* ija, double*
     for (int i = 0; i < n; i++) x[i] = x[i] + alpha * pinv[i] + omega * sinv[i];
     for (int i = 0; i < n; i++) r[i] = s[i] - omega * t[i];
     n2r = dotp(n, r, r);
-    if (iter == 1) n2r0 = (n2r <= 1.0e-6) ? 1.0 : n2r;
+    if (iter == 1) n2r0 = (n2r <= small) ? 1.0 : n2r;
     *resid = sqrt(n2r / n2r0);
     if (n2r <= tol * tol * n2r0) break;
     iter = iter + 1;
     rho0 = rho;
   }
   *iters = iter;
-  free(r); free(rb); free(v); free(ax); free(p); free(pinv); free(s); free(sinv); free(t);
+  free(r); free(rb); free(v); free(ax); free(p); free(pinv); free(s); free(sinv); free(t); free(M);
   return status;
 }
 
-/* Phase 10: ppe_solve_in_house_simple_dirichlet.F90:1-158 (closed domain) */
+int vbo_bicgstab_nonull(int n, int nnz, const double* A, const int* ija, double* x,
+                        const double* B, int itermax, double tol, int* iters, double* resid) {
+  (void)nnz;
+  return bicgstab(n, A, ija, x, B, itermax, tol, iters, resid, 0);
+}
+
+int vbo_bicgstab_open(int n, int nnz, const double* A, const int* ija, double* x,
+                      const double* B, int itermax, double tol, int* iters, double* resid) {
+  (void)nnz;
+  return bicgstab(n, A, ija, x, B, itermax, tol, iters, resid, 1);
+}
+
+/* Phase 10: ppe_solve_in_house_simple_dirichlet.F90:1-158 */
 static int phase_ppe(vbo_ctx* c) {
   const int npfb = c->npfb;
   double* B = (double*)xcalloc(npfb, 8);
@@ -1130,11 +1248,20 @@ static int phase_ppe(vbo_ctx* c) {
   }
   ija[npfb] = ia[npfb];
   free(ia); free(ijnum);
+  if (c->open_domain == 1) {                                /* PPE smoothing, :126-138 */
+    for (int i = 0; i < npfb; i++) {
+      if (c->nsurf[i] == 1) continue;
+      if (c->psmooth[i] == 1.0) continue;
+      B[i] = B[i] * c->psmooth[i];
+      for (int e = ija[i]; e <= ija[i + 1] - 1; e++) A[e - 1] = A[e - 1] * c->psmooth[i];
+    }
+  }
   c->ppe_a = (double*)xcalloc(nnz, 8); c->ppe_ija = (int*)xcalloc(nnz, 4); c->ppe_b = (double*)xcalloc(npfb, 8);
   memcpy(c->ppe_a, A, 8 * (size_t)nnz); memcpy(c->ppe_ija, ija, 4 * (size_t)nnz); memcpy(c->ppe_b, B, 8 * (size_t)npfb);
   c->ppe_nnz = nnz;
   for (int i = 0; i < npfb; i++) c->p[i] = 0.0;             /* :143-152 */
-  int st = vbo_bicgstab_nonull(npfb, nnz, A, ija, c->p, B, 1000, 1.0e-8, &c->ls_iters, &c->ls_resid);
+  int st = bicgstab(npfb, A, ija, c->p, B, 1000, 1.0e-8, &c->ls_iters, &c->ls_resid,   /* :148-152 */
+                    c->open_domain == 1);
   free(A); free(ija); free(B);
   return st;
 }
@@ -1239,6 +1366,16 @@ int vbo_get_ppe_nnz(const vbo_ctx* c) { return c->ppe_nnz; }
 void vbo_get_ppe(const vbo_ctx* c, int nnz, double* a, int* ija, double* b) {
   memcpy(a, c->ppe_a, 8 * (size_t)nnz); memcpy(ija, c->ppe_ija, 4 * (size_t)nnz);
   memcpy(b, c->ppe_b, 8 * (size_t)c->npfb);
+}
+
+void vbo_get_surface(const vbo_ctx* c, int n, int* nsurf, double* conc, double* divr, double* sn,
+                     double* psmooth) {
+  for (int i = 0; i < n; i++) {
+    const int in = i < c->npfb;
+    nsurf[i] = c->nsurf[i]; conc[i] = c->conc[i]; divr[i] = c->divr[i];
+    sn[i] = (in && c->snx) ? c->snx[i] : 0.0; sn[n + i] = (in && c->sny) ? c->sny[i] : 0.0;
+    psmooth[i] = (in && c->psmooth) ? c->psmooth[i] : 0.0;
+  }
 }
 
 void vbo_get_grad_p(const vbo_ctx* c, int n, double* gp) {

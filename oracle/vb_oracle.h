@@ -67,6 +67,11 @@ void vbo_secondary_properties(const vbo_ctx* c, int n, double* vort, double* u_r
  * fluid on each circle (valid after phase 11; force (nc,2) row-major) and the Nusselt number. */
 void vbo_circle_forces(const vbo_ctx* c, int nc, double* force, int* counts);
 double vbo_nusselt(const vbo_ctx* c);
+/* Free-surface quantities of rows 1..n (locate_free_surf_part, freesurf_mod.F90:17-68), valid from
+ * phase 6: n_surf, conc, div_r; in an open domain also the smoothed surface normal (n,2) and
+ * ppe_smooth until phase 12 (zeros otherwise). */
+void vbo_get_surface(const vbo_ctx* c, int n, int* nsurf, double* conc, double* divr, double* sn,
+                     double* psmooth);
 
 /* Stand-alone pieces. */
 void vbo_fac_wab(int n, const double* q, double ad_7, double ad_7h, double* fac, double* wab);
@@ -75,6 +80,9 @@ void vbo_logconf(int n, const double* tau_n4, const double* trc, const double* g
                  double dt, double wi, double eps_ptt, int use_kick, double* tau_out_n4, double* trc_out);
 int  vbo_bicgstab_nonull(int n, int nnz, const double* a, const int* ija, double* x,
                          const double* b, int itermax, double tol, int* iters, double* resid);
+/* SPARSE_PBICGSTAB (linear_solver.F90:82-227), the open-domain solve: Jacobi, no DC projection */
+int  vbo_bicgstab_open(int n, int nnz, const double* a, const int* ija, double* x,
+                       const double* b, int itermax, double tol, int* iters, double* resid);
 void vbo_spmv_nr(int n, const double* a, const int* ija, const double* x, double* y);
 /* nnf_models_mod.F90: model 0 newtonian(mu array), 1 power law, 2 bingham, 3 cross, 4 carreau */
 void vbo_nnf(int model, int n, const double* sr, double mu, double* mu_eff);

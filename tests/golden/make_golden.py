@@ -5,8 +5,8 @@ small seeded cases.
     python tests/golden/make_golden.py
 
 For every phase it records the SHA-256 of each array the phase produces (mirror relations and
-offsets, neighbour lists, kernel weights and KGC, shifting velocity, particle state, pressure
-gradient) and the step scalars; the final state is stored whole.  tests/test_golden.py checks the C
+offsets, neighbour lists, kernel weights and KGC, free-surface flags and quantities, shifting
+velocity, particle state, pressure gradient) and the step scalars; the final state is stored whole.  tests/test_golden.py checks the C
 restatement against these files, so the oracle stays pinned where the reference is absent.
 """
 import hashlib
@@ -29,6 +29,9 @@ CASES = {
     # which the reference writes past the end of ij_link (heap corruption); the GPU path returns
     # VB_ERR_CAPACITY there instead
     "cylinder_8k": (lambda: cases.cylinder(nx=128, ny=64, name="cylinder_8k"), 2),
+    # open domains: free surface, ramp rows (jitter 0.2 dx), Jacobi BiCGSTAB, surface shifting
+    "fs_convection_11k": (lambda: cases.free_surface_convection(jitter=0.2), 2),
+    "dam_break_3k": (lambda: cases.dam_break(jitter=0.2, shift_coeff=1), 2),
 }
 FINAL_STATE = ("poiseuille_2500", "convection_1250")     # whole final state kept (small files)
 
@@ -57,6 +60,8 @@ def phase_record(S, k):
         w = sc["maxneighbours"]
         rec["maxneighbours"] = w
         rec["weights"] = [digest(a) for a in S.weights(w)]
+    if k == 6:
+        rec["surface"] = {key: digest(v) for key, v in S.surface().items()}
     if k == 7:
         rec["ushift"] = digest(S.ushift())
     if k in (8, 9, 10, 11):

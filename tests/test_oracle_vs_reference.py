@@ -36,6 +36,10 @@ def _run_phases(case, steps):
             if k == 4:
                 cr, lr = R.neighbours(); co, lo = O.neighbours()
                 _same(cr, co); _same(lr, lo)
+            if k == 6:
+                a, b = R.surface(), O.surface()
+                for key in ("nsurf", "conc", "divr", "sn", "psmooth"):
+                    _same(a[key], b[key])
             if k == 5:
                 w = sr["maxneighbours"]
                 assert w == so["maxneighbours"]
@@ -69,6 +73,26 @@ def test_convection_phases_bit_exact():
 
 def test_cylinder_phases_bit_exact():
     _run_phases(cases.cylinder(nx=160, ny=80, name="cylinder_13k"), steps=2)
+
+
+def test_free_surface_convection_phases_bit_exact():
+    """Open domain (generator case 6): free-surface detection, normal smoothing, the PPE ramp rows,
+    Dirichlet surface rows, the Jacobi BiCGSTAB and the velocity-scaled surface shifting.  Jitter
+    0.2 dx puts ~40 rows on the ramp (0 < ppe_smooth < 1)."""
+    c = cases.free_surface_convection(jitter=0.2)
+    R, O = _pair(c)
+    R.phase(1); O.phase(1)
+    for k in range(2, 7):
+        R.phase(k); O.phase(k)
+    s = O.surface()
+    assert s["nsurf"].sum() > 100 and ((s["psmooth"] > 0) & (s["psmooth"] < 1)).sum() > 20
+    _run_phases(c, steps=3)
+
+
+def test_dam_break_phases_bit_exact():
+    """Open domain in a closed box of four walls (walls facing the empty box are surface rows),
+    velocity-scaled shifting."""
+    _run_phases(cases.dam_break(jitter=0.2, shift_coeff=1), steps=3)
 
 
 def test_whole_step_equals_phases():
@@ -129,3 +153,29 @@ def test_secondary_properties_bit_exact(tmp_path, monkeypatch, name):
     assert rows.shape == (2, 2)
     assert urms_o == rows[1, 1] and rows[1, 0] == R.scalars()["time"] - R.scalars()["dt"]
     assert np.abs(vort_o).max() > 0
+
+
+def test_reference_open_bicgstab_on_oracle_matrix():
+    """linear_solver.F90's SPARSE_PBICGSTAB (Jacobi, no DC projection, its integer 10-6 threshold),
+    called directly, on the oracle's assembled open-domain PPE (Dirichlet rows with 1e30 diagonals,
+    ramp rows scaled)."""
+    from oracle import oraclewrap, refwrap
+    c = cases.free_surface_convection(jitter=0.2)
+    refwrap.RefSolver(c)
+    O = oraclewrap.OracleSolver(c)
+    O.step()
+    for k in range(1, 11):
+        O.phase(k)
+    a, ija, b = O.ppe()
+    assert a.max() == 1.0e30
+    xr, itr, rr = refwrap.bicgstab_open(a, ija, b)
+    xo, ito, ro, st = oraclewrap.bicgstab_open(a, ija, b)
+    assert itr == ito and rr == ro
+    _same(xr, xo)
+    # a right-hand side with |b|^2 ~ 3.1, under the threshold 4: the first-iteration norms are then
+    # replaced by 1 (a 1.0d-6 threshold would keep them), which changes the returned residual
+    # sqrt(|r|^2 / norm2_r0) and the convergence test, both compared here
+    xr2, itr2, rr2 = refwrap.bicgstab_open(a, ija, b * 1e-3)
+    xo2, ito2, ro2, st = oraclewrap.bicgstab_open(a, ija, b * 1e-3)
+    assert itr2 == ito2 and rr2 == ro2
+    _same(xr2, xo2)

@@ -1,4 +1,4 @@
-"""Seeded synthetic inputs for the five benchmark workloads.
+"""Seeded synthetic inputs for the benchmark workloads and the open-domain (free-surface) cases.
 
 The reference reads its initial state from IPART/INDAT/IBOUND files written by its case generator
 (source/gen/datclass.F90).  This module builds the same kind of state in memory:
@@ -195,6 +195,75 @@ def channel_case(name, L, H, nx, ny, seed, *, Ra, Pr, El, beta, body_force=(0.0,
     return c
 
 
+def _f32(v: float) -> float:
+    """A REAL(4) literal of the generator widened to double (datclass.F90 mixes them into FP64)."""
+    return float(np.float32(v))
+
+
+def free_surface_convection(nx=150, L=1.0, H=1.0, depth=0.5, seed=6, Ra=2.0e4, Pr=1.0, El=0.01,
+                            beta=0.5, eps_ptt=0.1, steps=1, jitter=0.01, name="fs_convection_11k") -> Case:
+    """Free-surface Rayleigh-Benard flow (the generator's case 6, datclass.F90:433-488): a layer of
+    depth `depth` on a hot wall (theta = 1) in [0,L]x[0,H], periodic in x (patch types 1,2,0,2, the
+    top patch open), gravity (0,-1), open domain with the velocity-scaled shifting coefficient
+    (i_shift_coeff 1).  theta = 1 below y = 0.2 depth + 0.1 depth sin(2 pi x / L), else 0 (the
+    generator's REAL(4) constants)."""
+    dx = L / nx
+    h = 1.3 * dx
+    b_node = np.array([[0.0, 0.0], [L, 0.0], [L, H], [0.0, H]])
+    b_type = np.array([1, 2, 0, 2], dtype=np.int32)
+    b_vel = np.zeros(4)
+    b_thermal = np.array([1.0, 0.0, -1.0, 0.0])
+    b_pp = np.array([3, 4, 1, 2], dtype=np.int32)
+    bx, by, bu, bv, bt = _boundary_particles(b_node, b_type, b_vel, b_thermal, dx)
+    ny = int(math.ceil(depth / dx - 0.5))
+    while (ny - 0.5) * dx >= depth:
+        ny -= 1
+    ix, iy = np.meshgrid(np.arange(nx), np.arange(ny), indexing="ij")   # x outer, y inner
+    fx = (ix.ravel() + 0.5) * dx
+    fy = (iy.ravel() + 0.5) * dx
+    nf = fx.shape[0]
+    rnd = splitmix64_uniform(seed, 2 * nf).reshape(nf, 2)
+    fx = fx + (2.0 * rnd[:, 0] - 1.0) * (jitter * dx)
+    fy = fy + (2.0 * rnd[:, 1] - 1.0) * (jitter * dx)
+    edge = _f32(0.2) * depth + _f32(0.1) * depth * np.sin(_f32(2.0) * math.pi * fx / L)
+    ft = np.where(fy <= edge, 1.0, 0.0)
+    return Case(name=name, h=h, dx=dx, dv=dx * dx, Ra=Ra, Pr=Pr, El=El, beta=beta,
+                body_force=(0.0, -1.0), ve_nonlinearity=eps_ptt, i_open_domain=1, i_shift_coeff=1,
+                steps=steps, b_node=b_node, b_type=b_type, b_vel=b_vel, b_thermal=b_thermal,
+                b_periodic_parent=b_pp, nbS=bx.shape[0], x=np.concatenate([bx, fx]),
+                y=np.concatenate([by, fy]), u=np.concatenate([bu, np.zeros(nf)]),
+                v=np.concatenate([bv, np.zeros(nf)]), theta=np.concatenate([bt, np.zeros(nf) + ft]))
+
+
+def dam_break(ncol=40, h0=0.25, seed=7, Ra=1.0e4, Pr=1.0, El=0.1, beta=0.5, eps_ptt=0.1,
+              shift_coeff=0, steps=1, jitter=0.01, name="dam_break_3k") -> Case:
+    """A viscoelastic dam break: the geometry of the generator's case 1 (datclass.F90:56-121) -- a
+    column [0,h0]x[0,2 h0] collapsing under gravity (0,-1) in the closed box [0,1]^2 (four walls),
+    dx = h0/ncol -- with a polymeric phase (El > 0, beta < 1; the generator's case is Newtonian)."""
+    dx = h0 / ncol
+    h = 1.3 * dx
+    b_node = np.array([[0.0, 0.0], [1.0, 0.0], [1.0, 1.0], [0.0, 1.0]])
+    b_type = np.array([1, 1, 1, 1], dtype=np.int32)
+    b_vel = np.zeros(4)
+    b_thermal = np.zeros(4)
+    b_pp = np.zeros(4, dtype=np.int32)
+    bx, by, bu, bv, bt = _boundary_particles(b_node, b_type, b_vel, b_thermal, dx)
+    ix, iy = np.meshgrid(np.arange(ncol), np.arange(2 * ncol), indexing="ij")
+    fx = (ix.ravel() + 0.5) * dx
+    fy = (iy.ravel() + 0.5) * dx
+    nf = fx.shape[0]
+    rnd = splitmix64_uniform(seed, 2 * nf).reshape(nf, 2)
+    fx = fx + (2.0 * rnd[:, 0] - 1.0) * (jitter * dx)
+    fy = fy + (2.0 * rnd[:, 1] - 1.0) * (jitter * dx)
+    return Case(name=name, h=h, dx=dx, dv=dx * dx, Ra=Ra, Pr=Pr, El=El, beta=beta,
+                body_force=(0.0, -1.0), ve_nonlinearity=eps_ptt, i_open_domain=1,
+                i_shift_coeff=shift_coeff, steps=steps, b_node=b_node, b_type=b_type, b_vel=b_vel,
+                b_thermal=b_thermal, b_periodic_parent=b_pp, nbS=bx.shape[0],
+                x=np.concatenate([bx, fx]), y=np.concatenate([by, fy]),
+                u=np.concatenate([bu, np.zeros(nf)]), v=np.concatenate([bv, np.zeros(nf)]),
+                theta=np.concatenate([bt, np.zeros(nf)]))
+
+
 def poiseuille(nx=50, ny=50, L=1.0, H=1.0, seed=1, Wi=1.0, beta=0.1, Re=0.1, steps=1000, name="poiseuille_2500"):
     """Oldroyd-B start-up Poiseuille flow, body force (8,0), theta = 0 (config 1; config 4 shape)."""
     Ra = Re * Re          # Pr = 1, Re = sqrt(Ra/Pr)
@@ -223,6 +292,9 @@ CONFIGS = {
                                           name="poiseuille_250k"),
     "convection_1M": lambda: convection(nx=1414, ny=707, seed=5, Ra=5e4, Wi=1.0, beta=0.5, steps=500,
                                         name="convection_1M"),
+    "fs_convection_11k": lambda: free_surface_convection(),
+    "fs_convection_1M": lambda: free_surface_convection(nx=1414, seed=8, name="fs_convection_1M"),
+    "dam_break_3k": lambda: dam_break(),
 }
 
 

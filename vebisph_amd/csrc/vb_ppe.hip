@@ -1270,8 +1270,8 @@ vb_status upload_entry_table(vb_ctx* c) {
 }
 
 vb_status launch_ppe(vb_ctx* c, double tol, int itermax) {
-  const int nb = grid_for(c->npfb);
-  VB_LAUNCH(c, k_ppe_rhs, nb, c->g, c->st, c->cnt, c->link, c->wg1, c->wg2, c->k11, c->k12, c->k21,
+  const int nb = spmv_grid(c->npfb);    // a neighbour sum: one row per thread, like the SpMV
+  VB_LAUNCH_BS(c, k_ppe_rhs, nb, kSpmvBlock, c->g, c->st, c->cnt, c->link, c->wg1, c->wg2, c->k11, c->k12, c->k21,
             c->k22, c->rhs, c->partials, c->red, c->scal);
   VB_FIN(c, k_fin_ppe_rhs, c->partials, nb, c->red, c->npfb, c->scal);
   VB_LAUNCH(c, k_assemble, grid_1d(c->npfb), c->g, c->st, c->cnt, c->link, c->wl, c->nsurf, c->rhs,

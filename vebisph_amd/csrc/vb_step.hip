@@ -1117,8 +1117,8 @@ vb_status launch_final(vb_ctx* c) {
 }
 
 vb_status launch_secondary_properties(vb_ctx* c) {
-  const int nb = grid_for(c->npfb);
-  VB_LAUNCH(c, k_vorticity, nb, c->g, c->st, c->cnt, c->link, c->wg1, c->wg2, c->vort, c->partials, c->red);
+  const int nb = spmv_grid(c->npfb);    // a neighbour sum: one row per thread
+  VB_LAUNCH_BS(c, k_vorticity, nb, kSpmvBlock, c->g, c->st, c->cnt, c->link, c->wg1, c->wg2, c->vort, c->partials, c->red);
   hipLaunchKernelGGL(k_fin_vorticity, dim3(1), dim3(kBlock), 0, c->stream, c->partials, nb, c->red, c->npfb,
                      c->scal);
   VB_HIP(c, hipGetLastError());
