@@ -52,8 +52,10 @@ typedef struct {
   double ve_nonlinearity;        /* eps of linear PTT; 0 = Oldroyd-B */
   double time;                   /* start time */
   int32_t itime;                 /* steps already taken */
-  int32_t i_open_domain;         /* must be 0 (closed domain); the open-domain branch is not built */
-  int32_t i_shift_coeff;         /* must be 0 (D = 0.25 h^2) */
+  int32_t i_open_domain;         /* 0 closed domain; 1 open domain with a free surface
+                                    (freesurf_mod.F90, Dirichlet surface rows, Jacobi BiCGSTAB) */
+  int32_t i_shift_coeff;         /* 0: D = 0.25 h^2; 1: D = 4 h dt (|u| + 0.2 exp(-|u|/(0.2 maxU)) maxU)
+                                    (new_fick_shift.F90:43-51) */
   int32_t precision;             /* vb_precision */
   int32_t reference_kick;        /* 1: reproduce predictor_mod.F90:369-371 as the reference runs it */
   int32_t serial_reductions;     /* 1: global sums in index order on one thread, as the reference's
@@ -106,11 +108,15 @@ vb_status vb_put_in_bin(vb_ctx* ctx);
 vb_status vb_create_mirror_particles(vb_ctx* ctx);
 /* particle_allocation: divide + neighboring_list_parallel, neighbour_finding_mod.F90:15-127,261-365 */
 vb_status vb_particle_allocation(vb_ctx* ctx);
-/* kernel_calculation.F90:2-75 */
+/* kernel_calculation.F90:2-75; in an open domain the same pass also takes calc_part_dist_vars
+ * (freesurf_mod.F90:70-103), the free-surface criterion (:27-38) and ppe_smooth (:54-64) */
 vb_status vb_kernel_calculation(vb_ctx* ctx);
-/* locate_free_surf_part, closed-domain branch, freesurf_mod.F90:39-45 */
+/* locate_free_surf_part, freesurf_mod.F90:17-68: n_surf = 0 (closed domain), or the mirrors take
+ * their parent's flag (:43-45); smooth_normals (:105-137) is folded into vb_new_fick_shift, the only
+ * user of the smoothed normals */
 vb_status vb_locate_free_surf_part(vb_ctx* ctx);
-/* new_fick_shift.F90:15-92,117-210 and ushift, div_free.F90:65-78 */
+/* new_fick_shift.F90:15-92,117-210 and ushift, div_free.F90:65-78 (open-domain surface particles
+ * shift along the surface only, :60-73) */
 vb_status vb_new_fick_shift(vb_ctx* ctx);
 /* prediction_step, predictor_mod.F90:27-139 (log-conformation :141-331, mirror stress
  * mirror_boundaries_mod.F90:586-653) */
@@ -118,7 +124,9 @@ vb_status vb_prediction_step(vb_ctx* ctx);
 /* mirror_velocities, mirror_boundaries_mod.F90:509-584 */
 vb_status vb_mirror_velocities(vb_ctx* ctx);
 /* ppe_solve_in_house_simple_dirichlet.F90:1-158 with SPARSE_PBICGSTAB_nonull
- * (linear_solver.F90:275-425).  tol <= 0 / itermax <= 0 select the reference's 1e-8 / 1000.
+ * (linear_solver.F90:275-425), or in an open domain (i_open_domain = 1) with the Dirichlet surface
+ * rows, ppe_smooth-scaled rows near the surface and SPARSE_PBICGSTAB (:82-227).
+ * tol <= 0 / itermax <= 0 select the reference's 1e-8 / 1000.
  * iters and residual may be NULL; when given, the call synchronises to return them. */
 vb_status vb_ppe_solve(vb_ctx* ctx, double tol, int32_t itermax, int32_t* iters, double* residual);
 /* SPARSE_PBICGSTAB_nonull (linear_solver.F90:275-285) on a caller's system, same argument meaning:
@@ -129,6 +137,12 @@ vb_status vb_ppe_solve(vb_ctx* ctx, double tol, int32_t itermax, int32_t* iters,
 vb_status vb_sparse_pbicgstab_nonull(vb_ctx* ctx, const double* A, const int32_t* ija, double* x,
                                      const double* B, int32_t n, int32_t nmax, int32_t itermax,
                                      double tol, int32_t* iter, double* residual);
+/* SPARSE_PBICGSTAB (linear_solver.F90:82-227), the open-domain solver, on a caller's system with the
+ * arguments of vb_sparse_pbicgstab_nonull: Jacobi preconditioner M = A(1:n), no mean removal, and
+ * the reference's first-iteration norm threshold 10-6 (an integer expression, 4). */
+vb_status vb_sparse_pbicgstab(vb_ctx* ctx, const double* A, const int32_t* ija, double* x, const double* B,
+                              int32_t n, int32_t nmax, int32_t itermax, double tol, int32_t* iter,
+                              double* residual);
 /* calc_pressure_gradient (gradient_calculation.F90:16-48) and div_free.F90:93-102 */
 vb_status vb_pressure_update(vb_ctx* ctx);
 /* div_free.F90:111-117 final advection; advances time */
@@ -143,6 +157,11 @@ vb_status vb_div_free(vb_ctx* ctx);
  * computes them between the velocity update and the final advection, which changes neither, so
  * call this after vb_pressure_update or vb_div_free.  vort and u_rms may be NULL. */
 vb_status vb_secondary_properties(vb_ctx* ctx, int32_t n, double* vort, double* u_rms);
+/* The columns conc and n_surf of the PART files (output.F90:47-49) for rows 1..n, n <= np: the
+ * free-surface flag and, in an open domain, the concentration (calc_part_dist_vars,
+ * freesurf_mod.F90:95-100) of the last step (rows past npfb: mirror flags, conc 0; conc is 0 in a
+ * closed domain, where the reference never sets it).  Either pointer may be NULL. */
+vb_status vb_free_surface(vb_ctx* ctx, int32_t n, int32_t* n_surf, double* conc);
 /* maxval(abs(u(1:npfb,1))), maxval(abs(u(1:npfb,2))), maxval(abs(P(1:npfb))): the per-step line of
  * data_directory/TIME (output.F90:86), from the current state */
 vb_status vb_field_maxima(vb_ctx* ctx, double* maxima3);

@@ -1,0 +1,221 @@
+"""gfx950 open-domain (free-surface) path against the oracle, whose open-domain restatement is
+bit-exact against the reference (tests/test_oracle_vs_reference.py, tests/test_golden.py).
+
+Cases: the generator's free-surface Rayleigh-Benard flow (case 6: periodic in x, one wall,
+i_shift_coeff 1) and a viscoelastic dam break (four walls; the walls facing the empty part of the
+box are surface rows too), both with 0.2 dx jitter so that ~40 rows sit on the ppe_smooth ramp.
+
+Tolerances, stated where used:
+  * neighbour lists, surface flags (n_surf), PPE column structure: exact;
+  * concentration and the shifting of the first step: exact up to pow() ulps (device library vs
+    libm, as in the closed domain);
+  * PPE values and right-hand side of the first step: exact except the ppe_smooth scale of the ramp
+    rows, 0.5 (1 - cos(...)), whose cos() may differ by an ulp: elementwise relative 1e-14 (the
+    Dirichlet diagonal is 1e30, so the comparison is elementwise, not relative to the maximum);
+  * the BiCGSTAB iterates differ by the rounding of the parallel dot products: the pressure is held
+    to relative 1e-3 as in the closed domain, positions to the FP64 field bar 1e-10.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from vebisph_amd import cases, solver
+from oracle import oraclewrap as ow
+
+pytestmark = pytest.mark.gpu
+
+OPEN = {
+    "fs_convection": lambda: cases.free_surface_convection(jitter=0.2),
+    "dam_break": lambda: cases.dam_break(jitter=0.2, shift_coeff=1),
+}
+
+
+def _eq(a, b):
+    a = np.asarray(a); b = np.asarray(b)
+    assert a.shape == b.shape
+    assert np.array_equal(a, b), f"{np.count_nonzero(a != b)} of {a.size} differ"
+
+
+def _rel(a, b):
+    return np.max(np.abs(np.asarray(a) - np.asarray(b))) / max(np.max(np.abs(b)), 1e-300)
+
+
+def _elem(a, b):
+    """max elementwise relative difference (exact zeros must match)"""
+    a = np.asarray(a, dtype=np.float64); b = np.asarray(b, dtype=np.float64)
+    d = np.abs(a - b)
+    nz = b != 0
+    assert np.all(d[~nz] == 0.0)
+    return float(np.max(d[nz] / np.abs(b[nz]))) if nz.any() else 0.0
+
+
+def _open_system(case, steps=1):
+    """The oracle's assembled open-domain PPE after `steps` full steps."""
+    O = ow.OracleSolver(case)
+    for _ in range(steps):
+        O.step()
+    for k in range(1, 11):
+        O.phase(k)
+    return O.ppe()
+
+
+@pytest.mark.parametrize("name", list(OPEN))
+def test_open_step_phases_against_oracle(gpu_lib, name):
+    case = OPEN[name]()
+    print(f"case {case.name} npfb={case.npfb}", flush=True)
+    G = solver.Solver(case); O = ow.OracleSolver(case)
+    for step in range(2):
+        first = step == 0
+        for k in range(1, 13):
+            G.phase(k); O.phase(k)
+            sg, so = G.scalars(), O.scalars()
+            if k == 4:
+                cg, lg = G.neighbours(); co, lo = O.neighbours()
+                _eq(cg, co); _eq(lg, lo)
+            if k == 6:
+                n = so["np"]
+                ns_g, conc_g = G.free_surface(n)
+                so_ = O.surface(n)
+                _eq(ns_g, so_["nsurf"])
+                m = case.npfb
+                if first:
+                    _eq(conc_g[:m], so_["conc"][:m])
+                else:
+                    assert _rel(conc_g[:m], so_["conc"][:m]) <= 1e-12
+                ramp = int(((so_["psmooth"] > 0) & (so_["psmooth"] < 1)).sum())
+                print(f"step {step + 1}: {int(so_['nsurf'][:m].sum())} surface rows, {ramp} ramp rows",
+                      flush=True)
+                assert so_["nsurf"][:m].sum() > 100 and ramp > 10
+            if k == 7:
+                assert _rel(G.ushift(), O.ushift()) <= (1e-13 if first else 1e-8)
+            if k == 10:
+                w = so["maxneighbours"]
+                diag, col, val, rhs = G.ppe_ell(w)
+                cnt = G.neighbours()[0]
+                a_g, ija_g = solver.ell_to_nr(diag, col, val, cnt)
+                a_o, ija_o, b_o = O.ppe()
+                _eq(ija_g, ija_o)
+                assert np.count_nonzero(a_o == 1.0e30) == int(so_["nsurf"][:case.npfb].sum())
+                if first:
+                    assert _elem(a_g, a_o) <= 1e-14
+                    assert _elem(rhs, b_o) <= 1e-14
+                else:
+                    assert _rel(rhs, b_o) <= 1e-6
+                assert sg["LS_residual"] <= 1e-8 and 1 <= sg["LS_iters"] <= 1000
+                assert _rel(G.get_state()["p"], O.get_state()["p"]) <= 1e-3
+            if k == 12:
+                a, b = G.get_state(), O.get_state()
+                assert _rel(a["x"], b["x"]) <= 1e-10 and _rel(a["y"], b["y"]) <= 1e-10
+                assert _rel(a["u"], b["u"]) <= 1e-5 and _rel(a["v"], b["v"]) <= 1e-5
+
+
+@pytest.mark.parametrize("name", list(OPEN))
+def test_open_bicgstab_bit_exact_in_serial_order(gpu_lib, name):
+    """SPARSE_PBICGSTAB (Jacobi on the 1e30 Dirichlet diagonals, no DC projection, threshold 4) on
+    the oracle's assembled open-domain PPE, global sums in index order: the oracle's iterate bit for
+    bit, and the same iteration count and residual."""
+    case = OPEN[name]()
+    a, ija, b = _open_system(case)
+    xo, ito, ro, sto = ow.bicgstab_open(a, ija, b)
+    G = solver.Solver(case, serial_reductions=True)
+    xg, itg, rg, stg = G.sparse_pbicgstab_nonull(a, ija, b, open_domain=True)
+    assert (itg, stg) == (ito, sto)
+    assert rg == ro
+    _eq(xg, xo)
+    # a right-hand side under the first-iteration threshold 4, and a nonzero initial guess
+    x0 = np.linspace(-1e-6, 1e-6, case.npfb)
+    for bb, guess in ((b * 1e-3, None), (b, x0)):
+        xo2, ito2, ro2, _ = ow.bicgstab_open(a, ija, bb, x0=guess, itermax=60)
+        xg2, itg2, rg2, _ = G.sparse_pbicgstab_nonull(a, ija, bb, x0=guess, itermax=60, open_domain=True)
+        assert itg2 == ito2 and rg2 == ro2
+        _eq(xg2, xo2)
+
+
+def test_open_bicgstab_parallel_iterates_follow_reference(gpu_lib):
+    """The five-kernel parallel-order iteration with the Jacobi preconditioner: stopped after k
+    iterations it lands where the reference algorithm does after k, to the reference's own one-ulp
+    sensitivity (as for the closed domain's solver)."""
+    case = OPEN["fs_convection"]()
+    a, ija, b = _open_system(case)
+    G = solver.Solver(case)
+    b_ulp = np.nextafter(b, np.inf)
+    for itermax in (1, 2, 8, 9, 17, 33):
+        xo, ito, ro, sto = ow.bicgstab_open(a, ija, b, itermax=itermax)
+        xg, itg, rg, stg = G.sparse_pbicgstab_nonull(a, ija, b, itermax=itermax, open_domain=True)
+        xs, _, rs, _ = ow.bicgstab_open(a, ija, b_ulp, itermax=itermax)
+        sx, sr = _rel(xs, xo), abs(rs - ro) / ro
+        print(f"itermax {itermax}: iters {itg}/{ito} resid {rg:.6e}/{ro:.6e} dx {_rel(xg, xo):.2e} "
+              f"(one-ulp spread {sx:.2e})", flush=True)
+        assert itg == ito
+        assert stg == (6 if itg > itermax else sto)
+        assert _rel(xg, xo) <= max(100.0 * sx, 1e-13)
+        assert abs(rg - ro) / ro <= max(100.0 * sr, 1e-13)
+    # to convergence: the reference criterion, and the oracle's solution
+    xo, ito, ro, _ = ow.bicgstab_open(a, ija, b)
+    xg, itg, rg, stg = G.sparse_pbicgstab_nonull(a, ija, b, open_domain=True)
+    assert stg == 0 and rg <= 1e-8
+    assert _rel(xg, xo) <= 1e-3
+
+
+def _perturbed_state(case):
+    """The initial state with every coordinate moved by one ulp."""
+    n = case.npfb
+    return dict(x=np.nextafter(case.x, np.inf), y=np.nextafter(case.y, np.inf), u=case.u, v=case.v,
+                theta=case.theta, tau=np.zeros((n, 4)), trace=np.full(n, 2.0), p=np.zeros(n),
+                inbin=np.zeros(n, np.int32))
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("name", list(OPEN))
+def test_open_multi_step_within_reference_sensitivity(gpu_lib, name, precision):
+    """Ten steps against the oracle, held to the oracle's own one-ulp spread (as for the closed
+    domain), in FP64 and with FP32 matvecs (the surface and ramp rows use the full FP64 rows)."""
+    from vebisph_amd import _lib
+    case = OPEN[name]()
+    print(f"case {case.name} {precision}", flush=True)
+    O = ow.OracleSolver(case); O2 = ow.OracleSolver(case, state=_perturbed_state(case))
+    G = solver.Solver(case, precision=_lib.PREC_MIXED if precision == "mixed" else _lib.PREC_FP64)
+    for _ in range(10):
+        O.step(); O2.step(); G.div_free()
+        assert G.scalars()["status"] in (0, 6)
+    a, b, b2 = G.get_state(), O.get_state(), O2.get_state()
+    for key in ("x", "y", "u", "v", "theta", "p", "tau", "trace"):
+        spread = _rel(b2[key], b[key])
+        err = _rel(a[key], b[key])
+        print(f"{key}: gpu-vs-oracle {err:.2e}, oracle one-ulp spread {spread:.2e}", flush=True)
+        assert err <= max(10.0 * spread, 1e-12), key
+    ns, conc = G.free_surface()
+    s = O.surface()
+    assert np.count_nonzero(ns != s["nsurf"][:case.npfb]) <= 2
+    assert _rel(conc, s["conc"][:case.npfb]) <= 1e-6
+
+
+def _drive(run_dir, *args):
+    import subprocess
+    exe = os.path.join(os.path.dirname(solver.__file__), "vb_drive")
+    print(f"vb_drive run {run_dir} {' '.join(args)}", flush=True)
+    r = subprocess.run([exe, "run", str(run_dir), *args], capture_output=True, text=True, timeout=120)
+    print(r.stdout[-2000:], r.stderr[-2000:], flush=True)
+    assert r.returncode == 0
+
+
+def test_vb_drive_open_domain_part_columns(gpu_lib, tmp_path):
+    """vb_drive run on an open-domain run directory: the PART files (PART<k-1> after step k, the
+    driver's first dump coming after the first step) carry each step's n_surf and conc
+    (output.F90:47), the values the ctypes path gives for the same steps, bit for bit."""
+    from vebisph_amd import refio
+    case = OPEN["fs_convection"]()
+    refio.write_case_dir(str(tmp_path), case, tmax=1.0, dt_out=1e-12)
+    _drive(tmp_path, "fp64", "3")
+    G = solver.Solver(case)
+    dd = tmp_path / "data_directory"
+    for k in range(1, 4):
+        G.div_free()
+        ns, conc = G.free_surface()
+        part = refio.read_part(str(dd / f"PART{k - 1}"), case.npfb)
+        keep = G.get_state()["inbin"] == 0
+        assert ns.sum() > 100 and np.all(conc[keep] > 0)
+        _eq(part["n_surf"], ns[keep])
+        _eq(part["conc"], conc[keep])
+        _eq(part["x"], G.get_state()["x"][keep])

@@ -22,11 +22,12 @@ EXPORTS = [
     "vb_create", "vb_destroy", "vb_last_error", "vb_upload_state", "vb_download_state",
     "vb_set_time_step", "vb_put_in_bin", "vb_create_mirror_particles", "vb_particle_allocation",
     "vb_kernel_calculation", "vb_locate_free_surf_part", "vb_new_fick_shift", "vb_prediction_step",
-    "vb_mirror_velocities", "vb_ppe_solve", "vb_sparse_pbicgstab_nonull", "vb_pressure_update",
+    "vb_mirror_velocities", "vb_ppe_solve", "vb_sparse_pbicgstab_nonull", "vb_sparse_pbicgstab",
+    "vb_pressure_update",
     "vb_final_advection", "vb_div_free",
     "vb_get_scalars", "vb_export_mirrors", "vb_export_neighbours", "vb_export_weights",
     "vb_export_ushift", "vb_export_grad_p", "vb_export_ppe", "vb_secondary_properties",
-    "vb_field_maxima", "vb_circle_forces", "vb_nusselt", "vb_profile_spmv", "vb_profile_copy",
+    "vb_free_surface", "vb_field_maxima", "vb_circle_forces", "vb_nusselt", "vb_profile_spmv", "vb_profile_copy",
     "vb_nnf",
     "vb_kernel_functions", "vb_logconf",
 ]
@@ -89,6 +90,7 @@ def lib():
         L.vb_ppe_solve.argtypes = [vp, C.c_double, C.c_int32, ip, dp]
         L.vb_sparse_pbicgstab_nonull.argtypes = [vp, dp, ip, dp, dp, C.c_int32, C.c_int32, C.c_int32,
                                                  C.c_double, ip, dp]
+        L.vb_sparse_pbicgstab.argtypes = L.vb_sparse_pbicgstab_nonull.argtypes
         L.vb_get_scalars.argtypes = [vp, ip, dp]
         L.vb_export_mirrors.argtypes = [vp, C.c_int32, ip, ip, dp]
         L.vb_export_neighbours.argtypes = [vp, C.c_int32, C.c_int32, ip, ip]

@@ -85,8 +85,9 @@ int run_dir(const std::string& dir, int precision, long max_steps) {
   vbio_ibound ib;
   if (vbio_read_indat(at("INDAT").c_str(), &ind) || vbio_read_ibound(at("IBOUND").c_str(), &ib)) return io_fail();
   if (vbio_check_ibound(&ib)) return io_fail();                  // input.F90:236-238 STOPs
-  if (ind.i_open_domain != 0) return fail("INDAT: open domain (i_open_domain /= 0) is not built");
-  if (ind.i_shift_coeff != 0) return fail("INDAT: i_shift_coeff /= 0 is not built");
+  if (ind.i_open_domain != 0 && ind.i_open_domain != 1) return fail("INDAT: i_open_domain must be 0 or 1");
+  if (ind.i_shift_coeff != 0 && ind.i_shift_coeff != 1) return fail("INDAT: i_shift_coeff must be 0 or 1");
+  const bool open = ind.i_open_domain == 1;
   for (int i = 0; i < ib.nb_patches; i++)
     if (ib.b_type[i] == 3 || ib.b_type[i] == 4) return fail("IBOUND: inflow/outflow patches are not coded (input.F90:287-288)");
   // The device forms b_edge(ib) = b_node(ib+1) - b_node(ib), as the reference's generator writes it.
@@ -120,12 +121,16 @@ int run_dir(const std::string& dir, int precision, long max_steps) {
                              s.th.data(), &dv)) {
     return io_fail();
   }
+  // every particle starts on the free surface in an open domain (input.F90:147-153), until the
+  // first step flags them
+  std::fill(nsurf.begin(), nsurf.end(), open ? 1 : 0);
 
   vb_params prm{};
   prm.h = ind.h; prm.dx = ind.dx; prm.dv = dv; prm.Ra = ind.Ra; prm.Pr = ind.Pr; prm.El = ind.El;
   prm.beta = ind.beta; prm.body_force[0] = ind.body_force[0]; prm.body_force[1] = ind.body_force[1];
   prm.dt_coef_cfl = ind.dt_coef_cfl; prm.dt_coef_visc = ind.dt_coef_visc;
   prm.ve_nonlinearity = ind.ve_nonlinearity; prm.time = time; prm.itime = itime;
+  prm.i_open_domain = ind.i_open_domain; prm.i_shift_coeff = ind.i_shift_coeff;
   prm.precision = precision; prm.reference_kick = 1;
   std::vector<double> bn(2 * ib.nb_patches), cc(2 * ib.nb_circles), cv(2 * ib.nb_circles);
   for (int i = 0; i < ib.nb_patches; i++) { bn[2 * i] = ib.b_node[i][0]; bn[2 * i + 1] = ib.b_node[i][1]; }
@@ -164,7 +169,8 @@ int run_dir(const std::string& dir, int precision, long max_steps) {
 
   double next_dump = time, next_dump2 = time, dt = ind.dt, ls_res = 0.0;
   int ls_iters = 0, n_inbin = 0, np = ind.i_restart ? hd.np : ind.np;
-  std::vector<double> vort(npfb, 0.0), conc(npfb, 0.0);   // conc: never set on the closed-domain path
+  std::vector<double> vort(npfb, 0.0), conc(npfb, 0.0);   // conc: set by the open domain's steps only
+  bool stepped = false;
   int rc = 0;
   auto scalars = [&]() {
     int32_t iv[12];
@@ -179,6 +185,8 @@ int run_dir(const std::string& dir, int precision, long max_steps) {
       const int idp = n_dump;
       n_dump = n_dump + 1;
       if (s.download(ctx, npfb) != VB_OK) return fail(vb_last_error(ctx));
+      if (open && stepped && vb_free_surface(ctx, npfb, nsurf.data(), conc.data()) != VB_OK)
+        return fail(vb_last_error(ctx));
       const std::string part = dd + "/PART" + std::to_string(idp);
       // a_out = trace_conf (predictor_mod.F90:243); n_surf = 0 in a closed domain
       if (vbio_write_part(part.c_str(), npfb, s.x.data(), s.y.data(), s.u.data(), s.v.data(), s.th.data(),
@@ -221,6 +229,7 @@ int run_dir(const std::string& dir, int precision, long max_steps) {
       if (vbio_append_record(f_192.c_str(), "dd", row)) { rc = io_fail(); break; }
     }
     scalars();
+    stepped = true;
     steps++;
     if (itime % 10 == 0)
       printf("itime %d time %.9g dt %.6e LS_iters %d LS_residual %.3e n_inbin %d\n", itime, time, dt, ls_iters,

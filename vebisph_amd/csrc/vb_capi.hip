@@ -67,7 +67,7 @@ static void free_all(vb_ctx* c) {
                   c->perm, c->rank, c->pflag, c->pscan, c->ist_real, c->posP, c->idx16, c->lenP,
                   c->rowflagP, c->rowsumP, c->bperm, c->xperm, c->atab,
                   c->wg1, c->wg2, c->wl, c->k11, c->k12, c->k21, c->k22,
-                  c->usx, c->usy, c->kick, c->gpx, c->gpy, c->vort, c->a_diag, c->a_val, c->a_rowsum, c->rhs,
+                  c->usx, c->usy, c->kick, c->gpx, c->gpy, c->vort, c->conc, c->snx, c->sny, c->pscale, c->a_diag, c->a_val, c->a_rowsum, c->rhs,
                   c->a_col, c->a_len, c->rowflag, c->chunkg, c->choff, c->xflag, c->xscan, c->xrows,
                   c->lnk4, c->val4, c->val4f,
                   c->r, c->rb, c->vv,
@@ -88,6 +88,7 @@ static vb_status build_geom(vb_ctx* c, const vb_params* prm, const vb_boundary* 
     return VB_ERR_ARG;
   }
   g.nbp = b->nb_patches; g.ncirc = b->nb_circles; g.npfb = c->npfb; g.nbS = c->nbS;
+  g.open = prm->i_open_domain; g.shift_coeff = prm->i_shift_coeff;
   for (int i = 0; i < g.nbp; i++) {
     g.bnode[i][0] = b->b_node[2 * i]; g.bnode[i][1] = b->b_node[2 * i + 1];
     g.btype[i] = b->b_type[i]; g.bvel[i] = b->b_vel[i]; g.bth[i] = b->b_thermal[i];
@@ -128,7 +129,7 @@ static vb_status build_geom(vb_ctx* c, const vb_params* prm, const vb_boundary* 
   const double h = prm->h, h2 = h * h;
   g.h = h; g.dx = prm->dx; g.dv = prm->dv;
   g.sup = 3.0 * h; g.sup2 = 9.0 * h2; g.uno = 1.0 / (3.0 * h);
-  g.ad7 = 7.0 / 478.0 / 3.14159265358979323846 / h / h; g.ad7h = g.ad7 / h;
+  g.ad7 = 7.0 / 478.0 / kPi / h / h; g.ad7h = g.ad7 / h;
   g.eta2 = h * h * 1.0e-10;
   g.eta_small = (double)0.01f * h * h;
   g.oneoverh = 1.0 / h;
@@ -165,8 +166,8 @@ vb_status vb_create(const vb_params* prm, const vb_boundary* bnd, int32_t npfb, 
   c->itime = prm->itime;
   c->time = prm->time;
   vb_status s = VB_OK;
-  if (prm->i_open_domain != 0 || prm->i_shift_coeff != 0) {
-    c->err = "only closed domains with i_shift_coeff = 0 are supported";
+  if ((prm->i_open_domain != 0 && prm->i_open_domain != 1) || (prm->i_shift_coeff != 0 && prm->i_shift_coeff != 1)) {
+    c->err = "i_open_domain and i_shift_coeff must be 0 or 1";
     s = VB_ERR_ARG;
   }
   if (s == VB_OK) s = build_geom(c, prm, bnd);
@@ -184,7 +185,8 @@ vb_status vb_create(const vb_params* prm, const vb_boundary* bnd, int32_t npfb, 
   A(c->scan_bsum, (std::max<size_t>(cap, (size_t)c->g.nct) + 1023) / 1024 + 1);
   A(c->cnt, n); A(c->link, nw); A(c->wg1, nw); A(c->wg2, nw); A(c->wl, nw);
   A(c->k11, n); A(c->k12, n); A(c->k21, n); A(c->k22, n);
-  A(c->usx, n); A(c->usy, n); A(c->kick, n); A(c->gpx, n); A(c->gpy, n); A(c->vort, n);
+  A(c->usx, n); A(c->usy, n); A(c->kick, n); A(c->gpx, n); A(c->gpy, n); A(c->vort, n); A(c->conc, n);
+  if (prm->i_open_domain == 1) { A(c->snx, n); A(c->sny, n); A(c->pscale, n); }
   A(c->a_diag, n); A(c->a_val, nw); A(c->a_rowsum, n); A(c->rhs, n); A(c->a_col, nw); A(c->a_len, n);
   // compact grouped rows: up to VB_NPLINK slots rounded up to a multiple of 4
   // compact rows: at most VB_NPLINK slots rounded up to groups of 4, for every row of every chunk
@@ -198,6 +200,12 @@ vb_status vb_create(const vb_params* prm, const vb_boundary* bnd, int32_t npfb, 
   const char* sp = getenv("VB_SPMV");
   c->spmv_stored = sp && strcmp(sp, "stored") == 0;
   c->spmv_tiled = sp && strcmp(sp, "tiled") == 0 && prm->precision == VB_PREC_MIXED;
+  if (c->spmv_tiled && prm->i_open_domain == 1 && s == VB_OK) {
+    // its FP32 entries are recomputed from the pair distance, which the open domain's ramp rows
+    // (scaled by ppe_smooth) and Jacobi preconditioner do not follow
+    c->err = "VB_SPMV=tiled supports closed domains only";
+    s = VB_ERR_ARG;
+  }
   if (prm->precision == VB_PREC_MIXED) A(c->val4f, nw4);
   if (c->spmv_tiled) {
     A(c->perm, n); A(c->rank, n); A(c->pflag, cap + 1); A(c->pscan, cap + 1); A(c->ist_real, c->g.nct + 1);
@@ -325,17 +333,29 @@ vb_status vb_ppe_solve(vb_ctx* c, double tol, int32_t itermax, int32_t* iters, d
   return (vb_status)c->scal_host->status;
 }
 
-vb_status vb_sparse_pbicgstab_nonull(vb_ctx* c, const double* A, const int32_t* ija, double* x,
-                                     const double* B, int32_t n, int32_t nmax, int32_t itermax,
-                                     double tol, int32_t* iter, double* residual) {
+static vb_status caller_system(vb_ctx* c, const double* A, const int32_t* ija, double* x, const double* B,
+                               int32_t n, int32_t nmax, int32_t itermax, double tol, int32_t* iter,
+                               double* residual, bool open) {
   if (!c || !A || !ija || !x || !B || n < 1 || nmax < n + 1) return VB_ERR_ARG;
   if (ija[0] != n + 2) { c->err = "Sparse storage mode is not configured properly"; return VB_ERR_ARG; }
   if (ija[n] > nmax + 1) { c->err = "ija(n+1) beyond nmax"; return VB_ERR_ARG; }
-  vb_status s = solve_nr(c, A, ija, x, B, n, itermax, tol);
+  vb_status s = solve_nr(c, A, ija, x, B, n, itermax, tol, open);
   if (s != VB_OK) return s;
   if (iter) *iter = c->scal_host->iter;
   if (residual) *residual = c->scal_host->resid;
   return (vb_status)c->scal_host->status;
+}
+
+vb_status vb_sparse_pbicgstab_nonull(vb_ctx* c, const double* A, const int32_t* ija, double* x,
+                                     const double* B, int32_t n, int32_t nmax, int32_t itermax,
+                                     double tol, int32_t* iter, double* residual) {
+  return caller_system(c, A, ija, x, B, n, nmax, itermax, tol, iter, residual, false);
+}
+
+vb_status vb_sparse_pbicgstab(vb_ctx* c, const double* A, const int32_t* ija, double* x, const double* B,
+                              int32_t n, int32_t nmax, int32_t itermax, double tol, int32_t* iter,
+                              double* residual) {
+  return caller_system(c, A, ija, x, B, n, nmax, itermax, tol, iter, residual, true);
 }
 
 vb_status vb_pressure_update(vb_ctx* c) { return c ? launch_pressure_update(c) : VB_ERR_ARG; }
@@ -499,6 +519,18 @@ vb_status vb_secondary_properties(vb_ctx* c, int32_t n, double* vort, double* u_
     VB_HIP(c, hipMemcpyAsync(vort, c->vort, (size_t)n * 8, hipMemcpyDeviceToHost, c->stream));
   if ((s = sync_scalars(c)) != VB_OK) return s;
   if (u_rms) *u_rms = c->scal_host->u_rms;
+  return VB_OK;
+}
+
+vb_status vb_free_surface(vb_ctx* c, int32_t n, int32_t* n_surf, double* conc) {
+  if (!c || n < 0 || n > std::max(c->np_host, c->npfb)) return VB_ERR_ARG;
+  const int nr = std::min<int>(n, c->npfb);
+  if (n_surf && n > 0)
+    VB_HIP(c, hipMemcpyAsync(n_surf, c->nsurf, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
+  if (conc && nr > 0)
+    VB_HIP(c, hipMemcpyAsync(conc, c->conc, (size_t)nr * 8, hipMemcpyDeviceToHost, c->stream));
+  VB_HIP(c, hipStreamSynchronize(c->stream));
+  if (conc) for (int i = nr; i < n; i++) conc[i] = 0.0;
   return VB_OK;
 }
 

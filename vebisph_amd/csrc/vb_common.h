@@ -19,6 +19,7 @@ namespace vb {
 #ifndef VB_MAX_BLOCKS
 #define VB_MAX_BLOCKS 512
 #endif
+constexpr double kPi = 3.14159265358979323846;   // common_parameter.F90:12
 constexpr int kBlock = 256;                // threads per workgroup (4 wavefronts of 64)
 // SpMV launch: one row per thread (up to kSpmvMaxBlocks workgroups), so that every row's chain of
 // dependent gathers has its own wavefront slot.  Measured on the 1M-row PPE: one row per thread
@@ -62,6 +63,8 @@ struct Geom {
   double xmint, xmaxt, ymint, ymaxt;
   int ncx, ncy, nct;
   double bf[2];
+  int open;                                // i_open_domain: free surface (freesurf_mod.F90)
+  int shift_coeff;                         // i_shift_coeff (new_fick_shift.F90:43-51)
 };
 
 // Particle arrays (device pointers).  Rows 0..npfb-1 are wall+fluid, npfb..np-1 mirrors.
@@ -79,6 +82,7 @@ struct Scal {
   int np, nmirror, n_inbin, maxneigh, err, seg_base;
   // BiCGSTAB
   double rho, rho0, alpha, omega, beta, n2s0, n2r0, resid, mean_p, mean_s, tol;
+  double n2_small;                         // first-iteration norm threshold (1e-6, or 4 when open)
   int iter, itermax, done, exit_s, status;
   // BiCGSTAB in five kernels per iteration: rho of iteration k in rho_pp[k & 1]; sums of r, v and
   // of the projected p, each recorded by the kernel that folds it

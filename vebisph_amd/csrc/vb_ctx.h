@@ -33,6 +33,9 @@ struct vb_ctx {
   double* kick;                      // per-particle B perturbation (reference_kick)
   double *gpx, *gpy;
   double* vort;                      // secondary_properties: vorticity of rows 0..npfb-1
+  double* conc;                      // open domain: concentration (calc_part_dist_vars), else 0
+  double *snx, *sny;                 // open domain: surface normal -h sum w_G (before smoothing)
+  double* pscale;                    // open domain: PPE row scale, ppe_smooth or 1 (surface rows)
 
   // PPE (ELL, slot-major) and BiCGSTAB vectors
   double *a_diag, *a_val, *a_rowsum, *rhs;
@@ -87,7 +90,7 @@ vb_status launch_ppe(vb_ctx* c, double tol, int itermax);
 vb_status upload_entry_table(vb_ctx* c);
 vb_status profile_spmv(vb_ctx* c, int reps, double* avg_us, double* alg_bytes);
 vb_status solve_nr(vb_ctx* c, const double* A_nr, const int* ija, double* x, const double* B, int n,
-                   int itermax, double tol);
+                   int itermax, double tol, bool open);
 // out[0..n-1] = exclusive prefix sums of in[0..n-1], out[n] = total (vb_step.hip)
 vb_status launch_exclusive_scan(vb_ctx* c, const int* in, int* out, int n);
 // helpers (vb_capi.hip)

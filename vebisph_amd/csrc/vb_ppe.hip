@@ -33,7 +33,7 @@ constexpr int kRefresh = 50;                     // mixed path: FP64 residual re
 enum SpmvMode { SPMV_FP64 = 0, SPMV_MIXED = 1 };
 
 // ------------------------------------------------------------------------------------------------
-// RHS, ppe_solve_in_house_simple_dirichlet.F90:27-53 (DC removal for the closed domain)
+// RHS, ppe_solve_in_house_simple_dirichlet.F90:27-53 (DC removal for the closed domain only)
 // ------------------------------------------------------------------------------------------------
 __global__ void k_ppe_rhs(Geom g, State st, const int* cnt, const int* link, const double* wg1,
                           const double* wg2, const double* k11, const double* k12, const double* k21,
@@ -61,9 +61,9 @@ __global__ void k_ppe_rhs(Geom g, State st, const int* cnt, const int* link, con
   R.store(partials);
 }
 
-__global__ void k_fin_ppe_rhs(const double* partials, int nblk, const double* red, int n, Scal* sc) {
+__global__ void k_fin_ppe_rhs(const double* partials, int nblk, const double* red, int n, int open, Scal* sc) {
   double s = fold(partials, 0, nblk, red, n);
-  if (threadIdx.x == 0) sc->mean = s / (double)n;
+  if (threadIdx.x == 0) sc->mean = open ? 0.0 : s / (double)n;     // b - 0.0 == b bit for bit
 }
 
 __host__ __device__ inline size_t grouped(int k, int i, int n) { return ((size_t)(k >> 2) * n + i) * 4 + (k & 3); }
@@ -83,8 +83,8 @@ constexpr int kChunk = 64;
 // (rowflag 0: no mirror neighbour, every neighbour within 32767 rows) and records per chunk the
 // groups its compact rows need (chunkg); k_assemble_sliced then writes the compact rows.
 __global__ void k_assemble(Geom g, State st, const int* cnt, const int* link, const double* wl,
-                           const int* nsurf, double* rhs, double* a_diag, int* a_col, double* a_val,
-                           uint8_t* rowflag, double* rowsum, int* chunkg, const Scal* sc) {
+                           const int* nsurf, const double* pscale, double* rhs, double* a_diag, int* a_col,
+                           double* a_val, uint8_t* rowflag, double* rowsum, int* chunkg, const Scal* sc) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
   const int npfb = g.npfb;
   int ngroups = 0;
@@ -109,6 +109,12 @@ __global__ void k_assemble(Geom g, State st, const int* cnt, const int* link, co
     }
   }
   if (nsurf[i] != 0) { adiag = 1.0e30; explicit_row = true; }
+  // open domain, :126-138: the off-diagonals and right-hand side of rows near the surface are scaled
+  // by ppe_smooth (pscale is 1 elsewhere and on surface rows; x * 1.0 == x); scaled rows are stored
+  // full (k_assemble_explicit applies the scale), so the compact rows stay unscaled
+  const double ps = pscale ? pscale[i] : 1.0;
+  if (ps != 1.0) explicit_row = true;
+  bi = bi * ps;
   double d = 0.0 + adiag;
   double rs = d;
   if (!explicit_row) {
@@ -142,8 +148,8 @@ constexpr int kExplicitThreads = 128;
 static_assert(kExplicitThreads >= kNplink, "one thread per slot");
 __global__ void __launch_bounds__(kExplicitThreads)
 k_assemble_explicit(int npfb, const int* rows, int nrows, const int* cnt, const int* link,
-                    const double* wl, const int* irel, const double* a_diag, int* a_col, double* a_val,
-                    double* rowsum) {
+                    const double* wl, const int* irel, const double* pscale, const double* a_diag, int* a_col,
+                    double* a_val, double* rowsum) {
   if ((int)blockIdx.x >= nrows) return;
   __shared__ int cols[kExplicitThreads];
   __shared__ double mw[kExplicitThreads], vals[kExplicitThreads];
@@ -165,6 +171,7 @@ k_assemble_explicit(int npfb, const int* rows, int nrows, const int* cnt, const 
         double acc = 0.0;
         for (int k2 = 0; k2 <= k; k2++) if (cols[k2] == ck) acc = acc + mw[k2];
         val = acc;
+        if (pscale) val = val * pscale[i];               // ppe_smooth, :131-134
       }
     }
     a_col[k * npfb + i] = ck;
@@ -394,7 +401,15 @@ __device__ inline double spmv_row(const Mat& A, int i, const double* __restrict_
 // ------------------------------------------------------------------------------------------------
 struct Vecs {
   double *x, *b, *r, *rb, *v, *pn, *pinv, *s, *sinv, *t;   // pn, s: before remove_dc
+  // null: SPARSE_PBICGSTAB_nonull (M = 1, remove_dc of p and s); else SPARSE_PBICGSTAB, the open
+  // domain's solve (:82-227): Jacobi M = jac = diag(A) (:63-75), no mean removed
+  const double* jac;
 };
+
+// inverseM, :47-59: x / (M(i) + 1e-15)
+__device__ inline double inv_m(const Vecs& w, int i, double x) {
+  return w.jac ? x / (w.jac[i] + 1.0e-15) : x / kOnePlusEps;
+}
 
 enum { ST_P = 0, ST_SPMV_P = 1, ST_S = 2, ST_SPMV_S = 3, ST_XR = 4, ST_INIT = 5, ST_TRUE = 6 };
 static_assert(ST_TRUE < kSolverStages, "stage buffers");
@@ -435,13 +450,15 @@ __global__ void k_bi_init(Mat A, Vecs w, int x_given, RedBufs rb) {
   R.store(rb.P(ST_INIT));
 }
 
-__global__ void k_fin_init(RedBufs rb, int nblk, double tol, int itermax, Scal* sc) {
+// small: the first-iteration norm threshold, 1.0d-6 (nonull, :367, :404) or the integer expression
+// 10-6 = 4 (open, :171, :207)
+__global__ void k_fin_init(RedBufs rb, int nblk, double tol, int itermax, double small, Scal* sc) {
   double f[2];
   fold_n<2>(rb.P(ST_INIT), nblk, rb.R(ST_INIT), rb.n, f);
   const double rho = f[0];
   if (threadIdx.x == 0) {
     sc->rho = rho; sc->rho_pp[1] = rho; sc->sum_r = f[1]; sc->rho0 = 1.0; sc->alpha = 1.0; sc->omega = 1.0; sc->beta = 0.0;
-    sc->n2s0 = 1.0; sc->n2r0 = 1.0; sc->resid = 0.0; sc->tol = tol;
+    sc->n2s0 = 1.0; sc->n2r0 = 1.0; sc->resid = 0.0; sc->tol = tol; sc->n2_small = small;
     sc->iter = 1; sc->itermax = itermax; sc->exit_s = 0; sc->status = VB_OK;
     sc->mean_p = 0.0; sc->mean_s = 0.0; sc->sum_v = 0.0; sc->sum_p = 0.0;
     sc->done = 0;
@@ -466,7 +483,7 @@ __global__ void k_bi_p(int n, Vecs w, RedBufs rb, const Scal* sc) {
   R.store(rb.P(ST_P));
 }
 
-// :351-352 remove_dc(p), inverseM: p_inv = (pn - mean)/(1+1e-15)
+// :351-352 remove_dc(p), inverseM: p_inv = (pn - mean)/(M+1e-15) (open domain: mean 0)
 __global__ void k_bi_pinv(int n, Vecs w, RedBufs rb, Scal* sc) {
   if (finished(sc)) return;
   RowRange rr = block_rows(n, gridDim.x);
@@ -474,11 +491,11 @@ __global__ void k_bi_pinv(int n, Vecs w, RedBufs rb, Scal* sc) {
   double pv[kPre];
 #pragma unroll
   for (int u = 0; u < kPre; u++) if (i0 + u * bd < rr.end) pv[u] = w.pn[i0 + u * bd];
-  const double mean = fold(rb.P(ST_P), 0, gridDim.x, rb.R(ST_P), n) / (double)n;   // :243
+  const double mean = w.jac ? 0.0 : fold(rb.P(ST_P), 0, gridDim.x, rb.R(ST_P), n) / (double)n;   // :243
   if (blockIdx.x == 0 && threadIdx.x == 0) sc->mean_p = mean;
 #pragma unroll
-  for (int u = 0; u < kPre; u++) if (i0 + u * bd < rr.end) w.pinv[i0 + u * bd] = (pv[u] - mean) / kOnePlusEps;
-  for (int i = i0 + kPre * bd; i < rr.end; i += bd) w.pinv[i] = (w.pn[i] - mean) / kOnePlusEps;
+  for (int u = 0; u < kPre; u++) if (i0 + u * bd < rr.end) w.pinv[i0 + u * bd] = inv_m(w, i0 + u * bd, pv[u] - mean);
+  for (int i = i0 + kPre * bd; i < rr.end; i += bd) w.pinv[i] = inv_m(w, i, w.pn[i] - mean);
 }
 
 // :353-354  v = A p_inv; the terms of r_bar . v
@@ -536,7 +553,7 @@ __global__ void k_bi_sinv(int n, Vecs w, RedBufs rb, Scal* sc) {
   fold_n<2>(rb.P(ST_S), gridDim.x, rb.R(ST_S), n, f);
   const double n2s = f[0], sum = f[1];
   const bool first = sc->iter == 1;
-  const double n2s0 = first ? ((n2s <= 1.0e-6) ? 1.0 : n2s) : sc->n2s0;
+  const double n2s0 = first ? ((n2s <= sc->n2_small) ? 1.0 : n2s) : sc->n2s0;
   const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
   if (n2s <= sc->tol * sc->tol * n2s0) {
     const double alpha = sc->alpha;
@@ -544,11 +561,11 @@ __global__ void k_bi_sinv(int n, Vecs w, RedBufs rb, Scal* sc) {
     if (lead) { sc->n2s0 = n2s0; sc->resid = sqrt(n2s / n2s0); sc->exit_s = 1; }
     return;
   }
-  const double mean = sum / (double)n;
+  const double mean = w.jac ? 0.0 : sum / (double)n;
   if (lead) { sc->n2s0 = n2s0; sc->mean_s = mean; }
 #pragma unroll
-  for (int u = 0; u < kPre; u++) if (i0 + u * bd < rr.end) w.sinv[i0 + u * bd] = (sv[u] - mean) / kOnePlusEps;
-  for (int i = i0 + kPre * bd; i < rr.end; i += bd) w.sinv[i] = (w.s[i] - mean) / kOnePlusEps;
+  for (int u = 0; u < kPre; u++) if (i0 + u * bd < rr.end) w.sinv[i0 + u * bd] = inv_m(w, i0 + u * bd, sv[u] - mean);
+  for (int i = i0 + kPre * bd; i < rr.end; i += bd) w.sinv[i] = inv_m(w, i, w.s[i] - mean);
 }
 
 // :385-387  t = A s_inv; terms of t.s and t.t with the projected s
@@ -629,7 +646,7 @@ __global__ void k_bi_true_shift(int n, Vecs w, RedBufs rb, const Scal* sc, int w
   if (finished(sc)) return;
   double f[2];
   fold_n<2>(rb.P(ST_TRUE), ntrue, rb.R(ST_TRUE), n, f);
-  const double d = (f[0] - f[1]) / (double)n;
+  const double d = w.jac ? 0.0 : (f[0] - f[1]) / (double)n;   // open domain: no projections, r = B - A x
   RowRange rr = block_rows(n, gridDim.x);
   Red<3> R(rb.R(ST_XR), n);
   for (int i = rr.begin + threadIdx.x; i < rr.end; i += blockDim.x) {
@@ -649,7 +666,7 @@ __global__ void k_fin_xr(RedBufs rb, int nblk, Scal* sc) {
   fold_n<2>(rb.P(ST_XR), nblk, rb.R(ST_XR), rb.n, f);
   const double n2r = f[0], rho = f[1];
   if (threadIdx.x == 0) {
-    if (sc->iter == 1) sc->n2r0 = (n2r <= 1.0e-6) ? 1.0 : n2r;
+    if (sc->iter == 1) sc->n2r0 = (n2r <= sc->n2_small) ? 1.0 : n2r;
     sc->resid = sqrt(n2r / sc->n2r0);
     if (n2r <= sc->tol * sc->tol * sc->n2r0) {
       sc->done = 1;
@@ -698,7 +715,7 @@ struct FinState { bool done; double beta, sum_r; };
 // :403-419, :329-343 for iteration k from its folded |r|^2, r_bar.r and sum(r); lead records them
 __device__ inline FinState finish_iteration(const double (&f)[3], int k, Scal* sc, bool lead) {
   const double n2r = f[0], rho = f[1];
-  const double n2r0 = k == 1 ? ((n2r <= 1.0e-6) ? 1.0 : n2r) : sc->n2r0;
+  const double n2r0 = k == 1 ? ((n2r <= sc->n2_small) ? 1.0 : n2r) : sc->n2r0;
   FinState o{false, 0.0, f[2]};
   int status = VB_OK;
   const bool converged = n2r <= sc->tol * sc->tol * n2r0;
@@ -741,13 +758,13 @@ __global__ void k_fu_p(int n, Vecs w, RedBufs rb, Scal* sc, int k, int fin) {
   const bool first = k == 1;
   const double omega = sc->omega;
   const double sum_p = first ? sum_r : sum_r + beta * (sc->sum_p - omega * sc->sum_v);
-  const double mean = sum_p / (double)n;                                  // remove_dc, :243
+  const double mean = w.jac ? 0.0 : sum_p / (double)n;                   // remove_dc, :243
   Red<1> R(nullptr, n);
   auto row = [&](int i, double r, double p, double v) {
     const double pn = first ? r : r + beta * (p - omega * v);
     const double pp = pn - mean;
     w.pn[i] = pp;
-    w.pinv[i] = pp / kOnePlusEps;
+    w.pinv[i] = inv_m(w, i, pp);
     R.add(0, i, pp);
   };
 #pragma unroll
@@ -767,7 +784,7 @@ __global__ void k_fu_s(int n, Vecs w, RedBufs rb, Scal* sc, int nspmv, int k) {
   double f[2];
   fold_n<2>(rb.P(ST_SPMV_P), nspmv, nullptr, n, f);
   const double alpha = sc->rho_pp[k & 1] / (f[0] + 1.0e-15);
-  const double mean = (sc->sum_r - alpha * f[1]) / (double)n;
+  const double mean = w.jac ? 0.0 : (sc->sum_r - alpha * f[1]) / (double)n;
   if (blockIdx.x == 0 && threadIdx.x == 0) { sc->alpha = alpha; sc->sum_v = f[1]; sc->mean_s = mean; }
   Red<1> R(nullptr, n);
   auto row = [&](int i, double r, double v) {
@@ -775,7 +792,7 @@ __global__ void k_fu_s(int n, Vecs w, RedBufs rb, Scal* sc, int nspmv, int k) {
     R.add(0, i, s * s);
     const double sp = s - mean;
     w.s[i] = sp;
-    w.sinv[i] = sp / kOnePlusEps;
+    w.sinv[i] = inv_m(w, i, sp);
   };
 #pragma unroll
   for (int u = 0; u < kPre; u++) if (i0 + u * bd < rr.end) row(i0 + u * bd, rv[u], vv[u]);
@@ -797,7 +814,7 @@ __global__ void k_fu_xr(int n, Vecs w, RedBufs rb, Scal* sc, int nspmv, int k) {
   fold_n<2>(rb.P(ST_S), gridDim.x, nullptr, n, fs);          // |s|^2, sum(p)
   fold_n<2>(rb.P(ST_SPMV_S), nspmv, nullptr, n, ft);         // t.s, t.t
   const double n2s = fs[0];
-  const double n2s0 = k == 1 ? ((n2s <= 1.0e-6) ? 1.0 : n2s) : sc->n2s0;
+  const double n2s0 = k == 1 ? ((n2s <= sc->n2_small) ? 1.0 : n2s) : sc->n2s0;
   const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
   const double alpha = sc->alpha;
   if (lead) {
@@ -1120,8 +1137,9 @@ static vb_status solve(vb_ctx* c, const Mat& A, Vecs w, int x_given, double tol,
   RedBufs rb = red_bufs(c, n);
   Scal* sc = c->scal;
   const bool fused = rb.rows == nullptr;   // serial mode: the reference's eight-kernel order
+  const double small = w.jac ? (double)(10 - 6) : 1.0e-6;
   VB_LAUNCH(c, k_bi_init, nb, A, w, x_given, rb);
-  VB_FIN(c, k_fin_init, rb, nb, tol, itermax, sc);
+  VB_FIN(c, k_fin_init, rb, nb, tol, itermax, small, sc);
   int launched = 0, batch = 8;
   for (;;) {
     for (int b = 0; b < batch; b++) {
@@ -1163,8 +1181,9 @@ static vb_status solve(vb_ctx* c, const Mat& A, Vecs w, int x_given, double tol,
   return VB_OK;
 }
 
-static Vecs step_vecs(vb_ctx* c, double* x) {
-  return Vecs{x, c->rhs, c->r, c->rb, c->vv, c->pn, c->pinv, c->ss, c->sinv, c->tt};
+// open: the open domain's Jacobi solve on the assembled diagonal
+static Vecs step_vecs(vb_ctx* c, double* x, bool open) {
+  return Vecs{x, c->rhs, c->r, c->rb, c->vv, c->pn, c->pinv, c->ss, c->sinv, c->tt, open ? c->a_diag : nullptr};
 }
 
 static TileMat tile_mat(vb_ctx* c) {
@@ -1191,7 +1210,7 @@ static vb_status solve_tiled(vb_ctx* c, const TileMat& T, Vecs w, double tol, in
   Mat A{};                                         // unused: x is not given
   A.n = n;
   VB_LAUNCH(c, k_bi_init, nb, A, w, 0, rb);
-  VB_FIN(c, k_fin_init, rb, nb, tol, itermax, sc);
+  VB_FIN(c, k_fin_init, rb, nb, tol, itermax, 1.0e-6, sc);
   int launched = 0, batch = 8;
   for (;;) {
     for (int b = 0; b < batch; b++) {
@@ -1273,9 +1292,9 @@ vb_status launch_ppe(vb_ctx* c, double tol, int itermax) {
   const int nb = spmv_grid(c->npfb);    // a neighbour sum: one row per thread, like the SpMV
   VB_LAUNCH_BS(c, k_ppe_rhs, nb, kSpmvBlock, c->g, c->st, c->cnt, c->link, c->wg1, c->wg2, c->k11, c->k12, c->k21,
             c->k22, c->rhs, c->partials, c->red, c->scal);
-  VB_FIN(c, k_fin_ppe_rhs, c->partials, nb, c->red, c->npfb, c->scal);
-  VB_LAUNCH(c, k_assemble, grid_1d(c->npfb), c->g, c->st, c->cnt, c->link, c->wl, c->nsurf, c->rhs,
-            c->a_diag, c->a_col, c->a_val, c->rowflag, c->a_rowsum, c->chunkg, c->scal);
+  VB_FIN(c, k_fin_ppe_rhs, c->partials, nb, c->red, c->npfb, c->g.open, c->scal);
+  VB_LAUNCH(c, k_assemble, grid_1d(c->npfb), c->g, c->st, c->cnt, c->link, c->wl, c->nsurf, c->pscale,
+            c->rhs, c->a_diag, c->a_col, c->a_val, c->rowflag, c->a_rowsum, c->chunkg, c->scal);
   const int nch = (c->npfb + kChunk - 1) / kChunk;
   vb_status ss = launch_exclusive_scan(c, c->chunkg, c->choff, nch);
   if (ss != VB_OK) return ss;
@@ -1289,11 +1308,12 @@ vb_status launch_ppe(vb_ctx* c, double tol, int itermax) {
   const int nx = c->scal_host->nexplicit;
   if (nx > 0) {
     hipLaunchKernelGGL(k_assemble_explicit, dim3(nx), dim3(kExplicitThreads), 0, c->stream, c->npfb, c->xrows,
-                       nx, c->cnt, c->link, c->wl, c->st.irel, c->a_diag, c->a_col, c->a_val, c->a_rowsum);
+                       nx, c->cnt, c->link, c->wl, c->st.irel, c->pscale, c->a_diag, c->a_col, c->a_val,
+                       c->a_rowsum);
     VB_HIP(c, hipGetLastError());
   }
   Mat A = step_matrix(c);
-  Vecs w = step_vecs(c, c->st.p);
+  Vecs w = step_vecs(c, c->st.p, c->g.open != 0);
   if (c->prm.precision == VB_PREC_MIXED) {
     if (!c->spmv_tiled) return solve<SPMV_MIXED>(c, A, w, 0, tol, itermax);   // stored FP32 values
     vb_status s = tile_setup(c);
@@ -1308,9 +1328,10 @@ vb_status launch_ppe(vb_ctx* c, double tol, int itermax) {
   return solve<SPMV_FP64>(c, A, w, 0, tol, itermax);
 }
 
-// SPARSE_PBICGSTAB_nonull on a caller's system in Numerical-Recipes storage (host arrays).
+// SPARSE_PBICGSTAB_nonull (open = false) or SPARSE_PBICGSTAB (open = true) on a caller's system in
+// Numerical-Recipes storage (host arrays).
 vb_status solve_nr(vb_ctx* c, const double* A_nr, const int* ija, double* x, const double* B, int n,
-                   int itermax, double tol) {
+                   int itermax, double tol, bool open) {
   if (n != c->npfb) { c->err = "system size must equal the context's npfb"; return VB_ERR_ARG; }
   std::vector<int> len(n), col((size_t)n * kNplink, 0);
   std::vector<double> val((size_t)n * kNplink, 0.0), diag(n);
@@ -1337,7 +1358,7 @@ vb_status solve_nr(vb_ctx* c, const double* A_nr, const int* ija, double* x, con
   Mat A = step_matrix(c);
   A.len = c->a_len;
   A.rowflag = nullptr;
-  vb_status s = solve<SPMV_FP64>(c, A, step_vecs(c, c->xsys), 1, tol, itermax);
+  vb_status s = solve<SPMV_FP64>(c, A, step_vecs(c, c->xsys, open), 1, tol, itermax);
   if (s != VB_OK) return s;
   VB_HIP(c, hipMemcpyAsync(x, c->xsys, (size_t)n * 8, hipMemcpyDeviceToHost, c->stream));
   VB_HIP(c, hipStreamSynchronize(c->stream));
@@ -1368,7 +1389,7 @@ vb_status profile_spmv(vb_ctx* c, int reps, double* avg_us, double* alg_bytes) {
   const bool mixed = c->prm.precision == VB_PREC_MIXED;
   Mat A = step_matrix(c);
   if ((c->spmv_stored || c->red) && !mixed) A.rowflag = nullptr;
-  Vecs w = step_vecs(c, c->st.p);
+  Vecs w = step_vecs(c, c->st.p, c->g.open != 0);
   int* d_counts = c->seg_counts;
   hipLaunchKernelGGL(k_bi_set_live, dim3(1), dim3(kBlock), 0, c->stream, c->scal, d_counts, c->cnt,
                      A.rowflag, n);

@@ -517,10 +517,18 @@ __global__ void k_neighbours(Geom g, State st, const int* cellkey, const int* is
 }
 
 // ------------------------------------------------------------------------------------------------
-// Phase 5: kernel_calculation.F90:24-71 (weights, Morris Laplacian weight, KGC inverse)
+// Phase 5: kernel_calculation.F90:24-71 (weights, Morris Laplacian weight, KGC inverse).
+// OPEN: the same pass also gathers what locate_free_surf_part needs from the pair (freesurf_mod.F90,
+// calc_part_dist_vars :85-101, criterion :29-37, ppe_smooth :55-64), in slot order:
+//   conc = dv (sum_k W(|r_ij|/h) + W(0)),  div r = -sum_k r_ij . w_G,  sn = -h sum_k w_G,
+//   n_surf = (div r <= beta) with beta = 1.4 + 0.5 max(conc - 1, -0.2), and the PPE row scale:
+//   1 on surface rows (their Dirichlet row is not smoothed) and where div r > 1.2 beta, else
+//   0.5 (1 - cos((div r - beta) pi / (0.2 beta))).
 // ------------------------------------------------------------------------------------------------
+template <bool OPEN>
 __global__ void k_weights(Geom g, State st, const int* cnt, const int* link, double* wg1, double* wg2,
-                          double* wl, double* k11, double* k12, double* k21, double* k22, Scal* sc) {
+                          double* wl, double* k11, double* k12, double* k21, double* k22, double* conc,
+                          double* snx, double* sny, int* nsurf, double* pscale, Scal* sc) {
   int i = blockIdx.x * kBlock + threadIdx.x;
   int mx = 0;
   if (i < g.npfb) {
@@ -528,6 +536,7 @@ __global__ void k_weights(Geom g, State st, const int* cnt, const int* link, dou
     mx = n;
     const double xi = st.x[i], yi = st.y[i], dv = g.dv;
     double a11 = 0.0, a12 = 0.0, a21 = 0.0, a22 = 0.0;
+    double ct = 0.0, dr = 0.0, sx = 0.0, sy = 0.0;
     for_slots(link, i, n, npfb, [&](int j) { return make_double2(st.x[j], st.y[j]); },
               [&](int s, int, double2 pj) {
       double rx = xi - pj.x, ry = yi - pj.y;
@@ -542,46 +551,96 @@ __global__ void k_weights(Geom g, State st, const int* cnt, const int* link, dou
       wg1[s] = w1; wg2[s] = w2; wl[s] = l;
       a11 = a11 - dv * gx * rx; a12 = a12 - dv * gy * rx;
       a21 = a21 - dv * gx * ry; a22 = a22 - dv * gy * ry;
+      if (OPEN) {
+        ct = ct + wab(rad / g.h, g.ad7);
+        dr = dr - (rx * w1 + ry * w2);
+        sx = sx - w1; sy = sy - w2;
+      }
     });
     double t = 1.0 / ((a11 * a22 - a12 * a21) + 1.0e-15);
     k11[i] = t * a22; k12[i] = -t * a12; k21[i] = -t * a21; k22[i] = t * a11;
+    if (OPEN) {
+      const double c = ct * dv + dv * wab(0.0, g.ad7);
+      const double beta = 1.4 + 0.5 * fmax(c - 1.0, -0.2);
+      const int surf = dr > beta ? 0 : 1;
+      double ps = 1.0;
+      if (!(dr > 1.2 * beta) && !surf) ps = 0.5 * (1.0 - cos((dr - beta) * kPi / (0.2 * beta)));
+      conc[i] = c; snx[i] = sx * g.h; sny[i] = sy * g.h; nsurf[i] = surf; pscale[i] = ps;
+    }
   }
   // maxneighbours = maxval(ij_count)
   for (int o = 32; o > 0; o >>= 1) { int w = __shfl_down(mx, o, 64); mx = w > mx ? w : mx; }
   if ((threadIdx.x & 63) == 0 && mx > 0) atomicMax(&sc->maxneigh, mx);
 }
 
-// Phase 6: locate_free_surf_part, closed domain (freesurf_mod.F90:39-45): n_surf = 0 everywhere
-__global__ void k_freesurf(int* nsurf, int np) {
+// Phase 6: locate_free_surf_part (freesurf_mod.F90:39-45): closed domain n_surf = 0 everywhere; the
+// open domain's rows 0..npfb-1 were flagged by k_weights<true>; mirrors take their parent's flag.
+__global__ void k_freesurf(int* nsurf, const int* irel, int npfb, int np, int open) {
   int i = blockIdx.x * kBlock + threadIdx.x;
-  if (i < np) nsurf[i] = 0;
+  if (i >= np) return;
+  if (i >= npfb) nsurf[i] = open ? nsurf[irel[i]] : 0;
+  else if (!open) nsurf[i] = 0;
 }
 
 // ------------------------------------------------------------------------------------------------
-// Phase 7: shifting_forces + new_fick_shift (new_fick_shift.F90:15-92, 117-210), i_shift_coeff 0,
-// closed domain; ushift = dr_fs/dt and -u on walls (div_free.F90:65-78)
+// Phase 7: shifting_forces + new_fick_shift (new_fick_shift.F90:15-92, 117-210, akinci off);
+// ushift = dr_fs/dt and -u on walls (div_free.F90:65-78).
+//   D: shift_coeff 0, 0.25 h^2 (:44); shift_coeff 1, 4 h dt (|u| + 0.2 exp(-|u| / (0.2 maxU)) maxU)
+//      (:46-50) with the REAL(4) literal 0.2 widened, and maxU = max(1e-10, |u| of the last row),
+//      the value of the reference's assignment inside its max reduction (:31-37) with one thread.
+//   Open-domain surface rows (n_surf = 1) shift along the surface only: dr = -D (t . grad C) t, t the
+//      rotated unit normal n = sn / max(|sn|, 1e-8 widened from REAL(4)) (:60-73), sn the surface
+//      normal Shepard-filtered over the real neighbours, smooth_normals (freesurf_mod.F90:114-127):
+//      dv (sum_{j real} W(|r_ij|/h) sn_j + W(0) sn_i), summed here in the same slot loop.
 // ------------------------------------------------------------------------------------------------
 __global__ void k_shift(Geom g, State st, const int* cnt, const int* link, const double* wg1,
-                        const double* wg2, double* usx, double* usy, const Scal* sc) {
+                        const double* wg2, const int* nsurf, const double* snx, const double* sny,
+                        double* usx, double* usy, const Scal* sc) {
   int i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= g.npfb) return;
   const int npfb = g.npfb, n = cnt[i];
   const double xi = st.x[i], yi = st.y[i], wab1 = sc->wab1, h = g.h;
-  double sfx = 0.0, sfy = 0.0, stx = 0.0, sty = 0.0;
+  const bool surf = g.open && i >= g.nbS && nsurf[i] == 1;
+  double sfx = 0.0, sfy = 0.0, stx = 0.0, sty = 0.0, nsx = 0.0, nsy = 0.0;
   for_slots(link, i, n, npfb, [&](int j) { return make_double2(st.x[j], st.y[j]); },
-            [&](int s, int, double2 pj) {
+            [&](int s, int j, double2 pj) {
     double rx = xi - pj.x, ry = yi - pj.y;
     double rad = sqrt(rx * rx + ry * ry);
     double qq = rad / h;
-    double fab = 0.25 * pow(wab(qq, g.ad7) / wab1, 4.0);
+    double w = wab(qq, g.ad7);
+    double fab = 0.25 * pow(w / wab1, 4.0);
     double w1 = wg1[s], w2 = wg2[s];
     sfx = sfx + w1; sfy = sfy + w2;
     stx = stx + w1 * fab; sty = sty + w2 * fab;
+    if (surf && j < npfb) { nsx = nsx + w * snx[j]; nsy = nsy + w * sny[j]; }
   });
   double dt = sc->dt;
   if (i < g.nbS) { usx[i] = -st.u[i]; usy[i] = -st.v[i]; return; }
-  double D = 0.25 * h * h;
-  double drx = -D * (sfx + stx), dry = -D * (sfy + sty);
+  double D;
+  if (g.shift_coeff == 1) {
+    const double f02 = (double)0.2f;
+    const double ul = st.u[npfb - 1], vl = st.v[npfb - 1];
+    const double maxU = fmax(1.0e-10, sqrt(ul * ul + vl * vl));
+    const double absU = sqrt(st.u[i] * st.u[i] + st.v[i] * st.v[i]);
+    const double temp = f02 * exp(-absU / (f02 * maxU));
+    D = 4.0 * h * dt * (absU + temp * maxU);
+  } else {
+    D = 0.25 * h * h;
+  }
+  const double gcx = sfx + stx, gcy = sfy + sty;
+  double drx, dry;
+  if (surf) {
+    const double w0 = wab(0.0, g.ad7), dv = g.dv;
+    const double sxs = nsx * dv + snx[i] * w0 * dv, sys = nsy * dv + sny[i] * w0 * dv;
+    const double tn = sqrt(sxs * sxs + sys * sys);
+    const double den = fmax(tn, (double)1.0e-8f);
+    const double nx = sxs / den, ny = sys / den;
+    const double tx = -ny, ty = nx;
+    const double dCds = tx * gcx + ty * gcy;
+    drx = -D * dCds * tx; dry = -D * dCds * ty;
+  } else {
+    drx = -D * gcx; dry = -D * gcy;
+  }
   double t = sqrt(drx * drx + dry * dry);
   if (t > 0.1 * h) { drx = 0.1 * h * drx / t; dry = 0.1 * h * dry / t; }
   usx[i] = drx / dt; usy[i] = dry / dt;
@@ -1061,19 +1120,23 @@ vb_status launch_neighbours(vb_ctx* c) {
 }
 
 vb_status launch_weights(vb_ctx* c) {
-  VB_LAUNCH(c, k_weights, grid_1d(c->npfb), c->g, c->st, c->cnt, c->link, c->wg1, c->wg2, c->wl, c->k11,
-            c->k12, c->k21, c->k22, c->scal);
+  if (c->g.open)
+    VB_LAUNCH(c, k_weights<true>, grid_1d(c->npfb), c->g, c->st, c->cnt, c->link, c->wg1, c->wg2, c->wl,
+              c->k11, c->k12, c->k21, c->k22, c->conc, c->snx, c->sny, c->nsurf, c->pscale, c->scal);
+  else
+    VB_LAUNCH(c, k_weights<false>, grid_1d(c->npfb), c->g, c->st, c->cnt, c->link, c->wg1, c->wg2, c->wl,
+              c->k11, c->k12, c->k21, c->k22, nullptr, nullptr, nullptr, nullptr, nullptr, c->scal);
   return VB_OK;
 }
 
 vb_status launch_freesurf(vb_ctx* c) {
-  VB_LAUNCH(c, k_freesurf, grid_1d(c->np_host), c->nsurf, c->np_host);
+  VB_LAUNCH(c, k_freesurf, grid_1d(c->np_host), c->nsurf, c->st.irel, c->npfb, c->np_host, c->g.open);
   return VB_OK;
 }
 
 vb_status launch_shift(vb_ctx* c) {
-  VB_LAUNCH(c, k_shift, grid_1d(c->npfb), c->g, c->st, c->cnt, c->link, c->wg1, c->wg2, c->usx, c->usy,
-            c->scal);
+  VB_LAUNCH(c, k_shift, grid_1d(c->npfb), c->g, c->st, c->cnt, c->link, c->wg1, c->wg2, c->nsurf, c->snx,
+            c->sny, c->usx, c->usy, c->scal);
   return VB_OK;
 }
 

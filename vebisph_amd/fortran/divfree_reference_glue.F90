@@ -8,7 +8,8 @@
 !!   every call      one step on the device, then dt, np, nmirror, n_inbin, LS_iters, LS_residual;
 !!   output steps    the particle state is copied back into common_2d when the driver will write
 !!                   it (time + dt > next_dump, output.F90:17, or the run ends), so the state
-!!                   crosses PCIe only for output; when time > next_dump2 the vorticity and RMS
+!!                   crosses PCIe only for output (with n_surf and conc in an open domain); when
+!!                   time > next_dump2 the vorticity and RMS
 !!                   velocity of secondary_properties (div_free.F90:152-185) come from the device
 !!                   into common_2d's vort and unit 192, as the reference's step computes them.
 !! Build: add vebisph_amd/fortran/vebisph_gpu.F90 and this file to the reference's source list in
@@ -18,8 +19,8 @@ module divfree
   use kind_parameters
   use common_parameter
   use common_2d
-  use vebisph_gpu, only: vb_params, vb_boundary, vb_get_scalars, vb_secondary_properties, VB_PREC_FP64, &
-       VB_PREC_MIXED
+  use vebisph_gpu, only: vb_params, vb_boundary, vb_get_scalars, vb_secondary_properties, vb_free_surface, &
+       VB_PREC_FP64, VB_PREC_MIXED
   use divfree_gpu, only: gpu_div_free => div_free, vbf_init, vbf_upload, vbf_download, vbf_ctx, &
        vbf_status
   implicit none
@@ -62,8 +63,11 @@ contains
        write(192,*) time, u_rms
        flush(192)
     end if
-    if (time + dt > next_dump .or. time + dt > tmax) &
-         call vbf_download(int(npfb, c_int32_t), r, u, theta, tau_p, trace_conf, p, inbin)
+    if (time + dt > next_dump .or. time + dt > tmax) then
+       call vbf_download(int(npfb, c_int32_t), r, u, theta, tau_p, trace_conf, p, inbin)
+       !! the open domain's n_surf and conc columns of the PART files (freesurf_mod.F90:29-37, :95-100)
+       if (i_open_domain .eq. 1) s = vb_free_surface(vbf_ctx, int(npfb, c_int32_t), n_surf, conc)
+    end if
   end subroutine div_free
 
   subroutine start()

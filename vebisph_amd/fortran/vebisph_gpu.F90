@@ -154,6 +154,21 @@ module vebisph_gpu
        real(c_double), intent(out) :: residual
        integer(c_int) :: s
      end function vb_sparse_pbicgstab_nonull
+     !! SPARSE_PBICGSTAB(A,ija,x,B,n,nmax,itermax,tol,iter,residual), linear_solver.F90:82-227
+     function vb_sparse_pbicgstab(ctx, a, ija, x, b, n, nmax, itermax, tol, iter, residual) &
+          bind(C, name="vb_sparse_pbicgstab") result(s)
+       import :: c_int, c_int32_t, c_double, c_ptr
+       type(c_ptr), value :: ctx
+       real(c_double), intent(in) :: a(*)
+       integer(c_int32_t), intent(in) :: ija(*)
+       real(c_double), intent(inout) :: x(*)
+       real(c_double), intent(in) :: b(*)
+       integer(c_int32_t), value :: n, nmax, itermax
+       real(c_double), value :: tol
+       integer(c_int32_t), intent(out) :: iter
+       real(c_double), intent(out) :: residual
+       integer(c_int) :: s
+     end function vb_sparse_pbicgstab
      function vb_pressure_update(ctx) bind(C, name="vb_pressure_update") result(s)
        import :: c_int, c_ptr
        type(c_ptr), value :: ctx
@@ -231,6 +246,14 @@ module vebisph_gpu
        real(c_double), intent(out) :: vort(*), u_rms
        integer(c_int) :: s
      end function vb_secondary_properties
+     function vb_free_surface(ctx, n, n_surf, conc) bind(C, name="vb_free_surface") result(s)
+       import :: c_int, c_int32_t, c_double, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: n
+       integer(c_int32_t), intent(out) :: n_surf(*)
+       real(c_double), intent(out) :: conc(*)
+       integer(c_int) :: s
+     end function vb_free_surface
      function vb_field_maxima(ctx, maxima) bind(C, name="vb_field_maxima") result(s)
        import :: c_int, c_double, c_ptr
        type(c_ptr), value :: ctx

@@ -224,16 +224,17 @@ class Solver:
         self._chk(_lib.lib().vb_ppe_solve(self.h, tol, itermax, C.byref(it), C.byref(res)))
         return it.value, res.value
 
-    def sparse_pbicgstab_nonull(self, a, ija, b, x0=None, itermax=1000, tol=1e-8):
-        """SPARSE_PBICGSTAB_nonull (linear_solver.F90:275) on a caller's NR row-indexed system."""
+    def sparse_pbicgstab_nonull(self, a, ija, b, x0=None, itermax=1000, tol=1e-8, open_domain=False):
+        """SPARSE_PBICGSTAB_nonull (linear_solver.F90:275) on a caller's NR row-indexed system, or with
+        open_domain SPARSE_PBICGSTAB (:82, Jacobi, no mean removal)."""
         n = b.shape[0]
         x = np.ascontiguousarray(np.zeros(n) if x0 is None else x0, dtype=np.float64).copy()
         a = np.ascontiguousarray(a, dtype=np.float64); ija = np.ascontiguousarray(ija, dtype=np.int32)
         bb = np.ascontiguousarray(b, dtype=np.float64)
         it = C.c_int32(0); res = C.c_double(0.0)
-        st = self._chk(_lib.lib().vb_sparse_pbicgstab_nonull(self.h, _d(a), _i(ija), _d(x), _d(bb), n,
-                                                              a.shape[0], itermax, tol, C.byref(it),
-                                                              C.byref(res)))
+        fn = _lib.lib().vb_sparse_pbicgstab if open_domain else _lib.lib().vb_sparse_pbicgstab_nonull
+        st = self._chk(fn(self.h, _d(a), _i(ija), _d(x), _d(bb), n, a.shape[0], itermax, tol, C.byref(it),
+                          C.byref(res)))
         return x, it.value, res.value, st
 
     def calc_pressure_gradient(self):
@@ -294,6 +295,13 @@ class Solver:
         vort = np.zeros(n); u_rms = C.c_double(0.0)
         self._chk(_lib.lib().vb_secondary_properties(self.h, n, _d(vort), C.byref(u_rms)))
         return vort, u_rms.value
+
+    def free_surface(self, n=None):
+        """n_surf and conc of rows 1..n (the PART file columns, output.F90:47-49) of the last step."""
+        n = self.case.npfb if n is None else n
+        ns = np.zeros(n, np.int32); co = np.zeros(n)
+        self._chk(_lib.lib().vb_free_surface(self.h, n, _i(ns), _d(co)))
+        return ns, co
 
     def field_maxima(self):
         """max |u|, |v|, |p| over rows 1..npfb (the per-step line of data_directory/TIME)."""
