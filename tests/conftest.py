@@ -18,8 +18,9 @@ def pytest_configure(config):
         subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "oracle"], check=False,
                        stdout=subprocess.DEVNULL)
     ref_so = os.path.join(ROOT, "oracle", "_ref", "libvbref.so")
-    if not os.path.exists(ref_so) and os.path.isdir("/root/reference/source"):
-        subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "ref"], check=False,
+    gen = os.path.join(ROOT, "oracle", "_ref", "gen2D")
+    if not (os.path.exists(ref_so) and os.path.exists(gen)) and os.path.isdir("/root/reference/source"):
+        subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "ref", "gen"], check=False,
                        stdout=subprocess.DEVNULL)
 
 

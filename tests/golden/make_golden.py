@@ -7,7 +7,9 @@ small seeded cases.
 For every phase it records the SHA-256 of each array the phase produces (mirror relations and
 offsets, neighbour lists, kernel weights and KGC, free-surface flags and quantities, shifting
 velocity, particle state, pressure gradient) and the step scalars; the final state is stored whole.  tests/test_golden.py checks the C
-restatement against these files, so the oracle stays pinned where the reference is absent.
+restatement against these files, so the oracle stays pinned where the reference is absent.  It also
+runs the reference's case generator (oracle/_ref/gen2D) on its eight cases and records each run
+directory as the readers parse it (generator_cases.json, for tests/test_generator.py).
 """
 import hashlib
 import json
@@ -77,6 +79,39 @@ def phase_record(S, k):
     return rec
 
 
+GEN2D = os.path.join(ROOT, "oracle", "_ref", "gen2D")
+
+
+def run_dir_record(path):
+    """Digests of a run directory as the readers parse it (getdata's view), and INDAT's text."""
+    from vebisph_amd import refio
+    rd = refio.read_case_dir(path)
+    c = rd.case
+    rec = {"INDAT": open(os.path.join(path, "INDAT")).read(), "npfb": c.npfb, "nbS": c.nbS,
+           "scalars": [rd.tmax, rd.dt_out, rd.dt, rd.alpha_evss, c.h, c.dx, c.dv, c.Ra, c.Pr, c.El, c.beta,
+                       c.body_force[0], c.body_force[1], c.dt_coef_cfl, c.dt_coef_visc, c.ve_nonlinearity,
+                       c.i_open_domain, c.i_shift_coeff]}
+    for key in ("x", "y", "u", "v", "theta", "b_node", "b_type", "b_vel", "b_thermal", "b_periodic_parent",
+                "c_centre", "c_radius", "c_omega", "c_vel", "c_thermal"):
+        a = getattr(c, key)
+        rec[key] = None if a is None else digest(a)
+    return rec
+
+
+def generator_fixtures():
+    """The reference's own generator (oracle/_ref/gen2D, datclass.F90) on each of its eight cases."""
+    import subprocess
+    import tempfile
+    out = {}
+    for k in range(1, 9):
+        d = tempfile.mkdtemp()
+        subprocess.run([GEN2D], input=f"{k}\n", text=True, cwd=d, capture_output=True, check=True)
+        out[str(k)] = run_dir_record(d)
+        print("generator case", k, out[str(k)]["npfb"])
+    with open(os.path.join(HERE, "generator_cases.json"), "w") as f:
+        json.dump(out, f, indent=0)
+
+
 def main():
     from oracle import refwrap
     if not refwrap.available():
@@ -102,6 +137,9 @@ def main():
         print(name, case.npfb, "LS_iters", [r[9]["LS_iters"] for r in recs])
     with open(os.path.join(HERE, "reference_phases.json"), "w") as f:
         json.dump(index, f, indent=0)
+    if not os.path.exists(GEN2D):
+        sys.exit("oracle/_ref/gen2D is not built (make -C oracle gen)")
+    generator_fixtures()
 
 
 if __name__ == "__main__":

@@ -95,6 +95,17 @@ def test_dam_break_phases_bit_exact():
     _run_phases(cases.dam_break(jitter=0.2, shift_coeff=1), steps=3)
 
 
+@pytest.mark.parametrize("k", [3, 4, 5, 6, 7, 8])
+def test_generator_cases_phases_bit_exact(k, tmp_path):
+    """The reference generator's cases (vebisph_amd.generator, itself equal to the reference's
+    datclass.F90 output) as the reference reads them from the run directory: moving walls (7), a
+    heated circle (5), open domains (6, 8).  Cases 1 and 2 are Newtonian (El = 0), which the
+    viscoelastic build's polymeric coefficient (1 - beta) Pr / (Ra El) turns into 0/0."""
+    from vebisph_amd import generator, refio
+    generator.write_run_dir(k, str(tmp_path))
+    _run_phases(refio.read_case_dir(str(tmp_path)).case, steps=2)
+
+
 def test_whole_step_equals_phases():
     """The harness's phase split equals the reference's own div_free (the real routine)."""
     from oracle import oraclewrap, refwrap

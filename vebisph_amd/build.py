@@ -102,10 +102,11 @@ def build_cpp_driver() -> str:
 
 
 def build_checkers() -> None:
-    """oracle/liboracle.so always; oracle/_ref when the reference sources are present."""
+    """oracle/liboracle.so always; oracle/_ref (the reference's step and its case generator) when the
+    reference sources are present."""
     _run(["make", "-C", os.path.join(ROOT, "oracle"), "oracle"])
     if os.path.isdir("/root/reference/source"):
-        _run(["make", "-C", os.path.join(ROOT, "oracle"), "ref"])
+        _run(["make", "-C", os.path.join(ROOT, "oracle"), "ref", "gen"])
 
 
 def build_all() -> None:
