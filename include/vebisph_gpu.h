@@ -60,6 +60,9 @@ typedef struct {
   int32_t reference_kick;        /* 1: reproduce predictor_mod.F90:369-371 as the reference runs it */
   int32_t serial_reductions;     /* 1: global sums in index order on one thread, as the reference's
                                     single-thread loops (bit-for-bit parity runs; slow at scale) */
+  int32_t fluid_model;           /* the reference's compile-time fluid model (its -Dconst_mod): 0 or 2
+                                    viscoelastic (log-conformation linear PTT / Oldroyd-B), 1 Newtonian
+                                    (no polymeric stress; predictor_mod.F90:58-77) */
 } vb_params;
 
 /* Boundary patches and circles: IBOUND, input.F90:51-81.  b_node is (nb_patches,2) row-major;

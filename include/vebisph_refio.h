@@ -89,7 +89,8 @@ int32_t vbio_write_ipart(const char* path, int32_t npfb, const double* x, const 
                          const double* v, const double* theta, double dv);
 
 /* PART<n>, output.F90:44-51 (viscoelastic layout): for each of rows 1..n with inbin == 0 one list of
- * x y u v theta P conc a_out n_surf vort tau11 tau12 tau22 */
+ * x y u v theta P conc a_out n_surf vort tau11 tau12 tau22; with tau11, tau12, tau22 all NULL the
+ * Newtonian build's ten columns (output.F90:46-47) */
 int32_t vbio_write_part(const char* path, int32_t n, const double* x, const double* y, const double* u,
                         const double* v, const double* theta, const double* p, const double* conc,
                         const double* a_out, const int32_t* n_surf, const double* vort, const double* tau11,
@@ -97,9 +98,13 @@ int32_t vbio_write_part(const char* path, int32_t n, const double* x, const doub
 /* the 13 columns of a PART file back, for up to cap particles: column c of particle i at
  * cols13[c*cap + i]; *n gets the count */
 int32_t vbio_read_part(const char* path, int32_t cap, int32_t* n, double* cols13);
+/* the same for ncols = 13 (viscoelastic build) or 10 (Newtonian build, output.F90:46-47: no stress
+ * columns; vbio_write_part writes that layout when tau11, tau12, tau22 are all NULL) */
+int32_t vbio_read_part_ncols(const char* path, int32_t ncols, int32_t cap, int32_t* n, double* cols);
 
 /* TEMP_REC.dat, output.F90:60-81: the header, then for rows 1..n with inbin == 0 the lists
- * x y u v theta p dv n_surf and tau11 tau12 tau21 tau22 */
+ * x y u v theta p dv n_surf and tau11 tau12 tau21 tau22 (with all four stress arrays NULL: the
+ * Newtonian build's file, without the stress lines, output.F90:69-71; likewise for the reader) */
 int32_t vbio_write_temp_rec(const char* path, const vbio_restart* hd, int32_t n, const double* x,
                             const double* y, const double* u, const double* v, const double* theta,
                             const double* p, double dv, const int32_t* n_surf, const double* tau11,

@@ -78,7 +78,8 @@ class OracleSolver:
         L = lib()
         self.case = case
         nc = case.nb_circles
-        iprm = _c([case.npfb, case.nbS, case.i_open_domain, case.i_shift_coeff, 0, itime], np.int32)
+        iprm = _c([case.npfb, case.nbS, case.i_open_domain, case.i_shift_coeff, 0, itime,
+                   getattr(case, "const_mod", 2)], np.int32)
         prm = _c(list(case.params_vector()) + [time])
         if nc:
             cc, cr, co, cv, ct = (_c(case.c_centre), _c(case.c_radius), _c(case.c_omega),

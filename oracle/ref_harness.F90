@@ -120,6 +120,9 @@ contains
     allocate(inbin(cap))
     r = 0.0d0; u = 0.0d0; p = 0.0d0; theta = 0.0d0; tau_p = 0.0d0; a_out = 0.0d0
     conc = 0.0d0; div_r = 0.0d0
+    !! solver outputs as a fresh program has them: SPARSE_PBICGSTAB* leave them unset when rho = 0
+    !! at the first iteration (linear_solver.F90:138-141)
+    LS_iters = 0; LS_residual = 0.0d0
     n_surf(:) = 0_ikind
     n_free = 0
     allocate(p_free(cap))
@@ -217,8 +220,15 @@ contains
     integer(ikind) :: i
     do i = 1, n
        x(i) = r(i,1); y(i) = r(i,2); ux(i) = u(i,1); uy(i) = u(i,2); th(i) = theta(i)
-       tau(i,1) = tau_p(i,1,1); tau(i,2) = tau_p(i,1,2); tau(i,3) = tau_p(i,2,1); tau(i,4) = tau_p(i,2,2)
-       trc(i) = trace_conf(i); pr_out(i) = p(i)
+       !! the Newtonian build's getdata allocates no polymeric stress (input.F90:109-112)
+       if (allocated(tau_p)) then
+          tau(i,1) = tau_p(i,1,1); tau(i,2) = tau_p(i,1,2); tau(i,3) = tau_p(i,2,1); tau(i,4) = tau_p(i,2,2)
+       else
+          tau(i,:) = 0.0d0
+       end if
+       trc(i) = 2.0d0
+       if (allocated(trace_conf)) trc(i) = trace_conf(i)
+       pr_out(i) = p(i)
        inb(i) = merge(1, 0, logical(inbin(i)))
     end do
   end subroutine vbref_get_particles

@@ -13,22 +13,27 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "_ref", "libvbref.so")
+# the same sources built as the reference's Newtonian program (-Dconst_mod=1)
+LIB_PATH_NEWT = os.path.join(HERE, "_ref", "newt", "libvbref.so")
 
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="F_CONTIGUOUS")
 _ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="F_CONTIGUOUS")
-_lib = None
+_libs = {}
 
 
-def available() -> bool:
-    return os.path.exists(LIB_PATH)
+def available(const_mod: int = 2) -> bool:
+    return os.path.exists(LIB_PATH_NEWT if const_mod == 1 else LIB_PATH)
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not available():
-            raise RuntimeError(f"reference harness not built: {LIB_PATH} (run make -C oracle ref)")
-        L = C.CDLL(LIB_PATH)
+def lib(const_mod: int = 2):
+    """The harness of the reference built with -Dconst_mod=2 (viscoelastic, the default) or 1
+    (Newtonian).  Each is its own library with its own module globals (loaded RTLD_LOCAL)."""
+    key = 1 if const_mod == 1 else 2
+    if key not in _libs:
+        path = LIB_PATH_NEWT if key == 1 else LIB_PATH
+        if not os.path.exists(path):
+            raise RuntimeError(f"reference harness not built: {path} (run make -C oracle ref refnewt)")
+        L = C.CDLL(path)
         L.vbref_setup.argtypes = [_ip, _dp, C.c_int, _dp, _ip, _dp, _dp, _ip, C.c_int, _dp, _dp, _dp, _dp, _dp]
         L.vbref_set_particles.argtypes = [C.c_int] + [_dp] * 8 + [_ip]
         L.vbref_get_particles.argtypes = [C.c_int] + [_dp] * 8 + [_ip]
@@ -50,8 +55,8 @@ def lib():
         L.vbref_get_vort.argtypes = [C.c_int, _dp, _dp]
         L.vbref_open_units.argtypes = [C.c_int]
         L.vbref_ld_write.argtypes = [C.c_char_p, C.c_int, C.c_int, _dp]
-        _lib = L
-    return _lib
+        _libs[key] = L
+    return _libs[key]
 
 
 def _f(a, dt=np.float64):
@@ -62,7 +67,7 @@ class RefSolver:
     """The reference step on a Case; state lives in the Fortran module globals."""
 
     def __init__(self, case, time=0.0, itime=0, state=None):
-        L = lib()
+        L = self.L = lib(getattr(case, "const_mod", 2))
         self.case = case
         nb = case.nb_patches
         nc = case.nb_circles
@@ -87,7 +92,7 @@ class RefSolver:
 
     def set_state(self, s):
         n = self.case.npfb
-        lib().vbref_set_particles(n, _f(s["x"]), _f(s["y"]), _f(s["u"]), _f(s["v"]), _f(s["theta"]),
+        self.L.vbref_set_particles(n, _f(s["x"]), _f(s["y"]), _f(s["u"]), _f(s["v"]), _f(s["theta"]),
                                   _f(s["tau"]), _f(s["trace"]), _f(s["p"]), _f(s["inbin"], np.int32))
 
     def get_state(self, n=None):
@@ -95,7 +100,7 @@ class RefSolver:
         out = {k: np.zeros(n, order="F") for k in ("x", "y", "u", "v", "theta", "trace", "p")}
         tau = np.zeros((n, 4), order="F")
         inb = np.zeros(n, np.int32, order="F")
-        lib().vbref_get_particles(n, out["x"], out["y"], out["u"], out["v"], out["theta"], tau,
+        self.L.vbref_get_particles(n, out["x"], out["y"], out["u"], out["v"], out["theta"], tau,
                                   out["trace"], out["p"], inb)
         out["tau"] = tau
         out["inbin"] = inb
@@ -104,7 +109,7 @@ class RefSolver:
     def scalars(self):
         ints = np.zeros(11, np.int32, order="F")
         dbls = np.zeros(8, order="F")
-        lib().vbref_get_scalars(ints, dbls)
+        self.L.vbref_get_scalars(ints, dbls)
         keys_i = ("np", "npfb", "nbS", "nmirror", "maxneighbours", "LS_iters", "itime", "n_inbin",
                   "ncx", "ncy", "nct")
         keys_d = ("dt", "LS_residual", "time", "umax", "x_mint", "x_maxt", "y_mint", "y_maxt")
@@ -114,19 +119,19 @@ class RefSolver:
 
     def set_threads(self, n):
         """OpenMP threads of the reference loops (1 after construction: deterministic)."""
-        lib().vbref_set_threads(int(n))
+        self.L.vbref_set_threads(int(n))
 
     def step(self):
-        lib().vbref_step()
+        self.L.vbref_step()
 
     def phase(self, k):
-        lib().vbref_phase(k)
+        self.L.vbref_phase(k)
 
     def mirrors(self):
         m = self.scalars()["nmirror"]
         irel = np.zeros(m, np.int32, order="F"); vrel = np.zeros(m, np.int32, order="F")
         dp = np.zeros(m, order="F")
-        lib().vbref_get_mirrors(m, irel, vrel, dp)
+        self.L.vbref_get_mirrors(m, irel, vrel, dp)
         return irel, vrel, dp
 
     def neighbours(self, w=NotImplemented):
@@ -134,26 +139,26 @@ class RefSolver:
         from vebisph_amd.cases import NPLINK
         w = NPLINK if w is NotImplemented else w
         cnt = np.zeros(n, np.int32, order="F"); link = np.zeros((n, w), np.int32, order="F")
-        lib().vbref_get_neighbours(n, w, cnt, link)
+        self.L.vbref_get_neighbours(n, w, cnt, link)
         return cnt, link
 
     def weights(self, w):
         n = self.case.npfb
         wg = np.zeros((n, w, 2), order="F"); wl = np.zeros((n, w), order="F")
         kg = np.zeros((n, 4), order="F")
-        lib().vbref_get_weights(n, w, wg, wl, kg)
+        self.L.vbref_get_weights(n, w, wg, wl, kg)
         return wg, wl, kg
 
     def ushift(self):
         n = self.case.npfb
         us = np.zeros((n, 2), order="F")
-        lib().vbref_get_ushift(n, us)
+        self.L.vbref_get_ushift(n, us)
         return us
 
     def grad_p(self):
         n = self.case.npfb
         gp = np.zeros((n, 2), order="F")
-        lib().vbref_get_grad_p(n, gp)
+        self.L.vbref_get_grad_p(n, gp)
         return gp
 
     def surface(self, n=None):
@@ -161,7 +166,7 @@ class RefSolver:
         n = self.case.npfb if n is None else n
         ns = np.zeros(n, np.int32); co = np.zeros(n); dr = np.zeros(n)
         sn = np.zeros((n, 2), order="F"); ps = np.zeros(n)
-        lib().vbref_get_surface(n, ns, co, dr, sn, ps)
+        self.L.vbref_get_surface(n, ns, co, dr, sn, ps)
         return {"nsurf": ns, "conc": co, "divr": dr, "sn": sn, "psmooth": ps}
 
 
@@ -189,18 +194,20 @@ def bicgstab_open(a, ija, b, x0=None, itermax=1000, tol=1e-8):
 class RefRun:
     """The reference's module state as getdata (input.F90:12-180) leaves it after reading the run
     directory `path` (INDAT, IBOUND, IPART); steps and output() run in that directory, where the
-    reference writes its files (data_directory/ is created).  One per process at a time."""
+    reference writes its files (data_directory/ is created).  One per process at a time.
+    const_mod selects the reference's build: 2 viscoelastic (default), 1 Newtonian."""
 
-    def __init__(self, path):
+    def __init__(self, path, const_mod=2):
+        self.L = lib(const_mod)
         self.path = os.path.abspath(path)
         os.makedirs(os.path.join(self.path, "data_directory"), exist_ok=True)
         with _cwd(self.path):
-            lib().vbref_getdata()
+            self.L.vbref_getdata()
         self.params = self.run_params()
 
     def run_params(self):
         ints = np.zeros(10, np.int32, order="F"); dbls = np.zeros(17, order="F")
-        lib().vbref_get_run_params(ints, dbls)
+        self.L.vbref_get_run_params(ints, dbls)
         ki = ("i_restart", "i_open_domain", "i_shift_coeff", "np", "npfb", "nbS", "itime", "nb_patches",
               "nb_circles", "n_dump")
         kd = ("tmax", "dt_out", "h", "dx", "dt", "bf_x", "bf_y", "Ra", "Pr", "El", "beta", "dt_coef_cfl",
@@ -216,7 +223,7 @@ class RefRun:
             np.zeros(nb, np.int32, order="F")
         m = max(nc, 1)
         cc, cr, co, cv, ct = z(m, 2), z(m), z(m), z(m, 2), z(m)
-        lib().vbref_get_boundary(nb, nc, bn, be, bt, bv, bth, bp, cc, cr, co, cv, ct)
+        self.L.vbref_get_boundary(nb, nc, bn, be, bt, bv, bth, bp, cc, cr, co, cv, ct)
         return dict(b_node=bn, b_edge=be, b_type=bt, b_vel=bv, b_thermal=bth, b_periodic_parent=bp,
                     c_centre=cc[:nc], c_radius=cr[:nc], c_omega=co[:nc], c_vel=cv[:nc], c_thermal=ct[:nc])
 
@@ -225,35 +232,43 @@ class RefRun:
         out = {k: np.zeros(n, order="F") for k in ("x", "y", "u", "v", "theta", "trace", "p")}
         tau = np.zeros((n, 4), order="F")
         inb = np.zeros(n, np.int32, order="F")
-        lib().vbref_get_particles(n, out["x"], out["y"], out["u"], out["v"], out["theta"], tau,
+        self.L.vbref_get_particles(n, out["x"], out["y"], out["u"], out["v"], out["theta"], tau,
                                   out["trace"], out["p"], inb)
         out["tau"] = tau
         out["inbin"] = inb
         return out
 
     def set_dumps(self, next_dump, next_dump2, dt_out):
-        lib().vbref_set_dumps(next_dump, next_dump2, dt_out)
+        self.L.vbref_set_dumps(next_dump, next_dump2, dt_out)
 
     def step(self):
         with _cwd(self.path):
-            lib().vbref_step()
+            self.L.vbref_step()
 
     def vort_a_out(self, n=None):
         n = self.params["npfb"] if n is None else n
         vo = np.zeros(n, order="F"); ao = np.zeros(n, order="F")
-        lib().vbref_get_vort(n, vo, ao)
+        self.L.vbref_get_vort(n, vo, ao)
         return vo, ao
+
+    def surface(self, n=None):
+        """n_surf and conc of rows 1..n (the PART columns the open domain sets)."""
+        n = self.params["npfb"] if n is None else n
+        ns = np.zeros(n, np.int32); co = np.zeros(n); dr = np.zeros(n)
+        sn = np.zeros((n, 2), order="F"); ps = np.zeros(n)
+        self.L.vbref_get_surface(n, ns, co, dr, sn, ps)
+        return ns, co
 
     def open_units(self, restart=False):
         with _cwd(self.path):
-            lib().vbref_open_units(int(restart))
+            self.L.vbref_open_units(int(restart))
 
     def output(self):
         with _cwd(self.path):
-            lib().vbref_output()
+            self.L.vbref_output()
 
     def close_units(self):
-        lib().vbref_close_units()
+        self.L.vbref_close_units()
 
 
 def ld_write(path, vals):

@@ -25,6 +25,7 @@ struct vbo_ctx {
   /* parameters (input.F90:18-37) and derived constants (input.F90:164-172) */
   double h, dx, dv, Ra, Pr, El, beta, bf[2], cfl, visc, eps_ptt, time, dt, umax;
   int npfb, nbS, open_domain, shift_coeff, itime, cap;
+  int const_mod;          /* the reference's build: 1 Newtonian, else viscoelastic */
   double sup, sup2, uno, ad7, ad7h, eta2;
   double xmint, xmaxt, ymint, ymaxt;
   int ncx, ncy, nct;
@@ -110,6 +111,7 @@ vbo_ctx* vbo_create(const int* iprm, const double* prm, int nbp, const double* b
   c->beta = prm[6]; c->bf[0] = prm[7]; c->bf[1] = prm[8]; c->cfl = prm[9]; c->visc = prm[10];
   c->eps_ptt = prm[11]; c->time = prm[12];
   c->npfb = iprm[0]; c->nbS = iprm[1]; c->open_domain = iprm[2]; c->shift_coeff = iprm[3];
+  c->const_mod = iprm[6];
   c->itime = iprm[5];
   /* mirror rows need up to nmirror_esti extra rows (mirror_boundaries_mod.F90:53-54) */
   int esti = (c->npfb <= 500) ? 10 * c->npfb : 2 * c->npfb;
@@ -985,16 +987,17 @@ static void phase_predict(vbo_ctx* c) {
     grad_one(c, c->u, i, c->gvel + 4 * i);
     grad_one(c, c->v, i, c->gvel + 4 * i + 2);
   }
-  /* constitutive_eqn_evolve: all of 1..npfb including walls */
+  const int newtonian = c->const_mod == 1;
+  /* constitutive_eqn_evolve (viscoelastic build only, :43-46): all of 1..npfb including walls */
   const double wi = c->El * sqrt(c->Ra / c->Pr);
-  for (int i = 0; i < npfb; i++) {
+  for (int i = 0; i < npfb && !newtonian; i++) {
     double ti[4] = {c->t11[i], c->t12[i], c->t21[i], c->t22[i]}, to[4], trc;
     logconf_one(ti, c->trc[i], c->gvel + 4 * i, c->dt, wi, c->eps_ptt, c->kick[i], c->itime != 2, to, &trc);
     c->trc[i] = trc;
     c->t11[i] = to[0]; c->t12[i] = to[1]; c->t21[i] = to[2]; c->t22[i] = to[3];
   }
-  mirror_stress(c);
-  /* RHS, predictor_mod.F90:203-249 */
+  if (!newtonian) mirror_stress(c);
+  /* RHS: viscoelastic :79-100, Newtonian :58-77 (solvent and thermal terms only) */
   const double cs = (c->beta) * sqrt(c->Pr / c->Ra);
   const double cp = (1.0 - c->beta) * c->Pr / (c->Ra * c->El);
   const double ct = 1.0 / sqrt(c->Ra * c->Pr);
@@ -1007,6 +1010,12 @@ static void phase_predict(vbo_ctx* c) {
       size_t s = (size_t)k * npfb + i;
       int j = c->link[s];
       double uij = c->u[i] - c->u[j], vij = c->v[i] - c->v[j];
+      if (newtonian) {
+        ax = ax + cs * uij * c->wl[s];
+        ay = ay + cs * vij * c->wl[s];
+        at = at + ct * (c->th[i] - c->th[j]) * c->wl[s];
+        continue;
+      }
       double T11 = c->t11[j] - c->t11[i], T12 = c->t12[j] - c->t12[i];
       double T21 = c->t21[j] - c->t21[i], T22 = c->t22[j] - c->t22[i];
       double kw1 = c->k11[i] * c->wg1[s] + c->k12[i] * c->wg2[s];
@@ -1411,7 +1420,8 @@ void vbo_secondary_properties(const vbo_ctx* c, int n, double* vort, double* u_r
 void vbo_circle_forces(const vbo_ctx* c, int nc, double* force, int* counts) {
   const int npfb = c->npfb;
   const double cs = c->beta * sqrt(c->Pr / c->Ra);
-  const double cp = (1.0 - c->beta) * c->Pr / (c->Ra * c->El);
+  /* the Newtonian build has no polymeric stress (its c_p may be 0/0 there) */
+  const double cp = c->const_mod == 1 ? 0.0 : (1.0 - c->beta) * c->Pr / (c->Ra * c->El);
   for (int ib = 0; ib < nc && ib < c->ncirc; ib++) {
     const double cx = c->ccen[2 * ib], cy = c->ccen[2 * ib + 1], R = fabs(c->crad[ib]);
     int n = 0;

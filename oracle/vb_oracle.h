@@ -25,7 +25,8 @@ enum {
   VBO_WARN_BREAKDOWN = 5,      /* linear_solver.F90:332-335 */
 };
 
-/* iprm: npfb, nbS, i_open_domain, i_shift_coeff, capacity(rows), itime
+/* iprm: npfb, nbS, i_open_domain, i_shift_coeff, capacity(rows), itime, const_mod (the reference's
+ *       build: 1 Newtonian, 2 viscoelastic)
  * prm:  h, dx, dv, Ra, Pr, El, beta, bf_x, bf_y, dt_coef_cfl, dt_coef_visc, ve_nonlinearity, time
  * b_node is (nbp,2) row-major; c_centre, c_vel are (ncirc,2) row-major. */
 vbo_ctx* vbo_create(const int* iprm, const double* prm, int nbp, const double* b_node,

@@ -19,8 +19,9 @@ def pytest_configure(config):
                        stdout=subprocess.DEVNULL)
     ref_so = os.path.join(ROOT, "oracle", "_ref", "libvbref.so")
     gen = os.path.join(ROOT, "oracle", "_ref", "gen2D")
-    if not (os.path.exists(ref_so) and os.path.exists(gen)) and os.path.isdir("/root/reference/source"):
-        subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "ref", "gen"], check=False,
+    newt = os.path.join(ROOT, "oracle", "_ref", "newt", "libvbref.so")
+    if not all(map(os.path.exists, (ref_so, gen, newt))) and os.path.isdir("/root/reference/source"):
+        subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "ref", "refnewt", "gen"], check=False,
                        stdout=subprocess.DEVNULL)
 
 

@@ -24,6 +24,17 @@ sys.path.insert(0, ROOT)
 
 from vebisph_amd import cases  # noqa: E402
 
+def _generated(k, const_mod=2):
+    """Generator case k (vebisph_amd.generator) as a run reads it, for the given build."""
+    import tempfile
+    from vebisph_amd import generator, refio
+    d = tempfile.mkdtemp()
+    generator.write_run_dir(k, d)
+    c = refio.read_case_dir(d).case
+    c.const_mod = const_mod
+    return c
+
+
 CASES = {
     "poiseuille_2500": (lambda: cases.make("poiseuille_2500"), 2),
     "convection_1250": (lambda: cases.convection(nx=50, ny=25, name="convection_1250"), 2),
@@ -34,6 +45,9 @@ CASES = {
     # open domains: free surface, ramp rows (jitter 0.2 dx), Jacobi BiCGSTAB, surface shifting
     "fs_convection_11k": (lambda: cases.free_surface_convection(jitter=0.2), 2),
     "dam_break_3k": (lambda: cases.dam_break(jitter=0.2, shift_coeff=1), 2),
+    # the reference built as its Newtonian program (-Dconst_mod=1) on the generator's Newtonian cases
+    "gen1_dam_break_newtonian": (lambda: _generated(1, const_mod=1), 2),
+    "gen2_taylor_green_newtonian": (lambda: _generated(2, const_mod=1), 2),
 }
 FINAL_STATE = ("poiseuille_2500", "convection_1250")     # whole final state kept (small files)
 

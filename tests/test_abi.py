@@ -64,7 +64,7 @@ def _c_layout():
 #include "vebisph_gpu.h"
 int main(void) {
   printf("%zu %zu %zu %zu %zu %zu\n", sizeof(vb_params), offsetof(vb_params, body_force),
-         offsetof(vb_params, itime), offsetof(vb_params, serial_reductions),
+         offsetof(vb_params, itime), offsetof(vb_params, fluid_model),
          sizeof(vb_boundary), offsetof(vb_boundary, c_centre));
   return 0;
 }
@@ -79,7 +79,7 @@ int main(void) {
 
 def test_record_layout_c_vs_ctypes():
     P, B = _lib.VbParams, _lib.VbBoundary
-    assert _c_layout() == [C.sizeof(P), P.body_force.offset, P.itime.offset, P.serial_reductions.offset,
+    assert _c_layout() == [C.sizeof(P), P.body_force.offset, P.itime.offset, P.fluid_model.offset,
                            C.sizeof(B), B.c_centre.offset]
 
 
@@ -95,7 +95,7 @@ program layout
   type(vb_params), target :: p
   type(vb_boundary), target :: b
   print '(6i8)', c_sizeof(p), loc_diff(c_loc(p%body_force), c_loc(p)), loc_diff(c_loc(p%itime), c_loc(p)), &
-       loc_diff(c_loc(p%serial_reductions), c_loc(p)), c_sizeof(b), loc_diff(c_loc(b%c_centre), c_loc(b))
+       loc_diff(c_loc(p%fluid_model), c_loc(p)), c_sizeof(b), loc_diff(c_loc(b%c_centre), c_loc(b))
 contains
   integer function loc_diff(a, o)
     type(c_ptr), intent(in) :: a, o
@@ -116,18 +116,21 @@ end program
     assert got == _c_layout()
 
 
-def test_reference_glue_compiles_against_reference_modules():
+@pytest.mark.parametrize("const_mod", [2, 1])
+def test_reference_glue_compiles_against_reference_modules(const_mod):
     """divfree_reference_glue.F90 replaces the reference's module divfree inside the reference's own
-    build: it must compile against the reference's module files (oracle/_ref/obj, built from the
-    reference sources by oracle/Makefile)."""
-    ref_obj = os.path.join(ROOT, "oracle", "_ref", "obj")
+    build: it must compile against the reference's module files (oracle/_ref/obj, and oracle/_ref/newt/obj
+    for the Newtonian build, built from the reference sources by oracle/Makefile) with the build's
+    -Dconst_mod."""
+    ref_obj = os.path.join(ROOT, "oracle", "_ref", *(("newt", "obj") if const_mod == 1 else ("obj",)))
     if not os.path.exists(os.path.join(ref_obj, "common_2d.mod")) or not os.path.exists(build.FLANG):
         pytest.skip("reference module files not built here")
     if not os.path.exists(os.path.join(build.BUILD, "divfree_gpu.mod")):
         build.build_fortran()
-    out = os.path.join(build.BUILD, "glue_check")
+    out = os.path.join(build.BUILD, f"glue_check_{const_mod}")
     os.makedirs(out, exist_ok=True)
-    subprocess.run([build.FLANG, "-cpp", "-c", "-fPIC", "-module-dir", out, "-I", build.BUILD, "-I", ref_obj,
+    subprocess.run([build.FLANG, "-cpp", f"-Dconst_mod={const_mod}", "-c", "-fPIC", "-module-dir", out, "-I",
+                    build.BUILD, "-I", ref_obj,
                     os.path.join(ROOT, "vebisph_amd", "fortran", "divfree_reference_glue.F90"),
                     "-o", os.path.join(out, "glue.o")], check=True)
 

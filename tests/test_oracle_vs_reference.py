@@ -106,6 +106,21 @@ def test_generator_cases_phases_bit_exact(k, tmp_path):
     _run_phases(refio.read_case_dir(str(tmp_path)).case, steps=2)
 
 
+@pytest.mark.parametrize("k", [1, 2, 3, 5])
+def test_newtonian_build_phases_bit_exact(k, tmp_path):
+    """The reference built as its Newtonian program (-Dconst_mod=1, oracle/_ref/newt: no polymeric
+    stress, RHS predictor_mod.F90:58-77) on generator cases: the dam break and Taylor-Green vortices
+    it was written for (1, 2), Poiseuille flow (3) and the flow past a cylinder (5)."""
+    from oracle import refwrap
+    from vebisph_amd import generator, refio
+    if not refwrap.available(1):
+        pytest.skip("Newtonian reference build missing")
+    generator.write_run_dir(k, str(tmp_path))
+    case = refio.read_case_dir(str(tmp_path)).case
+    case.const_mod = 1
+    _run_phases(case, steps=2)
+
+
 def test_whole_step_equals_phases():
     """The harness's phase split equals the reference's own div_free (the real routine)."""
     from oracle import oraclewrap, refwrap

@@ -196,3 +196,43 @@ def test_reference_output_read_back_and_rewritten(tmp_path, name):
     # input.F90:126, would read it as the pressure)
     if name == "convection":
         assert np.abs(cols["theta"]).max() > 0 and not np.array_equal(cols["theta"], cols["p"])
+
+
+@needs_ref
+def test_newtonian_reference_output_read_back_and_rewritten(tmp_path):
+    """The reference built as its Newtonian program (-Dconst_mod=1) writes PART files of ten columns
+    and TEMP_REC.dat without stress lines (output.F90:46-50,69-71): our readers parse them to its
+    state bit for bit, and our writers (stress arrays NULL) write the same records."""
+    from oracle import refwrap
+    from vebisph_amd import generator
+    if not refwrap.available(1):
+        pytest.skip("Newtonian reference build missing")
+    generator.write_run_dir(1, str(tmp_path))                 # the dam break, a Newtonian case
+    R = refwrap.RefRun(str(tmp_path), const_mod=1)
+    R.open_units()
+    R.set_dumps(-1.0, -1.0, 1e-7)
+    R.step(); R.step()
+    R.output()
+    R.close_units()
+    dd = tmp_path / "data_directory"
+    st = R.get_state()
+    vort, a_out = R.vort_a_out()
+    n = st["x"].shape[0]
+    keep = st["inbin"] == 0
+    part = refio.read_part(str(dd / "PART0"), n + 10, newtonian=True)
+    for k in ("x", "y", "u", "v", "theta", "p"):
+        _same(part[k], st[k][keep])
+    _same(part["vort"], vort[keep])
+    assert "tau11" not in part and not np.any(part["a_out"])
+    with pytest.raises(refio.RefIOError):                     # ten values per record, not thirteen
+        refio.read_part(str(dd / "PART0"), n + 10)
+    hd, cols = refio.read_temp_rec(str(tmp_path / "TEMP_REC.dat"), newtonian=True)
+    assert (hd.itime, hd.npfb) == (2, int(keep.sum()))
+    for k in ("x", "y", "u", "v", "theta", "p"):
+        _same(cols[k], st[k][keep])
+    ns, conc = R.surface(n)                                  # the dam break is an open domain
+    _same(part["conc"], conc[keep]); _same(part["n_surf"], ns[keep])
+    assert np.all(conc[keep] > 0)
+    ours = str(tmp_path / "PART0.ours")
+    refio.write_part(ours, st, vort, a_out=a_out, conc=conc, n_surf=ns, newtonian=True)
+    _same_text_or_double(ours, str(dd / "PART0"))

@@ -40,7 +40,7 @@ class VbParams(C.Structure):
                 ("dt_coef_visc", C.c_double), ("ve_nonlinearity", C.c_double), ("time", C.c_double),
                 ("itime", C.c_int32), ("i_open_domain", C.c_int32), ("i_shift_coeff", C.c_int32),
                 ("precision", C.c_int32), ("reference_kick", C.c_int32),
-                ("serial_reductions", C.c_int32)]
+                ("serial_reductions", C.c_int32), ("fluid_model", C.c_int32)]
 
 
 class VbBoundary(C.Structure):

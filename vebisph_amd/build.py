@@ -106,7 +106,7 @@ def build_checkers() -> None:
     reference sources are present."""
     _run(["make", "-C", os.path.join(ROOT, "oracle"), "oracle"])
     if os.path.isdir("/root/reference/source"):
-        _run(["make", "-C", os.path.join(ROOT, "oracle"), "ref", "gen"])
+        _run(["make", "-C", os.path.join(ROOT, "oracle"), "ref", "refnewt", "gen"])
 
 
 def build_all() -> None:

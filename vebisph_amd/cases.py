@@ -53,6 +53,8 @@ class Case:
     ve_nonlinearity: float = 0.0
     i_open_domain: int = 0
     i_shift_coeff: int = 0
+    # the reference's compile-time fluid model: 2 viscoelastic (linear PTT / Oldroyd-B), 1 Newtonian
+    const_mod: int = 2
     steps: int = 1
     # boundaries (IBOUND, input.F90:51-81)
     b_node: np.ndarray = None        # (nb, 2)

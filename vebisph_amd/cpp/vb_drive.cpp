@@ -1,13 +1,14 @@
 // Stand-alone C++ driver of the C ABI (include/vebisph_gpu.h), two modes.
 //
-//   vb_drive run DIR [fp64|mixed] [MAX_STEPS]
+//   vb_drive run DIR [fp64|mixed] [MAX_STEPS] [newtonian]
 //
 // The reference's program (source/base/sph2D_incom.F90:51-129) on the GPU: reads DIR/INDAT, IBOUND
 // and IPART (or TEMP_REC.dat when INDAT says restart) with the reference's formats, steps while
 // time <= tmax (at most MAX_STEPS steps), and writes what the reference writes: data_directory/PART<n>
 // and TEMP_REC.dat every dt_out (output.F90:17-82), data_directory/TIME and LS every step and
 // TIME_OUT at each dump (output.F90:55,86-87), the RMS velocity to fort.192 when the reference's
-// secondary_properties runs (div_free.F90:152-185).  Formats: include/vebisph_refio.h.
+// secondary_properties runs (div_free.F90:152-185).  Formats: include/vebisph_refio.h.  `newtonian` runs
+// the reference's Newtonian program (built with -Dconst_mod=1), with its file layouts.
 //
 //   vb_drive CASE_FILE NSTEPS OUT_FILE
 //
@@ -79,7 +80,9 @@ struct Columns {
 };
 
 // sph2D_incom.F90:51-129 with getdata (input.F90:12-180) and output (output.F90), on the GPU path.
-int run_dir(const std::string& dir, int precision, long max_steps) {
+// newtonian: the reference built with -Dconst_mod=1 (no polymeric stress; its PART files have ten
+// columns and its TEMP_REC.dat no stress lines, output.F90:46-50,69-71; input.F90:128-130)
+int run_dir(const std::string& dir, int precision, long max_steps, bool newtonian) {
   auto at = [&](const char* f) { return dir + "/" + f; };
   vbio_indat ind;
   vbio_ibound ib;
@@ -112,8 +115,9 @@ int run_dir(const std::string& dir, int precision, long max_steps) {
   std::vector<int32_t> nsurf(npfb, 0);
   if (ind.i_restart) {
     if (vbio_read_temp_rec(at("TEMP_REC.dat").c_str(), &hd, npfb, s.x.data(), s.y.data(), s.u.data(), s.v.data(),
-                           s.th.data(), s.p.data(), &dv, nsurf.data(), s.t11.data(), s.t12.data(), s.t21.data(),
-                           s.t22.data()))
+                           s.th.data(), s.p.data(), &dv, nsurf.data(), newtonian ? nullptr : s.t11.data(),
+                           newtonian ? nullptr : s.t12.data(), newtonian ? nullptr : s.t21.data(),
+                           newtonian ? nullptr : s.t22.data()))
       return io_fail();
     // the file has no trace column: trace_conf = A11 + A22 with A = tau + I (predictor_mod.F90:240-242)
     for (int i = 0; i < npfb; i++) s.trc[i] = (s.t11[i] + 1.0) + (s.t22[i] + 1.0);
@@ -131,6 +135,7 @@ int run_dir(const std::string& dir, int precision, long max_steps) {
   prm.dt_coef_cfl = ind.dt_coef_cfl; prm.dt_coef_visc = ind.dt_coef_visc;
   prm.ve_nonlinearity = ind.ve_nonlinearity; prm.time = time; prm.itime = itime;
   prm.i_open_domain = ind.i_open_domain; prm.i_shift_coeff = ind.i_shift_coeff;
+  prm.fluid_model = newtonian ? 1 : 2;
   prm.precision = precision; prm.reference_kick = 1;
   std::vector<double> bn(2 * ib.nb_patches), cc(2 * ib.nb_circles), cv(2 * ib.nb_circles);
   for (int i = 0; i < ib.nb_patches; i++) { bn[2 * i] = ib.b_node[i][0]; bn[2 * i + 1] = ib.b_node[i][1]; }
@@ -170,6 +175,7 @@ int run_dir(const std::string& dir, int precision, long max_steps) {
   double next_dump = time, next_dump2 = time, dt = ind.dt, ls_res = 0.0;
   int ls_iters = 0, n_inbin = 0, np = ind.i_restart ? hd.np : ind.np;
   std::vector<double> vort(npfb, 0.0), conc(npfb, 0.0);   // conc: set by the open domain's steps only
+  const std::vector<double> zeros(npfb, 0.0);
   bool stepped = false;
   int rc = 0;
   auto scalars = [&]() {
@@ -188,18 +194,21 @@ int run_dir(const std::string& dir, int precision, long max_steps) {
       if (open && stepped && vb_free_surface(ctx, npfb, nsurf.data(), conc.data()) != VB_OK)
         return fail(vb_last_error(ctx));
       const std::string part = dd + "/PART" + std::to_string(idp);
-      // a_out = trace_conf (predictor_mod.F90:243); n_surf = 0 in a closed domain
+      // a_out = trace_conf (predictor_mod.F90:243; the Newtonian build never sets it: 0); n_surf = 0 in
+      // a closed domain
       if (vbio_write_part(part.c_str(), npfb, s.x.data(), s.y.data(), s.u.data(), s.v.data(), s.th.data(),
-                          s.p.data(), conc.data(), s.trc.data(), nsurf.data(), vort.data(), s.t11.data(),
-                          s.t12.data(), s.t22.data(), s.inbin.data()))
+                          s.p.data(), conc.data(), newtonian ? zeros.data() : s.trc.data(), nsurf.data(),
+                          vort.data(), newtonian ? nullptr : s.t11.data(), newtonian ? nullptr : s.t12.data(),
+                          newtonian ? nullptr : s.t22.data(), s.inbin.data()))
         return io_fail();
       const double tout[4] = {time, (double)npfb, (double)itime, dt};
       if (vbio_append_record(f_tout.c_str(), "didd", tout)) return io_fail();
       vbio_restart h{itime, time, n_dump, np - n_inbin, npfb - n_inbin, nbS, 0,
                      xb[0], xb[1], xb[2], xb[3], ind.dx, 0.0};
       if (vbio_write_temp_rec(f_rec.c_str(), &h, npfb, s.x.data(), s.y.data(), s.u.data(), s.v.data(),
-                              s.th.data(), s.p.data(), dv, nsurf.data(), s.t11.data(), s.t12.data(),
-                              s.t21.data(), s.t22.data(), s.inbin.data()))
+                              s.th.data(), s.p.data(), dv, nsurf.data(), newtonian ? nullptr : s.t11.data(),
+                              newtonian ? nullptr : s.t12.data(), newtonian ? nullptr : s.t21.data(),
+                              newtonian ? nullptr : s.t22.data(), s.inbin.data()))
         return io_fail();
     }
     double m[3];
@@ -248,15 +257,22 @@ int main(int argc, char** argv) {
   if (argc >= 3 && strcmp(argv[1], "run") == 0) {
     int precision = VB_PREC_FP64;
     long max_steps = 1L << 40;
-    if (argc >= 4) {
-      if (strcmp(argv[3], "mixed") == 0) precision = VB_PREC_MIXED;
-      else if (strcmp(argv[3], "fp64") != 0) return fail("precision must be fp64 or mixed");
+    bool newtonian = false;
+    std::vector<const char*> pos;
+    for (int k = 3; k < argc; k++) {
+      if (strcmp(argv[k], "newtonian") == 0) newtonian = true;
+      else pos.push_back(argv[k]);
     }
-    if (argc >= 5) max_steps = std::stol(argv[4]);
-    if (argc > 5) return fail("usage: vb_drive run DIR [fp64|mixed] [MAX_STEPS]");
-    return run_dir(argv[2], precision, max_steps);
+    if (pos.size() >= 1) {
+      if (strcmp(pos[0], "mixed") == 0) precision = VB_PREC_MIXED;
+      else if (strcmp(pos[0], "fp64") != 0) return fail("precision must be fp64 or mixed");
+    }
+    if (pos.size() >= 2) max_steps = std::stol(pos[1]);
+    if (pos.size() > 2) return fail("usage: vb_drive run DIR [fp64|mixed] [MAX_STEPS] [newtonian]");
+    return run_dir(argv[2], precision, max_steps, newtonian);
   }
-  if (argc != 4) return fail("usage: vb_drive run DIR [fp64|mixed] [MAX_STEPS] | vb_drive CASE_FILE NSTEPS OUT_FILE");
+  if (argc != 4)
+    return fail("usage: vb_drive run DIR [fp64|mixed] [MAX_STEPS] [newtonian] | vb_drive CASE_FILE NSTEPS OUT_FILE");
   const int nsteps = std::stoi(argv[2]);
   FILE* f = fopen(argv[1], "rb");
   if (!f) return fail("cannot open the case file");

@@ -435,40 +435,48 @@ int32_t vbio_write_part(const char* path, int32_t n, const double* x, const doub
                         const double* v, const double* theta, const double* p, const double* conc,
                         const double* a_out, const int32_t* n_surf, const double* vort, const double* tau11,
                         const double* tau12, const double* tau22, const int32_t* inbin) {
-  if (n < 0 || !x || !y || !u || !v || !theta || !p || !conc || !a_out || !n_surf || !vort || !tau11 ||
-      !tau12 || !tau22 || !inbin)
+  // no stress arrays: the Newtonian build's ten columns (output.F90:46-47)
+  const bool newt = !tau11 && !tau12 && !tau22;
+  if (n < 0 || !x || !y || !u || !v || !theta || !p || !conc || !a_out || !n_surf || !vort ||
+      (!newt && (!tau11 || !tau12 || !tau22)) || !inbin)
     return fail("vbio_write_part: bad arguments");
   Out o(path, "w");
   if (!o.open()) return cant_open(path);
   Record r;
   for (int i = 0; i < n; i++) {
     if (inbin[i]) continue;
-    r.d(x[i]).d(y[i]).d(u[i]).d(v[i]).d(theta[i]).d(p[i]).d(conc[i]).d(a_out[i]).i(n_surf[i]).d(vort[i])
-        .d(tau11[i]).d(tau12[i]).d(tau22[i]).put(o.f());
+    r.d(x[i]).d(y[i]).d(u[i]).d(v[i]).d(theta[i]).d(p[i]).d(conc[i]).d(a_out[i]).i(n_surf[i]).d(vort[i]);
+    if (!newt) r.d(tau11[i]).d(tau12[i]).d(tau22[i]);
+    r.put(o.f());
   }
   return o.close();
 }
 
-int32_t vbio_read_part(const char* path, int32_t cap, int32_t* n, double* cols) {
-  if (cap < 0 || !n || (cap > 0 && !cols)) return fail("vbio_read_part: bad arguments");
+int32_t vbio_read_part_ncols(const char* path, int32_t ncols, int32_t cap, int32_t* n, double* cols) {
+  if ((ncols != 10 && ncols != 13) || cap < 0 || !n || (cap > 0 && !cols))
+    return fail("vbio_read_part: bad arguments");
   Reader r(path);
   if (!r.in.open()) return cant_open(path);
   *n = 0;
   double row[13];
-  // a list of 13 values per particle until the end of the file
+  // a list of ncols values per particle until the end of the file
   for (;;) {
     r.what = "PART record";
-    if (!r.in.read(13, r.tok)) {
+    if (!r.in.read(ncols, r.tok)) {
       if (r.tok.empty()) break;                       // clean end of file
       return r.err("incomplete particle record");
     }
-    for (int k = 0; k < 13; k++)
+    for (int k = 0; k < ncols; k++)
       if (!parse_real(r.tok[k], &row[k])) return r.err("not a number: '" + r.tok[k] + "'");
     if (*n >= cap) return fail(std::string(path) + ": more than cap particles");
-    for (int k = 0; k < 13; k++) cols[(size_t)k * cap + *n] = row[k];
+    for (int k = 0; k < ncols; k++) cols[(size_t)k * cap + *n] = row[k];
     (*n)++;
   }
   return 0;
+}
+
+int32_t vbio_read_part(const char* path, int32_t cap, int32_t* n, double* cols) {
+  return vbio_read_part_ncols(path, 13, cap, n, cols);
 }
 
 // output.F90:60-81
@@ -477,8 +485,10 @@ int32_t vbio_write_temp_rec(const char* path, const vbio_restart* hd, int32_t n,
                             const double* p, double dv, const int32_t* n_surf, const double* tau11,
                             const double* tau12, const double* tau21, const double* tau22,
                             const int32_t* inbin) {
-  if (!hd || n < 0 || !x || !y || !u || !v || !theta || !p || !n_surf || !tau11 || !tau12 || !tau21 || !tau22 ||
-      !inbin)
+  // no stress arrays: the Newtonian build's file, without the stress lines (output.F90:69-71)
+  const bool newt = !tau11 && !tau12 && !tau21 && !tau22;
+  if (!hd || n < 0 || !x || !y || !u || !v || !theta || !p || !n_surf ||
+      (!newt && (!tau11 || !tau12 || !tau21 || !tau22)) || !inbin)
     return fail("vbio_write_temp_rec: bad arguments");
   Out o(path, "w");
   if (!o.open()) return cant_open(path);
@@ -489,7 +499,7 @@ int32_t vbio_write_temp_rec(const char* path, const vbio_restart* hd, int32_t n,
   for (int i = 0; i < n; i++) {
     if (inbin[i]) continue;
     r.d(x[i]).d(y[i]).d(u[i]).d(v[i]).d(theta[i]).d(p[i]).d(dv).i(n_surf[i]).put(o.f());
-    r.d(tau11[i]).d(tau12[i]).d(tau21[i]).d(tau22[i]).put(o.f());
+    if (!newt) r.d(tau11[i]).d(tau12[i]).d(tau21[i]).d(tau22[i]).put(o.f());
   }
   return o.close();
 }
@@ -516,12 +526,14 @@ int32_t vbio_read_temp_rec(const char* path, vbio_restart* hd, int32_t cap, doub
   hd->av_conc = d6[5];
   if (cap == 0) return 0;
   if (hd->npfb < 0 || hd->npfb > cap) return fail(std::string(path) + ": npfb exceeds cap");
-  if (!x || !y || !u || !v || !theta || !p || !dv || !n_surf || !tau11 || !tau12 || !tau21 || !tau22)
+  const bool newt = !tau11 && !tau12 && !tau21 && !tau22;     // the Newtonian file (input.F90:128-130)
+  if (!x || !y || !u || !v || !theta || !p || !dv || !n_surf || (!newt && (!tau11 || !tau12 || !tau21 || !tau22)))
     return fail("vbio_read_temp_rec: bad arguments");
   for (int i = 0; i < hd->npfb; i++) {
     double* d8[8] = {&x[i], &y[i], &u[i], &v[i], &theta[i], &p[i], dv, nullptr};
     int32_t* i8[8] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, &n_surf[i]};
     if (int32_t s = r.mixed("x y u v theta p dv n_surf", "dddddddi", d8, i8)) return s;
+    if (newt) continue;
     double t4[4];
     if (int32_t s = r.reals("tau11 tau12 tau21 tau22", 4, t4)) return s;
     tau11[i] = t4[0]; tau12[i] = t4[1]; tau21[i] = t4[2]; tau22[i] = t4[3];

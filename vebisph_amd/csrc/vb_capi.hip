@@ -170,6 +170,10 @@ vb_status vb_create(const vb_params* prm, const vb_boundary* bnd, int32_t npfb, 
     c->err = "i_open_domain and i_shift_coeff must be 0 or 1";
     s = VB_ERR_ARG;
   }
+  if (prm->fluid_model < 0 || prm->fluid_model > 2) {
+    c->err = "fluid_model must be 0 or 2 (viscoelastic) or 1 (Newtonian)";
+    s = VB_ERR_ARG;
+  }
   if (s == VB_OK) s = build_geom(c, prm, bnd);
   if (s == VB_OK) s = hip_check(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "stream");
   const size_t cap = c->cap, n = npfb, nw = (size_t)npfb * kNplink;

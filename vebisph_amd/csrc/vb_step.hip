@@ -726,6 +726,7 @@ __global__ void k_mirror_stress(Geom g, State st, const Scal* sc) {
 }
 
 // RHS, buoyancy, u* and theta* (predictor_mod.F90:203-282), written to the alternate buffers
+template <bool NEWTONIAN>
 __global__ void k_rhs(Geom g, State st, const int* cnt, const int* link, const double* wg1,
                       const double* wg2, const double* wl, const double* k11, const double* k12,
                       const double* k21, const double* k22, double cs, double cp, double ct,
@@ -735,6 +736,22 @@ __global__ void k_rhs(Geom g, State st, const int* cnt, const int* link, const d
   const int npfb = g.npfb, n = cnt[i];
   const double ui = st.u[i], vi = st.v[i], thi = st.th[i];
   if (i < g.nbS) { un[i] = ui; vn[i] = vi; thn[i] = thi; return; }
+  if (NEWTONIAN) {                               // predictor_mod.F90:58-77: solvent and thermal only
+    double ax = 0.0, ay = 0.0, at = 0.0;
+    for_slots(link, i, n, npfb, [&](int j) { return make_double4(st.u[j], st.v[j], st.th[j], 0.0); },
+              [&](int s, int, double4 q) {
+      const double l = wl[s];
+      ax = ax + cs * (ui - q.x) * l;
+      ay = ay + cs * (vi - q.y) * l;
+      at = at + ct * (thi - q.z) * l;
+    });
+    ay = ay + thi;
+    const double dt = sc->dt;
+    un[i] = ui + ax * dt;
+    vn[i] = vi + ay * dt;
+    thn[i] = thi + at * dt;
+    return;
+  }
   const double T11i = st.t11[i], T12i = st.t12[i], T21i = st.t21[i], T22i = st.t22[i];
   const double K11 = k11[i], K12 = k12[i], K21 = k21[i], K22 = k22[i];
   double ax = 0.0, ay = 0.0, at = 0.0;
@@ -1144,14 +1161,23 @@ vb_status launch_predict(vb_ctx* c) {
   const vb_params& p = c->prm;
   double wi = p.El * sqrt(p.Ra / p.Pr);
   int use_kick = p.reference_kick && c->itime != 2;
-  VB_LAUNCH(c, k_grad_logconf, grid_1d(c->npfb), c->g, c->st, c->cnt, c->link, c->wg1, c->wg2, c->k11,
-            c->k12, c->k21, c->k22, c->kick, use_kick, wi, p.ve_nonlinearity, c->scal);
-  VB_LAUNCH(c, k_mirror_stress, grid_1d(c->nmirror_host), c->g, c->st, c->scal);
+  const bool newtonian = p.fluid_model == 1;
+  // the velocity gradient feeds only constitutive_eqn_evolve (and its mirror stresses): the
+  // Newtonian build computes it and never reads it (predictor_mod.F90:35-46), so it is skipped
+  if (!newtonian) {
+    VB_LAUNCH(c, k_grad_logconf, grid_1d(c->npfb), c->g, c->st, c->cnt, c->link, c->wg1, c->wg2, c->k11,
+              c->k12, c->k21, c->k22, c->kick, use_kick, wi, p.ve_nonlinearity, c->scal);
+    VB_LAUNCH(c, k_mirror_stress, grid_1d(c->nmirror_host), c->g, c->st, c->scal);
+  }
   double cs = (p.beta) * sqrt(p.Pr / p.Ra);
   double cp = (1.0 - p.beta) * p.Pr / (p.Ra * p.El);
   double ct = 1.0 / sqrt(p.Ra * p.Pr);
-  VB_LAUNCH(c, k_rhs, grid_1d(c->npfb), c->g, c->st, c->cnt, c->link, c->wg1, c->wg2, c->wl, c->k11,
-            c->k12, c->k21, c->k22, cs, cp, ct, c->u_alt, c->v_alt, c->th_alt, c->scal);
+  if (newtonian)
+    VB_LAUNCH(c, k_rhs<true>, grid_1d(c->npfb), c->g, c->st, c->cnt, c->link, c->wg1, c->wg2, c->wl, c->k11,
+              c->k12, c->k21, c->k22, cs, cp, ct, c->u_alt, c->v_alt, c->th_alt, c->scal);
+  else
+    VB_LAUNCH(c, k_rhs<false>, grid_1d(c->npfb), c->g, c->st, c->cnt, c->link, c->wg1, c->wg2, c->wl, c->k11,
+              c->k12, c->k21, c->k22, cs, cp, ct, c->u_alt, c->v_alt, c->th_alt, c->scal);
   return VB_OK;
 }
 
@@ -1194,7 +1220,8 @@ static double* scal_field(vb_ctx* c, size_t off) { return reinterpret_cast<doubl
 vb_status launch_circle_forces(vb_ctx* c) {
   const vb_params& p = c->prm;
   const double cs = (p.beta) * sqrt(p.Pr / p.Ra);            // the coefficients of launch_predict
-  const double cp = (1.0 - p.beta) * p.Pr / (p.Ra * p.El);
+  // the Newtonian build has no polymeric stress (its c_p may be 0/0 there)
+  const double cp = p.fluid_model == 1 ? 0.0 : (1.0 - p.beta) * p.Pr / (p.Ra * p.El);
   hipLaunchKernelGGL(k_circle_forces, dim3(1), dim3(kDiagThreads), 0, c->stream, c->g, c->st, c->cnt, c->link,
                      c->wg1, c->wg2, c->k11, c->k12, c->k21, c->k22, cs, cp, scal_field(c, offsetof(Scal, circ)));
   VB_HIP(c, hipGetLastError());

@@ -34,6 +34,9 @@ module divfree
   real(c_double), allocatable, target, save :: bnode_rm(:), bvel_c(:), bth_c(:), ccen_rm(:), crad_c(:), &
        comg_c(:), cvel_rm(:), cth_c(:)
   integer(c_int32_t), allocatable, target, save :: btype_c(:), bpp_c(:)
+#if const_mod==1
+  real(c_double), allocatable, save :: tau_nt(:,:,:), trace_nt(:)
+#endif
 
 contains
 
@@ -64,7 +67,11 @@ contains
        flush(192)
     end if
     if (time + dt > next_dump .or. time + dt > tmax) then
+#if const_mod==1
+       call vbf_download(int(npfb, c_int32_t), r, u, theta, tau_nt, trace_nt, p, inbin)
+#else
        call vbf_download(int(npfb, c_int32_t), r, u, theta, tau_p, trace_conf, p, inbin)
+#endif
        !! the open domain's n_surf and conc columns of the PART files (freesurf_mod.F90:29-37, :95-100)
        if (i_open_domain .eq. 1) s = vb_free_surface(vbf_ctx, int(npfb, c_int32_t), n_surf, conc)
     end if
@@ -84,6 +91,12 @@ contains
     prm%i_open_domain = int(i_open_domain, c_int32_t)
     prm%i_shift_coeff = int(i_shift_coeff, c_int32_t)
     prm%precision = vbf_precision
+    !! the fluid model is the reference build's compile-time switch (Makefile: -Dconst_mod)
+#if const_mod==1
+    prm%fluid_model = 1
+#else
+    prm%fluid_model = 2
+#endif
     prm%reference_kick = 1
     prm%serial_reductions = 0
 
@@ -109,7 +122,14 @@ contains
     end if
 
     call vbf_init(prm, bnd, int(npfb, c_int32_t), int(nbS, c_int32_t))
+#if const_mod==1
+    !! the Newtonian build allocates no polymeric stress (input.F90:109-112): zero stand-ins
+    allocate(tau_nt(size(r,1),2,2), trace_nt(size(r,1)))
+    tau_nt = 0.0d0; trace_nt = 2.0d0
+    call vbf_upload(int(npfb, c_int32_t), r, u, theta, tau_nt, trace_nt, p, inbin)
+#else
     call vbf_upload(int(npfb, c_int32_t), r, u, theta, tau_p, trace_conf, p, inbin)
+#endif
     started = .true.
   end subroutine start
 

@@ -29,7 +29,7 @@ module vebisph_gpu
      real(c_double) :: body_force(2)
      real(c_double) :: dt_coef_cfl, dt_coef_visc, ve_nonlinearity, time
      integer(c_int32_t) :: itime, i_open_domain, i_shift_coeff, precision, reference_kick, &
-          serial_reductions
+          serial_reductions, fluid_model
   end type vb_params
 
   !! IBOUND boundary description (input.F90:51-81); arrays row-major, see vebisph_gpu.h

@@ -53,6 +53,13 @@ class Restart(C.Structure):
 _lib = None
 
 
+class _OptD:
+    """an FP64 array argument that may be None (a NULL pointer)"""
+    @classmethod
+    def from_param(cls, a):
+        return None if a is None else _d.from_param(a)
+
+
 def lib():
     global _lib
     if _lib is None:
@@ -67,10 +74,11 @@ def lib():
         L.vbio_check_ibound.argtypes = [C.POINTER(Ibound)]
         L.vbio_read_ipart.argtypes = [C.c_char_p, C.c_int32] + [_d] * 5 + [C.POINTER(C.c_double)]
         L.vbio_write_ipart.argtypes = [C.c_char_p, C.c_int32] + [_d] * 5 + [C.c_double]
-        L.vbio_write_part.argtypes = [C.c_char_p, C.c_int32] + [_d] * 8 + [_i] + [_d] * 4 + [_i]
+        L.vbio_write_part.argtypes = [C.c_char_p, C.c_int32] + [_d] * 8 + [_i] + [_d] + [_OptD] * 3 + [_i]
         L.vbio_read_part.argtypes = [C.c_char_p, C.c_int32, C.POINTER(C.c_int32), _d]
+        L.vbio_read_part_ncols.argtypes = [C.c_char_p, C.c_int32, C.c_int32, C.POINTER(C.c_int32), _d]
         L.vbio_write_temp_rec.argtypes = ([C.c_char_p, C.POINTER(Restart), C.c_int32] + [_d] * 6
-                                          + [C.c_double, _i] + [_d] * 4 + [_i])
+                                          + [C.c_double, _i] + [_OptD] * 4 + [_i])
         L.vbio_read_temp_rec.argtypes = ([C.c_char_p, C.POINTER(Restart), C.c_int32] + [C.c_void_p] * 12)
         L.vbio_append_record.argtypes = [C.c_char_p, C.c_char_p, _d]
         L.vbio_format_real.argtypes = [C.c_double, C.c_char_p, C.c_int32]
@@ -189,29 +197,34 @@ def write_case_dir(path, case: cases.Case, tmax: float, dt_out: float, dt: float
 PART_COLUMNS = ("x", "y", "u", "v", "theta", "p", "conc", "a_out", "n_surf", "vort", "tau11", "tau12", "tau22")
 
 
-def read_part(path, cap):
-    """The 13 columns of a PART file (output.F90:44-51) as a dict of arrays."""
-    cols = np.zeros(13 * cap)
+def read_part(path, cap, newtonian=False):
+    """The 13 columns of a PART file (output.F90:44-51) as a dict of arrays (the Newtonian build's
+    file: the first 10)."""
+    nc = 10 if newtonian else 13
+    cols = np.zeros(nc * cap)
     n = C.c_int32(0)
-    _chk(lib().vbio_read_part(_b(path), cap, C.byref(n), cols))
-    cols = cols.reshape(13, cap)[:, :n.value]
+    _chk(lib().vbio_read_part_ncols(_b(path), nc, cap, C.byref(n), cols))
+    cols = cols.reshape(nc, cap)[:, :n.value]
     return dict(zip(PART_COLUMNS, cols))
 
 
-def write_part(path, state, vort, a_out=None, conc=None, n_surf=None):
-    """A PART file of a state dict (Solver.get_state layout) with the reference's columns."""
+def write_part(path, state, vort, a_out=None, conc=None, n_surf=None, newtonian=False):
+    """A PART file of a state dict (Solver.get_state layout) with the reference's columns (the
+    Newtonian build's: without the stress columns)."""
     n = state["x"].shape[0]
     z = np.zeros(n)
     tau = state["tau"]
+    ts = (None, None, None) if newtonian else (_c(tau[:, 0]), _c(tau[:, 1]), _c(tau[:, 3]))
     _chk(lib().vbio_write_part(_b(path), n, _c(state["x"]), _c(state["y"]), _c(state["u"]), _c(state["v"]),
                                _c(state["theta"]), _c(state["p"]), _c(z if conc is None else conc),
                                _c(state["trace"] if a_out is None else a_out),
                                _c(np.zeros(n) if n_surf is None else n_surf, np.int32), _c(vort),
-                               _c(tau[:, 0]), _c(tau[:, 1]), _c(tau[:, 3]), _c(state["inbin"], np.int32)))
+                               *ts, _c(state["inbin"], np.int32)))
 
 
-def read_temp_rec(path):
-    """TEMP_REC.dat: the header and the particle columns (eight values per particle line)."""
+def read_temp_rec(path, newtonian=False):
+    """TEMP_REC.dat: the header and the particle columns (eight values per particle line; the
+    Newtonian build's file has no stress lines, and the tau columns come back as 0)."""
     hd = Restart()
     _chk(lib().vbio_read_temp_rec(_b(path), C.byref(hd), 0, *([None] * 12)))
     n = hd.npfb
@@ -221,8 +234,9 @@ def read_temp_rec(path):
     ptr = lambda a: a.ctypes.data_as(C.c_void_p)
     _chk(lib().vbio_read_temp_rec(_b(path), C.byref(hd), n, ptr(cols["x"]), ptr(cols["y"]), ptr(cols["u"]),
                                   ptr(cols["v"]), ptr(cols["theta"]), ptr(cols["p"]), C.cast(C.byref(dv), C.c_void_p),
-                                  ptr(ns), ptr(cols["tau11"]), ptr(cols["tau12"]), ptr(cols["tau21"]),
-                                  ptr(cols["tau22"])))
+                                  ptr(ns), *([None] * 4 if newtonian else
+                                             [ptr(cols["tau11"]), ptr(cols["tau12"]), ptr(cols["tau21"]),
+                                              ptr(cols["tau22"])])))
     cols["n_surf"] = ns
     cols["dv"] = dv.value
     return hd, cols

@@ -49,6 +49,7 @@ def make_params(case, precision=_lib.PREC_FP64, reference_kick=True, time=0.0, i
     p.precision = int(precision)
     p.reference_kick = 1 if reference_kick else 0
     p.serial_reductions = 1 if serial_reductions else 0
+    p.fluid_model = getattr(case, "const_mod", 2)
     return p
 
 
