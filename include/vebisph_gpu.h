@@ -160,6 +160,14 @@ vb_status vb_div_free(vb_ctx* ctx);
  * computes them between the velocity update and the final advection, which changes neither, so
  * call this after vb_pressure_update or vb_div_free.  vort and u_rms may be NULL. */
 vb_status vb_secondary_properties(vb_ctx* ctx, int32_t n, double* vort, double* u_rms);
+/* The assembled PPE as an operator, for an adjoint solve (no counterpart in the reference, which has
+ * no adjoint): y = A x (transpose = 0) or y = A^T x (transpose = 1), x and y host arrays of n = npfb
+ * rows, A the matrix the last vb_ppe_solve assembled from the step's lists
+ * (ppe_solve_in_house_simple_dirichlet.F90:55-138, mirrors folded onto their parents, Dirichlet and
+ * ramp rows included).  A^T comes from a transposed map built on the first transpose call after each
+ * assembly, with a fixed summation order.  VB_ERR_ARG once the lists are rebuilt by the next step's
+ * vb_particle_allocation, or after vb_sparse_pbicgstab* replaced the stored matrix. */
+vb_status vb_ppe_operator(vb_ctx* ctx, int32_t transpose, int32_t n, const double* x, double* y);
 /* The columns conc and n_surf of the PART files (output.F90:47-49) for rows 1..n, n <= np: the
  * free-surface flag and, in an open domain, the concentration (calc_part_dist_vars,
  * freesurf_mod.F90:95-100) of the last step (rows past npfb: mirror flags, conc 0; conc is 0 in a

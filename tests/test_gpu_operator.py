@@ -1,0 +1,92 @@
+"""vb_ppe_operator: the assembled PPE as an operator, y = A x and y = A^T x (the operator an adjoint
+solve needs; the reference has no adjoint, so A^T is pinned by its definition).  A is the matrix the
+GPU's own vb_ppe_solve assembled (exported with vb_export_ppe and itself pinned against the oracle by
+tests/test_gpu_parity.py); the reference values are scipy's products with that matrix and its
+transpose.  Tolerance: elementwise 1e-13 of sum_j |A_ij| |x_j| (the summation orders differ), and the
+adjoint identity <A x, y> = <x, A^T y> to 1e-12 of its terms' scale.
+
+Cases cover every source of entries: walls and periodic mirrors (Poiseuille), circle mirrors
+(cylinder), the open domain's Dirichlet rows (1e30 diagonals) and ppe_smooth ramp rows, which make A
+unsymmetric (free-surface convection), and wall corners (dam break)."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from vebisph_amd import cases, solver
+
+pytestmark = pytest.mark.gpu
+
+CASES = {
+    "poiseuille": lambda: cases.make("poiseuille_2500"),
+    "cylinder": lambda: cases.cylinder(nx=160, ny=80, name="cylinder_13k"),
+    "fs_convection": lambda: cases.free_surface_convection(jitter=0.2),
+    "dam_break": lambda: cases.dam_break(jitter=0.2, shift_coeff=1),
+}
+
+
+def _matrix(G):
+    w = G.scalars()["maxneighbours"]
+    diag, col, val, rhs = G.ppe_ell(w)
+    cnt = G.neighbours()[0]
+    a, ija = solver.ell_to_nr(diag, col, val, cnt)
+    n = diag.shape[0]
+    lens = ija[1:n + 1] - ija[:n]
+    rows = np.concatenate([np.arange(n), np.repeat(np.arange(n), lens)])
+    cols = np.concatenate([np.arange(n), ija[ija[0] - 1:ija[n] - 1] - 1])
+    vals = np.concatenate([a[:n], a[ija[0] - 1:ija[n] - 1]])
+    return sp.csr_matrix((vals, (rows, cols)), shape=(n, n))     # duplicate slots summed
+
+
+def _close(got, want, scale):
+    err = np.abs(got - want)
+    bad = err > 1e-13 * scale + 1e-300
+    assert not bad.any(), f"{bad.sum()} rows off, worst {np.max(err / np.maximum(scale, 1e-300)):.3e}"
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_ppe_operator_and_transpose(gpu_lib, name):
+    case = CASES[name]()
+    print(f"case {case.name} npfb={case.npfb}", flush=True)
+    G = solver.Solver(case)
+    G.div_free()
+    for k in range(1, 11):
+        G.phase(k)
+    A = _matrix(G)
+    absA = abs(A)
+    rng = np.random.default_rng(11)
+    x = rng.standard_normal(case.npfb)
+    y = rng.standard_normal(case.npfb)
+    ax = G.ppe_operator(x)
+    atx = G.ppe_operator(x, transpose=True)
+    _close(ax, A @ x, absA @ np.abs(x))
+    _close(atx, A.T @ x, absA.T @ np.abs(x))
+    aty = G.ppe_operator(y, transpose=True)
+    lhs, rhs = float(np.dot(A @ x, y)), float(np.dot(x, aty))
+    scale = float(np.abs(y) @ (absA @ np.abs(x)))
+    print(f"<Ax,y> {lhs:.17e} <x,A^T y> {rhs:.17e} (scale {scale:.3e}); A symmetric: "
+          f"{abs(A - A.T).max() == 0}", flush=True)
+    assert abs(lhs - rhs) <= 1e-12 * scale
+    # the same bits from a second context built the same way (fixed summation order)
+    G2 = solver.Solver(case)
+    G2.div_free()
+    for k in range(1, 11):
+        G2.phase(k)
+    assert np.array_equal(G2.ppe_operator(x, transpose=True), atx)
+    # the next step's lists no longer match the stored matrix: refused
+    for k in range(1, 5):
+        G.phase(k)
+    with pytest.raises(solver.VbError):
+        G.ppe_operator(x, transpose=True)
+
+
+def test_ppe_operator_refused_for_a_callers_system(gpu_lib):
+    """after vb_sparse_pbicgstab_nonull the stored matrix is the caller's (no lists): refused"""
+    case = CASES["poiseuille"]()
+    G = solver.Solver(case)
+    G.div_free()
+    w = G.scalars()["maxneighbours"]
+    diag, col, val, rhs = G.ppe_ell(w)
+    a, ija = solver.ell_to_nr(diag, col, val, G.neighbours()[0])
+    G.sparse_pbicgstab_nonull(a, ija, rhs, itermax=5)
+    with pytest.raises(solver.VbError):
+        G.ppe_operator(np.ones(case.npfb))

@@ -75,6 +75,7 @@ static void free_all(vb_ctx* c) {
                   c->xsys, c->partials, c->red, c->scal};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
+  free_operator(c);
   if (c->scal_host) (void)hipHostFree(c->scal_host);
   if (c->stream) (void)hipStreamDestroy(c->stream);
 }
@@ -523,6 +524,20 @@ vb_status vb_secondary_properties(vb_ctx* c, int32_t n, double* vort, double* u_
     VB_HIP(c, hipMemcpyAsync(vort, c->vort, (size_t)n * 8, hipMemcpyDeviceToHost, c->stream));
   if ((s = sync_scalars(c)) != VB_OK) return s;
   if (u_rms) *u_rms = c->scal_host->u_rms;
+  return VB_OK;
+}
+
+vb_status vb_ppe_operator(vb_ctx* c, int32_t transpose, int32_t n, const double* x, double* y) {
+  if (!c || n != c->npfb || !x || !y) {
+    if (c) c->err = "vb_ppe_operator: n must equal the context's npfb";
+    return VB_ERR_ARG;
+  }
+  const size_t bytes = (size_t)n * sizeof(double);
+  VB_HIP(c, hipMemcpyAsync(c->vv, x, bytes, hipMemcpyHostToDevice, c->stream));
+  vb_status s = apply_ppe_operator(c, transpose != 0, c->vv, c->tt);
+  if (s != VB_OK) return s;
+  VB_HIP(c, hipMemcpyAsync(y, c->tt, bytes, hipMemcpyDeviceToHost, c->stream));
+  VB_HIP(c, hipStreamSynchronize(c->stream));
   return VB_OK;
 }
 

@@ -6,6 +6,19 @@
 
 #include "vb_common.h"
 
+// The assembled PPE operator's transposed map (vb_operator.hip), built on first use after each
+// assembly.
+struct OperatorMap {
+  double* tval = nullptr;             // real part of A^T, slot-major like the neighbour list
+  int* tcol = nullptr;
+  int *mcount = nullptr, *moff = nullptr, *mfill = nullptr, *nmiss = nullptr;
+  int *mkey = nullptr, *mrow = nullptr;   // mirror entries per column, in (row, slot) order
+  double* mval = nullptr;
+  size_t cap_m = 0;
+  bool valid = false;                 // the map matches the stored matrix
+  bool step_matrix = false;           // the stored matrix is a step's PPE (not a caller's system)
+};
+
 struct vb_ctx {
   vb_params prm;
   vb::Geom g;
@@ -63,6 +76,7 @@ struct vb_ctx {
 
   double* partials;                  // reduction partials, 2 * kPartialStride per solver stage
   double* red;                       // per-row reduction terms, 2 * npfb per stage (serial mode only)
+  OperatorMap op;                    // A and A^T of the assembled PPE (the adjoint's operator)
   vb::Scal* scal;                    // device scalars
   vb::Scal* scal_host;               // pinned mirror
   int nmirror_host, np_host;
@@ -91,6 +105,9 @@ vb_status upload_entry_table(vb_ctx* c);
 vb_status profile_spmv(vb_ctx* c, int reps, double* avg_us, double* alg_bytes);
 vb_status solve_nr(vb_ctx* c, const double* A_nr, const int* ija, double* x, const double* B, int n,
                    int itermax, double tol, bool open);
+// the assembled PPE operator (vb_operator.hip): y = A x or A^T x on device vectors of npfb rows
+vb_status apply_ppe_operator(vb_ctx* c, bool transpose, const double* x, double* y);
+void free_operator(vb_ctx* c);
 // out[0..n-1] = exclusive prefix sums of in[0..n-1], out[n] = total (vb_step.hip)
 vb_status launch_exclusive_scan(vb_ctx* c, const int* in, int* out, int n);
 // helpers (vb_capi.hip)

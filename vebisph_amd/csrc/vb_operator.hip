@@ -1,0 +1,210 @@
+// The assembled PPE operator and its transpose, for the adjoint of the step (the reference has no
+// adjoint: this is the operator such a solve needs, A^T lambda = g on the matrix that
+// ppe_solve_in_house_simple_dirichlet.F90:55-138 assembles).
+//
+// The transposed map.  Row c of A^T holds A(i, c) for every row i whose stored row references
+// column c.  Between real particles the neighbour relation is symmetric (the pair test
+// 0 < |r_ij|^2 <= 9h^2 of neighbour_finding_mod.F90:353-359 is the same both ways), so the real
+// entries of column c come from exactly the real neighbours i = link(c, m) of c, and slot m of row c
+// of A^T can hold A(i, c) = a_val(i, k) with k the slot of c in i's list ("reverse slot"): A^T's
+// real part has the forward ELL shape and gathers the same columns.  Mirror slots of c's own list
+// are gaps (value 0, column c).  Entries whose column is the parent of a mirror (a row i near a
+// wall referencing a mirror of c; these occur only in the rows with mirror neighbours) go to a
+// per-column bucket: counted, scanned, filled, then each bucket sorted by (i, k), so the order of
+// every sum is fixed and the result the same run after run.  Self-mirror slots sit in the
+// diagonal already (value 0 in the row).
+//
+// Cost: the reverse-slot search reads ~cnt/2 list entries per slot (~10 ms per build at 1M); it is
+// built once per assembled matrix (on first use) and each application is one SpMV-shaped pass.
+#include <vector>
+
+#include "vb_ctx.h"
+
+namespace vb {
+
+// reverse slots: tval/tcol[m*npfb + c] for the real part of row c of A^T; nmiss counts real pairs
+// whose reverse slot is absent (an asymmetric list, which the pair test rules out)
+__global__ void k_transpose_real(int npfb, const int* cnt, const int* link, const double* a_val, double* tval,
+                                 int* tcol, int* nmiss) {
+  const int c = blockIdx.x * kBlock + threadIdx.x;
+  if (c >= npfb) return;
+  const int n = cnt[c];
+  int miss = 0;
+  for (int m = 0; m < n; m++) {
+    const size_t s = (size_t)m * npfb + c;
+    const int i = link[s];
+    double v = 0.0;
+    int col = c;
+    if (i < npfb) {
+      const int ni = cnt[i];
+      int k = 0;
+      while (k < ni && link[(size_t)k * npfb + i] != c) k++;
+      if (k < ni) { v = a_val[(size_t)k * npfb + i]; col = i; }
+      else miss++;
+    }
+    tval[s] = v;
+    tcol[s] = col;
+  }
+  if (miss) atomicAdd(nmiss, miss);
+}
+
+// Mirror entries of the explicit rows: (i, k) with link(i, k) a mirror whose parent c != i.
+__device__ inline int mirror_parent(int npfb, const int* irel, int j) { return j >= npfb ? irel[j] : -1; }
+
+__global__ void k_transpose_mirror_count(int npfb, const int* rows, int nrows, const int* cnt, const int* link,
+                                         const int* irel, int* mcount) {
+  const int r = blockIdx.x * kBlock + threadIdx.x;
+  if (r >= nrows) return;
+  const int i = rows[r], n = cnt[i];
+  for (int k = 0; k < n; k++) {
+    const int c = mirror_parent(npfb, irel, link[(size_t)k * npfb + i]);
+    if (c >= 0 && c != i) atomicAdd(&mcount[c], 1);
+  }
+}
+
+__global__ void k_transpose_mirror_fill(int npfb, const int* rows, int nrows, const int* cnt, const int* link,
+                                        const int* irel, const int* moff, int* mfill, int* mkey) {
+  const int r = blockIdx.x * kBlock + threadIdx.x;
+  if (r >= nrows) return;
+  const int i = rows[r], n = cnt[i];
+  for (int k = 0; k < n; k++) {
+    const int c = mirror_parent(npfb, irel, link[(size_t)k * npfb + i]);
+    if (c >= 0 && c != i) {
+      const int pos = atomicAdd(&mfill[c], 1);
+      mkey[moff[c] + pos] = i * kNplink + k;                // (row, slot), unique per entry
+    }
+  }
+}
+
+// each column's bucket in increasing (row, slot) order, then the values
+__global__ void k_transpose_mirror_sort(int npfb, const int* moff, int* mkey, int* mrow, double* mval,
+                                        const double* a_val) {
+  const int c = blockIdx.x * kBlock + threadIdx.x;
+  if (c >= npfb) return;
+  const int b = moff[c], e = moff[c + 1];
+  for (int p = b + 1; p < e; p++) {
+    const int key = mkey[p];
+    int q = p - 1;
+    while (q >= b && mkey[q] > key) { mkey[q + 1] = mkey[q]; q--; }
+    mkey[q + 1] = key;
+  }
+  for (int p = b; p < e; p++) {
+    const int i = mkey[p] / kNplink, k = mkey[p] % kNplink;
+    mrow[p] = i;
+    mval[p] = a_val[(size_t)k * npfb + i];
+  }
+}
+
+// y = A^T x: the diagonal, the real part in c's slot order, then c's mirror bucket
+__global__ void k_apply_transpose(int npfb, const int* cnt, const double* diag, const double* tval,
+                                  const int* tcol, const int* moff, const int* mrow, const double* mval,
+                                  const double* x, double* y) {
+  const int c = blockIdx.x * kBlock + threadIdx.x;
+  if (c >= npfb) return;
+  double acc = diag[c] * x[c];
+  const int n = cnt[c];
+  for (int m = 0; m < n; m++) {
+    const size_t s = (size_t)m * npfb + c;
+    acc = acc + tval[s] * x[tcol[s]];
+  }
+  for (int p = moff[c]; p < moff[c + 1]; p++) acc = acc + mval[p] * x[mrow[p]];
+  y[c] = acc;
+}
+
+// y = A x on the full stored rows in the reference's slot order (sparse_atime, linear_solver.F90:264-272)
+__global__ void k_apply_forward(int npfb, const int* cnt, const double* diag, const int* a_col,
+                                const double* a_val, const double* x, double* y) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= npfb) return;
+  double bb = diag[i] * x[i];
+  const int n = cnt[i];
+  for (int k = 0; k < n; k++) {
+    const size_t s = (size_t)k * npfb + i;
+    bb = bb + a_val[s] * x[a_col[s]];
+  }
+  y[i] = bb;
+}
+
+__global__ void k_op_zero(int* a, int n) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) a[i] = 0;
+}
+
+// The transposed map of the matrix the last vb_ppe_solve assembled (kept until the next assembly).
+vb_status build_transpose(vb_ctx* c) {
+  const int npfb = c->npfb;
+  OperatorMap& op = c->op;
+  if (!op.tval) {                                        // per-context, allocated on first use
+    const size_t nw = (size_t)npfb * kNplink;
+    VB_HIP(c, hipMalloc((void**)&op.tval, nw * sizeof(double)));
+    VB_HIP(c, hipMalloc((void**)&op.tcol, nw * sizeof(int)));
+    VB_HIP(c, hipMalloc((void**)&op.mcount, ((size_t)npfb + 1) * sizeof(int)));
+    VB_HIP(c, hipMalloc((void**)&op.moff, ((size_t)npfb + 1) * sizeof(int)));
+    VB_HIP(c, hipMalloc((void**)&op.mfill, ((size_t)npfb + 1) * sizeof(int)));
+    VB_HIP(c, hipMalloc((void**)&op.nmiss, sizeof(int)));
+  }
+  VB_LAUNCH(c, k_op_zero, grid_1d(npfb + 1), op.mcount, npfb + 1);
+  VB_LAUNCH(c, k_op_zero, grid_1d(npfb + 1), op.mfill, npfb + 1);
+  VB_LAUNCH(c, k_op_zero, 1, op.nmiss, 1);
+  VB_LAUNCH(c, k_transpose_real, grid_1d(npfb), npfb, c->cnt, c->link, c->a_val, op.tval, op.tcol, op.nmiss);
+  // the explicit rows listed by the last assembly (the rows with mirror neighbours)
+  const int nx = c->scal_host->nexplicit;
+  if (nx > 0)
+    VB_LAUNCH(c, k_transpose_mirror_count, grid_1d(nx), npfb, c->xrows, nx, c->cnt, c->link, c->st.irel,
+              op.mcount);
+  vb_status s = launch_exclusive_scan(c, op.mcount, op.moff, npfb);
+  if (s != VB_OK) return s;
+  int hv[2] = {0, 0};
+  VB_HIP(c, hipMemcpyAsync(&hv[0], op.moff + npfb, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  VB_HIP(c, hipMemcpyAsync(&hv[1], op.nmiss, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  VB_HIP(c, hipStreamSynchronize(c->stream));
+  if (hv[1] != 0) {
+    c->err = "transposed PPE map: a neighbour pair without its reverse entry";
+    return VB_ERR_CAPACITY;
+  }
+  const size_t nm = (size_t)hv[0];
+  if (nm > op.cap_m || !op.mkey) {                       // grow-only bucket storage
+    if (op.mkey) { VB_HIP(c, hipFree(op.mkey)); VB_HIP(c, hipFree(op.mrow)); VB_HIP(c, hipFree(op.mval)); }
+    op.mkey = op.mrow = nullptr; op.mval = nullptr;
+    const size_t cap = nm + nm / 4 + 64;
+    VB_HIP(c, hipMalloc((void**)&op.mkey, cap * sizeof(int)));
+    VB_HIP(c, hipMalloc((void**)&op.mrow, cap * sizeof(int)));
+    VB_HIP(c, hipMalloc((void**)&op.mval, cap * sizeof(double)));
+    op.cap_m = cap;
+  }
+  if (nx > 0)
+    VB_LAUNCH(c, k_transpose_mirror_fill, grid_1d(nx), npfb, c->xrows, nx, c->cnt, c->link, c->st.irel, op.moff,
+              op.mfill, op.mkey);
+  VB_LAUNCH(c, k_transpose_mirror_sort, grid_1d(npfb), npfb, op.moff, op.mkey, op.mrow, op.mval, c->a_val);
+  op.valid = true;
+  return VB_OK;
+}
+
+vb_status apply_ppe_operator(vb_ctx* c, bool transpose, const double* x, double* y) {
+  const int npfb = c->npfb;
+  if (!c->op.step_matrix) {
+    c->err = "the PPE operator needs the matrix of a step's vb_ppe_solve (none, or replaced by a caller's system)";
+    return VB_ERR_ARG;
+  }
+  if (transpose) {
+    if (!c->op.valid) {
+      vb_status s = build_transpose(c);
+      if (s != VB_OK) return s;
+    }
+    VB_LAUNCH(c, k_apply_transpose, grid_1d(npfb), npfb, c->cnt, c->a_diag, c->op.tval, c->op.tcol, c->op.moff,
+              c->op.mrow, c->op.mval, x, y);
+  } else {
+    VB_LAUNCH(c, k_apply_forward, grid_1d(npfb), npfb, c->cnt, c->a_diag, c->a_col, c->a_val, x, y);
+  }
+  return VB_OK;
+}
+
+void free_operator(vb_ctx* c) {
+  void* ptrs[] = {c->op.tval, c->op.tcol, c->op.mcount, c->op.moff, c->op.mfill, c->op.mkey, c->op.mrow,
+                  c->op.mval, c->op.nmiss};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  c->op = OperatorMap{};
+}
+
+}  // namespace vb

@@ -1312,6 +1312,7 @@ vb_status launch_ppe(vb_ctx* c, double tol, int itermax) {
                        c->a_rowsum);
     VB_HIP(c, hipGetLastError());
   }
+  c->op.valid = false; c->op.step_matrix = true;        // a new matrix: the transposed map is rebuilt on use
   Mat A = step_matrix(c);
   Vecs w = step_vecs(c, c->st.p, c->g.open != 0);
   if (c->prm.precision == VB_PREC_MIXED) {
@@ -1333,6 +1334,7 @@ vb_status launch_ppe(vb_ctx* c, double tol, int itermax) {
 vb_status solve_nr(vb_ctx* c, const double* A_nr, const int* ija, double* x, const double* B, int n,
                    int itermax, double tol, bool open) {
   if (n != c->npfb) { c->err = "system size must equal the context's npfb"; return VB_ERR_ARG; }
+  c->op.valid = false; c->op.step_matrix = false;       // the stored matrix becomes the caller's
   std::vector<int> len(n), col((size_t)n * kNplink, 0);
   std::vector<double> val((size_t)n * kNplink, 0.0), diag(n);
   for (int i = 0; i < n; i++) {

@@ -1124,6 +1124,7 @@ vb_status launch_exclusive_scan(vb_ctx* c, const int* in, int* out, int n) {
 
 vb_status launch_neighbours(vb_ctx* c) {
   const Geom& g = c->g;
+  c->op.valid = false; c->op.step_matrix = false;       // the stored PPE no longer matches the lists
   int np = c->np_host;
   VB_LAUNCH(c, k_fill_int, grid_1d(g.nct), c->nc, g.nct, 0);
   VB_LAUNCH(c, k_cell_count, grid_1d(np), c->g, c->st, np, c->cellkey, c->cellslot, c->nc, c->scal);

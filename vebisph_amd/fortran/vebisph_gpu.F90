@@ -246,6 +246,15 @@ module vebisph_gpu
        real(c_double), intent(out) :: vort(*), u_rms
        integer(c_int) :: s
      end function vb_secondary_properties
+     !! the assembled PPE as an operator (y = A x, or A^T x with transpose = 1), for an adjoint solve
+     function vb_ppe_operator(ctx, transpose, n, x, y) bind(C, name="vb_ppe_operator") result(s)
+       import :: c_int, c_int32_t, c_double, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: transpose, n
+       real(c_double), intent(in) :: x(*)
+       real(c_double), intent(out) :: y(*)
+       integer(c_int) :: s
+     end function vb_ppe_operator
      function vb_free_surface(ctx, n, n_surf, conc) bind(C, name="vb_free_surface") result(s)
        import :: c_int, c_int32_t, c_double, c_ptr
        type(c_ptr), value :: ctx

@@ -297,6 +297,14 @@ class Solver:
         self._chk(_lib.lib().vb_secondary_properties(self.h, n, _d(vort), C.byref(u_rms)))
         return vort, u_rms.value
 
+    def ppe_operator(self, x, transpose=False):
+        """y = A x or A^T x with A the PPE the last solve assembled (the adjoint's operator)."""
+        n = self.case.npfb
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.zeros(n)
+        self._chk(_lib.lib().vb_ppe_operator(self.h, 1 if transpose else 0, n, _d(x), _d(y)))
+        return y
+
     def free_surface(self, n=None):
         """n_surf and conc of rows 1..n (the PART file columns, output.F90:47-49) of the last step."""
         n = self.case.npfb if n is None else n
