@@ -410,6 +410,11 @@ struct Vecs {
 __device__ inline double inv_m(const Vecs& w, int i, double x) {
   return w.jac ? x / (w.jac[i] + 1.0e-15) : x / kOnePlusEps;
 }
+// the same with the solve's kind fixed at compile time (OPEN: w.jac is the diagonal)
+template <bool OPEN>
+__device__ inline double inv_m_t(const Vecs& w, int i, double x) {
+  return OPEN ? x / (w.jac[i] + 1.0e-15) : x / kOnePlusEps;
+}
 
 enum { ST_P = 0, ST_SPMV_P = 1, ST_S = 2, ST_SPMV_S = 3, ST_XR = 4, ST_INIT = 5, ST_TRUE = 6 };
 static_assert(ST_TRUE < kSolverStages, "stage buffers");
@@ -735,6 +740,8 @@ __device__ inline FinState finish_iteration(const double (&f)[3], int k, Scal* s
 }
 
 // Iteration k; fin = 1: finish iteration k-1 here (its x,r partials), 0: k_fu_fin or k_fin_init did
+// (OPEN: the open domain's SPARSE_PBICGSTAB, Jacobi and no mean removed)
+template <bool OPEN>
 __global__ void k_fu_p(int n, Vecs w, RedBufs rb, Scal* sc, int k, int fin) {
   if (finished(sc)) return;
   RowRange rr = block_rows(n, gridDim.x);
@@ -758,13 +765,13 @@ __global__ void k_fu_p(int n, Vecs w, RedBufs rb, Scal* sc, int k, int fin) {
   const bool first = k == 1;
   const double omega = sc->omega;
   const double sum_p = first ? sum_r : sum_r + beta * (sc->sum_p - omega * sc->sum_v);
-  const double mean = w.jac ? 0.0 : sum_p / (double)n;                   // remove_dc, :243
+  const double mean = OPEN ? 0.0 : sum_p / (double)n;                    // remove_dc, :243
   Red<1> R(nullptr, n);
   auto row = [&](int i, double r, double p, double v) {
     const double pn = first ? r : r + beta * (p - omega * v);
     const double pp = pn - mean;
     w.pn[i] = pp;
-    w.pinv[i] = inv_m(w, i, pp);
+    w.pinv[i] = inv_m_t<OPEN>(w, i, pp);
     R.add(0, i, pp);
   };
 #pragma unroll
@@ -773,6 +780,7 @@ __global__ void k_fu_p(int n, Vecs w, RedBufs rb, Scal* sc, int k, int fin) {
   R.store(rb.P(ST_S) + gridDim.x);
 }
 
+template <bool OPEN>
 __global__ void k_fu_s(int n, Vecs w, RedBufs rb, Scal* sc, int nspmv, int k) {
   if (finished(sc)) return;
   RowRange rr = block_rows(n, gridDim.x);
@@ -784,7 +792,7 @@ __global__ void k_fu_s(int n, Vecs w, RedBufs rb, Scal* sc, int nspmv, int k) {
   double f[2];
   fold_n<2>(rb.P(ST_SPMV_P), nspmv, nullptr, n, f);
   const double alpha = sc->rho_pp[k & 1] / (f[0] + 1.0e-15);
-  const double mean = w.jac ? 0.0 : (sc->sum_r - alpha * f[1]) / (double)n;
+  const double mean = OPEN ? 0.0 : (sc->sum_r - alpha * f[1]) / (double)n;
   if (blockIdx.x == 0 && threadIdx.x == 0) { sc->alpha = alpha; sc->sum_v = f[1]; sc->mean_s = mean; }
   Red<1> R(nullptr, n);
   auto row = [&](int i, double r, double v) {
@@ -792,7 +800,7 @@ __global__ void k_fu_s(int n, Vecs w, RedBufs rb, Scal* sc, int nspmv, int k) {
     R.add(0, i, s * s);
     const double sp = s - mean;
     w.s[i] = sp;
-    w.sinv[i] = inv_m(w, i, sp);
+    w.sinv[i] = inv_m_t<OPEN>(w, i, sp);
   };
 #pragma unroll
   for (int u = 0; u < kPre; u++) if (i0 + u * bd < rr.end) row(i0 + u * bd, rv[u], vv[u]);
@@ -1146,9 +1154,11 @@ static vb_status solve(vb_ctx* c, const Mat& A, Vecs w, int x_given, double tol,
       const int k = launched + b + 1;
       const bool refresh = MODE == SPMV_MIXED && k % kRefresh == 0;
       if (fused) {
-        VB_LAUNCH(c, k_fu_p, nb, n, w, rb, sc, k, b > 0 ? 1 : 0);
+        if (w.jac) VB_LAUNCH(c, k_fu_p<true>, nb, n, w, rb, sc, k, b > 0 ? 1 : 0);
+        else VB_LAUNCH(c, k_fu_p<false>, nb, n, w, rb, sc, k, b > 0 ? 1 : 0);
         VB_LAUNCH_BS(c, (k_bi_spmv_p<MODE, true>), ns, kSpmvBlock, A, w, rb, sc);
-        VB_LAUNCH(c, k_fu_s, nb, n, w, rb, sc, ns, k);
+        if (w.jac) VB_LAUNCH(c, k_fu_s<true>, nb, n, w, rb, sc, ns, k);
+        else VB_LAUNCH(c, k_fu_s<false>, nb, n, w, rb, sc, ns, k);
         VB_LAUNCH_BS(c, (k_bi_spmv_s<MODE, true>), ns, kSpmvBlock, A, w, rb, sc);
         VB_LAUNCH(c, k_fu_xr, nb, n, w, rb, sc, ns, k);
         if (refresh) {
