@@ -63,11 +63,26 @@ def _f(a, dt=np.float64):
     return np.asfortranarray(np.asarray(a, dtype=dt))
 
 
+def _ensure_stack(n):
+    """The reference's open-domain SPARSE_PBICGSTAB keeps its ten n-vectors as automatic arrays
+    (linear_solver.F90:85-93), i.e. on the calling thread's stack: 80 MB at a million particles,
+    beyond the usual 8 MB limit (the reference's own runs need `ulimit -s` raised too).  The harness
+    is called on this process's main thread, whose stack grows on demand up to the soft limit, so the
+    limit is raised here (the kernel keeps at least 128 MB below the stack free for this growth)."""
+    import resource
+    need = 96 * int(n) + (64 << 20)
+    soft, hard = resource.getrlimit(resource.RLIMIT_STACK)
+    if soft != resource.RLIM_INFINITY and soft < need:
+        new = need if hard == resource.RLIM_INFINITY else min(need, hard)
+        resource.setrlimit(resource.RLIMIT_STACK, (new, hard))
+
+
 class RefSolver:
     """The reference step on a Case; state lives in the Fortran module globals."""
 
     def __init__(self, case, time=0.0, itime=0, state=None):
         L = self.L = lib(getattr(case, "const_mod", 2))
+        _ensure_stack(case.npfb)
         self.case = case
         nb = case.nb_patches
         nc = case.nb_circles
@@ -199,6 +214,7 @@ class RefRun:
 
     def __init__(self, path, const_mod=2):
         self.L = lib(const_mod)
+        _ensure_stack(2_000_000)
         self.path = os.path.abspath(path)
         os.makedirs(os.path.join(self.path, "data_directory"), exist_ok=True)
         with _cwd(self.path):

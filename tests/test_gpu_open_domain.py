@@ -219,3 +219,37 @@ def test_vb_drive_open_domain_part_columns(gpu_lib, tmp_path):
         _eq(part["n_surf"], ns[keep])
         _eq(part["conc"], conc[keep])
         _eq(part["x"], G.get_state()["x"][keep])
+
+
+def test_open_domain_large_invariants(gpu_lib):
+    """fs_convection_1M (1,001,113 particles, 1414 lattice columns): properties that need no oracle at
+    full size.  After one step the free surface is the top particle of every column (1414 rows, none of
+    them walls), the mirrors carry their parents' flags, the interior concentration is 1 to within the
+    lattice's 1% jitter, and the Dirichlet rows (A_ii = 1e30) leave p there ~1e30 times smaller than
+    the field."""
+    case = cases.make("fs_convection_1M")
+    print(f"fs_convection_1M npfb={case.npfb}", flush=True)
+    G = solver.Solver(case)
+    G.div_free()
+    s = G.scalars()
+    assert s["status"] in (0, 6) and 1 <= s["LS_iters"] <= 1001
+    n = s["np"]
+    ns, conc = G.free_surface(n)
+    m = case.npfb
+    surf = np.flatnonzero(ns[:m])
+    assert surf.size == 1414 and surf.min() >= case.nbS
+    st = G.get_state(n)
+    y = st["y"][:m]
+    # the highest fluid particle of each lattice column
+    fluid = np.arange(m) >= case.nbS
+    fi = np.flatnonzero(fluid)
+    col = np.floor(st["x"][fi] / case.dx).astype(np.int64)
+    order = np.lexsort((y[fi], col))
+    last = np.r_[col[order][1:] != col[order][:-1], True]
+    assert np.array_equal(surf, np.sort(fi[order][last]))
+    mirrors = G.mirrors()[0] - 1                       # parents of the mirror rows (1-based)
+    assert np.array_equal(ns[m:n], ns[mirrors])
+    interior = fluid & (y < 0.4) & (y > 0.1)
+    assert abs(conc[:m][interior].mean() - 1.0) < 0.01
+    p = st["p"][:m]
+    assert np.abs(p[surf]).max() <= 1e-20 * np.abs(p).max()
