@@ -168,6 +168,13 @@ vb_status vb_secondary_properties(vb_ctx* ctx, int32_t n, double* vort, double* 
  * assembly, with a fixed summation order.  VB_ERR_ARG once the lists are rebuilt by the next step's
  * vb_particle_allocation, or after vb_sparse_pbicgstab* replaced the stored matrix. */
 vb_status vb_ppe_operator(vb_ctx* ctx, int32_t transpose, int32_t n, const double* x, double* y);
+/* The adjoint PPE solve A^T lambda = g (no counterpart in the reference), A as for vb_ppe_operator:
+ * the reference's own BiCGSTAB with A^T as its matrix -- SPARSE_PBICGSTAB_nonull in a closed domain
+ * (g should then have zero mean: A^T maps into the mean-zero vectors), SPARSE_PBICGSTAB in an open
+ * one -- from lambda = 0, tol <= 0 / itermax <= 0 selecting 1e-8 / 1000.  g, lambda host arrays of
+ * n = npfb; iters and residual may be NULL; returns VB_WARN_* like vb_ppe_solve. */
+vb_status vb_ppe_adjoint_solve(vb_ctx* ctx, int32_t n, const double* g, double* lambda, double tol,
+                               int32_t itermax, int32_t* iters, double* residual);
 /* The columns conc and n_surf of the PART files (output.F90:47-49) for rows 1..n, n <= np: the
  * free-surface flag and, in an open domain, the concentration (calc_part_dist_vars,
  * freesurf_mod.F90:95-100) of the last step (rows past npfb: mirror flags, conc 0; conc is 0 in a

@@ -90,3 +90,32 @@ def test_ppe_operator_refused_for_a_callers_system(gpu_lib):
     G.sparse_pbicgstab_nonull(a, ija, rhs, itermax=5)
     with pytest.raises(solver.VbError):
         G.ppe_operator(np.ones(case.npfb))
+
+
+@pytest.mark.parametrize("name", ["poiseuille", "fs_convection"])
+def test_ppe_adjoint_solve(gpu_lib, name):
+    """A^T lambda = g with the reference's BiCGSTAB on the transposed PPE: the residual meets the
+    reference criterion on A^T itself (checked with scipy), and in the open domain, where A is
+    nonsingular, lambda is scipy's direct solution to the accuracy the criterion allows
+    (cond(A) * 1e-8: 1e-3 relative).  In the closed domain g has zero mean (A^T maps into the
+    mean-zero vectors)."""
+    import scipy.sparse.linalg as spla
+    case = CASES[name]()
+    G = solver.Solver(case)
+    G.div_free()
+    for k in range(1, 11):
+        G.phase(k)
+    A = _matrix(G)
+    g = np.random.default_rng(5).standard_normal(case.npfb)
+    if not case.i_open_domain:
+        g -= g.mean()
+    lam, it, res, st = G.ppe_adjoint_solve(g)
+    print(f"{case.name}: adjoint BiCGSTAB {it} iterations, residual {res:.3e}, status {st}", flush=True)
+    assert st == 0 and res <= 1e-8 and 1 <= it <= 1000
+    r = g - A.T @ lam
+    assert np.linalg.norm(r) <= 1e-6 * np.linalg.norm(g)
+    if case.i_open_domain:
+        ref = spla.spsolve(A.T.tocsc(), g)
+        err = np.linalg.norm(lam - ref) / np.linalg.norm(ref)
+        print(f"  against scipy's direct solve: {err:.2e}", flush=True)
+        assert err <= 1e-3

@@ -541,6 +541,29 @@ vb_status vb_ppe_operator(vb_ctx* c, int32_t transpose, int32_t n, const double*
   return VB_OK;
 }
 
+vb_status vb_ppe_adjoint_solve(vb_ctx* c, int32_t n, const double* g, double* lambda, double tol,
+                               int32_t itermax, int32_t* iters, double* residual) {
+  if (!c || n != c->npfb || !g || !lambda) {
+    if (c) c->err = "vb_ppe_adjoint_solve: n must equal the context's npfb";
+    return VB_ERR_ARG;
+  }
+  if (tol <= 0.0) tol = 1.0e-8;
+  if (itermax <= 0) itermax = 1000;
+  const size_t bytes = (size_t)n * sizeof(double);
+  if (!c->op.g) {
+    VB_HIP(c, hipMalloc((void**)&c->op.g, bytes));
+    VB_HIP(c, hipMalloc((void**)&c->op.lam, bytes));
+  }
+  VB_HIP(c, hipMemcpyAsync(c->op.g, g, bytes, hipMemcpyHostToDevice, c->stream));
+  vb_status s = adjoint_solve(c, tol, itermax);
+  if (s != VB_OK) return s;
+  VB_HIP(c, hipMemcpyAsync(lambda, c->op.lam, bytes, hipMemcpyDeviceToHost, c->stream));
+  if ((s = sync_scalars(c)) != VB_OK) return s;
+  if (iters) *iters = c->scal_host->iter;
+  if (residual) *residual = c->scal_host->resid;
+  return (vb_status)c->scal_host->status;
+}
+
 vb_status vb_free_surface(vb_ctx* c, int32_t n, int32_t* n_surf, double* conc) {
   if (!c || n < 0 || n > std::max(c->np_host, c->npfb)) return VB_ERR_ARG;
   const int nr = std::min<int>(n, c->npfb);

@@ -201,7 +201,7 @@ vb_status apply_ppe_operator(vb_ctx* c, bool transpose, const double* x, double*
 
 void free_operator(vb_ctx* c) {
   void* ptrs[] = {c->op.tval, c->op.tcol, c->op.mcount, c->op.moff, c->op.mfill, c->op.mkey, c->op.mrow,
-                  c->op.mval, c->op.nmiss};
+                  c->op.mval, c->op.nmiss, c->op.g, c->op.lam};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   c->op = OperatorMap{};

@@ -1,5 +1,7 @@
-// gfx950 pressure Poisson equation: RHS, assembly and the BiCGSTAB of the reference
-// (ppe_solve_in_house_simple_dirichlet.F90:1-158, linear_solver.F90:231-425).
+// gfx950 pressure Poisson equation: RHS, assembly and the BiCGSTABs of the reference
+// (ppe_solve_in_house_simple_dirichlet.F90:1-158, linear_solver.F90:82-425): SPARSE_PBICGSTAB_nonull
+// for closed domains, SPARSE_PBICGSTAB (Jacobi, no DC projection) for open ones, whose free-surface
+// rows are Dirichlet (A_ii = 1e30) and whose rows near the surface are scaled by ppe_smooth.
 //
 // Matrix.  The assembly writes the reference's row layout as ELL, slot-major like the neighbour
 // list (col/val[k*npfb + i]) plus a diagonal: slot k of row i is the k-th neighbour of i with
@@ -30,7 +32,8 @@ namespace vb {
 constexpr double kOnePlusEps = 1.0 + 1.0e-15;   // M(i) + 1e-15 with M = 1, linear_solver.F90:55,318
 constexpr int kRefresh = 50;                     // mixed path: FP64 residual replacement period
 
-enum SpmvMode { SPMV_FP64 = 0, SPMV_MIXED = 1 };
+// SPMV_TRANSPOSE: y = A^T x through the transposed map (vb_operator.hip), for the adjoint solve
+enum SpmvMode { SPMV_FP64 = 0, SPMV_MIXED = 1, SPMV_TRANSPOSE = 2 };
 
 // ------------------------------------------------------------------------------------------------
 // RHS, ppe_solve_in_house_simple_dirichlet.F90:27-53 (DC removal for the closed domain only)
@@ -247,6 +250,9 @@ struct Mat {
   const short4* lnk4;              // column offsets (column = row + offset)
   const double2* val4;             // FP64 values, two double2 planes per group
   const float4* val4f;             // FP32 values (mixed path), one float4 per group
+  // the transposed map (SPMV_TRANSPOSE): real part slot-major like the lists, mirror buckets per column
+  const double *tval, *mval;
+  const int *tcol, *moff, *mrow;
   int n;
 };
 
@@ -383,13 +389,28 @@ __device__ inline double spmv_row_f32(const Mat& A, int i, const double* __restr
   return row_f32(A, i, m, x, xi);
 }
 
+// Row c of A^T: the diagonal, the real part in c's slot order, then c's mirror bucket (the order of
+// k_apply_transpose in vb_operator.hip)
+__device__ inline double spmv_row_transpose(const Mat& A, int c, const double* __restrict__ x) {
+  const int n = A.n, m = A.len[c];
+  double acc = A.diag[c] * x[c];
+  for (int k = 0; k < m; k++) {
+    const size_t s = (size_t)k * n + c;
+    acc = acc + A.tval[s] * x[A.tcol[s]];
+  }
+  for (int p = A.moff[c]; p < A.moff[c + 1]; p++) acc = acc + A.mval[p] * x[A.mrow[p]];
+  return acc;
+}
+
 template <int MODE>
 __device__ inline double spmv_row(const Mat& A, int i, const double* __restrict__ x) {
+  if (MODE == SPMV_TRANSPOSE) return spmv_row_transpose(A, i, x);
   return MODE == SPMV_MIXED ? spmv_row_f32(A, i, x) : spmv_row_fp64(A, i, x);
 }
 
 // ------------------------------------------------------------------------------------------------
-// SPARSE_PBICGSTAB_nonull, linear_solver.F90:275-425.
+// SPARSE_PBICGSTAB_nonull, linear_solver.F90:275-425, and SPARSE_PBICGSTAB, :82-227 (Vecs::jac set:
+// the same kernels with the Jacobi divisor, zero means and the threshold 4).
 //
 // Per iteration: p -> [p_inv] -> SpMV -> s -> [s_inv] -> SpMV -> x,r -> finalize.  Each row kernel
 // that needs a global sum from the kernel before it folds that kernel's block partials itself (every
@@ -1132,6 +1153,7 @@ static Mat step_matrix(vb_ctx* c) {
   A.diag = c->a_diag; A.val = c->a_val; A.rowsum = c->a_rowsum; A.col = c->a_col; A.len = c->cnt;
   A.rowflag = c->rowflag; A.lnk4 = (const short4*)c->lnk4; A.val4 = (const double2*)c->val4;
   A.val4f = (const float4*)c->val4f; A.choff = c->choff; A.n = c->npfb;
+  A.tval = c->op.tval; A.mval = c->op.mval; A.tcol = c->op.tcol; A.moff = c->op.moff; A.mrow = c->op.mrow;
   return A;
 }
 
@@ -1375,6 +1397,27 @@ vb_status solve_nr(vb_ctx* c, const double* A_nr, const int* ija, double* x, con
   VB_HIP(c, hipMemcpyAsync(x, c->xsys, (size_t)n * 8, hipMemcpyDeviceToHost, c->stream));
   VB_HIP(c, hipStreamSynchronize(c->stream));
   return VB_OK;
+}
+
+// The adjoint PPE solve A^T lambda = g on the matrix the last step assembled: the reference's own
+// BiCGSTAB (SPARSE_PBICGSTAB_nonull for closed domains -- A^T maps into the mean-zero vectors, the
+// complement of A's constant null space, so the DC projections keep the iterates there as in the
+// forward solve -- or SPARSE_PBICGSTAB for open ones, Jacobi on diag(A^T) = diag(A)) with the
+// transposed map as its matrix, lambda_0 = 0.  g is read from and lambda written to the operator's
+// device vectors op.g and op.lam.
+vb_status adjoint_solve(vb_ctx* c, double tol, int itermax) {
+  if (!c->op.step_matrix) {
+    c->err = "the adjoint solve needs the matrix of a step's vb_ppe_solve";
+    return VB_ERR_ARG;
+  }
+  if (!c->op.valid) {
+    vb_status s = build_transpose(c);
+    if (s != VB_OK) return s;
+  }
+  Mat A = step_matrix(c);
+  Vecs w{c->op.lam, c->op.g, c->r, c->rb, c->vv, c->pn, c->pinv, c->ss, c->sinv, c->tt,
+         c->g.open ? c->a_diag : nullptr};
+  return solve<SPMV_TRANSPOSE>(c, A, w, 0, tol, itermax);
 }
 
 // ------------------------------------------------------------------------------------------------

@@ -255,6 +255,19 @@ module vebisph_gpu
        real(c_double), intent(out) :: y(*)
        integer(c_int) :: s
      end function vb_ppe_operator
+     !! A^T lambda = g with the reference's BiCGSTAB on the transposed PPE (an adjoint solve)
+     function vb_ppe_adjoint_solve(ctx, n, g, lambda, tol, itermax, iters, residual) &
+          bind(C, name="vb_ppe_adjoint_solve") result(s)
+       import :: c_int, c_int32_t, c_double, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: n, itermax
+       real(c_double), intent(in) :: g(*)
+       real(c_double), intent(out) :: lambda(*)
+       real(c_double), value :: tol
+       integer(c_int32_t), intent(out) :: iters
+       real(c_double), intent(out) :: residual
+       integer(c_int) :: s
+     end function vb_ppe_adjoint_solve
      function vb_free_surface(ctx, n, n_surf, conc) bind(C, name="vb_free_surface") result(s)
        import :: c_int, c_int32_t, c_double, c_ptr
        type(c_ptr), value :: ctx

@@ -305,6 +305,17 @@ class Solver:
         self._chk(_lib.lib().vb_ppe_operator(self.h, 1 if transpose else 0, n, _d(x), _d(y)))
         return y
 
+    def ppe_adjoint_solve(self, g, tol=1e-8, itermax=1000):
+        """lambda with A^T lambda = g (the reference's BiCGSTAB on the transposed PPE): (lambda, iters,
+        residual, status)."""
+        n = self.case.npfb
+        g = np.ascontiguousarray(g, dtype=np.float64)
+        lam = np.zeros(n)
+        it = C.c_int32(0); res = C.c_double(0.0)
+        st = self._chk(_lib.lib().vb_ppe_adjoint_solve(self.h, n, _d(g), _d(lam), C.c_double(tol), itermax,
+                                                      C.byref(it), C.byref(res)))
+        return lam, it.value, res.value, st
+
     def free_surface(self, n=None):
         """n_surf and conc of rows 1..n (the PART file columns, output.F90:47-49) of the last step."""
         n = self.case.npfb if n is None else n
