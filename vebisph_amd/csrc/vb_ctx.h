@@ -16,6 +16,7 @@ struct OperatorMap {
   double* mval = nullptr;
   size_t cap_m = 0;
   double *g = nullptr, *lam = nullptr;   // adjoint solve: right-hand side and solution (npfb)
+  double* val4T = nullptr;            // A^T values of the compact rows (layout of vb_ctx::val4)
   bool valid = false;                 // the map matches the stored matrix
   bool step_matrix = false;           // the stored matrix is a step's PPE (not a caller's system)
 };
@@ -109,6 +110,7 @@ vb_status solve_nr(vb_ctx* c, const double* A_nr, const int* ija, double* x, con
 // the assembled PPE operator (vb_operator.hip): y = A x or A^T x on device vectors of npfb rows
 vb_status apply_ppe_operator(vb_ctx* c, bool transpose, const double* x, double* y);
 vb_status build_transpose(vb_ctx* c);
+vb_status launch_transpose_sliced(vb_ctx* c);
 // A^T lambda = g with the reference's BiCGSTAB on the transposed map (vb_ppe.hip); g, lambda in op
 vb_status adjoint_solve(vb_ctx* c, double tol, int itermax);
 void free_operator(vb_ctx* c);

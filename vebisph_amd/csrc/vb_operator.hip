@@ -134,6 +134,7 @@ __global__ void k_op_zero(int* a, int n) {
 vb_status build_transpose(vb_ctx* c) {
   const int npfb = c->npfb;
   OperatorMap& op = c->op;
+  vb_status s0;
   if (!op.tval) {                                        // per-context, allocated on first use
     const size_t nw = (size_t)npfb * kNplink;
     VB_HIP(c, hipMalloc((void**)&op.tval, nw * sizeof(double)));
@@ -142,11 +143,14 @@ vb_status build_transpose(vb_ctx* c) {
     VB_HIP(c, hipMalloc((void**)&op.moff, ((size_t)npfb + 1) * sizeof(int)));
     VB_HIP(c, hipMalloc((void**)&op.mfill, ((size_t)npfb + 1) * sizeof(int)));
     VB_HIP(c, hipMalloc((void**)&op.nmiss, sizeof(int)));
+    const size_t nw4 = ((size_t)npfb + 63) / 64 * 64 * ((kNplink + 3) & ~3);     // as vb_ctx::val4
+    VB_HIP(c, hipMalloc((void**)&op.val4T, nw4 * sizeof(double)));
   }
   VB_LAUNCH(c, k_op_zero, grid_1d(npfb + 1), op.mcount, npfb + 1);
   VB_LAUNCH(c, k_op_zero, grid_1d(npfb + 1), op.mfill, npfb + 1);
   VB_LAUNCH(c, k_op_zero, 1, op.nmiss, 1);
   VB_LAUNCH(c, k_transpose_real, grid_1d(npfb), npfb, c->cnt, c->link, c->a_val, op.tval, op.tcol, op.nmiss);
+  if ((s0 = launch_transpose_sliced(c)) != VB_OK) return s0;
   // the explicit rows listed by the last assembly (the rows with mirror neighbours)
   const int nx = c->scal_host->nexplicit;
   if (nx > 0)
@@ -201,7 +205,7 @@ vb_status apply_ppe_operator(vb_ctx* c, bool transpose, const double* x, double*
 
 void free_operator(vb_ctx* c) {
   void* ptrs[] = {c->op.tval, c->op.tcol, c->op.mcount, c->op.moff, c->op.mfill, c->op.mkey, c->op.mrow,
-                  c->op.mval, c->op.nmiss, c->op.g, c->op.lam};
+                  c->op.mval, c->op.nmiss, c->op.g, c->op.lam, c->op.val4T};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   c->op = OperatorMap{};

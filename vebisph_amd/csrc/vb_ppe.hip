@@ -238,6 +238,39 @@ __global__ void k_assemble_sliced(int npfb, const int* cnt, const int* link, con
   for (int k = n; k < padded; k++) put(k, 0, 0.0);
 }
 
+// The compact rows of A^T (the adjoint solve): a row without mirror neighbours has the same columns in
+// A and A^T (the real neighbour relation is symmetric), so A^T reuses the column offsets written
+// above and takes a second FP64 value plane, val4T, filled from the transposed map's real part in the
+// same rank order.
+__global__ void k_transpose_sliced(int npfb, const int* cnt, const int* link, const double* tval,
+                                   const uint8_t* rowflag, const int* choff, double* val4T) {
+  __shared__ int16_t off[kNplink][kBlock];
+  const int t = threadIdx.x, i = blockIdx.x * kBlock + t;
+  if (i >= npfb || rowflag[i]) return;
+  const int n = cnt[i], padded = (n + 3) & ~3;
+  const size_t P = (size_t)choff[i / kChunk], lane = i % kChunk;
+  for (int k = 0; k < n; k++) off[k][t] = (int16_t)(link[k * npfb + i] - i);
+  auto put = [&](int k, double val) {
+    val4T[((2 * (P + (k >> 2)) + ((k & 3) >> 1)) * kChunk + lane) * 2 + (k & 1)] = val;
+  };
+  for (int k = 0; k < n; k++) {
+    const int16_t ok = off[k][t];
+    int pos = 0;
+    for (int j = 0; j < n; j++) {
+      const int16_t oj = off[j][t];
+      pos += (oj < ok) || (oj == ok && j < k);
+    }
+    put(pos, tval[(size_t)k * npfb + i]);
+  }
+  for (int k = n; k < padded; k++) put(k, 0.0);
+}
+
+vb_status launch_transpose_sliced(vb_ctx* c) {
+  VB_LAUNCH(c, k_transpose_sliced, grid_1d(c->npfb), c->npfb, c->cnt, c->link, c->op.tval, c->rowflag, c->choff,
+            c->op.val4T);
+  return VB_OK;
+}
+
 // ------------------------------------------------------------------------------------------------
 // SpMV rows.  x_j = (src_j - mean) / (1 + 1e-15) reproduces remove_dc + inverseM exactly.
 // ------------------------------------------------------------------------------------------------
@@ -250,7 +283,9 @@ struct Mat {
   const short4* lnk4;              // column offsets (column = row + offset)
   const double2* val4;             // FP64 values, two double2 planes per group
   const float4* val4f;             // FP32 values (mixed path), one float4 per group
-  // the transposed map (SPMV_TRANSPOSE): real part slot-major like the lists, mirror buckets per column
+  // the transposed map (SPMV_TRANSPOSE): real part slot-major like the lists, mirror buckets per column,
+  // and the compact rows' A^T values (same offsets as lnk4)
+  const double2* val4T;
   const double *tval, *mval;
   const int *tcol, *moff, *mrow;
   int n;
@@ -389,14 +424,22 @@ __device__ inline double spmv_row_f32(const Mat& A, int i, const double* __restr
   return row_f32(A, i, m, x, xi);
 }
 
-// Row c of A^T: the diagonal, the real part in c's slot order, then c's mirror bucket (the order of
-// k_apply_transpose in vb_operator.hip)
+// Row c of A^T: rows without mirror neighbours as compact FP64 rows (the forward row's offsets, the
+// A^T value plane, column order), the others from the transposed map in c's slot order; then c's
+// mirror bucket.
 __device__ inline double spmv_row_transpose(const Mat& A, int c, const double* __restrict__ x) {
-  const int n = A.n, m = A.len[c];
-  double acc = A.diag[c] * x[c];
-  for (int k = 0; k < m; k++) {
-    const size_t s = (size_t)k * n + c;
-    acc = acc + A.tval[s] * x[A.tcol[s]];
+  double acc;
+  if (A.rowflag && !A.rowflag[c]) {
+    Mat At = A;
+    At.val4 = A.val4T;
+    acc = spmv_row_fp64(At, c, x);
+  } else {
+    const int n = A.n, m = A.len[c];
+    acc = A.diag[c] * x[c];
+    for (int k = 0; k < m; k++) {
+      const size_t s = (size_t)k * n + c;
+      acc = acc + A.tval[s] * x[A.tcol[s]];
+    }
   }
   for (int p = A.moff[c]; p < A.moff[c + 1]; p++) acc = acc + A.mval[p] * x[A.mrow[p]];
   return acc;
@@ -1154,6 +1197,7 @@ static Mat step_matrix(vb_ctx* c) {
   A.rowflag = c->rowflag; A.lnk4 = (const short4*)c->lnk4; A.val4 = (const double2*)c->val4;
   A.val4f = (const float4*)c->val4f; A.choff = c->choff; A.n = c->npfb;
   A.tval = c->op.tval; A.mval = c->op.mval; A.tcol = c->op.tcol; A.moff = c->op.moff; A.mrow = c->op.mrow;
+  A.val4T = (const double2*)c->op.val4T;
   return A;
 }
 
