@@ -106,14 +106,43 @@ def run(case, precision, steps, warmup):
     sc = s.scalars()
     force = s.circle_forces()[0].tolist() if case.nb_circles else None
     us, by = s.profile_spmv(reps=20)
+    adj = adjoint_cost(s, case)
     elapsed = t1 - t0
     res = {"value": case.npfb * steps / elapsed, "ms_per_step": 1e3 * elapsed / steps,
            "ls_iters_last_step": sc["LS_iters"], "ls_residual_last_step": sc["LS_residual"],
-           "spmv_us": us, "spmv_bytes": by, "spmv_GBs": by / (us * 1e-6) / 1e9, "force": force}
+           "spmv_us": us, "spmv_bytes": by, "spmv_GBs": by / (us * 1e-6) / 1e9, "force": force,
+           "adjoint_ppe": adj}
     log(f"{precision}: {res['value']:.4g} particle-steps/s, {res['ms_per_step']:.1f} ms/step, "
         f"BiCGSTAB {sc['LS_iters']} iterations (residual {sc['LS_residual']:.3g}); "
         f"SpMV {us:.1f} us, {res['spmv_GBs']:.0f} GB/s")
+    log(f"{precision}: PPE solve {adj['forward_ms']:.1f} ms ({adj['forward_iters']} iterations), adjoint "
+        f"PPE solve {adj['adjoint_ms']:.1f} ms ({adj['adjoint_iters']} iterations): {adj['ratio']:.2f}x")
     return s, res, state, sc0
+
+
+def adjoint_cost(s, case):
+    """The PPE solve of one more step (assembly + BiCGSTAB, phase 10) against the adjoint PPE solve
+    A^T lambda = g on the matrix it assembled (vb_ppe_adjoint_solve, FP64 BiCGSTAB; g a seeded
+    mean-zero vector; the transposed map built beforehand, untimed; host copies of g and lambda
+    included).  Run after the timed steps."""
+    import numpy as np
+    for k in range(1, 10):
+        s.phase(k)
+    s.scalars()
+    t = time.perf_counter()
+    s.phase(10)
+    sc = s.scalars()
+    tf = time.perf_counter() - t
+    g = np.random.default_rng(5).standard_normal(case.npfb)
+    if not case.i_open_domain:
+        g -= g.mean()
+    s.ppe_adjoint_solve(g, itermax=2)
+    t = time.perf_counter()
+    _, it, res, _ = s.ppe_adjoint_solve(g)
+    ta = time.perf_counter() - t
+    return {"forward_ms": 1e3 * tf, "forward_iters": sc["LS_iters"], "adjoint_ms": 1e3 * ta,
+            "adjoint_iters": it, "adjoint_residual": res, "ratio": ta / tf,
+            "scope": "PPE solve only; the rest of the adjoint step is not built"}
 
 
 def main():
@@ -149,7 +178,8 @@ def main():
         so.close()
         other = {"precision": oprec, "value": ores["value"], "ms_per_step": ores["ms_per_step"],
                  "ls_iters_last_step": ores["ls_iters_last_step"],
-                 "spmv_us": ores["spmv_us"], "spmv_GBs": ores["spmv_GBs"], "force_on_cylinder": ores["force"]}
+                 "spmv_us": ores["spmv_us"], "spmv_GBs": ores["spmv_GBs"], "force_on_cylinder": ores["force"],
+                 "adjoint_ppe": ores["adjoint_ppe"]}
 
     cpu = None
     if not a.no_cpu_baseline:
@@ -186,6 +216,8 @@ def main():
         "force_on_cylinder": main_res["force"],
         "force_rel_diff_vs_other_precision": force_diff(main_res["force"], other),
         "ratio_vs_other_precision": (main_res["value"] / other["value"]) if other else None,
+        # adjoint/forward cost of the PPE solve at this precision's forward solve
+        "adjoint_ppe": main_res["adjoint_ppe"],
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -238,6 +238,17 @@ vb_status vb_kernel_functions(int32_t n, const double* d_q, double ad_7, double 
  * row-major device arrays, trace in/out; kick selects the reference's B perturbation */
 vb_status vb_logconf(int32_t n, const double* d_tau, const double* d_trace, const double* d_gradvel,
                      double dt, double wi, double eps_ptt, int32_t kick, double* d_tau_out, double* d_trace_out);
+/* Adjoint of vb_logconf (no counterpart in the reference, which has no adjoint; one piece of an
+ * adjoint step): given the cotangents of its outputs, d_tau_out_bar (n,4) and d_trace_out_bar (n),
+ * writes J^T times them for the inputs at the same (tau, trace, grad_vel, Wi, dt): d_tau_bar (n,4),
+ * d_trace_bar (n), d_gradvel_bar (n,4), and per-particle contributions d_wi_bar (n) and d_dt_bar (n)
+ * to the shared Wi and dt (the caller sums them).  Exact FP64 derivatives of the branch each value
+ * takes (the A21 = 0 branch and the discriminant clamp of predictor_mod.F90:268,280 included).
+ * All pointers are device arrays and required. */
+vb_status vb_logconf_adjoint(int32_t n, const double* d_tau, const double* d_trace, const double* d_gradvel,
+                             double dt, double wi, double eps_ptt, int32_t kick, const double* d_tau_out_bar,
+                             const double* d_trace_out_bar, double* d_tau_bar, double* d_trace_bar,
+                             double* d_gradvel_bar, double* d_wi_bar, double* d_dt_bar);
 
 #ifdef __cplusplus
 }

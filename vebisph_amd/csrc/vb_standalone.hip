@@ -50,6 +50,36 @@ __global__ void k_logconf_n(int n, const double* tau, const double* trc, const d
   trout[i] = tr;
 }
 
+// Adjoint of the map for n particles: with the map's outputs (tau', trace') and their cotangents
+// (tau'_bar, trace'_bar), the cotangents of its inputs J^T (tau'_bar, trace'_bar) for
+// (tau, trace, grad_vel, Wi, dt), per particle (Wi and dt are shared by all particles: a caller sums
+// their per-particle contributions).  Column j of the Jacobian J is one pass of the map in Dual
+// arithmetic seeded along input j: eleven passes, exact derivatives of the branch each value takes.
+__global__ void k_logconf_adjoint(int n, const double* tau, const double* trc, const double* gv, double dt,
+                                  double wi, double eps, const double* kick, int use_kick, const double* tbar,
+                                  const double* trbar, double* tau_bar, double* trc_bar, double* gv_bar,
+                                  double* wi_bar, double* dt_bar) {
+  int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  // inputs: tau (0-3), trace (4), grad_vel (5-8), Wi (9), dt (10)
+  const double x[11] = {tau[4 * i], tau[4 * i + 1], tau[4 * i + 2], tau[4 * i + 3], trc[i],
+                        gv[4 * i],  gv[4 * i + 1],  gv[4 * i + 2],  gv[4 * i + 3],  wi, dt};
+  const double ob[5] = {tbar[4 * i], tbar[4 * i + 1], tbar[4 * i + 2], tbar[4 * i + 3], trbar[i]};
+  const double k = use_kick ? kick[i] : 0.0;
+  double xb[11];
+  for (int j = 0; j < 11; j++) {
+    Dual in[11];
+    for (int q = 0; q < 11; q++) in[q] = Dual(x[q], q == j ? 1.0 : 0.0);
+    Dual o[4], tr;
+    logconf(in, in[4], in + 5, in[10], in[9], eps, k, use_kick != 0, o, &tr);
+    xb[j] = (((ob[0] * o[0].d + ob[1] * o[1].d) + ob[2] * o[2].d) + ob[3] * o[3].d) + ob[4] * tr.d;
+  }
+  for (int q = 0; q < 4; q++) { tau_bar[4 * i + q] = xb[q]; gv_bar[4 * i + q] = xb[5 + q]; }
+  trc_bar[i] = xb[4];
+  wi_bar[i] = xb[9];
+  dt_bar[i] = xb[10];
+}
+
 // minstd stream of predictor_mod.F90:357,369-371 (see vb_capi.hip), computed on the device by
 // jumping the generator ahead: x_k = 48271^k mod (2^31-1).
 __device__ inline uint64_t mulmod(uint64_t a, uint64_t b) { return (a * b) % 2147483647ull; }
@@ -99,6 +129,30 @@ vb_status vb_logconf(int32_t n, const double* d_tau, const double* d_trace, cons
   }
   hipLaunchKernelGGL(k_logconf_n, dim3(grid_1d(n)), dim3(kBlock), 0, 0, n, d_tau, d_trace, d_gradvel, dt,
                      wi, eps_ptt, dk, kick, d_tau_out, d_trace_out);
+  hipError_t e = hipGetLastError();
+  hipError_t e2 = hipStreamSynchronize(0);
+  if (dk) (void)hipFree(dk);
+  return (e != hipSuccess || e2 != hipSuccess) ? VB_ERR_HIP : VB_OK;
+}
+
+vb_status vb_logconf_adjoint(int32_t n, const double* d_tau, const double* d_trace, const double* d_gradvel,
+                             double dt, double wi, double eps_ptt, int32_t kick, const double* d_tau_out_bar,
+                             const double* d_trace_out_bar, double* d_tau_bar, double* d_trace_bar,
+                             double* d_gradvel_bar, double* d_wi_bar, double* d_dt_bar) {
+  if (n < 0) return VB_ERR_ARG;
+  if (n == 0) return VB_OK;
+  const void* need[] = {d_tau, d_trace, d_gradvel, d_tau_out_bar, d_trace_out_bar, d_tau_bar, d_trace_bar,
+                        d_gradvel_bar, d_wi_bar, d_dt_bar};
+  for (const void* p : need)
+    if (!p) return VB_ERR_ARG;
+  double* dk = nullptr;
+  if (kick) {
+    if (hipMalloc((void**)&dk, (size_t)n * 8) != hipSuccess) return VB_ERR_HIP;
+    hipLaunchKernelGGL(k_kick_table, dim3(grid_1d(n)), dim3(kBlock), 0, 0, n, dk);
+  }
+  hipLaunchKernelGGL(k_logconf_adjoint, dim3(grid_1d(n)), dim3(kBlock), 0, 0, n, d_tau, d_trace, d_gradvel,
+                     dt, wi, eps_ptt, dk, kick, d_tau_out_bar, d_trace_out_bar, d_tau_bar, d_trace_bar,
+                     d_gradvel_bar, d_wi_bar, d_dt_bar);
   hipError_t e = hipGetLastError();
   hipError_t e2 = hipStreamSynchronize(0);
   if (dk) (void)hipFree(dk);
