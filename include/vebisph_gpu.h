@@ -168,6 +168,13 @@ vb_status vb_secondary_properties(vb_ctx* ctx, int32_t n, double* vort, double* 
  * assembly, with a fixed summation order.  VB_ERR_ARG once the lists are rebuilt by the next step's
  * vb_particle_allocation, or after vb_sparse_pbicgstab* replaced the stored matrix. */
 vb_status vb_ppe_operator(vb_ctx* ctx, int32_t transpose, int32_t n, const double* x, double* y);
+/* The SPH gradient of calc_pressure_gradient (gradient_calculation.F90:16-48) as a linear operator, and
+ * its transpose, for an adjoint step (no counterpart in the reference): row i of G x is
+ * K_i sum_k (x_parent(j_k) - x_i) grad W_ik with the step's lists, kernel gradients and KGC matrices,
+ * mirrors taking their parent's value.  transpose = 0: x has n = npfb values, y = [gx; gy] 2n;
+ * transpose = 1: x = [gx; gy] 2n, y = G^T x, n values.  Host arrays; valid, as vb_ppe_operator, from
+ * the step's vb_ppe_solve until the next vb_particle_allocation. */
+vb_status vb_grad_operator(vb_ctx* ctx, int32_t transpose, int32_t n, const double* x, double* y);
 /* The adjoint PPE solve A^T lambda = g (no counterpart in the reference), A as for vb_ppe_operator:
  * the reference's own BiCGSTAB with A^T as its matrix -- SPARSE_PBICGSTAB_nonull in a closed domain
  * (g should then have zero mean: A^T maps into the mean-zero vectors), SPARSE_PBICGSTAB in an open

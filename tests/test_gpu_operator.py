@@ -119,3 +119,68 @@ def test_ppe_adjoint_solve(gpu_lib, name):
         err = np.linalg.norm(lam - ref) / np.linalg.norm(ref)
         print(f"  against scipy's direct solve: {err:.2e}", flush=True)
         assert err <= 1e-3
+
+
+def _grad_matrix(G, n):
+    """The gradient operator as scipy matrices (Gx, Gy) from the exported lists, kernel gradients, KGC
+    matrices and mirror parents (calc_pressure_gradient, gradient_calculation.F90:16-48)."""
+    w = G.scalars()["maxneighbours"]
+    cnt, link = G.neighbours(w)
+    wg, _, kg = G.weights(w)
+    irel, _, dp = G.mirrors()
+    rows = np.repeat(np.arange(n), cnt)
+    mask = np.arange(w)[None, :] < cnt[:, None]
+    j = link[mask] - 1                                               # the exports are 1-based
+    par = np.where(j < n, j, irel[np.maximum(j - n, 0)] - 1)
+    wx, wy = wg[mask][:, 0], wg[mask][:, 1]
+    S = lambda v: (sp.csr_matrix((v, (rows, par)), shape=(n, n))
+                   - sp.csr_matrix((v, (rows, rows)), shape=(n, n)))
+    Sx, Sy = S(wx), S(wy)
+    K = [sp.diags(kg[:, q]) for q in range(4)]
+    # the magnitudes of the terms before they are summed (a wall row's mirror and real terms cancel,
+    # so |G| itself is no scale for the rounding of the sums)
+    T = lambda v: (sp.csr_matrix((np.abs(v), (rows, par)), shape=(n, n))
+                   + sp.csr_matrix((np.abs(v), (rows, rows)), shape=(n, n)))
+    aK = [abs(k) for k in K]
+    mag = (aK[0] @ T(wx) + aK[1] @ T(wy), aK[2] @ T(wx) + aK[3] @ T(wy))
+    off = np.where(j < n, 0.0, dp[np.maximum(j - n, 0)])          # the pressure mirrors' offsets
+    ox = np.bincount(rows, off * wx, n); oy = np.bincount(rows, off * wy, n)
+    offset = np.stack([kg[:, 0] * ox + kg[:, 1] * oy, kg[:, 2] * ox + kg[:, 3] * oy], axis=1)
+    return K[0] @ Sx + K[1] @ Sy, K[2] @ Sx + K[3] @ Sy, offset, mag
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_grad_operator_and_transpose(gpu_lib, name):
+    """vb_grad_operator: G x and G^T g against scipy's products with the matrix built from the
+    exports (1e-13 of the sum of |terms| per row, terms taken before the wall rows' mirror and real
+    terms cancel), the adjoint identity to 1e-12, and G p plus the mirrors' pressure offsets equal to
+    the step's own pressure gradient (phase 11) to 1e-13."""
+    case = CASES[name]()
+    n = case.npfb
+    print(f"case {case.name} npfb={n}", flush=True)
+    G = solver.Solver(case)
+    G.div_free()
+    for k in range(1, 11):
+        G.phase(k)
+    Gx, Gy, offset, (aGx, aGy) = _grad_matrix(G, n)
+    rng = np.random.default_rng(17)
+    x = rng.standard_normal(n)
+    g = rng.standard_normal((n, 2))
+    gxv = G.grad_operator(x)
+    _close(gxv[:, 0], Gx @ x, aGx @ np.abs(x))
+    _close(gxv[:, 1], Gy @ x, aGy @ np.abs(x))
+    gt = G.grad_operator(g, transpose=True)
+    _close(gt, Gx.T @ g[:, 0] + Gy.T @ g[:, 1], aGx.T @ np.abs(g[:, 0]) + aGy.T @ np.abs(g[:, 1]))
+    lhs, rhs = float(np.sum(gxv * g)), float(np.dot(x, gt))
+    scale = float(np.abs(g[:, 0]) @ (aGx @ np.abs(x)) + np.abs(g[:, 1]) @ (aGy @ np.abs(x)))
+    print(f"<Gx,g> {lhs:.17e} <x,G^T g> {rhs:.17e} (scale {scale:.3e})", flush=True)
+    assert abs(lhs - rhs) <= 1e-12 * scale
+    # the step's pressure gradient is G p plus the mirror offsets
+    p = G.get_state()["p"]
+    gp_op = G.grad_operator(p) + offset
+    G.phase(11)
+    gp = G.grad_p()
+    sc = aGx @ np.abs(p) + aGy @ np.abs(p) + np.abs(offset).sum(axis=1) + 1e-300
+    err = np.max(np.abs(gp_op - gp) / sc[:, None])
+    print(f"step's grad p against G p + offsets: {err:.2e}", flush=True)
+    assert err <= 1e-13

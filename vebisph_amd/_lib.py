@@ -27,7 +27,7 @@ EXPORTS = [
     "vb_final_advection", "vb_div_free",
     "vb_get_scalars", "vb_export_mirrors", "vb_export_neighbours", "vb_export_weights",
     "vb_export_ushift", "vb_export_grad_p", "vb_export_ppe", "vb_secondary_properties",
-    "vb_ppe_operator", "vb_ppe_adjoint_solve", "vb_free_surface", "vb_field_maxima", "vb_circle_forces", "vb_nusselt", "vb_profile_spmv", "vb_profile_copy",
+    "vb_ppe_operator", "vb_grad_operator", "vb_ppe_adjoint_solve", "vb_free_surface", "vb_field_maxima", "vb_circle_forces", "vb_nusselt", "vb_profile_spmv", "vb_profile_copy",
     "vb_nnf",
     "vb_kernel_functions", "vb_logconf", "vb_logconf_adjoint",
 ]

@@ -541,6 +541,24 @@ vb_status vb_ppe_operator(vb_ctx* c, int32_t transpose, int32_t n, const double*
   return VB_OK;
 }
 
+vb_status vb_grad_operator(vb_ctx* c, int32_t transpose, int32_t n, const double* x, double* y) {
+  if (!c || n != c->npfb || !x || !y) {
+    if (c) c->err = "vb_grad_operator: n must equal the context's npfb";
+    return VB_ERR_ARG;
+  }
+  const size_t n1 = (size_t)n * sizeof(double), n2 = 2 * n1;
+  if (!c->op.gin) {
+    VB_HIP(c, hipMalloc((void**)&c->op.gin, n2));
+    VB_HIP(c, hipMalloc((void**)&c->op.gout, n2));
+  }
+  VB_HIP(c, hipMemcpyAsync(c->op.gin, x, transpose ? n2 : n1, hipMemcpyHostToDevice, c->stream));
+  vb_status s = apply_grad_operator(c, transpose != 0, c->op.gin, c->op.gout);
+  if (s != VB_OK) return s;
+  VB_HIP(c, hipMemcpyAsync(y, c->op.gout, transpose ? n1 : n2, hipMemcpyDeviceToHost, c->stream));
+  VB_HIP(c, hipStreamSynchronize(c->stream));
+  return VB_OK;
+}
+
 vb_status vb_ppe_adjoint_solve(vb_ctx* c, int32_t n, const double* g, double* lambda, double tol,
                                int32_t itermax, int32_t* iters, double* residual) {
   if (!c || n != c->npfb || !g || !lambda) {

@@ -17,6 +17,9 @@ struct OperatorMap {
   size_t cap_m = 0;
   double *g = nullptr, *lam = nullptr;   // adjoint solve: right-hand side and solution (npfb)
   double* val4T = nullptr;            // A^T values of the compact rows (layout of vb_ctx::val4)
+  int* rslot = nullptr;               // reverse slot: the slot of c in i's list, beside tcol
+  double *ga = nullptr, *gb = nullptr;     // transposed gradient: per-row coefficients (npfb)
+  double *gin = nullptr, *gout = nullptr;  // gradient operator staging (2 npfb each)
   bool valid = false;                 // the map matches the stored matrix
   bool step_matrix = false;           // the stored matrix is a step's PPE (not a caller's system)
 };
@@ -111,6 +114,7 @@ vb_status solve_nr(vb_ctx* c, const double* A_nr, const int* ija, double* x, con
 vb_status apply_ppe_operator(vb_ctx* c, bool transpose, const double* x, double* y);
 vb_status build_transpose(vb_ctx* c);
 vb_status launch_transpose_sliced(vb_ctx* c);
+vb_status apply_grad_operator(vb_ctx* c, bool transpose, const double* x, double* y);
 // A^T lambda = g with the reference's BiCGSTAB on the transposed map (vb_ppe.hip); g, lambda in op
 vb_status adjoint_solve(vb_ctx* c, double tol, int itermax);
 void free_operator(vb_ctx* c);

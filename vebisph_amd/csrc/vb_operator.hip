@@ -25,7 +25,7 @@ namespace vb {
 // reverse slots: tval/tcol[m*npfb + c] for the real part of row c of A^T; nmiss counts real pairs
 // whose reverse slot is absent (an asymmetric list, which the pair test rules out)
 __global__ void k_transpose_real(int npfb, const int* cnt, const int* link, const double* a_val, double* tval,
-                                 int* tcol, int* nmiss) {
+                                 int* tcol, int* rslot, int* nmiss) {
   const int c = blockIdx.x * kBlock + threadIdx.x;
   if (c >= npfb) return;
   const int n = cnt[c];
@@ -34,16 +34,17 @@ __global__ void k_transpose_real(int npfb, const int* cnt, const int* link, cons
     const size_t s = (size_t)m * npfb + c;
     const int i = link[s];
     double v = 0.0;
-    int col = c;
+    int col = c, rk = -1;
     if (i < npfb) {
       const int ni = cnt[i];
       int k = 0;
       while (k < ni && link[(size_t)k * npfb + i] != c) k++;
-      if (k < ni) { v = a_val[(size_t)k * npfb + i]; col = i; }
+      if (k < ni) { v = a_val[(size_t)k * npfb + i]; col = i; rk = k; }
       else miss++;
     }
     tval[s] = v;
     tcol[s] = col;
+    rslot[s] = rk;
   }
   if (miss) atomicAdd(nmiss, miss);
 }
@@ -139,6 +140,7 @@ vb_status build_transpose(vb_ctx* c) {
     const size_t nw = (size_t)npfb * kNplink;
     VB_HIP(c, hipMalloc((void**)&op.tval, nw * sizeof(double)));
     VB_HIP(c, hipMalloc((void**)&op.tcol, nw * sizeof(int)));
+    VB_HIP(c, hipMalloc((void**)&op.rslot, nw * sizeof(int)));
     VB_HIP(c, hipMalloc((void**)&op.mcount, ((size_t)npfb + 1) * sizeof(int)));
     VB_HIP(c, hipMalloc((void**)&op.moff, ((size_t)npfb + 1) * sizeof(int)));
     VB_HIP(c, hipMalloc((void**)&op.mfill, ((size_t)npfb + 1) * sizeof(int)));
@@ -149,7 +151,8 @@ vb_status build_transpose(vb_ctx* c) {
   VB_LAUNCH(c, k_op_zero, grid_1d(npfb + 1), op.mcount, npfb + 1);
   VB_LAUNCH(c, k_op_zero, grid_1d(npfb + 1), op.mfill, npfb + 1);
   VB_LAUNCH(c, k_op_zero, 1, op.nmiss, 1);
-  VB_LAUNCH(c, k_transpose_real, grid_1d(npfb), npfb, c->cnt, c->link, c->a_val, op.tval, op.tcol, op.nmiss);
+  VB_LAUNCH(c, k_transpose_real, grid_1d(npfb), npfb, c->cnt, c->link, c->a_val, op.tval, op.tcol, op.rslot,
+            op.nmiss);
   if ((s0 = launch_transpose_sliced(c)) != VB_OK) return s0;
   // the explicit rows listed by the last assembly (the rows with mirror neighbours)
   const int nx = c->scal_host->nexplicit;
@@ -203,9 +206,103 @@ vb_status apply_ppe_operator(vb_ctx* c, bool transpose, const double* x, double*
   return VB_OK;
 }
 
+// The SPH gradient of calc_pressure_gradient (gradient_calculation.F90:16-48) as a linear map of a
+// particle field x (npfb): row i gives K_i sum_k (x_parent(j_k) - x_i) grad W_ik, mirrors taking
+// their parent's value (the pressure mirrors' offset dP is constant, so it is no part of the map).
+// Forward: y = [gx; gy] (2 npfb), the step's slot order.
+__global__ void k_grad_forward(int npfb, const int* cnt, const int* link, const int* irel, const double* wg1,
+                               const double* wg2, const double* k11, const double* k12, const double* k21,
+                               const double* k22, const double* x, double* y) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= npfb) return;
+  const int n = cnt[i];
+  const double xi = x[i];
+  double gx = 0.0, gy = 0.0;
+  for (int k = 0; k < n; k++) {
+    const size_t s = (size_t)k * npfb + i;
+    const int j = link[s];
+    const double t = x[j < npfb ? j : irel[j]] - xi;
+    gx = gx + t * wg1[s]; gy = gy + t * wg2[s];
+  }
+  y[i] = k11[i] * gx + k12[i] * gy;
+  y[npfb + i] = k21[i] * gx + k22[i] * gy;
+}
+
+// Transposed gradient, y = G^T [gx; gy]: with a_i = K11 gx + K21 gy and b_i = K12 gx + K22 gy (the
+// weights of row i's two sums), column c collects a_i dW/dx + b_i dW/dy of every slot (i, k) whose
+// particle is c or a mirror of c, less its own row's slots (the -x_i term; a slot holding c's own
+// mirror cancels and is skipped).  Real pairs through the reverse slots in c's slot order, then c's
+// mirror bucket in (row, slot) order, then the own-row term: a fixed order.
+__global__ void k_grad_coef(int npfb, const double* k11, const double* k12, const double* k21,
+                            const double* k22, const double* g, double* a, double* b) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= npfb) return;
+  const double gx = g[i], gy = g[npfb + i];
+  a[i] = k11[i] * gx + k21[i] * gy;
+  b[i] = k12[i] * gx + k22[i] * gy;
+}
+
+__global__ void k_grad_transpose(int npfb, const int* cnt, const int* link, const int* irel, const double* wg1,
+                                 const double* wg2, const int* tcol, const int* rslot, const int* moff,
+                                 const int* mkey, const double* a, const double* b, double* y) {
+  const int c = blockIdx.x * kBlock + threadIdx.x;
+  if (c >= npfb) return;
+  const int n = cnt[c];
+  double acc = 0.0;
+  for (int m = 0; m < n; m++) {
+    const size_t s = (size_t)m * npfb + c;
+    const int k = rslot[s];
+    if (k < 0) continue;                                   // a mirror slot of c's own list
+    const int i = tcol[s];
+    const size_t r = (size_t)k * npfb + i;
+    acc = acc + (a[i] * wg1[r] + b[i] * wg2[r]);
+  }
+  for (int p = moff[c]; p < moff[c + 1]; p++) {
+    const int i = mkey[p] / kNplink, k = mkey[p] % kNplink;
+    const size_t r = (size_t)k * npfb + i;
+    acc = acc + (a[i] * wg1[r] + b[i] * wg2[r]);
+  }
+  double own = 0.0;
+  const double ac = a[c], bc = b[c];
+  for (int k = 0; k < n; k++) {
+    const size_t s = (size_t)k * npfb + c;
+    const int j = link[s];
+    if (j >= npfb && irel[j] == c) continue;
+    own = own + (ac * wg1[s] + bc * wg2[s]);
+  }
+  y[c] = acc - own;
+}
+
+vb_status apply_grad_operator(vb_ctx* c, bool transpose, const double* x, double* y) {
+  const int npfb = c->npfb;
+  if (!c->op.step_matrix) {
+    c->err = "the gradient operator needs the lists and weights of a step up to its vb_ppe_solve";
+    return VB_ERR_ARG;
+  }
+  if (!transpose) {
+    VB_LAUNCH(c, k_grad_forward, grid_1d(npfb), npfb, c->cnt, c->link, c->st.irel, c->wg1, c->wg2, c->k11,
+              c->k12, c->k21, c->k22, x, y);
+    return VB_OK;
+  }
+  if (!c->op.valid) {
+    vb_status s = build_transpose(c);
+    if (s != VB_OK) return s;
+  }
+  OperatorMap& op = c->op;
+  if (!op.ga) {
+    VB_HIP(c, hipMalloc((void**)&op.ga, (size_t)npfb * sizeof(double)));
+    VB_HIP(c, hipMalloc((void**)&op.gb, (size_t)npfb * sizeof(double)));
+  }
+  VB_LAUNCH(c, k_grad_coef, grid_1d(npfb), npfb, c->k11, c->k12, c->k21, c->k22, x, op.ga, op.gb);
+  VB_LAUNCH(c, k_grad_transpose, grid_1d(npfb), npfb, c->cnt, c->link, c->st.irel, c->wg1, c->wg2, op.tcol,
+            op.rslot, op.moff, op.mkey, op.ga, op.gb, y);
+  return VB_OK;
+}
+
 void free_operator(vb_ctx* c) {
   void* ptrs[] = {c->op.tval, c->op.tcol, c->op.mcount, c->op.moff, c->op.mfill, c->op.mkey, c->op.mrow,
-                  c->op.mval, c->op.nmiss, c->op.g, c->op.lam, c->op.val4T};
+                  c->op.mval, c->op.nmiss, c->op.g, c->op.lam, c->op.val4T, c->op.rslot, c->op.ga,
+                  c->op.gb, c->op.gin, c->op.gout};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   c->op = OperatorMap{};

@@ -255,6 +255,15 @@ module vebisph_gpu
        real(c_double), intent(out) :: y(*)
        integer(c_int) :: s
      end function vb_ppe_operator
+     !! the SPH pressure gradient as an operator (x: n values -> y: 2n) and its transpose (2n -> n)
+     function vb_grad_operator(ctx, transpose, n, x, y) bind(C, name="vb_grad_operator") result(s)
+       import :: c_int, c_int32_t, c_double, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: transpose, n
+       real(c_double), intent(in) :: x(*)
+       real(c_double), intent(out) :: y(*)
+       integer(c_int) :: s
+     end function vb_grad_operator
      !! A^T lambda = g with the reference's BiCGSTAB on the transposed PPE (an adjoint solve)
      function vb_ppe_adjoint_solve(ctx, n, g, lambda, tol, itermax, iters, residual) &
           bind(C, name="vb_ppe_adjoint_solve") result(s)

@@ -305,6 +305,19 @@ class Solver:
         self._chk(_lib.lib().vb_ppe_operator(self.h, 1 if transpose else 0, n, _d(x), _d(y)))
         return y
 
+    def grad_operator(self, x, transpose=False):
+        """The SPH pressure-gradient operator of the step's lists (mirrors at their parent's value):
+        G x as an (npfb, 2) array, or G^T g for g of shape (npfb, 2)."""
+        n = self.case.npfb
+        if transpose:
+            xin = np.ascontiguousarray(np.asarray(x, dtype=np.float64).T).ravel()
+            y = np.zeros(n)
+        else:
+            xin = np.ascontiguousarray(x, dtype=np.float64)
+            y = np.zeros(2 * n)
+        self._chk(_lib.lib().vb_grad_operator(self.h, 1 if transpose else 0, n, _d(xin), _d(y)))
+        return y if transpose else y.reshape(2, n).T
+
     def ppe_adjoint_solve(self, g, tol=1e-8, itermax=1000):
         """lambda with A^T lambda = g (the reference's BiCGSTAB on the transposed PPE): (lambda, iters,
         residual, status)."""
