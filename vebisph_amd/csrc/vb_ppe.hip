@@ -332,7 +332,9 @@ __device__ inline float4 ld_nt(const float4* p) {
 // One FP64 SpMV row y_i = A(i) x_i + sum_k a_k x_col(k).  Rows with mirror neighbours use the full
 // stored row; the others (~98%) load four slots per instruction from the grouped compact arrays: a
 // 2-byte column offset and the FP64 value, slots in increasing column order.
-__device__ inline double spmv_row_fp64(const Mat& A, int i, const double* __restrict__ x) {
+// (val4: the value plane, A.val4 for A and A.val4T for the compact rows of A^T)
+__device__ inline double spmv_row_fp64_v(const Mat& A, const double2* __restrict__ val4, int i,
+                                         const double* __restrict__ x) {
   const int m = A.len[i];
   const double xi = x[i];
   if (!A.rowflag || A.rowflag[i]) return row_full(A, i, m, xi, x);
@@ -353,16 +355,16 @@ __device__ inline double spmv_row_fp64(const Mat& A, int i, const double* __rest
   {
     // FP64 values: double2 plane 2g holds slots 4g, 4g+1; plane 2g+1 slots 4g+2, 4g+3
     const double2 z2 = make_double2(0.0, 0.0);
-    double2 a0 = ld_nt(A.val4 + vs(0, 0)), b0 = ld_nt(A.val4 + vs(0, 1)), a1 = z2, b1 = z2, a2 = z2, b2 = z2;
-    if (ng > 1) { a1 = ld_nt(A.val4 + vs(1, 0)); b1 = ld_nt(A.val4 + vs(1, 1)); }
+    double2 a0 = ld_nt(val4 + vs(0, 0)), b0 = ld_nt(val4 + vs(0, 1)), a1 = z2, b1 = z2, a2 = z2, b2 = z2;
+    if (ng > 1) { a1 = ld_nt(val4 + vs(1, 0)); b1 = ld_nt(val4 + vs(1, 1)); }
     g0[0] = x[i + o0.x]; g0[1] = x[i + o0.y]; g0[2] = x[i + o0.z]; g0[3] = x[i + o0.w];
     double bb = A.diag[i] * xi;
     for (int gi = 0; gi < ng; gi++) {
       if (gi + 2 < ng) {
         const size_t g2 = (size_t)(gi + 2);
         o2 = ld_nt(A.lnk4 + gs(g2));
-        a2 = ld_nt(A.val4 + vs(g2, 0));
-        b2 = ld_nt(A.val4 + vs(g2, 1));
+        a2 = ld_nt(val4 + vs(g2, 0));
+        b2 = ld_nt(val4 + vs(g2, 1));
       }
       if (gi + 1 < ng) { g1[0] = x[i + o1.x]; g1[1] = x[i + o1.y]; g1[2] = x[i + o1.z]; g1[3] = x[i + o1.w]; }
       const int k = 4 * gi;
@@ -417,6 +419,10 @@ __device__ inline double row_f32(const Mat& A, int i, int m, const double* __res
   return (double)acc + A.rowsum[i] * xhat_i;
 }
 
+__device__ inline double spmv_row_fp64(const Mat& A, int i, const double* __restrict__ x) {
+  return spmv_row_fp64_v(A, A.val4, i, x);
+}
+
 __device__ inline double spmv_row_f32(const Mat& A, int i, const double* __restrict__ x) {
   const int m = A.len[i];
   const double xi = x[i];
@@ -428,11 +434,11 @@ __device__ inline double spmv_row_f32(const Mat& A, int i, const double* __restr
 // A^T value plane, column order), the others from the transposed map in c's slot order; then c's
 // mirror bucket.
 __device__ inline double spmv_row_transpose(const Mat& A, int c, const double* __restrict__ x) {
+  // the bucket bounds first: their loads overlap the row's chain instead of following it
+  const int pb = A.moff[c], pe = A.moff[c + 1];
   double acc;
   if (A.rowflag && !A.rowflag[c]) {
-    Mat At = A;
-    At.val4 = A.val4T;
-    acc = spmv_row_fp64(At, c, x);
+    acc = spmv_row_fp64_v(A, A.val4T, c, x);
   } else {
     const int n = A.n, m = A.len[c];
     acc = A.diag[c] * x[c];
@@ -441,7 +447,7 @@ __device__ inline double spmv_row_transpose(const Mat& A, int c, const double* _
       acc = acc + A.tval[s] * x[A.tcol[s]];
     }
   }
-  for (int p = A.moff[c]; p < A.moff[c + 1]; p++) acc = acc + A.mval[p] * x[A.mrow[p]];
+  for (int p = pb; p < pe; p++) acc = acc + A.mval[p] * x[A.mrow[p]];
   return acc;
 }
 
