@@ -47,11 +47,42 @@ def measured_traffic(workload, precision):
     return None
 
 
-def cpu_baseline(case, state, time0, itime0, threads, budget_s=20.0):
+def host_cpu():
+    """The host's CPU: model name, logical CPUs (nproc), physical cores (distinct core/socket pairs
+    of lscpu -p), the CPUs this process may run on, and the thread count the CPU baseline uses: the
+    physical cores available to the process, capped by OMP_NUM_THREADS where the job sets it (the
+    host's CPU share per GPU)."""
+    import subprocess
+    info = {"model": None, "nproc": os.cpu_count(), "physical_cores": None,
+            "affinity": len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                info["model"] = line.split(":", 1)[1].strip()
+        par = subprocess.run(["lscpu", "-p=CORE,SOCKET"], capture_output=True, text=True, timeout=10).stdout
+        cores = {ln for ln in par.splitlines() if ln and not ln.startswith("#")}
+        info["physical_cores"] = len(cores) or None
+    except (OSError, subprocess.SubprocessError):
+        pass
+    avail = info["affinity"] or info["nproc"] or 1
+    threads = min(info["physical_cores"] or avail, avail)
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    if omp.isdigit() and int(omp) > 0:
+        threads = min(threads, int(omp))
+        info["omp_num_threads"] = int(omp)
+    info["threads"] = max(1, threads)
+    return info
+
+
+def cpu_baseline(case, state, time0, itime0, threads, cpu_info, budget_s=20.0):
     """The reference itself (oracle/_ref, compiled from its Fortran sources) on the host cores,
-    stepping the same workload from the same state for about `budget_s` seconds; falls back to the
-    C restatement (one thread)."""
+    stepping the same workload from the same state: one untimed warm-up step, then steps timed for
+    about `budget_s` seconds; falls back to the C restatement (one thread)."""
     def sample(S, kind, cores, label):
+        t = time.perf_counter()
+        S.step()                                   # warm-up: page-in, thread start-up, first touch
+        warm = time.perf_counter() - t
         dts, its = [], []
         while True:
             t = time.perf_counter()
@@ -62,8 +93,10 @@ def cpu_baseline(case, state, time0, itime0, threads, budget_s=20.0):
                 break
         tot = sum(dts)
         return {"value": case.npfb * len(dts) / tot, "unit": "particle-steps/s", "cores": cores,
-                "kind": kind, "sample": f"{len(dts)} step(s) of the same workload from the GPU state "
-                f"after warmup ({label}, BiCGSTAB iterations {its}, {tot:.1f} s)"}
+                "kind": kind, "host": cpu_info,
+                "sample": f"{len(dts)} timed step(s) of the same workload from the GPU state after "
+                f"warmup, after one untimed warm-up step ({warm:.1f} s) ({label}, BiCGSTAB iterations "
+                f"{its}, {tot:.1f} s)"}
     try:
         from oracle import refwrap
         if refwrap.available():
@@ -115,8 +148,10 @@ def run(case, precision, steps, warmup):
     log(f"{precision}: {res['value']:.4g} particle-steps/s, {res['ms_per_step']:.1f} ms/step, "
         f"BiCGSTAB {sc['LS_iters']} iterations (residual {sc['LS_residual']:.3g}); "
         f"SpMV {us:.1f} us, {res['spmv_GBs']:.0f} GB/s")
-    log(f"{precision}: PPE solve {adj['forward_ms']:.1f} ms ({adj['forward_iters']} iterations), adjoint "
-        f"PPE solve {adj['adjoint_ms']:.1f} ms ({adj['adjoint_iters']} iterations): {adj['ratio']:.2f}x")
+    log(f"{precision}: PPE solve {adj['forward_ms']:.1f} ms ({adj['forward_iters']} iterations, true "
+        f"residual {adj['forward_true_residual']:.3g}), adjoint PPE solve {adj['adjoint_ms']:.1f} ms "
+        f"({adj['adjoint_iters']} iterations, true residual {adj['adjoint_true_residual']:.3g}): "
+        f"{adj['ratio']:.2f}x {adj['ratio_kind']}")
     return s, res, state, sc0
 
 
@@ -124,7 +159,12 @@ def adjoint_cost(s, case):
     """The PPE solve of one more step (assembly + BiCGSTAB, phase 10) against the adjoint PPE solve
     A^T lambda = g on the matrix it assembled (vb_ppe_adjoint_solve, FP64 BiCGSTAB; g a seeded
     mean-zero vector; the transposed map built beforehand, untimed; host copies of g and lambda
-    included).  Run after the timed steps."""
+    included), each with its status and its true FP64 residual on the assembled system: |B - A p| /
+    |B| for the forward solve (in a closed domain with the constant component removed, the part the
+    DC-projected iteration controls; also given raw) and |g - A^T lambda| / |g| for the adjoint.  Both
+    solves stop at the same tolerance (1e-8) and iteration limit (1000); where either stops at the
+    limit the ratio compares equal iteration counts, so it is a per-iteration cost, and is labelled
+    so.  Run after the timed steps."""
     import numpy as np
     for k in range(1, 10):
         s.phase(k)
@@ -133,16 +173,30 @@ def adjoint_cost(s, case):
     s.phase(10)
     sc = s.scalars()
     tf = time.perf_counter() - t
+    fstatus = sc["status"]
+    _, _, _, rhs = s.ppe_ell(1)
+    p = s.get_state()["p"]
+    rf = rhs - s.ppe_operator(p)
+    nb = max(np.linalg.norm(rhs), 1e-300)
+    fwd_true_raw = float(np.linalg.norm(rf) / nb)
+    fwd_true = float(np.linalg.norm(rf - (0.0 if case.i_open_domain else rf.mean())) / nb)
     g = np.random.default_rng(5).standard_normal(case.npfb)
     if not case.i_open_domain:
         g -= g.mean()
     s.ppe_adjoint_solve(g, itermax=2)
     t = time.perf_counter()
-    _, it, res, _ = s.ppe_adjoint_solve(g)
+    lam, it, res, ast = s.ppe_adjoint_solve(g)
     ta = time.perf_counter() - t
-    return {"forward_ms": 1e3 * tf, "forward_iters": sc["LS_iters"], "adjoint_ms": 1e3 * ta,
-            "adjoint_iters": it, "adjoint_residual": res, "ratio": ta / tf,
-            "scope": "PPE solve only; the rest of the adjoint step is not built"}
+    adj_true = float(np.linalg.norm(g - s.ppe_operator(lam, transpose=True)) / np.linalg.norm(g))
+    converged = fstatus == 0 and ast == 0
+    return {"forward_ms": 1e3 * tf, "forward_iters": sc["LS_iters"], "forward_status": fstatus,
+            "forward_residual": sc["LS_residual"], "forward_true_residual": fwd_true,
+            "forward_true_residual_raw": fwd_true_raw,
+            "adjoint_ms": 1e3 * ta, "adjoint_iters": it, "adjoint_status": ast, "adjoint_residual": res,
+            "adjoint_true_residual": adj_true, "converged": converged, "ratio": ta / tf,
+            "ratio_kind": "per solve (both converged)" if converged else
+                          "per iteration (a solve stopped at the iteration limit)",
+            "scope": "the PPE solve of one step and its adjoint solve"}
 
 
 def main():
@@ -183,8 +237,9 @@ def main():
 
     cpu = None
     if not a.no_cpu_baseline:
-        threads = min(16, os.cpu_count() or 1)
-        cpu = cpu_baseline(case, state, sc0["time"], sc0["itime"], threads)
+        info = host_cpu()
+        log(f"host CPU: {info}")
+        cpu = cpu_baseline(case, state, sc0["time"], sc0["itime"], info["threads"], info)
         if cpu:
             log(f"CPU baseline ({cpu['kind']}, {cpu['cores']} cores): {cpu['value']:.4g} particle-steps/s")
 

@@ -86,7 +86,8 @@ typedef struct vb_ctx vb_ctx;
 
 /* Context: replaces the module globals of common_2d.F90:20-80 and the allocations of
  * input.F90:105-114.  capacity rows = npfb + mirror capacity (2*npfb, 10*npfb when npfb <= 500,
- * mirror_boundaries_mod.F90:53-54). */
+ * mirror_boundaries_mod.F90:53-54).  Neighbour slots are addressed with 32-bit indices, so npfb is
+ * at most (2^31 - 1) / VB_NPLINK = 28,633,115: larger returns VB_ERR_CAPACITY before any device call. */
 vb_status vb_create(const vb_params* prm, const vb_boundary* bnd, int32_t npfb, int32_t nbS, vb_ctx** out);
 vb_status vb_destroy(vb_ctx* ctx);
 const char* vb_last_error(const vb_ctx* ctx);
@@ -165,8 +166,11 @@ vb_status vb_secondary_properties(vb_ctx* ctx, int32_t n, double* vort, double* 
  * rows, A the matrix the last vb_ppe_solve assembled from the step's lists
  * (ppe_solve_in_house_simple_dirichlet.F90:55-138, mirrors folded onto their parents, Dirichlet and
  * ramp rows included).  A^T comes from a transposed map built on the first transpose call after each
- * assembly, with a fixed summation order.  VB_ERR_ARG once the lists are rebuilt by the next step's
- * vb_particle_allocation, or after vb_sparse_pbicgstab* replaced the stored matrix. */
+ * assembly, with a fixed summation order.  transpose = 2 / 3 apply A^T / A through the solvers' own
+ * SpMV rows instead (the adjoint solve's compact A^T rows and map rows; the forward FP64 solve's
+ * compact rows in column order): the same operator, summed in the solvers' order.  VB_ERR_ARG once
+ * the lists are rebuilt by the next step's vb_particle_allocation, or after vb_sparse_pbicgstab*
+ * replaced the stored matrix. */
 vb_status vb_ppe_operator(vb_ctx* ctx, int32_t transpose, int32_t n, const double* x, double* y);
 /* The SPH gradient of calc_pressure_gradient (gradient_calculation.F90:16-48) as a linear operator, and
  * its transpose, for an adjoint step (no counterpart in the reference): row i of G x is
@@ -176,10 +180,11 @@ vb_status vb_ppe_operator(vb_ctx* ctx, int32_t transpose, int32_t n, const doubl
  * the step's vb_ppe_solve until the next vb_particle_allocation. */
 vb_status vb_grad_operator(vb_ctx* ctx, int32_t transpose, int32_t n, const double* x, double* y);
 /* The adjoint PPE solve A^T lambda = g (no counterpart in the reference), A as for vb_ppe_operator:
- * the reference's own BiCGSTAB with A^T as its matrix -- SPARSE_PBICGSTAB_nonull in a closed domain
- * (g should then have zero mean: A^T maps into the mean-zero vectors), SPARSE_PBICGSTAB in an open
- * one -- from lambda = 0, tol <= 0 / itermax <= 0 selecting 1e-8 / 1000.  g, lambda host arrays of
- * n = npfb; iters and residual may be NULL; returns VB_WARN_* like vb_ppe_solve. */
+ * the reference's own BiCGSTAB with A^T as its matrix -- SPARSE_PBICGSTAB_nonull in a closed domain,
+ * where the mean of g is removed first (A^T maps into the mean-zero vectors; this projection is the
+ * adjoint of the forward solve's removal of the mean of its right-hand side), SPARSE_PBICGSTAB in an
+ * open one -- from lambda = 0, tol <= 0 / itermax <= 0 selecting 1e-8 / 1000.  g, lambda host arrays
+ * of n = npfb; iters and residual may be NULL; returns VB_WARN_* like vb_ppe_solve. */
 vb_status vb_ppe_adjoint_solve(vb_ctx* ctx, int32_t n, const double* g, double* lambda, double tol,
                                int32_t itermax, int32_t* iters, double* residual);
 /* The columns conc and n_surf of the PART files (output.F90:47-49) for rows 1..n, n <= np: the

@@ -26,6 +26,7 @@ module vbref_harness
   use predictor_mod
   use calculation_gradient
   use linear_solver
+  use nnf_models
   implicit none
 
   real(rkind), save :: hx_mint, hx_maxt, hy_mint, hy_maxt
@@ -542,5 +543,34 @@ contains
     end do
     close(77)
   end subroutine vbref_ld_write
+
+  !! The reference's non-Newtonian viscosity models (nnf_models_mod.F90:8-110, no call sites in the
+  !! reference) on n shear rates: model 0 nnf_newtonian (mu_eff = its per-particle mu array, given
+  !! here in sr), 1 nnf_power_law, 2 nnf_bingham, 3 nnf_cross, 4 nnf_carreau with viscosity mu.
+  subroutine vbref_nnf(model, n, sr, mu, out) bind(C, name="vbref_nnf")
+    integer(c_int), value :: model, n
+    real(c_double), value :: mu
+    real(c_double), intent(in) :: sr(n)
+    real(c_double), intent(out) :: out(n)
+    real(rkind), allocatable :: s(:), o(:)
+    real(rkind) :: m
+    integer(ikind) :: nn
+    allocate(s(n), o(n))
+    s = sr; nn = n; m = mu; o = 0.0d0
+    select case (model)
+    case (0)
+       call nnf_newtonian(s, nn, o)
+    case (1)
+       call nnf_power_law(s, m, nn, o)
+    case (2)
+       call nnf_bingham(s, m, nn, o)
+    case (3)
+       call nnf_cross(s, m, nn, o)
+    case (4)
+       call nnf_carreau(s, m, nn, o)
+    end select
+    out = o
+    deallocate(s, o)
+  end subroutine vbref_nnf
 
 end module vbref_harness

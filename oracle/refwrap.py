@@ -55,6 +55,7 @@ def lib(const_mod: int = 2):
         L.vbref_get_vort.argtypes = [C.c_int, _dp, _dp]
         L.vbref_open_units.argtypes = [C.c_int]
         L.vbref_ld_write.argtypes = [C.c_char_p, C.c_int, C.c_int, _dp]
+        L.vbref_nnf.argtypes = [C.c_int, C.c_int, _dp, C.c_double, _dp]
         _libs[key] = L
     return _libs[key]
 
@@ -203,6 +204,15 @@ def bicgstab_open(a, ija, b, x0=None, itermax=1000, tol=1e-8):
     lib().vbref_bicgstab_open(n, a.shape[0], _f(a), _f(ija, np.int32), x, _f(b), itermax, tol,
                               C.byref(it), C.byref(res))
     return x, it.value, res.value
+
+
+def nnf(model, sr, mu, const_mod=2):
+    """The reference's nnf_models subroutine `model` (0 newtonian, 1 power law, 2 Bingham, 3 Cross,
+    4 Carreau; nnf_models_mod.F90:8-110) on the shear rates sr with viscosity mu."""
+    sr = _f(sr)
+    out = np.zeros_like(sr)
+    lib(const_mod).vbref_nnf(model, sr.shape[0], sr, mu, out)
+    return out
 
 
 # ---- the reference's own run-file reader and writer (input.F90 getdata, output.F90) ----

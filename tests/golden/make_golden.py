@@ -2,14 +2,15 @@
 the reference's Fortran sources by oracle/Makefile, one OpenMP thread) stepped phase by phase on
 small seeded cases.
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [nnf]      (nnf: only the viscosity-model fixture)
 
 For every phase it records the SHA-256 of each array the phase produces (mirror relations and
 offsets, neighbour lists, kernel weights and KGC, free-surface flags and quantities, shifting
 velocity, particle state, pressure gradient) and the step scalars; the final state is stored whole.  tests/test_golden.py checks the C
 restatement against these files, so the oracle stays pinned where the reference is absent.  It also
 runs the reference's case generator (oracle/_ref/gen2D) on its eight cases and records each run
-directory as the readers parse it (generator_cases.json, for tests/test_generator.py).
+directory as the readers parse it (generator_cases.json, for tests/test_generator.py), and the
+reference's nnf_models subroutines on a shear-rate sweep (nnf_reference.npz).
 """
 import hashlib
 import json
@@ -126,10 +127,40 @@ def generator_fixtures():
         json.dump(out, f, indent=0)
 
 
+NNF_MU = (1.7, 0.02)          # viscosities of the nnf_models sweep
+
+
+def nnf_sweep(mu):
+    """Shear rates that exercise every term of nnf_models_mod.F90:8-110: 0, the Bingham threshold
+    tau_y/(mu alpha) and the doubles either side of it, a log sweep over 1e-8..1e8 (the Cross
+    transition near 1/kk = 1e3, the Carreau one near 1/kk = 7) and a linear sweep over 0..20."""
+    thr = 1.0e-2 / (mu * 1.0e2)
+    return np.concatenate([[0.0, thr, np.nextafter(thr, 0.0), np.nextafter(thr, np.inf),
+                            np.nextafter(np.nextafter(thr, 0.0), 0.0)],
+                           np.logspace(-8.0, 8.0, 161), np.linspace(0.0, 20.0, 81)[1:]])
+
+
+def nnf_fixtures():
+    """The reference's five nnf_models subroutines on the sweep, for each viscosity (nnf_reference.npz)."""
+    from oracle import refwrap
+    out = {}
+    for q, mu in enumerate(NNF_MU):
+        sr = nnf_sweep(mu)
+        out[f"sr{q}"] = sr
+        for model in range(5):
+            out[f"mu{q}_model{model}"] = refwrap.nnf(model, sr, mu)
+    out["mu"] = np.array(NNF_MU)
+    np.savez_compressed(os.path.join(HERE, "nnf_reference.npz"), **out)
+    print("nnf fixtures", {k: v.shape for k, v in out.items()})
+
+
 def main():
     from oracle import refwrap
     if not refwrap.available():
         sys.exit("oracle/_ref/libvbref.so is not built (make -C oracle ref)")
+    nnf_fixtures()
+    if sys.argv[1:] == ["nnf"]:
+        return
     index = {}
     for name, (mk, steps) in CASES.items():
         case = mk()

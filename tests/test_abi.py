@@ -196,3 +196,23 @@ int main(void) {
     I, B, R = refio.Indat, refio.Ibound, refio.Restart
     assert got == [C.sizeof(I), I.body_force.offset, I.alpha_evss.offset, C.sizeof(B), B.u_inflow.offset,
                    B.c_thermal.offset, C.sizeof(R), R.xb_min.offset, R.av_conc.offset]
+
+
+def test_create_refuses_npfb_beyond_32bit_slot_indices():
+    """Neighbour slots are addressed k*npfb + i with 32-bit ints: vb_create refuses npfb above
+    (2^31 - 1) / VB_NPLINK with VB_ERR_CAPACITY before it makes any device call (so this runs
+    without a GPU), and leaves the output handle NULL."""
+    from vebisph_amd import cases, solver
+    L = _lib.lib()
+    case = cases.make("poiseuille_2500")
+    keep = []
+    p = solver.make_params(case)
+    b = solver.make_boundary(case, keep)
+    h = C.c_void_p(1234)
+    limit = (2 ** 31 - 1) // _lib.NPLINK
+    assert limit == 28_633_115
+    for npfb in (limit + 1, 2 ** 31 - 1):
+        assert L.vb_create(C.byref(p), C.byref(b), npfb, 0, C.byref(h)) == 4       # VB_ERR_CAPACITY
+        assert h.value is None
+        h = C.c_void_p(1234)
+    assert L.vb_create(C.byref(p), C.byref(b), 0, 0, C.byref(h)) == 8               # VB_ERR_ARG

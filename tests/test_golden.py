@@ -38,3 +38,32 @@ def test_oracle_reproduces_reference_fixtures(name):
         ref = np.load(path)
         for k in ref.files:
             assert np.array_equal(np.asarray(final[k]), ref[k]), k
+
+
+def test_oracle_nnf_reproduces_reference_fixture():
+    """The oracle's nnf_models restatement against the reference's own five subroutines
+    (nnf_models_mod.F90:8-110, run by make_golden.py over a shear-rate sweep that crosses the
+    Bingham threshold and the Cross and Carreau transitions): bit for bit."""
+    from oracle import oraclewrap
+    ref = np.load(os.path.join(GOLDEN, "nnf_reference.npz"))
+    for q, mu in enumerate(ref["mu"]):
+        sr = ref[f"sr{q}"]
+        assert sr.size == make_golden.nnf_sweep(mu).size
+        for model in range(5):
+            got = oraclewrap.nnf(model, sr, float(mu))
+            assert np.array_equal(got, ref[f"mu{q}_model{model}"]), (mu, model)
+
+
+def test_nnf_fixture_discriminates_model_constants():
+    """A one-constant slip in any model (the exponent 1.5, the Bingham tau_y or alpha, the Cross
+    kk or mm, the Carreau kk, mm or aa) moves the fixture's values: the sweep pins every constant."""
+    ref = np.load(os.path.join(GOLDEN, "nnf_reference.npz"))
+    mu = float(ref["mu"][0]); sr = ref["sr0"]
+    slips = {
+        1: lambda s: mu * (s + 1.0) ** (1.4 - 1.0),
+        2: lambda s: np.where(s <= 1.1e-2 / (mu * 1e2), mu * 1e2, mu + 1.1e-2 / np.maximum(s, 1e-300)),
+        3: lambda s: 0.05 * mu + (mu - 0.05 * mu) / (1.0 + (1.1e-3 * s) ** 3.0),
+        4: lambda s: 1e-3 + (mu - 1e-3) * (1.0 + (0.14 * s) ** 1.6) ** ((0.1 - 1.0) / 1.6),
+    }
+    for model, f in slips.items():
+        assert not np.allclose(f(sr), ref[f"mu0_model{model}"], rtol=1e-6, atol=0), model

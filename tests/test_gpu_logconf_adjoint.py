@@ -9,11 +9,12 @@ has no adjoint, so the derivative is pinned by its definition:
   reference), per input, to 1e-6 of each input's cotangent scale, with and without the reference's
   B perturbation ("kick") and with the PTT trace term (eps_ptt > 0).
 
-Rows are generic: |A21| > 0.02 and an eigenvalue gap > 0.1, where the map is smooth.  The reference's
-non-smooth points (the A21 = 0 branch at A = I, where the 1e-10 in the rotation-rate denominator
-makes the map vary on a 1e-10 scale, and the discriminant clamp) are outside what a finite
-difference can check; there the adjoint is the derivative of the branch taken, and the test checks
-that it is finite."""
+Rows are generic (|A21| > 0.02 and an eigenvalue gap > 0.1) or diagonal with A11 > A22 + 0.1 (A21 = 0
+exactly, where the map is smooth and the derivative pass takes the eigenvector branch), so the map is
+smooth at every row.  The reference's non-smooth points (A21 = 0 with A11 <= A22, among them the
+degenerate start A = I, where the 1e-10 in the rotation-rate denominator makes the map vary on a 1e-10
+scale, and the discriminant clamp) are outside what a finite difference can check; there the adjoint
+is the derivative of the branch taken, and the test checks that it is finite."""
 import numpy as np
 import pytest
 
@@ -34,6 +35,13 @@ def _inputs(n, seed):
     # when A11 < A22, and a finite difference then divides that rounding by 2h
     keep = (gap > 0.1) & (np.abs(tau[:, 2]) > 0.02)
     tau = tau[keep]
+    # and diagonal rows with A11 > A22 (tau_xy = tau_yx = 0, gap 0.1..0.5): the map is smooth across
+    # A21 = 0 there, and the derivative in A21 is that of the eigenvector branch on either side
+    nd = n // 8
+    diag = np.zeros((nd, 4))
+    diag[:, 3] = rng.uniform(-0.2, 0.3, nd)
+    diag[:, 0] = diag[:, 3] + rng.uniform(0.1, 0.5, nd)
+    tau = np.concatenate([tau, diag])
     m = tau.shape[0]
     trc = 2.0 + tau[:, 0] + tau[:, 3]
     gv = rng.uniform(-2, 2, (m, 4))

@@ -60,6 +60,10 @@ def test_ppe_operator_and_transpose(gpu_lib, name):
     atx = G.ppe_operator(x, transpose=True)
     _close(ax, A @ x, absA @ np.abs(x))
     _close(atx, A.T @ x, absA.T @ np.abs(x))
+    # the solvers' own rows: the adjoint solve's A^T row (compact A^T plane, map rows, mirror
+    # buckets) and the forward FP64 solve's compact rows, one application each
+    _close(G.ppe_operator(x, transpose=True, solver_rows=True), A.T @ x, absA.T @ np.abs(x))
+    _close(G.ppe_operator(x, solver_rows=True), A @ x, absA @ np.abs(x))
     aty = G.ppe_operator(y, transpose=True)
     lhs, rhs = float(np.dot(A @ x, y)), float(np.dot(x, aty))
     scale = float(np.abs(y) @ (absA @ np.abs(x)))
@@ -92,13 +96,14 @@ def test_ppe_operator_refused_for_a_callers_system(gpu_lib):
         G.ppe_operator(np.ones(case.npfb))
 
 
-@pytest.mark.parametrize("name", ["poiseuille", "fs_convection"])
+@pytest.mark.parametrize("name", list(CASES))
 def test_ppe_adjoint_solve(gpu_lib, name):
     """A^T lambda = g with the reference's BiCGSTAB on the transposed PPE: the residual meets the
-    reference criterion on A^T itself (checked with scipy), and in the open domain, where A is
+    reference criterion on A^T itself (checked with scipy), and in the open domains, where A is
     nonsingular, lambda is scipy's direct solution to the accuracy the criterion allows
-    (cond(A) * 1e-8: 1e-3 relative).  In the closed domain g has zero mean (A^T maps into the
-    mean-zero vectors)."""
+    (cond(A) * 1e-8: 1e-3 relative).  Closed domains (Poiseuille; the cylinder, whose circle mirrors
+    make A unsymmetric) take g with zero mean (A^T maps into the mean-zero vectors); a g with a
+    constant added gives the same lambda, since the solve removes g's mean first."""
     import scipy.sparse.linalg as spla
     case = CASES[name]()
     G = solver.Solver(case)
@@ -119,6 +124,12 @@ def test_ppe_adjoint_solve(gpu_lib, name):
         err = np.linalg.norm(lam - ref) / np.linalg.norm(ref)
         print(f"  against scipy's direct solve: {err:.2e}", flush=True)
         assert err <= 1e-3
+    else:
+        lam2, it2, res2, st2 = G.ppe_adjoint_solve(g + 0.37)
+        print(f"  g + 0.37: {it2} iterations, residual {res2:.3e}, status {st2}", flush=True)
+        assert st2 == 0 and res2 <= 1e-8
+        r2 = g - A.T @ lam2
+        assert np.linalg.norm(r2) <= 1e-6 * np.linalg.norm(g)
 
 
 def _grad_matrix(G, n):

@@ -179,17 +179,14 @@ def test_bicgstab_parallel_iterates_follow_reference(gpu_lib):
             assert abs(rg - ro) / ro <= max(100.0 * sr, 1e-13)
 
 
-@pytest.mark.parametrize("name,spmv", [("poiseuille", "compact"), ("convection", "compact"),
-                                       ("cylinder", "compact"), ("cylinder", "tiled")])
-def test_mixed_precision_fields(gpu_lib, name, spmv, monkeypatch):
+@pytest.mark.parametrize("name", list(SMALL))
+def test_mixed_precision_fields(gpu_lib, name):
     """FP32 matvecs inside the FP64 BiCGSTAB: ten steps against the FP64 oracle, held to the same
     reference-sensitivity scale as the FP64 path, with the FP64 residual of the final solve checked
-    on the assembled system.  spmv: stored FP32 rows (default) or the cell-tile SpMV (VB_SPMV=tiled)."""
+    on the assembled system."""
     from vebisph_amd import _lib
     case = SMALL[name]()
-    print(f"case {case.name} spmv {spmv}", flush=True)
-    if spmv == "tiled":
-        monkeypatch.setenv("VB_SPMV", "tiled")
+    print(f"case {case.name}", flush=True)
     O = ow.OracleSolver(case); O2 = ow.OracleSolver(case, state=_perturbed_state(case))
     G = solver.Solver(case, precision=_lib.PREC_MIXED)
     for _ in range(10):
@@ -260,6 +257,65 @@ def test_multi_step_fields_within_reference_sensitivity(gpu_lib, name):
     assert G.scalars()["time"] == O.scalars()["time"]
 
 
+@pytest.mark.parametrize("name", list(SMALL))
+def test_whole_steps_serial_order_against_oracle(gpu_lib, name):
+    """Ten whole steps with every global sum in index order (serial_reductions: the reference's
+    single-thread loops; the SpMVs then use the stored rows in the reference's slot order, and the
+    BiCGSTAB follows the reference's iterates bit for bit on identical systems).  What remains between
+    the GPU and the oracle is the device's exp/log (log-conformation) and pow (the shift's tensile
+    term) against the host's libm, ulps that the chaotic 1e-8-tolerance solve then carries.  Every
+    field whose oracle one-ulp spread (the oracle against itself from fluid positions moved by one
+    ulp) is below 1e-10 is held to 1e-10; the others to ten times that spread."""
+    case = SMALL[name]()
+    print(f"case {case.name} npfb={case.npfb}", flush=True)
+    # the spread moves the fluid positions only: wall particles sit exactly on the periodic patches,
+    # where one ulp decides whether a corner particle gets a periodic mirror (a discrete change of the
+    # mirror set, and through the capped shift a jump of ~0.1 h in nearby positions)
+    fl = _perturbed_state(case)
+    fl["x"][:case.nbS] = case.x[:case.nbS]; fl["y"][:case.nbS] = case.y[:case.nbS]
+    O = ow.OracleSolver(case); O2 = ow.OracleSolver(case, state=fl)
+    G = solver.Solver(case, serial_reductions=True)
+    iters = []
+    for _ in range(10):
+        O.step(); O2.step(); G.div_free()
+        iters.append((G.scalars()["LS_iters"], O.scalars()["LS_iters"]))
+    print(f"BiCGSTAB iterations (gpu, oracle): {iters}", flush=True)
+    a, b, b2 = G.get_state(), O.get_state(), O2.get_state()
+    tight = []
+    for key in ("x", "y", "u", "v", "theta", "p", "tau", "trace"):
+        spread = _rel(b2[key], b[key])
+        err = _rel(a[key], b[key])
+        print(f"serial {key}: gpu-vs-oracle {err:.2e}, oracle one-ulp spread {spread:.2e}", flush=True)
+        if spread < 1e-10:
+            tight.append(key)
+            assert err <= 1e-10, key
+        else:
+            assert err <= 10.0 * spread, key
+    assert {"x", "y"} <= set(tight)
+    assert G.scalars()["time"] == O.scalars()["time"]
+
+
+def test_newtonian_serial_step_differs_only_through_pow(gpu_lib):
+    """Where the attribution can be made exact: the Newtonian program (no log-conformation, so no
+    exp/log) in serial order.  The first step from rest then gives the oracle's velocities and
+    pressure bit for bit (its right-hand side, matrix and BiCGSTAB iterates are the same operations),
+    and positions differ only through the shift's pow (ushift to ulps, positions to 1e-15)."""
+    case = SMALL["poiseuille"]()
+    case.const_mod = 1
+    O = ow.OracleSolver(case)
+    G = solver.Solver(case, serial_reductions=True)
+    for k in range(1, 13):
+        G.phase(k); O.phase(k)
+        if k == 7:
+            us_err = _rel(G.ushift(), O.ushift())
+    a, b = G.get_state(), O.get_state()
+    print(f"ushift {us_err:.2e}; x {_rel(a['x'], b['x']):.2e}, y {_rel(a['y'], b['y']):.2e}", flush=True)
+    for key in ("u", "v", "p", "theta"):
+        _eq(a[key], b[key])
+    assert us_err <= 1e-13
+    assert _rel(a["x"], b["x"]) <= 1e-15 and _rel(a["y"], b["y"]) <= 1e-15
+
+
 def test_large_case_invariants(gpu_lib):
     """1.0M-particle convection: properties that do not need the oracle at full size."""
     case = cases.make("convection_1M")
@@ -307,11 +363,20 @@ def test_standalone_kernels(gpu_lib):
     assert L.vb_kernel_functions(q.size, dq.ptr, ad7, ad7 / h, df.ptr, dw.ptr) == 0
     f, w = ow.fac_wab(q, ad7, ad7 / h)
     _eq(df.numpy(), f); _eq(dw.numpy(), w)
-    sr = np.array([0.0, 0.3, 2.0, 50.0, 1e4])
-    for model in range(5):
-        ds = D(sr); do = D(np.zeros_like(sr))
-        assert L.vb_nnf(model, sr.size, ds.ptr, 1.7, do.ptr) == 0
-        assert np.allclose(do.numpy(), ow.nnf(model, sr, 1.7), rtol=1e-14, atol=0)
+    # nnf_models against the reference's own subroutines (tests/golden/nnf_reference.npz, written by
+    # the reference build over a sweep across the Bingham threshold and the Cross/Carreau
+    # transitions): exact where no pow is taken (models 0, 2), else the device pow within 1e-14
+    ref = np.load(os.path.join(os.path.dirname(__file__), "golden", "nnf_reference.npz"))
+    for q, mu in enumerate(ref["mu"]):
+        sr = ref[f"sr{q}"]
+        for model in range(5):
+            ds = D(sr); do = D(np.zeros_like(sr))
+            assert L.vb_nnf(model, sr.size, ds.ptr, float(mu), do.ptr) == 0
+            want = ref[f"mu{q}_model{model}"]
+            if model in (0, 2):
+                _eq(do.numpy(), want)
+            else:
+                assert np.allclose(do.numpy(), want, rtol=1e-14, atol=0), (mu, model)
     rng = np.random.default_rng(3)
     n = 4096
     tau = np.zeros((n, 4)); tau[:, 0] = rng.uniform(-0.2, 0.6, n); tau[:, 3] = rng.uniform(-0.2, 0.6, n)

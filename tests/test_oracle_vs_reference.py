@@ -205,3 +205,17 @@ def test_reference_open_bicgstab_on_oracle_matrix():
     xo2, ito2, ro2, st = oraclewrap.bicgstab_open(a, ija, b * 1e-3)
     assert itr2 == ito2 and rr2 == ro2
     _same(xr2, xo2)
+
+
+def test_nnf_models_bit_exact():
+    """nnf_models (nnf_models_mod.F90:8-110, no callers in the reference) called directly in the
+    reference build against the oracle's restatement over a shear-rate sweep (the golden sweep):
+    bit for bit, for two viscosities and all five models."""
+    import sys, os
+    from oracle import oraclewrap, refwrap
+    sys.path.insert(0, os.path.join(os.path.dirname(__file__), "golden"))
+    import make_golden
+    for mu in make_golden.NNF_MU:
+        sr = make_golden.nnf_sweep(mu)
+        for model in range(5):
+            _same(oraclewrap.nnf(model, sr, mu), refwrap.nnf(model, sr, mu))

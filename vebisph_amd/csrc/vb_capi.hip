@@ -64,8 +64,6 @@ static void free_all(vb_ctx* c) {
                   c->st.t22, c->st.trc, c->st.p, c->st.inbin, c->st.x0, c->st.y0, c->st.u0, c->st.v0,
                   c->st.irel, c->st.vrel, c->st.dP, c->u_alt, c->v_alt, c->th_alt, c->nsurf,
                   c->seg_counts, c->cellkey, c->cellslot, c->nc, c->ist, c->ip, c->scan_bsum, c->cnt, c->link,
-                  c->perm, c->rank, c->pflag, c->pscan, c->ist_real, c->posP, c->idx16, c->lenP,
-                  c->rowflagP, c->rowsumP, c->bperm, c->xperm, c->atab,
                   c->wg1, c->wg2, c->wl, c->k11, c->k12, c->k21, c->k22,
                   c->usx, c->usy, c->kick, c->gpx, c->gpy, c->vort, c->conc, c->snx, c->sny, c->pscale, c->a_diag, c->a_val, c->a_rowsum, c->rhs,
                   c->a_col, c->a_len, c->rowflag, c->chunkg, c->choff, c->xflag, c->xscan, c->xrows,
@@ -158,6 +156,10 @@ extern "C" {
 vb_status vb_create(const vb_params* prm, const vb_boundary* bnd, int32_t npfb, int32_t nbS, vb_ctx** out) {
   if (!prm || !bnd || !out || npfb < 1 || nbS < 0 || nbS > npfb) return VB_ERR_ARG;
   *out = nullptr;
+  // Slot indices k*npfb + i (k < VB_NPLINK) and the transposed map's keys i*VB_NPLINK + k are 32-bit:
+  // more than (2^31 - 1) / VB_NPLINK = 28,633,115 particles per context would wrap them.  Checked
+  // before any device call.
+  if ((int64_t)npfb * kNplink > (int64_t)INT32_MAX) return VB_ERR_CAPACITY;
   vb_ctx* c = new vb_ctx();
   memset((void*)&c->st, 0, sizeof c->st);
   c->prm = *prm;
@@ -201,22 +203,9 @@ vb_status vb_create(const vb_params* prm, const vb_boundary* bnd, int32_t npfb, 
   A(c->xflag, n + 1); A(c->xscan, n + 1); A(c->xrows, n);
   A(c->rowflag, n); A(c->lnk4, nw4); A(c->val4, nw4);
   // VB_SPMV=stored: every FP64 SpMV row from the stored matrix (measurement switch; same results).
-  // VB_SPMV=tiled: the mixed path's SpMV on LDS-staged cell tiles with recomputed FP32 entries.
   const char* sp = getenv("VB_SPMV");
   c->spmv_stored = sp && strcmp(sp, "stored") == 0;
-  c->spmv_tiled = sp && strcmp(sp, "tiled") == 0 && prm->precision == VB_PREC_MIXED;
-  if (c->spmv_tiled && prm->i_open_domain == 1 && s == VB_OK) {
-    // its FP32 entries are recomputed from the pair distance, which the open domain's ramp rows
-    // (scaled by ppe_smooth) and Jacobi preconditioner do not follow
-    c->err = "VB_SPMV=tiled supports closed domains only";
-    s = VB_ERR_ARG;
-  }
   if (prm->precision == VB_PREC_MIXED) A(c->val4f, nw4);
-  if (c->spmv_tiled) {
-    A(c->perm, n); A(c->rank, n); A(c->pflag, cap + 1); A(c->pscan, cap + 1); A(c->ist_real, c->g.nct + 1);
-    A(c->posP, 2 * n); A(c->idx16, nw4); A(c->lenP, n); A(c->rowflagP, n); A(c->rowsumP, n);
-    A(c->bperm, n); A(c->xperm, n); A(c->atab, 4 * kTabN);
-  }
   A(c->r, n); A(c->rb, n); A(c->vv, n); A(c->pn, n); A(c->pinv, n); A(c->ss, n); A(c->sinv, n);
   A(c->tt, n); A(c->xsys, n);
   A(c->partials, (size_t)kSolverStages * 2 * kPartialStride); A(c->scal, 1);
@@ -241,16 +230,6 @@ vb_status vb_create(const vb_params* prm, const vb_boundary* bnd, int32_t npfb, 
   double a = sqrt(prm->Ra / prm->Pr), b = sqrt(prm->Ra * prm->Pr);
   c->dt_visc = prm->dt_coef_visc * prm->h * prm->h * (a < b ? a : b);
   c->np_host = npfb; c->nmirror_host = 0;
-  // mixed-path SpMV tiles: 16 cells along x (about 240 rows, a 3 x 18-cell window), coarser when
-  // the tile count would exceed the partial-sum buffers
-  c->tile_k = kTileK;
-  for (;;) {
-    c->ntx = (c->g.ncx + c->tile_k - 1) / c->tile_k;
-    c->ntiles = c->ntx * c->g.ncy;
-    if (c->ntiles + 1 <= kSpmvMaxBlocks) break;
-    c->tile_k *= 2;
-  }
-  if (s == VB_OK && c->spmv_tiled) s = upload_entry_table(c);
   if (s != VB_OK) {
     free_all(c);
     delete c;
@@ -532,9 +511,14 @@ vb_status vb_ppe_operator(vb_ctx* c, int32_t transpose, int32_t n, const double*
     if (c) c->err = "vb_ppe_operator: n must equal the context's npfb";
     return VB_ERR_ARG;
   }
+  if (transpose < 0 || transpose > 3) {
+    c->err = "vb_ppe_operator: transpose must be 0..3";
+    return VB_ERR_ARG;
+  }
   const size_t bytes = (size_t)n * sizeof(double);
   VB_HIP(c, hipMemcpyAsync(c->vv, x, bytes, hipMemcpyHostToDevice, c->stream));
-  vb_status s = apply_ppe_operator(c, transpose != 0, c->vv, c->tt);
+  vb_status s = transpose >= 2 ? apply_solver_spmv(c, transpose == 2, c->vv, c->tt)
+                               : apply_ppe_operator(c, transpose != 0, c->vv, c->tt);
   if (s != VB_OK) return s;
   VB_HIP(c, hipMemcpyAsync(y, c->tt, bytes, hipMemcpyDeviceToHost, c->stream));
   VB_HIP(c, hipStreamSynchronize(c->stream));

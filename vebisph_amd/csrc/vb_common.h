@@ -26,16 +26,6 @@ constexpr int kBlock = 256;                // threads per workgroup (4 wavefront
 // beats two by 8-15%, and 256-thread workgroups beat 512.
 constexpr int kSpmvBlock = VB_SPMV_BLOCK;
 constexpr int kSpmvMaxBlocks = 16384;
-#ifndef VB_TILE_K
-#define VB_TILE_K 8
-#endif
-#ifndef VB_TILE_THREADS
-#define VB_TILE_THREADS 128
-#endif
-// Tiled mixed SpMV (VB_SPMV=tiled): cells per tile along x and workgroup size; measured best of
-// 8/128, 8/256, 16/256, 32/512 on the 1M-particle PPE.
-constexpr int kTileK = VB_TILE_K;
-constexpr int kTileThreads = VB_TILE_THREADS;
 // Grid cap of the solver's streaming vector kernels (several rows per thread).  The kernel after
 // each reduction folds the previous kernel's per-workgroup partials in every workgroup, so this
 // cap also bounds the fold traffic (workgroups x partials).  Measured on the 1M-row PPE (ms per
@@ -46,7 +36,6 @@ static_assert(kSpmvBlock % 64 == 0 && kSpmvBlock <= 1024, "SpMV block");
 static_assert(kMaxBlocks % 8 == 0 && kSpmvMaxBlocks % 8 == 0, "grid caps");
 constexpr int kSolverStages = 7;           // BiCGSTAB reduction stages with their own buffers
 constexpr int kNplink = VB_NPLINK;
-constexpr int kTabN = 256;                 // mixed SpMV entry table: cubics on 256 intervals of q
 constexpr double kEps = 2.220446049250313e-16;   // epsilon(1d0)
 
 // Boundary description and derived constants, passed by value to kernels (< 4 KiB).
@@ -87,8 +76,6 @@ struct Scal {
   // BiCGSTAB in five kernels per iteration: rho of iteration k in rho_pp[k & 1]; sums of r, v and
   // of the projected p, each recorded by the kernel that folds it
   double rho_pp[2], sum_r, sum_v, sum_p;
-  // mixed path: real particles binned in cells (the cell-ordered rows), largest tile window
-  int nreal_cells, wmax;
   int nexplicit;                           // PPE rows with mirror neighbours (full stored rows)
   // output quantities: RMS velocity (secondary_properties), max |u|, |v|, |p| over rows 1..npfb
   double u_rms, fmax[3];

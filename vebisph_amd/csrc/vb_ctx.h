@@ -65,18 +65,7 @@ struct vb_ctx {
   double* val4;                      //   FP64 values
   float* val4f;                      //   FP32 values (mixed path, stored-value SpMV)
 
-  // mixed path: the solve runs on the real particles in cell order (rows k = rank of particle
-  // perm[k]) with the SpMV on tiles of cells staged in LDS
-  int *perm, *rank, *pflag, *pscan, *ist_real;
-  double* posP;                      // (x, y) of perm[k], interleaved
-  uint16_t* idx16;                   // slot-major [s*npfb + k]: neighbour's index in its tile window
-  uint8_t *lenP, *rowflagP;          // neighbour count, 1 = full stored row
-  double *rowsumP, *bperm, *xperm;
-  int tile_k, ntx, ntiles;           // cells per tile along x, tiles per cell row, tiles
-  int wlds;                          // staged window rows the SpMV is launched with
-  float* atab;                       // entry table of the tiled SpMV (4 floats per interval)
   bool spmv_stored;                  // FP64 path: full stored rows everywhere (VB_SPMV=stored)
-  bool spmv_tiled;                   // mixed path: SpMV on cell tiles (VB_SPMV=tiled)
   double *r, *rb, *vv, *pn, *pinv, *ss, *sinv, *tt, *xsys;
 
   double* partials;                  // reduction partials, 2 * kPartialStride per solver stage
@@ -106,17 +95,20 @@ vb_status launch_circle_forces(vb_ctx* c);          // Scal::circ
 vb_status launch_nusselt(vb_ctx* c, double scale);  // Scal::nusselt
 // PPE (vb_ppe.hip)
 vb_status launch_ppe(vb_ctx* c, double tol, int itermax);
-vb_status upload_entry_table(vb_ctx* c);
 vb_status profile_spmv(vb_ctx* c, int reps, double* avg_us, double* alg_bytes);
 vb_status solve_nr(vb_ctx* c, const double* A_nr, const int* ija, double* x, const double* B, int n,
                    int itermax, double tol, bool open);
 // the assembled PPE operator (vb_operator.hip): y = A x or A^T x on device vectors of npfb rows
 vb_status apply_ppe_operator(vb_ctx* c, bool transpose, const double* x, double* y);
+// the same through the solvers' own SpMV rows (forward FP64 compact rows, adjoint A^T rows; vb_ppe.hip)
+vb_status apply_solver_spmv(vb_ctx* c, bool transpose, const double* x, double* y);
 vb_status build_transpose(vb_ctx* c);
 vb_status launch_transpose_sliced(vb_ctx* c);
 vb_status apply_grad_operator(vb_ctx* c, bool transpose, const double* x, double* y);
 // A^T lambda = g with the reference's BiCGSTAB on the transposed map (vb_ppe.hip); g, lambda in op
 vb_status adjoint_solve(vb_ctx* c, double tol, int itermax);
+// v -= mean(v) on the device (vb_operator.hip; uses the reduction partials)
+vb_status remove_mean(vb_ctx* c, double* v, int n);
 void free_operator(vb_ctx* c);
 // out[0..n-1] = exclusive prefix sums of in[0..n-1], out[n] = total (vb_step.hip)
 vb_status launch_exclusive_scan(vb_ctx* c, const int* in, int* out, int n);

@@ -33,6 +33,20 @@ struct Dual {
   }
 };
 
+// Which branch of calc_eigens a matrix takes.  The reference takes the eigenvector branch when
+// A21 != 0 and otherwise fixes R = I, L = (A11, A22) (predictor_mod.F90:280-288).  Where A21 = 0 and
+// A11 > A22 with the discriminant unclamped (gap above ~3e-8), the map is smooth across A21 = 0 and
+// the eigenvector branch gives the same values there (r11 = L1 - A22 = A11 - A22 > 0, r21 = 0, so
+// R = I; L differs from (A11, A22) only by the rounding of its formula), but the A21 = 0 branch
+// drops the derivative of R in A21 (dR = dA21 / (A11 - A22)).  So the derivative pass (Dual) takes
+// the eigenvector branch there; the step's FP64 map keeps the reference's test.  Left non-smooth:
+// A21 = 0 with A11 <= A22 (the map's two sides differ by the 1e-10 of the rotation-rate denominator)
+// and the clamped discriminant at A21 = 0.
+__host__ __device__ inline bool eigenvector_branch(const double* A, double) { return A[2] != 0.0; }
+__host__ __device__ inline bool eigenvector_branch(const Dual* A, Dual tmp) {
+  return A[2].v != 0.0 || (A[0].v > A[3].v && tmp.v > 0.0);
+}
+
 // calc_eigens, predictor_mod.F90:253-290 (A, R as a11,a12,a21,a22)
 template <class T>
 __host__ __device__ inline void calc_eigens(const T* A, T* L, T* R) {
@@ -43,7 +57,7 @@ __host__ __device__ inline void calc_eigens(const T* A, T* L, T* R) {
   T sq = sqrt(tmp);
   L[0] = 0.5 * tr + sq;
   L[1] = 0.5 * tr - sq;
-  if (A[2] != 0.0) {
+  if (eigenvector_branch(A, tmp)) {
     T r11 = L[0] - A[3], r21 = A[2];
     T nrm = sqrt(r11 * r11 + r21 * r21);
     r11 = r11 / nrm; r21 = r21 / nrm;

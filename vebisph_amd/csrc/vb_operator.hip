@@ -299,6 +299,28 @@ vb_status apply_grad_operator(vb_ctx* c, bool transpose, const double* x, double
   return VB_OK;
 }
 
+// v -= mean(v) over n rows: per-workgroup partial sums, then every workgroup of the second kernel
+// folds them in the same fixed order (same bits) and subtracts.
+__global__ void k_sum_partials(const double* v, int n, double* partials) {
+  RowRange rr = block_rows(n, gridDim.x);
+  Red<1> R(nullptr, n);
+  for (int i = rr.begin + threadIdx.x; i < rr.end; i += blockDim.x) R.add(0, i, v[i]);
+  R.store(partials);
+}
+
+__global__ void k_sub_mean(double* v, int n, const double* partials, int nblk) {
+  const double m = fold(partials, 0, nblk, nullptr, n) / (double)n;
+  RowRange rr = block_rows(n, gridDim.x);
+  for (int i = rr.begin + threadIdx.x; i < rr.end; i += blockDim.x) v[i] = v[i] - m;
+}
+
+vb_status remove_mean(vb_ctx* c, double* v, int n) {
+  const int nb = grid_for(n);
+  VB_LAUNCH(c, k_sum_partials, nb, v, n, c->partials);
+  VB_LAUNCH(c, k_sub_mean, nb, v, n, c->partials, nb);
+  return VB_OK;
+}
+
 void free_operator(vb_ctx* c) {
   void* ptrs[] = {c->op.tval, c->op.tcol, c->op.mcount, c->op.moff, c->op.mfill, c->op.mkey, c->op.mrow,
                   c->op.mval, c->op.nmiss, c->op.g, c->op.lam, c->op.val4T, c->op.rslot, c->op.ga,

@@ -930,267 +930,6 @@ __global__ void k_fu_fin(RedBufs rb, int nblk, Scal* sc, int k) {
   finish_iteration(f, k, sc, threadIdx.x == 0);
 }
 
-// ------------------------------------------------------------------------------------------------
-// Mixed path, VB_SPMV=tiled: the solve in cell order, SpMV on LDS-staged tiles of cells.  It reads
-// 2 bytes per slot from HBM instead of 6, but on the 1M-particle PPE it runs at 101 us per SpMV
-// against 89 us for the stored-FP32 rows (the per-slot LDS reads and entry evaluation, and the
-// window staging, cost more than the bytes saved), so the stored rows are the default.
-//
-// Rows.  The real particles are renumbered in cell order (cells x-fastest, the stable in-cell
-// order of the binning): row k is particle perm[k], rank[perm[k]] = k; particles binned out of the
-// grid follow the cell-ordered ones.  Every solver vector lives in this order for the whole solve;
-// the right-hand side is gathered in and the pressure scattered out once.
-//
-// Tiles.  A tile is kTileK consecutive cells of one cell row; its rows are contiguous in cell
-// order, and every neighbour of a tile row lies in the 3 x (kTileK + 2) cells around the tile,
-// which are three contiguous row ranges (one per cell row).  The SpMV stages that window -- the
-// positions and the input vector, from coalesced loads -- in LDS, and each compact row reads its
-// neighbours there through 2-byte window indices (written once per step by k_tile_assemble), so
-// the only per-slot HBM traffic is the index.  The FP32 entries are recomputed from the pair
-// distance: on a compact row every neighbour is a real particle and the entry is -w_L
-// (ppe_...F90:76-115) with w_L = 2 (r_ij . w_G)/(|r_ij|^2 + 0.01 h^2), w_G = dv fac(q) r_ij/|r_ij|
-// (kernel_calculation.F90:46-53), a function of the distance alone:
-//     a_ij = -2 dv fac(|r_ij|/h) |r_ij| / (|r_ij|^2 + 0.01 h^2),
-// with r_ij formed in FP64 from the staged positions (the positions the weights were computed
-// from: nothing moves between kernel_calculation and the PPE) and a_ij(q), q = |r_ij|/h, read from a
-// per-context table of cubics on 256 intervals of [0, 3] (relative error below 1e-9 of max|a| for
-// q > 0.5, 3e-6 near q = 0, where no pair lies), staged in LDS; so a slot costs two LDS reads,
-// three FP64 subtractions, a square root and four FMAs.  The row is applied to FP32 differences of
-// the FP64 vector, y_k = sum a (x_j - x_k) + rowsum_k x_k, as the stored-value mixed SpMV.  Rows
-// with a mirror neighbour, rows of a tile whose window exceeds the LDS budget, and binned rows use
-// the full FP64 stored row, gathered through rank[].
-// ------------------------------------------------------------------------------------------------
-constexpr int kTileWindowCap = 2560;      // staged window rows (24 bytes each: 60 KB of LDS)
-
-struct TileGeo {
-  int row0, row1;                         // the tile's rows
-  int ws[3], we[3], base[3], W;           // window: cell rows cy-1, cy, cy+1
-};
-
-__device__ inline TileGeo tile_geo(int t, const int* __restrict__ ist_real, int ncx, int ncy, int ntx, int K) {
-  TileGeo G;
-  const int cy = t / ntx, cx0 = (t % ntx) * K, cx1 = min(cx0 + K, ncx) - 1;
-  G.row0 = ist_real[cy * ncx + cx0];
-  G.row1 = ist_real[cy * ncx + cx1 + 1];
-  const int lo = max(cx0 - 1, 0), hi = min(cx1 + 1, ncx - 1);
-  int W = 0;
-#pragma unroll
-  for (int d = 0; d < 3; d++) {
-    const int yy = cy + d - 1;
-    if (yy < 0 || yy >= ncy) { G.ws[d] = 0; G.we[d] = 0; }
-    else { G.ws[d] = ist_real[yy * ncx + lo]; G.we[d] = ist_real[yy * ncx + hi + 1]; }
-    G.base[d] = W;
-    W += G.we[d] - G.ws[d];
-  }
-  G.W = W;
-  return G;
-}
-
-struct TileMat {
-  const uint16_t* idx;                    // [s*n + k]
-  const uint8_t *len, *flag;
-  const double *rowsum, *pos;             // by row k; pos = (x, y) pairs
-  const int *perm, *rank, *ist_real;
-  const double *diag, *val;               // full stored rows, original order
-  const int* col;
-  const float4* tab;                      // kTabN cubics in t: c.x + t (c.y + t (c.z + t c.w))
-  int n, ncx, ncy, ntx, K, ntiles, wlds;
-  float qscale;                           // kTabN / (3 h): interval coordinate of |r_ij|
-};
-
-// A full stored row in cell order: y_k = A(i) x_k + sum_s a_s x_rank(col_s), i = perm[k]
-__device__ inline double tile_row_full(const TileMat& T, int k, const double* __restrict__ x) {
-  const int i = T.perm[k], m = T.len[k], n = T.n;
-  double bb = T.diag[i] * x[k];
-  for (int s = 0; s < m; s++) bb = bb + T.val[(size_t)s * n + i] * x[T.rank[T.col[(size_t)s * n + i]]];
-  return bb;
-}
-
-// One row from the staged window (spos: positions, sx: input vector, stab: entry table)
-__device__ inline double tile_row_lds(const TileMat& T, const double2* spos, const double* sx, const float4* stab,
-                                      int k, int own, double xk) {
-  const double2 pi = spos[own];
-  const int ng = (T.len[k] + 3) >> 2;
-  const size_t n = (size_t)T.n;
-  const ushort4* idx4 = reinterpret_cast<const ushort4*>(T.idx);
-  float acc = 0.0f;
-  auto slot = [&](int q) {
-    const double2 pj = spos[q];
-    const float rx = (float)(pi.x - pj.x), ry = (float)(pi.y - pj.y);
-    float t = __builtin_amdgcn_sqrtf(rx * rx + ry * ry) * T.qscale;
-    const int it = min((int)t, kTabN - 1);
-    t = t - (float)it;
-    const float4 c = stab[it];
-    const float a = fmaf(fmaf(fmaf(c.w, t, c.z), t, c.y), t, c.x);
-    acc = fmaf(a, (float)(sx[q] - xk), acc);
-  };
-  // the next group's four indices are loaded while this group is computed
-  ushort4 g = idx4[k];
-  for (int gi = 0; gi < ng; gi++) {
-    ushort4 gn = g;
-    if (gi + 1 < ng) gn = idx4[(size_t)(gi + 1) * n + k];
-    slot(g.x); slot(g.y); slot(g.z); slot(g.w);
-    g = gn;
-  }
-  return (double)acc + T.rowsum[k] * xk;
-}
-
-// VARIANT 0: v = A p_inv, terms of r_bar . v (:353-354); VARIANT 1: t = A s_inv, terms of t.s and
-// t.t with the projected s (:385-387).  Workgroup t < ntiles takes tile t; the last workgroup takes
-// the binned rows.
-template <int VARIANT>
-__global__ void __launch_bounds__(kTileThreads)
-k_tile_spmv(TileMat T, Vecs w, RedBufs rb, const Scal* sc) {
-  if (finished(sc)) return;
-  extern __shared__ double sm[];
-  const double* __restrict__ x = VARIANT == 0 ? w.pinv : w.sinv;
-  double* y = VARIANT == 0 ? w.v : w.t;
-  const double mean_s = VARIANT == 1 ? sc->mean_s : 0.0;
-  Red<VARIANT == 0 ? 1 : 2> R(rb.R(VARIANT == 0 ? ST_SPMV_P : ST_SPMV_S), T.n);
-  auto emit = [&](int k, double yk) {
-    y[k] = yk;
-    if (VARIANT == 0) {
-      R.add(0, k, w.rb[k] * yk);
-    } else {
-      const double si = w.s[k] - mean_s;
-      R.add(0, k, yk * si);
-      R.add(1, k, yk * yk);
-    }
-  };
-  if ((int)blockIdx.x < T.ntiles) {
-    __shared__ float4 stab[kTabN];
-    double2* spos = reinterpret_cast<double2*>(sm);           // [wlds] positions
-    double* sx = sm + 2 * (size_t)T.wlds;                      // [wlds] input vector
-    for (int q = threadIdx.x; q < kTabN; q += blockDim.x) stab[q] = T.tab[q];
-    const TileGeo G = tile_geo(blockIdx.x, T.ist_real, T.ncx, T.ncy, T.ntx, T.K);
-    const bool staged = G.W <= T.wlds;
-    if (staged) {
-      const double2* pos2 = reinterpret_cast<const double2*>(T.pos);
-      for (int q = threadIdx.x; q < G.W; q += blockDim.x) {
-        const int d = q < G.base[1] ? 0 : (q < G.base[2] ? 1 : 2);
-        const int g = G.ws[d] + (q - G.base[d]);
-        spos[q] = pos2[g];
-        sx[q] = x[g];
-      }
-    }
-    __syncthreads();
-    for (int k = G.row0 + threadIdx.x; k < G.row1; k += blockDim.x) {
-      const double xk = x[k];
-      const double yk = (T.flag[k] || !staged)
-                            ? tile_row_full(T, k, x)
-                            : tile_row_lds(T, spos, sx, stab, k, G.base[1] + (k - G.ws[1]), xk);
-      emit(k, yk);
-    }
-  } else {
-    for (int k = sc->nreal_cells + threadIdx.x; k < T.n; k += blockDim.x) emit(k, tile_row_full(T, k, x));
-  }
-  R.store(rb.P(VARIANT == 0 ? ST_SPMV_P : ST_SPMV_S));
-}
-
-// The cell-order renumbering.  flag[q] = 1 for real particles among the binned list ip[0..n_in_cells).
-__global__ void k_perm_flag(const int* ip, const int* ist, int nct, int np, int npfb, int* flag) {
-  const int q = blockIdx.x * kBlock + threadIdx.x;
-  if (q < np) flag[q] = (q < ist[nct] && ip[q] < npfb) ? 1 : 0;
-}
-
-// excl = exclusive scan of flag: ranks of the cell-ordered real particles; ist_real[c] = first row
-// of cell c (c = 0..nct, ist_real[nct] = rows in cells)
-__global__ void k_perm_build(const int* ip, const int* ist, const int* excl, int nct, int np, int npfb,
-                             State st, int* perm, int* rank, double* posP, int* ist_real, Scal* sc) {
-  const int q = blockIdx.x * kBlock + threadIdx.x;
-  if (q < np && q < ist[nct]) {
-    const int i = ip[q];
-    if (i < npfb) {
-      const int r = excl[q];
-      perm[r] = i; rank[i] = r;
-      posP[2 * r] = st.x[i]; posP[2 * r + 1] = st.y[i];
-    }
-  }
-  if (q <= nct) ist_real[q] = excl[ist[q]];
-  if (q == 0) sc->nreal_cells = excl[ist[nct]];
-}
-
-__global__ void k_perm_tail_flag(const int* cellkey, int npfb, int* flag) {
-  const int i = blockIdx.x * kBlock + threadIdx.x;
-  if (i < npfb) flag[i] = cellkey[i] < 1 ? 1 : 0;
-}
-
-// real particles outside every cell (binned): rows after the cell-ordered ones, in index order
-__global__ void k_perm_tail(const int* flag, const int* excl, int npfb, State st, int* perm, int* rank,
-                            double* posP, const Scal* sc) {
-  const int i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= npfb || !flag[i]) return;
-  const int r = sc->nreal_cells + excl[i];
-  perm[r] = i; rank[i] = r;
-  posP[2 * r] = st.x[i]; posP[2 * r + 1] = st.y[i];
-}
-
-// Per row k of the cell order: count, flag, row sum, and the window indices of a compact row.  A row
-// falls back to its full stored row when it has a mirror neighbour (rowflag), when its tile's
-// window exceeds wcap, or when it is a binned row.
-__global__ void k_tile_assemble(TileMat T, const int* cnt, const int* link, const int* cellkey,
-                                const uint8_t* rowflag, const double* rowsum, uint16_t* idx,
-                                uint8_t* lenP, uint8_t* flagP, double* rowsumP, int wcap, Scal* sc) {
-  const int k = blockIdx.x * kBlock + threadIdx.x;
-  const int n = T.n;
-  if (k >= n) return;
-  const int i = T.perm[k], m = cnt[i];
-  lenP[k] = (uint8_t)m;
-  rowsumP[k] = rowsum[i];
-  bool expl = rowflag[i] != 0 || k >= sc->nreal_cells;
-  if (!expl) {
-    const int c = cellkey[i] - 1, cx = c % T.ncx, cy = c / T.ncx;
-    const TileGeo G = tile_geo(cy * T.ntx + cx / T.K, T.ist_real, T.ncx, T.ncy, T.ntx, T.K);
-    atomicMax(&sc->wmax, G.W);
-    if (G.W > wcap) expl = true;
-    // grouped by four slots, (s/4, k, s%4); the padding slots name the row itself (zero entry)
-    const int own = G.base[1] + (k - G.ws[1]);
-    const int padded = (m + 3) & ~3;
-    for (int s = 0; s < padded && !expl; s++) {
-      int q = own;
-      if (s < m) {
-        const int j = link[(size_t)s * n + i];
-        const int d = (cellkey[j] - 1) / T.ncx - cy + 1;
-        const int r = T.rank[j];
-        if (d < 0 || d > 2 || r < G.ws[d] || r >= G.we[d]) { expl = true; break; }
-        q = G.base[d] + (r - G.ws[d]);
-      }
-      idx[grouped(s, k, n)] = (uint16_t)q;
-    }
-  }
-  flagP[k] = expl ? 1 : 0;
-}
-
-__global__ void k_reset_wmax(Scal* sc) {
-  if (threadIdx.x == 0) sc->wmax = 0;
-}
-
-__global__ void k_perm_gather(const int* perm, const double* src, double* dst, int n) {
-  const int k = blockIdx.x * kBlock + threadIdx.x;
-  if (k < n) dst[k] = src[perm[k]];
-}
-
-__global__ void k_perm_scatter(const int* perm, const double* src, double* dst, int n) {
-  const int k = blockIdx.x * kBlock + threadIdx.x;
-  if (k < n) dst[perm[k]] = src[k];
-}
-
-// Mixed path residual replacement in cell order (the FP64 full rows through rank[]), pass 1 of the
-// mean-preserving replacement of k_bi_true_residual.
-__global__ void k_tile_true_residual(TileMat T, Vecs w, RedBufs rb, const Scal* sc) {
-  if (finished(sc)) return;
-  const int n = T.n;
-  RowRange rr = block_rows(n, gridDim.x);
-  Red<2> R(rb.R(ST_TRUE), n);
-  for (int k = rr.begin + threadIdx.x; k < rr.end; k += blockDim.x) {
-    const double rt = w.b[k] - tile_row_full(T, k, w.x);
-    R.add(0, k, w.r[k]);
-    R.add(1, k, rt);
-    w.t[k] = rt;
-  }
-  R.store(rb.P(ST_TRUE));
-}
-
 #define VB_FIN(c, kern, ...)                                                               \
   do {                                                                                     \
     hipLaunchKernelGGL(kern, dim3(1), dim3(kBlock), 0, (c)->stream, __VA_ARGS__);          \
@@ -1268,108 +1007,6 @@ static Vecs step_vecs(vb_ctx* c, double* x, bool open) {
   return Vecs{x, c->rhs, c->r, c->rb, c->vv, c->pn, c->pinv, c->ss, c->sinv, c->tt, open ? c->a_diag : nullptr};
 }
 
-static TileMat tile_mat(vb_ctx* c) {
-  TileMat T;
-  T.idx = c->idx16; T.len = c->lenP; T.flag = c->rowflagP; T.rowsum = c->rowsumP; T.pos = c->posP;
-  T.perm = c->perm; T.rank = c->rank; T.ist_real = c->ist_real;
-  T.diag = c->a_diag; T.val = c->a_val; T.col = c->a_col;
-  T.n = c->npfb; T.ncx = c->g.ncx; T.ncy = c->g.ncy; T.ntx = c->ntx; T.K = c->tile_k;
-  T.ntiles = c->ntiles; T.wlds = c->wlds;
-  T.tab = reinterpret_cast<const float4*>(c->atab);
-  T.qscale = (float)((double)kTabN / (3.0 * c->g.h));
-  return T;
-}
-
-// The mixed solve in cell order: the BiCGSTAB of solve() with the tiled SpMV and the cell-order
-// residual replacement.
-static vb_status solve_tiled(vb_ctx* c, const TileMat& T, Vecs w, double tol, int itermax) {
-  const int n = T.n;
-  const int nb = grid_for(n);
-  const int ns = T.ntiles + 1;
-  const size_t lds = (size_t)3 * sizeof(double) * (T.wlds > 0 ? T.wlds : 1);
-  RedBufs rb = red_bufs(c, n);
-  Scal* sc = c->scal;
-  Mat A{};                                         // unused: x is not given
-  A.n = n;
-  VB_LAUNCH(c, k_bi_init, nb, A, w, 0, rb);
-  VB_FIN(c, k_fin_init, rb, nb, tol, itermax, 1.0e-6, sc);
-  int launched = 0, batch = 8;
-  for (;;) {
-    for (int b = 0; b < batch; b++) {
-      VB_LAUNCH(c, k_bi_p, nb, n, w, rb, sc);
-      VB_LAUNCH(c, k_bi_pinv, nb, n, w, rb, sc);
-      hipLaunchKernelGGL(k_tile_spmv<0>, dim3(ns), dim3(kTileThreads), lds, c->stream, T, w, rb, sc);
-      VB_HIP(c, hipGetLastError());
-      VB_LAUNCH(c, k_bi_s, nb, n, w, rb, sc, ns);
-      VB_LAUNCH(c, k_bi_sinv, nb, n, w, rb, sc);
-      hipLaunchKernelGGL(k_tile_spmv<1>, dim3(ns), dim3(kTileThreads), lds, c->stream, T, w, rb, sc);
-      VB_HIP(c, hipGetLastError());
-      VB_LAUNCH(c, k_bi_xr, nb, n, w, rb, sc, ns);
-      if ((launched + b + 1) % kRefresh == 0) {
-        VB_LAUNCH(c, k_tile_true_residual, nb, T, w, rb, sc);
-        VB_LAUNCH(c, k_bi_true_shift, nb, n, w, rb, sc, 0, nb);
-      }
-      VB_FIN(c, k_fin_xr, rb, nb, sc);
-    }
-    launched += batch;
-    VB_HIP(c, hipMemcpyAsync(c->scal_host, c->scal, sizeof(Scal), hipMemcpyDeviceToHost, c->stream));
-    VB_HIP(c, hipStreamSynchronize(c->stream));
-    if (c->scal_host->done || c->scal_host->exit_s) break;
-    if (launched > itermax + 2) break;
-    batch = launched < 32 ? 8 : 32;
-  }
-  return VB_OK;
-}
-
-// Cell-order renumbering, the tiles' window indices and the right-hand side in cell order.
-static vb_status tile_setup(vb_ctx* c) {
-  const int np = c->np_host, npfb = c->npfb, nct = c->g.nct;
-  VB_LAUNCH(c, k_perm_flag, grid_1d(np), c->ip, c->ist, nct, np, npfb, c->pflag);
-  vb_status s = launch_exclusive_scan(c, c->pflag, c->pscan, np);
-  if (s != VB_OK) return s;
-  VB_LAUNCH(c, k_perm_build, grid_1d(std::max(np, nct + 1)), c->ip, c->ist, c->pscan, nct, np, npfb,
-            c->st, c->perm, c->rank, c->posP, c->ist_real, c->scal);
-  VB_LAUNCH(c, k_perm_tail_flag, grid_1d(npfb), c->cellkey, npfb, c->pflag);
-  if ((s = launch_exclusive_scan(c, c->pflag, c->pscan, npfb)) != VB_OK) return s;
-  VB_LAUNCH(c, k_perm_tail, grid_1d(npfb), c->pflag, c->pscan, npfb, c->st, c->perm, c->rank, c->posP,
-            c->scal);
-  VB_LAUNCH(c, k_reset_wmax, 1, c->scal);
-  TileMat T = tile_mat(c);
-  VB_LAUNCH(c, k_tile_assemble, grid_1d(npfb), T, c->cnt, c->link, c->cellkey, c->rowflag, c->a_rowsum,
-            c->idx16, c->lenP, c->rowflagP, c->rowsumP, kTileWindowCap, c->scal);
-  VB_LAUNCH(c, k_perm_gather, grid_1d(npfb), c->perm, c->rhs, c->bperm, npfb);
-  if ((s = sync_scalars(c)) != VB_OK) return s;
-  c->wlds = std::min(c->scal_host->wmax, kTileWindowCap);
-  return VB_OK;
-}
-
-// The tiled SpMV's entry table: on interval k of q in [0, 3] the cubic through the four values of
-// a(q) = -2 dv fac(q) (q h) / ((q h)^2 + 0.01f h^2) at t = 0, 1/3, 2/3, 1 (FP64 Newton form, stored
-// as FP32 monomial coefficients in t).
-vb_status upload_entry_table(vb_ctx* c) {
-  const Geom& g = c->g;
-  auto a = [&](double q) {
-    const double r = q * g.h;
-    return -2.0 * g.dv * fac(q, g.ad7h) * r / (r * r + g.eta_small);
-  };
-  std::vector<float> tab(4 * (size_t)kTabN);
-  const double dq = 3.0 / kTabN, hs = 1.0 / 3.0;
-  for (int k = 0; k < kTabN; k++) {
-    double y[4];
-    for (int j = 0; j < 4; j++) y[j] = a((k + j * hs) * dq);
-    const double f01 = (y[1] - y[0]) / hs, f12 = (y[2] - y[1]) / hs, f23 = (y[3] - y[2]) / hs;
-    const double f012 = (f12 - f01) / (2 * hs), f123 = (f23 - f12) / (2 * hs);
-    const double f0123 = (f123 - f012) / (3 * hs);
-    tab[4 * k + 0] = (float)y[0];
-    tab[4 * k + 1] = (float)(f01 - f012 * hs + f0123 * 2 * hs * hs);
-    tab[4 * k + 2] = (float)(f012 - 3 * hs * f0123);
-    tab[4 * k + 3] = (float)f0123;
-  }
-  VB_HIP(c, hipMemcpyAsync(c->atab, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
-  VB_HIP(c, hipStreamSynchronize(c->stream));
-  return VB_OK;
-}
-
 vb_status launch_ppe(vb_ctx* c, double tol, int itermax) {
   const int nb = spmv_grid(c->npfb);    // a neighbour sum: one row per thread, like the SpMV
   VB_LAUNCH_BS(c, k_ppe_rhs, nb, kSpmvBlock, c->g, c->st, c->cnt, c->link, c->wg1, c->wg2, c->k11, c->k12, c->k21,
@@ -1397,15 +1034,7 @@ vb_status launch_ppe(vb_ctx* c, double tol, int itermax) {
   c->op.valid = false; c->op.step_matrix = true;        // a new matrix: the transposed map is rebuilt on use
   Mat A = step_matrix(c);
   Vecs w = step_vecs(c, c->st.p, c->g.open != 0);
-  if (c->prm.precision == VB_PREC_MIXED) {
-    if (!c->spmv_tiled) return solve<SPMV_MIXED>(c, A, w, 0, tol, itermax);   // stored FP32 values
-    vb_status s = tile_setup(c);
-    if (s != VB_OK) return s;
-    Vecs wp{c->xperm, c->bperm, c->r, c->rb, c->vv, c->pn, c->pinv, c->ss, c->sinv, c->tt};
-    if ((s = solve_tiled(c, tile_mat(c), wp, tol, itermax)) != VB_OK) return s;
-    VB_LAUNCH(c, k_perm_scatter, grid_1d(c->npfb), c->perm, c->xperm, c->st.p, c->npfb);
-    return VB_OK;
-  }
+  if (c->prm.precision == VB_PREC_MIXED) return solve<SPMV_MIXED>(c, A, w, 0, tol, itermax);
   // FP64 from the stored rows in the reference's slot order: on request, and in serial mode
   if (c->spmv_stored || c->red) A.rowflag = nullptr;
   return solve<SPMV_FP64>(c, A, w, 0, tol, itermax);
@@ -1454,7 +1083,9 @@ vb_status solve_nr(vb_ctx* c, const double* A_nr, const int* ija, double* x, con
 // complement of A's constant null space, so the DC projections keep the iterates there as in the
 // forward solve -- or SPARSE_PBICGSTAB for open ones, Jacobi on diag(A^T) = diag(A)) with the
 // transposed map as its matrix, lambda_0 = 0.  g is read from and lambda written to the operator's
-// device vectors op.g and op.lam.
+// device vectors op.g and op.lam.  In a closed domain g's mean is removed first (in place): that is
+// the adjoint of the forward solve's DC removal of its right-hand side, and without it a g with a
+// constant component would make the system inconsistent.
 vb_status adjoint_solve(vb_ctx* c, double tol, int itermax) {
   if (!c->op.step_matrix) {
     c->err = "the adjoint solve needs the matrix of a step's vb_ppe_solve";
@@ -1464,10 +1095,38 @@ vb_status adjoint_solve(vb_ctx* c, double tol, int itermax) {
     vb_status s = build_transpose(c);
     if (s != VB_OK) return s;
   }
+  if (!c->g.open) {
+    vb_status s = remove_mean(c, c->op.g, c->npfb);
+    if (s != VB_OK) return s;
+  }
   Mat A = step_matrix(c);
   Vecs w{c->op.lam, c->op.g, c->r, c->rb, c->vv, c->pn, c->pinv, c->ss, c->sinv, c->tt,
          c->g.open ? c->a_diag : nullptr};
   return solve<SPMV_TRANSPOSE>(c, A, w, 0, tol, itermax);
+}
+
+// One application of a solver's own SpMV row to x (no reductions): the forward FP64 solve's row
+// (compact rows in column order, the others full) or the adjoint solve's A^T row, for direct checks.
+template <int MODE>
+__global__ void k_apply_solver_row(Mat A, const double* x, double* y) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < A.n) y[i] = spmv_row<MODE>(A, i, x);
+}
+
+vb_status apply_solver_spmv(vb_ctx* c, bool transpose, const double* x, double* y) {
+  if (!c->op.step_matrix) {
+    c->err = "the solver's SpMV needs the matrix of a step's vb_ppe_solve";
+    return VB_ERR_ARG;
+  }
+  if (transpose && !c->op.valid) {
+    vb_status s = build_transpose(c);
+    if (s != VB_OK) return s;
+  }
+  Mat A = step_matrix(c);
+  if (c->spmv_stored || c->red) A.rowflag = nullptr;
+  if (transpose) VB_LAUNCH(c, k_apply_solver_row<SPMV_TRANSPOSE>, grid_1d(c->npfb), A, x, y);
+  else VB_LAUNCH(c, k_apply_solver_row<SPMV_FP64>, grid_1d(c->npfb), A, x, y);
+  return VB_OK;
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -1502,17 +1161,11 @@ vb_status profile_spmv(vb_ctx* c, int reps, double* avg_us, double* alg_bytes) {
   hipEvent_t e0, e1;
   VB_HIP(c, hipEventCreate(&e0));
   VB_HIP(c, hipEventCreate(&e1));
-  const bool tiled = mixed && c->spmv_tiled, fused = c->red == nullptr;
-  const TileMat T = tile_mat(c);
-  const size_t lds = (size_t)3 * sizeof(double) * (c->wlds > 0 ? c->wlds : 1);
+  const bool fused = c->red == nullptr;
   for (int r = 0; r <= reps; r++) {
     if (r == 1) VB_HIP(c, hipEventRecord(e0, c->stream));
     RedBufs rb{c->partials, nullptr, n};
-    if (tiled) {
-      hipLaunchKernelGGL(k_tile_spmv<0>, dim3(c->ntiles + 1), dim3(kTileThreads), lds, c->stream, T, w, rb,
-                         c->scal);
-      VB_HIP(c, hipGetLastError());
-    } else if (fused) {                     // the kernel the solve launches (five-kernel iteration)
+    if (fused) {                     // the kernel the solve launches (five-kernel iteration)
       if (mixed) VB_LAUNCH_BS(c, (k_bi_spmv_p<SPMV_MIXED, true>), nb, kSpmvBlock, A, w, rb, c->scal);
       else VB_LAUNCH_BS(c, (k_bi_spmv_p<SPMV_FP64, true>), nb, kSpmvBlock, A, w, rb, c->scal);
     } else {
@@ -1532,12 +1185,9 @@ vb_status profile_spmv(vb_ctx* c, int reps, double* avg_us, double* alg_bytes) {
   // Algorithmic bytes of one launch.  FP64: per row length 4, row flag 1, diagonal 8, the gathered
   // vector 8 (each element once), r_bar 8, v written 8; per slot the 2-byte column offset and the
   // 8-byte value on compact rows, 4 + 8 on full rows.  Mixed, stored values: the same per row (row
-  // sum for the diagonal) and 2 + 4 bytes per compact slot.  Mixed, tiled: per row count 1, flag 1,
-  // row sum 8, the input vector 8 and the position pair 16 (each staged element once), r_bar 8, v 8;
-  // per compact slot the 2-byte window index; per full-row slot 4 + 8 and the 4-byte rank.
+  // sum for the diagonal) and 2 + 4 bytes per compact slot.
   const double nnz = counts[0], nnz_full = A.rowflag ? counts[1] : counts[0];
-  if (tiled) *alg_bytes = 50.0 * n + 2.0 * (nnz - nnz_full) + 16.0 * nnz_full;
-  else *alg_bytes = 37.0 * n + (mixed ? 6.0 : 10.0) * (nnz - nnz_full) + 12.0 * nnz_full;
+  *alg_bytes = 37.0 * n + (mixed ? 6.0 : 10.0) * (nnz - nnz_full) + 12.0 * nnz_full;
   return VB_OK;
 }
 

@@ -297,12 +297,15 @@ class Solver:
         self._chk(_lib.lib().vb_secondary_properties(self.h, n, _d(vort), C.byref(u_rms)))
         return vort, u_rms.value
 
-    def ppe_operator(self, x, transpose=False):
-        """y = A x or A^T x with A the PPE the last solve assembled (the adjoint's operator)."""
+    def ppe_operator(self, x, transpose=False, solver_rows=False):
+        """y = A x or A^T x with A the PPE the last solve assembled (the adjoint's operator); with
+        solver_rows through the solvers' own SpMV rows (the adjoint solve's A^T rows, the forward FP64
+        solve's compact rows)."""
         n = self.case.npfb
         x = np.ascontiguousarray(x, dtype=np.float64)
         y = np.zeros(n)
-        self._chk(_lib.lib().vb_ppe_operator(self.h, 1 if transpose else 0, n, _d(x), _d(y)))
+        mode = (2 if transpose else 3) if solver_rows else (1 if transpose else 0)
+        self._chk(_lib.lib().vb_ppe_operator(self.h, mode, n, _d(x), _d(y)))
         return y
 
     def grad_operator(self, x, transpose=False):
