@@ -250,6 +250,16 @@ vb_status vb_kernel_functions(int32_t n, const double* d_q, double ad_7, double 
  * row-major device arrays, trace in/out; kick selects the reference's B perturbation */
 vb_status vb_logconf(int32_t n, const double* d_tau, const double* d_trace, const double* d_gradvel,
                      double dt, double wi, double eps_ptt, int32_t kick, double* d_tau_out, double* d_trace_out);
+/* vb_logconf without any device allocation (no counterpart in the reference: for callers that
+ * cannot allocate -- code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments and results as vb_logconf; with kick = 1 the per-particle kick table
+ * is built in a fixed device scratch array of the library (one per GPU, held until the process
+ * ends) instead of a hipMalloc'd buffer, so n is at most 1,048,576 (VB_ERR_CAPACITY above).
+ * Every call on a GPU shares that GPU's scratch: calls that overlap on one GPU (several host
+ * threads, or other streams) race and must be serialised by the caller. */
+vb_status vb_logconf_noalloc(int32_t n, const double* d_tau, const double* d_trace, const double* d_gradvel,
+                             double dt, double wi, double eps_ptt, int32_t kick, double* d_tau_out,
+                             double* d_trace_out);
 /* Adjoint of vb_logconf (no counterpart in the reference, which has no adjoint; one piece of an
  * adjoint step): given the cotangents of its outputs, d_tau_out_bar (n,4) and d_trace_out_bar (n),
  * writes J^T times them for the inputs at the same (tau, trace, grad_vel, Wi, dt): d_tau_bar (n,4),

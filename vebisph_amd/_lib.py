@@ -29,7 +29,7 @@ EXPORTS = [
     "vb_export_ushift", "vb_export_grad_p", "vb_export_ppe", "vb_secondary_properties",
     "vb_ppe_operator", "vb_grad_operator", "vb_ppe_adjoint_solve", "vb_free_surface", "vb_field_maxima", "vb_circle_forces", "vb_nusselt", "vb_profile_spmv", "vb_profile_copy",
     "vb_nnf",
-    "vb_kernel_functions", "vb_logconf", "vb_logconf_adjoint",
+    "vb_kernel_functions", "vb_logconf", "vb_logconf_noalloc", "vb_logconf_adjoint",
 ]
 
 
@@ -104,6 +104,7 @@ def lib():
         L.vb_kernel_functions.argtypes = [C.c_int32, vp, C.c_double, C.c_double, vp, vp]
         L.vb_logconf.argtypes = [C.c_int32, vp, vp, vp, C.c_double, C.c_double, C.c_double,
                                  C.c_int32, vp, vp]
+        L.vb_logconf_noalloc.argtypes = L.vb_logconf.argtypes
         L.vb_logconf_adjoint.argtypes = [C.c_int32, vp, vp, vp, C.c_double, C.c_double, C.c_double,
                                          C.c_int32, vp, vp, vp, vp, vp, vp, vp]
         for name in EXPORTS:

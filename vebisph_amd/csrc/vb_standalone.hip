@@ -1,5 +1,7 @@
 // Stand-alone device kernels of the C ABI: elementwise pieces of the step that callers (and the
 // parity tests) can run on their own device arrays.
+#include <mutex>
+
 #include "vb_ctx.h"
 #include "vb_logconf.h"
 
@@ -96,6 +98,30 @@ __global__ void k_kick_table(int n, double* kick) {
   kick[i] = (1.0e13 * kEps) * (ldexp((double)f, -54) - 0.5);
 }
 
+// Scratch of vb_logconf_noalloc: the per-particle kick table lives in this file-scope device
+// array instead of a hipMalloc'd buffer.  Each GPU holds its own copy until the process ends; host
+// code reaches it only through the address hipGetSymbolAddress returns for the current device.
+constexpr int kScratchMax = 1 << 20;            // particles per call (8 MiB of doubles)
+static __device__ __align__(256) double scratch[kScratchMax];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double* g_scratch_ptr[kMaxDevices];      // device address of `scratch`, per device
+
+// The current device's copy of `scratch`, looked up once per device.
+static vb_status scratch_address(double** out) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return VB_ERR_HIP;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return VB_ERR_HIP;
+    g_scratch_ptr[dev] = static_cast<double*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return VB_OK;
+}
+
 }  // namespace vb
 
 using namespace vb;
@@ -132,6 +158,25 @@ vb_status vb_logconf(int32_t n, const double* d_tau, const double* d_trace, cons
   hipError_t e = hipGetLastError();
   hipError_t e2 = hipStreamSynchronize(0);
   if (dk) (void)hipFree(dk);
+  return (e != hipSuccess || e2 != hipSuccess) ? VB_ERR_HIP : VB_OK;
+}
+
+vb_status vb_logconf_noalloc(int32_t n, const double* d_tau, const double* d_trace, const double* d_gradvel,
+                             double dt, double wi, double eps_ptt, int32_t kick, double* d_tau_out,
+                             double* d_trace_out) {
+  if (n < 0) return VB_ERR_ARG;
+  if (n > kScratchMax) return VB_ERR_CAPACITY;
+  if (n == 0) return VB_OK;
+  double* dk = nullptr;
+  if (kick) {
+    vb_status s = scratch_address(&dk);
+    if (s != VB_OK) return s;
+    hipLaunchKernelGGL(k_kick_table, dim3(grid_1d(n)), dim3(kBlock), 0, 0, n, dk);
+  }
+  hipLaunchKernelGGL(k_logconf_n, dim3(grid_1d(n)), dim3(kBlock), 0, 0, n, d_tau, d_trace, d_gradvel, dt,
+                     wi, eps_ptt, dk, kick, d_tau_out, d_trace_out);
+  hipError_t e = hipGetLastError();
+  hipError_t e2 = hipStreamSynchronize(0);
   return (e != hipSuccess || e2 != hipSuccess) ? VB_ERR_HIP : VB_OK;
 }
 

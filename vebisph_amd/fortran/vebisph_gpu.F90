@@ -348,6 +348,14 @@ module vebisph_gpu
        real(c_double), value :: dt, wi, eps_ptt
        integer(c_int) :: s
      end function vb_logconf
+     function vb_logconf_noalloc(n, d_tau, d_trace, d_gradvel, dt, wi, eps_ptt, kick, d_tau_out, d_trace_out) &
+          bind(C, name="vb_logconf_noalloc") result(s)
+       import :: c_int, c_int32_t, c_double, c_ptr
+       integer(c_int32_t), value :: n, kick
+       type(c_ptr), value :: d_tau, d_trace, d_gradvel, d_tau_out, d_trace_out
+       real(c_double), value :: dt, wi, eps_ptt
+       integer(c_int) :: s
+     end function vb_logconf_noalloc
      function vb_logconf_adjoint(n, d_tau, d_trace, d_gradvel, dt, wi, eps_ptt, kick, d_tau_out_bar, &
           d_trace_out_bar, d_tau_bar, d_trace_bar, d_gradvel_bar, d_wi_bar, d_dt_bar) &
           bind(C, name="vb_logconf_adjoint") result(s)
