@@ -187,6 +187,32 @@ vb_status vb_grad_operator(vb_ctx* ctx, int32_t transpose, int32_t n, const doub
  * of n = npfb; iters and residual may be NULL; returns VB_WARN_* like vb_ppe_solve. */
 vb_status vb_ppe_adjoint_solve(vb_ctx* ctx, int32_t n, const double* g, double* lambda, double tol,
                                int32_t itermax, int32_t* iters, double* residual);
+/* ---- the adjoint of a run of steps (no counterpart in the reference, which has no adjoint) ---- */
+/* Records a forward run for vb_adjoint_backward: nsteps steps of vb_div_free from the current state
+ * (each PPE solved to tol / itermax; <= 0 selects the reference's 1e-8 / 1000), keeping in device
+ * memory the state at the start of every ckpt_every-th step and the pressure of every step, plus the
+ * sweep's work arrays -- all allocated here, none during the steps or the sweep.  The context ends at
+ * the run's final state.  ppe_iters (nsteps) may be NULL.  Closed domains without circles and with
+ * i_shift_coeff = 0 only (VB_ERR_ARG otherwise).  Returns VB_WARN_MAXITER or
+ * VB_WARN_BICGSTAB_BREAKDOWN when a solve ended so. */
+vb_status vb_adjoint_record(vb_ctx* ctx, int32_t nsteps, int32_t ckpt_every, double tol, int32_t itermax,
+                            int32_t* ppe_iters);
+/* The reverse sweep over the recorded run, for a loss L of its final state: given dL/d(final state)
+ * (host arrays of n = npfb, each may be NULL for zero: positions, velocities, temperature, the four
+ * stress components, the conformation trace), returns grads3 = (dL/dWi, dL/dbeta, dL/dRa) with Pr
+ * fixed and El = Wi sqrt(Pr/Ra) (so dL/dRa includes dt's viscous limit, div_free.F90:142), and, when
+ * the *_bar0 arrays are given, dL/d(start state).  Each step is differentiated with its neighbour
+ * lists and mirror set frozen and its PPE solve taken as exact; the adjoint PPE solves A^T lambda = g
+ * with the reference's BiCGSTAB to tol / itermax.  Segments between checkpoints are recomputed from
+ * their checkpoint with the stored pressures.  adj_iters (nsteps) may be NULL.  The context is left at
+ * the recorded final state; the sweep may be run again with other cotangents. */
+vb_status vb_adjoint_backward(vb_ctx* ctx, int32_t n, const double* x_bar, const double* y_bar,
+                              const double* u_bar, const double* v_bar, const double* theta_bar,
+                              const double* tau11_bar, const double* tau12_bar, const double* tau21_bar,
+                              const double* tau22_bar, const double* trace_bar, double tol, int32_t itermax,
+                              double* grads3, double* x_bar0, double* y_bar0, double* u_bar0, double* v_bar0,
+                              double* theta_bar0, double* tau11_bar0, double* tau12_bar0, double* tau21_bar0,
+                              double* tau22_bar0, double* trace_bar0, int32_t* adj_iters);
 /* The columns conc and n_surf of the PART files (output.F90:47-49) for rows 1..n, n <= np: the
  * free-surface flag and, in an open domain, the concentration (calc_part_dist_vars,
  * freesurf_mod.F90:95-100) of the last step (rows past npfb: mirror flags, conc 0; conc is 0 in a

@@ -27,7 +27,8 @@ EXPORTS = [
     "vb_final_advection", "vb_div_free",
     "vb_get_scalars", "vb_export_mirrors", "vb_export_neighbours", "vb_export_weights",
     "vb_export_ushift", "vb_export_grad_p", "vb_export_ppe", "vb_secondary_properties",
-    "vb_ppe_operator", "vb_grad_operator", "vb_ppe_adjoint_solve", "vb_free_surface", "vb_field_maxima", "vb_circle_forces", "vb_nusselt", "vb_profile_spmv", "vb_profile_copy",
+    "vb_ppe_operator", "vb_grad_operator", "vb_ppe_adjoint_solve", "vb_adjoint_record",
+    "vb_adjoint_backward", "vb_free_surface", "vb_field_maxima", "vb_circle_forces", "vb_nusselt", "vb_profile_spmv", "vb_profile_copy",
     "vb_nnf",
     "vb_kernel_functions", "vb_logconf", "vb_logconf_noalloc", "vb_logconf_adjoint",
 ]
@@ -98,6 +99,9 @@ def lib():
         L.vb_export_ushift.argtypes = [vp, C.c_int32, dp]
         L.vb_export_grad_p.argtypes = [vp, C.c_int32, dp]
         L.vb_export_ppe.argtypes = [vp, C.c_int32, C.c_int32, dp, ip, dp, dp]
+        L.vb_adjoint_record.argtypes = [vp, C.c_int32, C.c_int32, C.c_double, C.c_int32, ip]
+        L.vb_adjoint_backward.argtypes = ([vp, C.c_int32] + [dp] * 10 + [C.c_double, C.c_int32, dp]
+                                          + [dp] * 10 + [ip])
         L.vb_profile_spmv.argtypes = [vp, C.c_int32, dp, dp]
         L.vb_profile_copy.argtypes = [vp, C.c_int64, C.c_int32, dp]
         L.vb_nnf.argtypes = [C.c_int32, C.c_int32, vp, C.c_double, vp]

@@ -74,6 +74,7 @@ static void free_all(vb_ctx* c) {
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   free_operator(c);
+  free_adjoint(c);
   if (c->scal_host) (void)hipHostFree(c->scal_host);
   if (c->stream) (void)hipStreamDestroy(c->stream);
 }
@@ -553,17 +554,19 @@ vb_status vb_ppe_adjoint_solve(vb_ctx* c, int32_t n, const double* g, double* la
   if (itermax <= 0) itermax = 1000;
   const size_t bytes = (size_t)n * sizeof(double);
   if (!c->op.g) {
-    VB_HIP(c, hipMalloc((void**)&c->op.g, bytes));
-    VB_HIP(c, hipMalloc((void**)&c->op.lam, bytes));
+    vb_status sa = alloc_operator(c, 0);
+    if (sa != VB_OK) return sa;
   }
   VB_HIP(c, hipMemcpyAsync(c->op.g, g, bytes, hipMemcpyHostToDevice, c->stream));
-  vb_status s = adjoint_solve(c, tol, itermax);
+  int it = 0, st = VB_OK;
+  double res = 0.0;
+  vb_status s = adjoint_solve(c, tol, itermax, &it, &res, &st);
   if (s != VB_OK) return s;
   VB_HIP(c, hipMemcpyAsync(lambda, c->op.lam, bytes, hipMemcpyDeviceToHost, c->stream));
-  if ((s = sync_scalars(c)) != VB_OK) return s;
-  if (iters) *iters = c->scal_host->iter;
-  if (residual) *residual = c->scal_host->resid;
-  return (vb_status)c->scal_host->status;
+  VB_HIP(c, hipStreamSynchronize(c->stream));
+  if (iters) *iters = it;
+  if (residual) *residual = res;
+  return (vb_status)st;
 }
 
 vb_status vb_free_surface(vb_ctx* c, int32_t n, int32_t* n_surf, double* conc) {

@@ -65,6 +65,21 @@ struct State {
   double* dP;
 };
 
+// Linear parts of the affine maps the mirror phase applies, recorded (only during the adjoint's
+// re-run of a step; all null otherwise) for the reverse sweep:
+//   mjac[6m..6m+5]  mirror m (row npfb + m): J, the 2x2 Jacobian (row-major) of its position in its
+//                   parent's position, then d dP_m / d x_parent (2);
+//   msgn[m]         bit 0: its phase-3 velocity is minus the parent's (plus a constant), bit 1: so is
+//                   its temperature, bit 2: the map is not affine (a circle mirror);
+//   ejac[4i..]      particle i: the 2x2 Jacobian of replace_escapees' move (identity if not moved);
+//   eflag[i]        1 moved across a patch, 2 moved by a circle (not affine), 0 not moved.
+struct LinRec {
+  double* mjac;
+  int* msgn;
+  double* ejac;
+  int* eflag;
+};
+
 // Device scalars of one step (one struct in device memory).
 struct Scal {
   double dt, umax, mean, wab1;

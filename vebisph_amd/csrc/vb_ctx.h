@@ -18,11 +18,15 @@ struct OperatorMap {
   double *g = nullptr, *lam = nullptr;   // adjoint solve: right-hand side and solution (npfb)
   double* val4T = nullptr;            // A^T values of the compact rows (layout of vb_ctx::val4)
   int* rslot = nullptr;               // reverse slot: the slot of c in i's list, beside tcol
+  double *rref = nullptr, *bref = nullptr, *dref = nullptr;   // adjoint refinement: residual,
+  double* refs = nullptr;                                       // scaled residual, correction, scalars
   double *ga = nullptr, *gb = nullptr;     // transposed gradient: per-row coefficients (npfb)
   double *gin = nullptr, *gout = nullptr;  // gradient operator staging (2 npfb each)
   bool valid = false;                 // the map matches the stored matrix
   bool step_matrix = false;           // the stored matrix is a step's PPE (not a caller's system)
 };
+
+struct AdjointCtx;                   // the adjoint sweep's record and work arrays (vb_adjoint.hip)
 
 struct vb_ctx {
   vb_params prm;
@@ -34,6 +38,7 @@ struct vb_ctx {
   std::string err;
 
   vb::State st;                      // current particle arrays
+  vb::LinRec rec{};                  // mirror/escape Jacobians: set only during the adjoint's re-run
   double *u_alt, *v_alt, *th_alt;    // prediction writes u*, v*, theta* here; then swapped in
   int* nsurf;
 
@@ -71,6 +76,7 @@ struct vb_ctx {
   double* partials;                  // reduction partials, 2 * kPartialStride per solver stage
   double* red;                       // per-row reduction terms, 2 * npfb per stage (serial mode only)
   OperatorMap op;                    // A and A^T of the assembled PPE (the adjoint's operator)
+  AdjointCtx* adj = nullptr;         // recorded forward run for vb_adjoint_backward
   vb::Scal* scal;                    // device scalars
   vb::Scal* scal_host;               // pinned mirror
   int nmirror_host, np_host;
@@ -89,12 +95,15 @@ vb_status launch_predict(vb_ctx* c);
 vb_status launch_mirror_vel(vb_ctx* c);
 vb_status launch_pressure_update(vb_ctx* c);
 vb_status launch_final(vb_ctx* c);
+vb_status launch_mirror_p(vb_ctx* c);               // mirror pressures p_parent + dP only
+void free_adjoint(vb_ctx* c);                       // (vb_adjoint.hip)
 vb_status launch_secondary_properties(vb_ctx* c);   // vort, Scal::u_rms
 vb_status launch_field_maxima(vb_ctx* c);           // Scal::fmax
 vb_status launch_circle_forces(vb_ctx* c);          // Scal::circ
 vb_status launch_nusselt(vb_ctx* c, double scale);  // Scal::nusselt
 // PPE (vb_ppe.hip)
 vb_status launch_ppe(vb_ctx* c, double tol, int itermax);
+vb_status assemble_ppe(vb_ctx* c);   // right-hand side and matrix only (launch_ppe = this + the solve)
 vb_status profile_spmv(vb_ctx* c, int reps, double* avg_us, double* alg_bytes);
 vb_status solve_nr(vb_ctx* c, const double* A_nr, const int* ija, double* x, const double* B, int n,
                    int itermax, double tol, bool open);
@@ -103,10 +112,12 @@ vb_status apply_ppe_operator(vb_ctx* c, bool transpose, const double* x, double*
 // the same through the solvers' own SpMV rows (forward FP64 compact rows, adjoint A^T rows; vb_ppe.hip)
 vb_status apply_solver_spmv(vb_ctx* c, bool transpose, const double* x, double* y);
 vb_status build_transpose(vb_ctx* c);
+vb_status alloc_operator(vb_ctx* c, size_t bucket_cap);
 vb_status launch_transpose_sliced(vb_ctx* c);
 vb_status apply_grad_operator(vb_ctx* c, bool transpose, const double* x, double* y);
-// A^T lambda = g with the reference's BiCGSTAB on the transposed map (vb_ppe.hip); g, lambda in op
-vb_status adjoint_solve(vb_ctx* c, double tol, int itermax);
+// A^T lambda = g, FP64 iterative refinement around the reference's BiCGSTAB on the transposed map
+// (vb_ppe.hip); g, lambda in op; total inner iterations, |g - A^T lambda| / |g| and a vb_status
+vb_status adjoint_solve(vb_ctx* c, double tol, int itermax, int* iters, double* resid, int* status);
 // v -= mean(v) on the device (vb_operator.hip; uses the reduction partials)
 vb_status remove_mean(vb_ctx* c, double* v, int n);
 void free_operator(vb_ctx* c);

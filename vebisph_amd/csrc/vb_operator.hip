@@ -16,6 +16,7 @@
 //
 // Cost: the reverse-slot search reads ~cnt/2 list entries per slot (~10 ms per build at 1M); it is
 // built once per assembled matrix (on first use) and each application is one SpMV-shaped pass.
+#include <algorithm>
 #include <vector>
 
 #include "vb_ctx.h"
@@ -131,12 +132,13 @@ __global__ void k_op_zero(int* a, int n) {
   if (i < n) a[i] = 0;
 }
 
-// The transposed map of the matrix the last vb_ppe_solve assembled (kept until the next assembly).
-vb_status build_transpose(vb_ctx* c) {
+// The operator's buffers (per context, allocated on first use, or up front by the adjoint sweep,
+// which runs no allocation inside its step loop): the transposed map, its mirror buckets of at least
+// bucket_cap entries (grow-only), the adjoint solve's g and lambda.
+vb_status alloc_operator(vb_ctx* c, size_t bucket_cap) {
   const int npfb = c->npfb;
   OperatorMap& op = c->op;
-  vb_status s0;
-  if (!op.tval) {                                        // per-context, allocated on first use
+  if (!op.tval) {
     const size_t nw = (size_t)npfb * kNplink;
     VB_HIP(c, hipMalloc((void**)&op.tval, nw * sizeof(double)));
     VB_HIP(c, hipMalloc((void**)&op.tcol, nw * sizeof(int)));
@@ -148,6 +150,34 @@ vb_status build_transpose(vb_ctx* c) {
     const size_t nw4 = ((size_t)npfb + 63) / 64 * 64 * ((kNplink + 3) & ~3);     // as vb_ctx::val4
     VB_HIP(c, hipMalloc((void**)&op.val4T, nw4 * sizeof(double)));
   }
+  if (!op.g) {
+    VB_HIP(c, hipMalloc((void**)&op.g, (size_t)npfb * sizeof(double)));
+    VB_HIP(c, hipMalloc((void**)&op.lam, (size_t)npfb * sizeof(double)));
+  }
+  if (!op.rref) {
+    VB_HIP(c, hipMalloc((void**)&op.rref, (size_t)npfb * sizeof(double)));
+    VB_HIP(c, hipMalloc((void**)&op.bref, (size_t)npfb * sizeof(double)));
+    VB_HIP(c, hipMalloc((void**)&op.dref, (size_t)npfb * sizeof(double)));
+    VB_HIP(c, hipMalloc((void**)&op.refs, 4 * sizeof(double)));
+  }
+  if (bucket_cap > op.cap_m || !op.mkey) {
+    if (op.mkey) { VB_HIP(c, hipFree(op.mkey)); VB_HIP(c, hipFree(op.mrow)); VB_HIP(c, hipFree(op.mval)); }
+    op.mkey = op.mrow = nullptr; op.mval = nullptr;
+    const size_t cap = std::max<size_t>(bucket_cap, 64);
+    VB_HIP(c, hipMalloc((void**)&op.mkey, cap * sizeof(int)));
+    VB_HIP(c, hipMalloc((void**)&op.mrow, cap * sizeof(int)));
+    VB_HIP(c, hipMalloc((void**)&op.mval, cap * sizeof(double)));
+    op.cap_m = cap;
+  }
+  return VB_OK;
+}
+
+// The transposed map of the matrix the last vb_ppe_solve assembled (kept until the next assembly).
+vb_status build_transpose(vb_ctx* c) {
+  const int npfb = c->npfb;
+  OperatorMap& op = c->op;
+  vb_status s0;
+  if (!op.tval && (s0 = alloc_operator(c, 0)) != VB_OK) return s0;
   VB_LAUNCH(c, k_op_zero, grid_1d(npfb + 1), op.mcount, npfb + 1);
   VB_LAUNCH(c, k_op_zero, grid_1d(npfb + 1), op.mfill, npfb + 1);
   VB_LAUNCH(c, k_op_zero, 1, op.nmiss, 1);
@@ -170,15 +200,7 @@ vb_status build_transpose(vb_ctx* c) {
     return VB_ERR_CAPACITY;
   }
   const size_t nm = (size_t)hv[0];
-  if (nm > op.cap_m || !op.mkey) {                       // grow-only bucket storage
-    if (op.mkey) { VB_HIP(c, hipFree(op.mkey)); VB_HIP(c, hipFree(op.mrow)); VB_HIP(c, hipFree(op.mval)); }
-    op.mkey = op.mrow = nullptr; op.mval = nullptr;
-    const size_t cap = nm + nm / 4 + 64;
-    VB_HIP(c, hipMalloc((void**)&op.mkey, cap * sizeof(int)));
-    VB_HIP(c, hipMalloc((void**)&op.mrow, cap * sizeof(int)));
-    VB_HIP(c, hipMalloc((void**)&op.mval, cap * sizeof(double)));
-    op.cap_m = cap;
-  }
+  if (nm > op.cap_m && (s = alloc_operator(c, nm + nm / 4 + 64)) != VB_OK) return s;   // grow-only
   if (nx > 0)
     VB_LAUNCH(c, k_transpose_mirror_fill, grid_1d(nx), npfb, c->xrows, nx, c->cnt, c->link, c->st.irel, op.moff,
               op.mfill, op.mkey);
@@ -324,6 +346,7 @@ vb_status remove_mean(vb_ctx* c, double* v, int n) {
 void free_operator(vb_ctx* c) {
   void* ptrs[] = {c->op.tval, c->op.tcol, c->op.mcount, c->op.moff, c->op.mfill, c->op.mkey, c->op.mrow,
                   c->op.mval, c->op.nmiss, c->op.g, c->op.lam, c->op.val4T, c->op.rslot, c->op.ga,
+                  c->op.rref, c->op.bref, c->op.dref, c->op.refs,
                   c->op.gb, c->op.gin, c->op.gout};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);

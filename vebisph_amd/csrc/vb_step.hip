@@ -97,17 +97,52 @@ __device__ inline double pos_along(const Geom& g, double px, double py, int ib, 
   return (py - g.bnode[ib][1] - pd * g.bnorm[ib][1]) / g.bedge[ib][1];
 }
 
+// Gradient of pos_along(x, ib, bound_dist(x, ib)) in x (bound_dist's gradient is bnorm[ib]).
+__device__ inline void grad_pos_along(const Geom& g, int ib, double* d) {
+  const double nx = g.bnorm[ib][0], ny = g.bnorm[ib][1];
+  if (fabs(g.bedge[ib][0]) > fabs(g.bedge[ib][1])) {
+    d[0] = (1.0 - nx * nx) / g.bedge[ib][0]; d[1] = (-nx * ny) / g.bedge[ib][0];
+  } else {
+    d[0] = (-ny * nx) / g.bedge[ib][1]; d[1] = (1.0 - ny * ny) / g.bedge[ib][1];
+  }
+}
+
+// Jacobian of the reflection x - 2 bound_dist(x, ib) n_ib across patch ib's line: I - 2 n n^T.
+__device__ inline void jac_reflect(const Geom& g, int ib, double* J) {
+  const double nx = g.bnorm[ib][0], ny = g.bnorm[ib][1];
+  J[0] = 1.0 - 2.0 * nx * nx; J[1] = -2.0 * nx * ny; J[2] = -2.0 * ny * nx; J[3] = 1.0 - 2.0 * ny * ny;
+}
+
+// Jacobian of the periodic transfer from patch `from` to patch `to`:
+// x' = node_to + e_to - t e_to - pd n_to with t = pos_along(x, from), pd = bound_dist(x, from).
+__device__ inline void jac_periodic(const Geom& g, int from, int to, double* J) {
+  double dt[2];
+  grad_pos_along(g, from, dt);
+  const double nfx = g.bnorm[from][0], nfy = g.bnorm[from][1];
+  J[0] = -g.bedge[to][0] * dt[0] - g.bnorm[to][0] * nfx; J[1] = -g.bedge[to][0] * dt[1] - g.bnorm[to][0] * nfy;
+  J[2] = -g.bedge[to][1] * dt[0] - g.bnorm[to][1] * nfx; J[3] = -g.bedge[to][1] * dt[1] - g.bnorm[to][1] * nfy;
+}
+
+__device__ inline void mat2_mul(const double* A, const double* B, double* C) {
+  const double c0 = A[0] * B[0] + A[1] * B[2], c1 = A[0] * B[1] + A[1] * B[3];
+  const double c2 = A[2] * B[0] + A[3] * B[2], c3 = A[2] * B[1] + A[3] * B[3];
+  C[0] = c0; C[1] = c1; C[2] = c2; C[3] = c3;
+}
+
 // replace_escapees, :336-407.  Per particle the circle and patch tests run in the reference's order.
-__global__ void k_escapees(Geom g, State st, const Scal* sc) {
+// rec.ejac (the adjoint's re-run only): the Jacobian of the particle's move is recorded.
+__global__ void k_escapees(Geom g, State st, const Scal* sc, LinRec rec) {
   int i = g.nbS + blockIdx.x * kBlock + threadIdx.x;
   if (i >= g.npfb) return;
   double x = st.x[i], y = st.y[i];
+  double J[4] = {1.0, 0.0, 0.0, 1.0};
+  int flag = 0;
   for (int ib = 0; ib < g.ncirc; ib++) {
     double R = g.crad[ib];
     double rx = x - g.ccen[ib][0], ry = y - g.ccen[ib][1];
     double rad = sqrt(rx * rx + ry * ry);
-    if (rad < fabs(R) && R > 0.0) { x = x + (rx / rad) * 2.0 * (R - rad); y = y + (ry / rad) * 2.0 * (R - rad); }
-    if (rad > fabs(R) && R < 0.0) { x = x - (rx / rad) * 2.0 * (rad + R); y = y - (ry / rad) * 2.0 * (rad + R); }
+    if (rad < fabs(R) && R > 0.0) { x = x + (rx / rad) * 2.0 * (R - rad); y = y + (ry / rad) * 2.0 * (R - rad); flag = 2; }
+    if (rad > fabs(R) && R < 0.0) { x = x - (rx / rad) * 2.0 * (rad + R); y = y - (ry / rad) * 2.0 * (rad + R); flag = 2; }
   }
   bool moved = false;
   if (!st.inbin[i]) {
@@ -118,17 +153,22 @@ __global__ void k_escapees(Geom g, State st, const Scal* sc) {
       if (pd > 0.0) continue;
       if (fabs(pd) > g.sup + 1.0e-10) continue;
       double tmp = pos_along(g, x, y, ib, pd);
+      double M[4];
       if (t == 1) {
         if (tmp < 0.0 || tmp > 1.0) continue;
         x = x - 2.0 * pd * g.bnorm[ib][0];
         y = y - 2.0 * pd * g.bnorm[ib][1];
+        jac_reflect(g, ib, M);
       } else {
         int ibp = g.bpp[ib];
         double tx = -1.0 * tmp * g.bedge[ibp][0] - pd * g.bnorm[ibp][0];
         double ty = -1.0 * tmp * g.bedge[ibp][1] - pd * g.bnorm[ibp][1];
         x = g.bnode[ibp][0] + g.bedge[ibp][0] + tx;
         y = g.bnode[ibp][1] + g.bedge[ibp][1] + ty;
+        jac_periodic(g, ib, ibp, M);
       }
+      mat2_mul(M, J, J);
+      if (!flag) flag = 1;
       double dt = sc->dt;
       st.x0[i] = x - st.u[i] * dt; st.y0[i] = y - st.v[i] * dt;
       st.u0[i] = st.u[i]; st.v0[i] = st.v[i];
@@ -136,6 +176,10 @@ __global__ void k_escapees(Geom g, State st, const Scal* sc) {
     }
   }
   if (g.ncirc || moved) { st.x[i] = x; st.y[i] = y; }
+  if (rec.ejac) {
+    rec.ejac[4 * i] = J[0]; rec.ejac[4 * i + 1] = J[1]; rec.ejac[4 * i + 2] = J[2]; rec.ejac[4 * i + 3] = J[3];
+    rec.eflag[i] = flag;
+  }
 }
 
 // A mirror segment is one reference loop that appends mirrors in candidate order.  The append order
@@ -346,7 +390,45 @@ __global__ void k_seg_scan(int* counts, int nb, Scal* sc, int mcap) {
   }
 }
 
-__global__ void k_seg_emit(Geom g, Seg s, State st, const Scal* sc, const int* offsets, int mcap) {
+// The linear part of mirror m's maps in its parent's values (see LinRec), for the segment that
+// emitted it from candidate c: J of the position, d dP / d x_parent = (J - I)^T f where
+// dP = f . (x_m - x_parent) (a mirror of a mirror adds to the intermediate mirror's), and the signs
+// of the velocity and temperature maps.
+__device__ void seg_record(const Geom& g, const Seg& s, int c, LinRec rec, int m) {
+  double J[4] = {1.0, 0.0, 0.0, 1.0}, dp[2] = {0.0, 0.0};
+  int sg = 0;
+  const double fx = g.bf[0], fy = g.bf[1];
+  switch (s.kind) {
+    case SEG_CIRCLE: sg = 4; break;
+    case SEG_WALL:
+      jac_reflect(g, s.ib, J);
+      dp[0] = (J[0] - 1.0) * fx + J[2] * fy; dp[1] = J[1] * fx + (J[3] - 1.0) * fy;
+      sg = 3;
+      break;
+    case SEG_PERIODIC: jac_periodic(g, s.ibp, s.ib, J); break;
+    case SEG_CORNER_WW:
+      J[0] = -1.0; J[3] = -1.0;
+      dp[0] = -2.0 * fx; dp[1] = -2.0 * fy;
+      sg = 1;
+      break;
+    case SEG_CORNER_PP: break;
+    case SEG_CORNER_MIX: {
+      const double* Jj = rec.mjac + 6 * (size_t)c;       // the intermediate mirror npfb + c
+      double R[4];
+      jac_reflect(g, s.ibp, R);
+      mat2_mul(R, Jj, J);
+      dp[0] = Jj[4] + (J[0] - Jj[0]) * fx + (J[2] - Jj[2]) * fy;
+      dp[1] = Jj[5] + (J[1] - Jj[1]) * fx + (J[3] - Jj[3]) * fy;
+      sg = rec.msgn[c] ^ 3;
+      break;
+    }
+  }
+  double* o = rec.mjac + 6 * (size_t)m;
+  o[0] = J[0]; o[1] = J[1]; o[2] = J[2]; o[3] = J[3]; o[4] = dp[0]; o[5] = dp[1];
+  rec.msgn[m] = sg;
+}
+
+__global__ void k_seg_emit(Geom g, Seg s, State st, const Scal* sc, const int* offsets, int mcap, LinRec rec) {
   int c = blockIdx.x * kBlock + threadIdx.x;
   int base = sc->seg_base;
   int mlimit = s.kind == SEG_CORNER_MIX ? base : 0;
@@ -360,6 +442,7 @@ __global__ void k_seg_emit(Geom g, Seg s, State st, const Scal* sc, const int* o
       int k = g.npfb + m;
       st.irel[k] = r.irel; st.vrel[k] = r.vrel; st.x[k] = r.x; st.y[k] = r.y;
       st.u[k] = r.u; st.v[k] = r.v; st.th[k] = r.th; st.dP[k] = r.dP;
+      if (rec.mjac) seg_record(g, s, c, rec, m);
     }
   }
 }
@@ -1050,7 +1133,7 @@ static vb_status run_segment(vb_ctx* c, const Seg& s) {
   VB_LAUNCH(c, k_seg_count, nb, c->g, s, c->st, c->scal, c->seg_counts);
   hipLaunchKernelGGL(k_seg_scan, dim3(1), dim3(1024), 0, c->stream, c->seg_counts, nb, c->scal, c->mcap);
   VB_HIP(c, hipGetLastError());
-  VB_LAUNCH(c, k_seg_emit, nb, c->g, s, c->st, c->scal, c->seg_counts, c->mcap);
+  VB_LAUNCH(c, k_seg_emit, nb, c->g, s, c->st, c->scal, c->seg_counts, c->mcap, c->rec);
   return VB_OK;
 }
 
@@ -1064,7 +1147,7 @@ static vb_status read_nmirror(vb_ctx* c) {
 vb_status launch_mirrors(vb_ctx* c) {
   const Geom& g = c->g;
   int nf = c->npfb - c->nbS;
-  if (nf > 0) VB_LAUNCH(c, k_escapees, grid_1d(nf), c->g, c->st, c->scal);
+  if (nf > 0) VB_LAUNCH(c, k_escapees, grid_1d(nf), c->g, c->st, c->scal, c->rec);
   vb_status s;
   if (g.ncirc) {
     Seg sg{SEG_CIRCLE, 0, 0, nf * g.ncirc, 0.0};
@@ -1197,6 +1280,11 @@ vb_status launch_pressure_update(vb_ctx* c) {
   if (c->nmirror_host > 0) VB_LAUNCH(c, k_mirror_p, grid_1d(c->nmirror_host), c->g, c->st, c->scal);
   VB_LAUNCH(c, k_pressure_update, grid_1d(c->npfb), c->g, c->st, c->cnt, c->link, c->wg1, c->wg2, c->k11,
             c->k12, c->k21, c->k22, c->gpx, c->gpy, c->scal);
+  return VB_OK;
+}
+
+vb_status launch_mirror_p(vb_ctx* c) {
+  if (c->nmirror_host > 0) VB_LAUNCH(c, k_mirror_p, grid_1d(c->nmirror_host), c->g, c->st, c->scal);
   return VB_OK;
 }
 

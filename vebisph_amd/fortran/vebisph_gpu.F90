@@ -348,6 +348,27 @@ module vebisph_gpu
        real(c_double), value :: dt, wi, eps_ptt
        integer(c_int) :: s
      end function vb_logconf
+     function vb_adjoint_record(ctx, nsteps, ckpt_every, tol, itermax, ppe_iters) &
+          bind(C, name="vb_adjoint_record") result(s)
+       import :: c_int, c_int32_t, c_double, c_ptr
+       type(c_ptr), value :: ctx, ppe_iters
+       integer(c_int32_t), value :: nsteps, ckpt_every, itermax
+       real(c_double), value :: tol
+       integer(c_int) :: s
+     end function vb_adjoint_record
+     function vb_adjoint_backward(ctx, n, x_bar, y_bar, u_bar, v_bar, theta_bar, tau11_bar, tau12_bar, &
+          tau21_bar, tau22_bar, trace_bar, tol, itermax, grads3, x_bar0, y_bar0, u_bar0, v_bar0, theta_bar0, &
+          tau11_bar0, tau12_bar0, tau21_bar0, tau22_bar0, trace_bar0, adj_iters) &
+          bind(C, name="vb_adjoint_backward") result(s)
+       import :: c_int, c_int32_t, c_double, c_ptr
+       type(c_ptr), value :: ctx, x_bar, y_bar, u_bar, v_bar, theta_bar, tau11_bar, tau12_bar, tau21_bar, &
+            tau22_bar, trace_bar, x_bar0, y_bar0, u_bar0, v_bar0, theta_bar0, tau11_bar0, tau12_bar0, &
+            tau21_bar0, tau22_bar0, trace_bar0, adj_iters
+       integer(c_int32_t), value :: n, itermax
+       real(c_double), value :: tol
+       real(c_double), intent(out) :: grads3(3)
+       integer(c_int) :: s
+     end function vb_adjoint_backward
      function vb_logconf_noalloc(n, d_tau, d_trace, d_gradvel, dt, wi, eps_ptt, kick, d_tau_out, d_trace_out) &
           bind(C, name="vb_logconf_noalloc") result(s)
        import :: c_int, c_int32_t, c_double, c_ptr

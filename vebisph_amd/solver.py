@@ -332,6 +332,34 @@ class Solver:
                                                       C.byref(it), C.byref(res)))
         return lam, it.value, res.value, st
 
+    # ---- the adjoint of a run of steps (no counterpart in the reference) ----
+    ADJ_FIELDS = ("x", "y", "u", "v", "theta", "tau11", "tau12", "tau21", "tau22", "trace")
+
+    def adjoint_record(self, nsteps, ckpt_every, tol=1e-8, itermax=1000):
+        """nsteps steps from the current state, recorded for adjoint_backward (the state every
+        ckpt_every steps and every step's pressure kept on the device).  Returns the PPE iteration
+        counts and the status (VB_WARN_* when a solve stopped at itermax)."""
+        it = np.zeros(nsteps, np.int32)
+        st = self._chk(_lib.lib().vb_adjoint_record(self.h, nsteps, ckpt_every, C.c_double(tol), itermax,
+                                                    _i(it)))
+        self._adj_nsteps = nsteps
+        return it, st
+
+    def adjoint_backward(self, bars, tol=1e-8, itermax=1000, initial=False):
+        """The reverse sweep for dL/d(final state) = bars (a dict over ADJ_FIELDS, missing = 0):
+        dict(grads = [dL/dWi, dL/dbeta, dL/dRa], initial = dL/d(start state) when asked for,
+        adj_iters = adjoint PPE iterations per step, status)."""
+        n = self.case.npfb
+        ins = [np.ascontiguousarray(bars[k], dtype=np.float64) if bars.get(k) is not None else None
+               for k in self.ADJ_FIELDS]
+        outs = [np.zeros(n) for _ in self.ADJ_FIELDS] if initial else [None] * len(self.ADJ_FIELDS)
+        g = np.zeros(3)
+        it = np.zeros(self._adj_nsteps, np.int32)
+        st = self._chk(_lib.lib().vb_adjoint_backward(
+            self.h, n, *[_d(a) for a in ins], C.c_double(tol), itermax, _d(g), *[_d(a) for a in outs], _i(it)))
+        return {"grads": g, "initial": dict(zip(self.ADJ_FIELDS, outs)) if initial else None,
+                "adj_iters": it, "status": st}
+
     def free_surface(self, n=None):
         """n_surf and conc of rows 1..n (the PART file columns, output.F90:47-49) of the last step."""
         n = self.case.npfb if n is None else n
