@@ -199,6 +199,52 @@ def adjoint_cost(s, case):
             "scope": "the PPE solve of one step and its adjoint solve"}
 
 
+def adjoint_run_cost(workload, nsteps, every, precision="fp64"):
+    """The adjoint's cost at the adjoint Poiseuille inversion's size: a plain forward run of nsteps
+    steps (vb_div_free), the same run recorded for the adjoint (vb_adjoint_record: checkpoints every
+    `every` steps, every step's pressure kept), and the reverse sweep (vb_adjoint_backward) for the
+    velocity-mismatch loss 1/2 |u - u_target|^2 with u_target = 0.9 u_final.  The start is the fluid at
+    rest with a pre-stretched conformation (vebisph_amd.adjoint.prestretched_state).  Reports the
+    sweep's time over the plain forward's ("ratio"), and record + sweep over the plain forward."""
+    import numpy as np
+    from vebisph_amd import _lib, adjoint, cases, solver
+    case = cases.make(workload)
+    prec = _lib.PREC_MIXED if precision == "mixed" else _lib.PREC_FP64
+    start = adjoint.prestretched_state(case)
+    s = solver.Solver(case, precision=prec, state=start)
+    s.scalars()
+    t = time.perf_counter()
+    for _ in range(nsteps):
+        s.div_free()
+    s.scalars()
+    t_fwd = time.perf_counter() - t
+    s.close()
+    s = solver.Solver(case, precision=prec, state=start)
+    t = time.perf_counter()
+    fit, fst = s.adjoint_record(nsteps, every)
+    t_rec = time.perf_counter() - t
+    fin = s.get_state()
+    nb = case.nbS
+    bars = {"u": np.zeros(case.npfb), "v": np.zeros(case.npfb)}
+    bars["u"][nb:] = 0.1 * fin["u"][nb:]
+    bars["v"][nb:] = 0.1 * fin["v"][nb:]
+    t = time.perf_counter()
+    res = s.adjoint_backward(bars)
+    t_bwd = time.perf_counter() - t
+    s.close()
+    out = {"workload": case.name, "particles": case.npfb, "steps": nsteps, "ckpt_every": every,
+           "precision": precision, "forward_s": t_fwd, "record_s": t_rec, "backward_s": t_bwd,
+           "ratio": t_bwd / t_fwd, "ratio_with_record": (t_rec + t_bwd) / t_fwd, "target_ratio": 4.0,
+           "met": t_bwd / t_fwd <= 4.0,
+           "forward_ppe_iters_mean": float(np.mean(fit)), "adjoint_ppe_iters_mean": float(np.mean(res["adj_iters"])),
+           "adjoint_status": res["status"], "record_status": fst,
+           "grads_Wi_beta_Ra": [float(g) for g in res["grads"]]}
+    log(f"adjoint at {case.name}: forward {t_fwd:.2f} s, record {t_rec:.2f} s, backward {t_bwd:.2f} s over "
+        f"{nsteps} steps: backward/forward {out['ratio']:.2f}x (with the record {out['ratio_with_record']:.2f}x); "
+        f"PPE iterations forward {out['forward_ppe_iters_mean']:.0f}, adjoint {out['adjoint_ppe_iters_mean']:.0f}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -208,6 +254,10 @@ def main():
     ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
     ap.add_argument("--no-compare", action="store_true", help="skip the run at the other precision")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--adjoint-workload", default="poiseuille_250k",
+                    help="the adjoint cost's workload ('' skips it)")
+    ap.add_argument("--adjoint-steps", type=int, default=200)
+    ap.add_argument("--adjoint-every", type=int, default=20)
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: the step has no multi-GPU path in this library (the box "
@@ -234,6 +284,10 @@ def main():
                  "ls_iters_last_step": ores["ls_iters_last_step"],
                  "spmv_us": ores["spmv_us"], "spmv_GBs": ores["spmv_GBs"], "force_on_cylinder": ores["force"],
                  "adjoint_ppe": ores["adjoint_ppe"]}
+
+    adj_run = None
+    if a.adjoint_workload:
+        adj_run = adjoint_run_cost(a.adjoint_workload, a.adjoint_steps, a.adjoint_every)
 
     cpu = None
     if not a.no_cpu_baseline:
@@ -273,6 +327,8 @@ def main():
         "ratio_vs_other_precision": (main_res["value"] / other["value"]) if other else None,
         # adjoint/forward cost of the PPE solve at this precision's forward solve
         "adjoint_ppe": main_res["adjoint_ppe"],
+        # adjoint/forward cost of a whole run at the adjoint inversion's size (checkpointed sweep)
+        "adjoint": adj_run,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

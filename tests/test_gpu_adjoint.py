@@ -148,23 +148,10 @@ def _reduced_c4():
 
 
 def prestretched(case, amp=1e-2):
-    """The fluid at rest with a smooth, non-isotropic conformation (tau11 - tau22 >= amp / 2,
-    tau12 = tau21 of amplitude amp).  From the isotropic A = I the reference's log-conformation map
-    divides its rotation rate by the 1e-10 regularisation (predictor_mod.F90:203-207), so the ulp-level
-    rounding of A11 - A22 -- which moves with Wi through exp(-dt/Wi) -- reaches the shear stress ~1e9
-    times amplified: central differences of a run from A = I change with their step (measured on this
-    case, dL/dWi: 6.9e-8, -1.0e-4, 1.7e-4, 5.4e-5 at eps = 1e-4 .. 1e-7), i.e. that map has no
-    derivative a finite difference resolves.  A pre-stretched start keeps the eigenvalues apart."""
-    a = _state_arrays(solver.initial_state(case))
-    L = case.b_node[:, 0].max() - case.b_node[:, 0].min()
-    H = case.b_node[:, 1].max() - case.b_node[:, 1].min()
-    x, y = a["x"], a["y"]
-    a["tau11"] = 2.0 * amp + amp * np.sin(2 * np.pi * x / L) * np.cos(np.pi * y / H)
-    a["tau22"] = 0.5 * amp + 0.2 * amp * np.cos(2 * np.pi * x / L)
-    t12 = amp * np.cos(2 * np.pi * x / L) * np.sin(np.pi * y / H)
-    a["tau12"] = t12; a["tau21"] = t12.copy()
-    a["trace"] = 2.0 + a["tau11"] + a["tau22"]
-    return _to_state(a, case)
+    """The inversions' start state (vebisph_amd.adjoint.prestretched_state): measured on the reduced
+    Poiseuille case from A = I instead, central differences of dL/dWi change with their step (6.9e-8,
+    -1.0e-4, 1.7e-4, 5.4e-5 at eps = 1e-4 .. 1e-7) -- no derivative to compare with."""
+    return adjoint.prestretched_state(case, amp)
 
 
 def test_adjoint_gradients_reduced_poiseuille_inversion(gpu_lib):

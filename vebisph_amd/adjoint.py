@@ -32,6 +32,25 @@ def case_with(case, wi, beta, ra):
     return c
 
 
+def prestretched_state(case, amp=1e-2):
+    """The start state of the inversions: the fluid at rest with a smooth, non-isotropic conformation
+    (tau11 - tau22 >= amp / 2, tau12 = tau21 of amplitude amp).  From the isotropic A = I the reference's
+    log-conformation map divides its rotation rate by the 1e-10 regularisation
+    (predictor_mod.F90:203-207), so the ulp-level rounding of A11 - A22 -- which moves with Wi through
+    exp(-dt/Wi) -- reaches the shear stress amplified ~1e9 times, and the run has no derivative in Wi
+    that a finite difference resolves; a pre-stretched start keeps the eigenvalues apart."""
+    st = solver.initial_state(case)
+    L = float(case.b_node[:, 0].max() - case.b_node[:, 0].min())
+    H = float(case.b_node[:, 1].max() - case.b_node[:, 1].min())
+    x, y = np.asarray(case.x), np.asarray(case.y)
+    t11 = 2.0 * amp + amp * np.sin(2 * np.pi * x / L) * np.cos(np.pi * y / H)
+    t22 = 0.5 * amp + 0.2 * amp * np.cos(2 * np.pi * x / L)
+    t12 = amp * np.cos(2 * np.pi * x / L) * np.sin(np.pi * y / H)
+    st["tau"] = np.stack([t11, t12, t12.copy(), t22], axis=1)
+    st["trace"] = 2.0 + t11 + t22
+    return st
+
+
 def run_forward(case, wi, beta, ra, nsteps, tol=1e-8, itermax=1000, precision=_lib.PREC_FP64, state=None):
     """The final state (dict) of nsteps plain steps at the given parameters (no record)."""
     s = solver.Solver(case_with(case, wi, beta, ra), precision=precision, state=state)

@@ -1104,12 +1104,12 @@ vb_status solve_nr(vb_ctx* c, const double* A_nr, const int* ija, double* x, con
 // same), so a single solve cannot meet 1e-8.  Each round here solves A^T d = r / |r| to 1e-6 (the
 // unit scale also keeps the reference's first-iteration threshold from turning the criterion
 // absolute), updates lambda += |r| d, and recomputes r = g - A^T lambda in FP64; two or three rounds
-// reach 1e-12.  Closed domains remove the means of g and of each r first (the adjoint of the
+// reach 1e-12.  The last round's inner tolerance is only what the requested one still needs.  Closed domains remove the means of g and of each r first (the adjoint of the
 // forward's DC removal of its right-hand side).  Stops when |r| <= tol |g| or after itermax inner
 // iterations in total.
 __global__ void k_fill_zero(double* a, int n);
 constexpr double kRefineInnerTol = 1.0e-6;
-constexpr int kRefineRounds = 8;
+constexpr int kRefineRounds = 12;
 
 // r = g - A^T lambda (lambda = 0: r = g), partial sums of r and r^2
 __global__ void k_ref_resid(Mat A, const double* g, const double* lam, int first, double* r, double* partials) {
@@ -1174,8 +1174,11 @@ vb_status adjoint_solve(vb_ctx* c, double tol, int itermax, int* iters, double* 
       break;
     }
     if (total >= itermax) { st = VB_WARN_MAXITER; break; }
+    // a round solves to 1e-6, or only as far as the requested tolerance still needs (with a factor
+    // 3 of margin, and at least a tenfold reduction) once the residual is close to it
+    const double inner = std::min(0.1, std::max(kRefineInnerTol, 0.3 * tol * gnorm / rnorm));
     VB_LAUNCH(c, k_ref_scale, grid_1d(n), n, op.rref, op.refs, op.bref);
-    if ((s = solve<SPMV_TRANSPOSE>(c, A, w, 0, kRefineInnerTol, itermax - total)) != VB_OK) return s;
+    if ((s = solve<SPMV_TRANSPOSE>(c, A, w, 0, inner, itermax - total)) != VB_OK) return s;
     total += std::min(c->scal_host->iter, itermax - total);
     if (c->scal_host->status == VB_WARN_BICGSTAB_BREAKDOWN) st = VB_WARN_BICGSTAB_BREAKDOWN;
     VB_LAUNCH(c, k_ref_update, grid_1d(n), n, op.lam, op.dref, op.refs, round == 0 ? 1 : 0);
