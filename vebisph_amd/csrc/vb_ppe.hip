@@ -1104,8 +1104,9 @@ vb_status solve_nr(vb_ctx* c, const double* A_nr, const int* ija, double* x, con
 // same), so a single solve cannot meet 1e-8.  Each round here solves A^T d = r / |r| to 1e-6 (the
 // unit scale also keeps the reference's first-iteration threshold from turning the criterion
 // absolute), updates lambda += |r| d, and recomputes r = g - A^T lambda in FP64; two or three rounds
-// reach 1e-12.  The last round's inner tolerance is only what the requested one still needs.  Closed domains remove the means of g and of each r first (the adjoint of the
-// forward's DC removal of its right-hand side).  Stops when |r| <= tol |g| or after itermax inner
+// reach 1e-12.  A round's inner tolerance is only what the requested one still needs (0.3 tol |g| /
+// |r|, within [1e-6, 0.1]).  Closed domains remove the means of g and of each r first (the adjoint of
+// the forward's DC removal of its right-hand side).  Stops when |r| <= tol |g| or after itermax inner
 // iterations in total.
 __global__ void k_fill_zero(double* a, int n);
 constexpr double kRefineInnerTol = 1.0e-6;
