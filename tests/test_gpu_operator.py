@@ -118,7 +118,8 @@ def test_ppe_adjoint_solve(gpu_lib, name):
     print(f"{case.name}: adjoint BiCGSTAB {it} iterations, residual {res:.3e}, status {st}", flush=True)
     assert st == 0 and res <= 1e-8 and 1 <= it <= 1000
     r = g - A.T @ lam
-    assert np.linalg.norm(r) <= 1e-6 * np.linalg.norm(g)
+    print(f"  scipy's residual {np.linalg.norm(r) / np.linalg.norm(g):.3e}", flush=True)
+    assert np.linalg.norm(r) <= 1e-7 * np.linalg.norm(g)
     if case.i_open_domain:
         ref = spla.spsolve(A.T.tocsc(), g)
         err = np.linalg.norm(lam - ref) / np.linalg.norm(ref)
@@ -128,8 +129,13 @@ def test_ppe_adjoint_solve(gpu_lib, name):
         lam2, it2, res2, st2 = G.ppe_adjoint_solve(g + 0.37)
         print(f"  g + 0.37: {it2} iterations, residual {res2:.3e}, status {st2}", flush=True)
         assert st2 == 0 and res2 <= 1e-8
+        # the reported residual is that of the mean-free system, not the constant's rounding: the
+        # refinement's norm of r - mean(r) is taken in a second pass (sum(r^2) - n mean^2 cancels
+        # here, where the constant dominates)
         r2 = g - A.T @ lam2
-        assert np.linalg.norm(r2) <= 1e-6 * np.linalg.norm(g)
+        print(f"  scipy's residual {np.linalg.norm(r2) / np.linalg.norm(g):.3e}", flush=True)
+        assert res2 > 0.0
+        assert np.linalg.norm(r2) <= 1e-7 * np.linalg.norm(g)
 
 
 def _grad_matrix(G, n):

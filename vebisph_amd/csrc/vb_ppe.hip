@@ -1112,30 +1112,42 @@ __global__ void k_fill_zero(double* a, int n);
 constexpr double kRefineInnerTol = 1.0e-6;
 constexpr int kRefineRounds = 12;
 
-// r = g - A^T lambda (lambda = 0: r = g), partial sums of r and r^2
+// r = g - A^T lambda (lambda = 0: r = g), partial sums of r
 __global__ void k_ref_resid(Mat A, const double* g, const double* lam, int first, double* r, double* partials) {
   const int n = A.n;
   RowRange rr = block_rows(n, gridDim.x);
-  Red<2> R(nullptr, n);
+  Red<1> R(nullptr, n);
   for (int i = rr.begin + threadIdx.x; i < rr.end; i += blockDim.x) {
     const double v = first ? g[i] : g[i] - spmv_row_transpose(A, i, lam);
     r[i] = v;
     R.add(0, i, v);
-    R.add(1, i, v * v);
   }
   R.store(partials);
 }
 
-// out[0] = mean(r) (0 when open), out[1] = |r - mean|
-__global__ void k_ref_fin(const double* partials, int nblk, int n, int open, double* out) {
-  double f[2];
-  fold_n<2>(partials, nblk, nullptr, n, f);
-  if (threadIdx.x == 0) {
-    const double m = open ? 0.0 : f[0] / (double)n;
-    const double q = f[1] - (double)n * m * m;
-    out[0] = m;
-    out[1] = sqrt(q > 0.0 ? q : 0.0);
+// out[0] = mean(r) (0 when open)
+__global__ void k_ref_mean(const double* partials, int nblk, int n, int open, double* out) {
+  const double f = fold(partials, 0, nblk, nullptr, n);
+  if (threadIdx.x == 0) out[0] = open ? 0.0 : f / (double)n;
+}
+
+// partial sums of (r - mean)^2: a second pass, since sum(r^2) - n mean^2 cancels when r's constant
+// part dominates (a right-hand side with a constant added)
+__global__ void k_ref_dev(int n, const double* r, const double* sc, double* partials) {
+  RowRange rr = block_rows(n, gridDim.x);
+  const double m = sc[0];
+  Red<1> R(nullptr, n);
+  for (int i = rr.begin + threadIdx.x; i < rr.end; i += blockDim.x) {
+    const double d = r[i] - m;
+    R.add(0, i, d * d);
   }
+  R.store(partials);
+}
+
+// out[1] = |r - mean|
+__global__ void k_ref_fin(const double* partials, int nblk, int n, double* out) {
+  const double q = fold(partials, 0, nblk, nullptr, n);
+  if (threadIdx.x == 0) out[1] = sqrt(q);
 }
 
 __global__ void k_ref_scale(int n, const double* r, const double* sc, double* b) {
@@ -1164,8 +1176,9 @@ vb_status adjoint_solve(vb_ctx* c, double tol, int itermax, int* iters, double* 
   int total = 0, st = VB_OK;
   for (int round = 0; round < kRefineRounds; round++) {
     VB_LAUNCH(c, k_ref_resid, nb, A, op.g, op.lam, round == 0 ? 1 : 0, op.rref, c->partials);
-    hipLaunchKernelGGL(k_ref_fin, dim3(1), dim3(kBlock), 0, c->stream, c->partials, nb, n, open, op.refs);
-    VB_HIP(c, hipGetLastError());
+    VB_FIN(c, k_ref_mean, c->partials, nb, n, open, op.refs);
+    VB_LAUNCH(c, k_ref_dev, nb, n, op.rref, op.refs, c->partials);
+    VB_FIN(c, k_ref_fin, c->partials, nb, n, op.refs);
     VB_HIP(c, hipMemcpyAsync(h, op.refs, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     VB_HIP(c, hipStreamSynchronize(c->stream));
     rnorm = h[1];
