@@ -180,11 +180,15 @@ vb_status vb_ppe_operator(vb_ctx* ctx, int32_t transpose, int32_t n, const doubl
  * the step's vb_ppe_solve until the next vb_particle_allocation. */
 vb_status vb_grad_operator(vb_ctx* ctx, int32_t transpose, int32_t n, const double* x, double* y);
 /* The adjoint PPE solve A^T lambda = g (no counterpart in the reference), A as for vb_ppe_operator:
- * the reference's own BiCGSTAB with A^T as its matrix -- SPARSE_PBICGSTAB_nonull in a closed domain,
- * where the mean of g is removed first (A^T maps into the mean-zero vectors; this projection is the
- * adjoint of the forward solve's removal of the mean of its right-hand side), SPARSE_PBICGSTAB in an
- * open one -- from lambda = 0, tol <= 0 / itermax <= 0 selecting 1e-8 / 1000.  g, lambda host arrays
- * of n = npfb; iters and residual may be NULL; returns VB_WARN_* like vb_ppe_solve. */
+ * FP64 iterative refinement around the reference's own BiCGSTAB with A^T as its matrix --
+ * SPARSE_PBICGSTAB_nonull in a closed domain, where the mean of g and of each refinement residual is
+ * removed (A^T maps into the mean-zero vectors; this projection is the adjoint of the forward solve's
+ * removal of the mean of its right-hand side), SPARSE_PBICGSTAB in an open one.  Each round solves
+ * A^T d = r / |r| and adds |r| d to lambda (from lambda = 0), then recomputes r = g - A^T lambda in
+ * FP64, until |r| <= tol |g| (BiCGSTAB alone stalls near 3e-9 on A^T, which is not symmetric in
+ * wall-bounded domains); tol <= 0 / itermax <= 0 select 1e-8 / 1000, itermax bounding the inner
+ * iterations in total.  g, lambda host arrays of n = npfb; iters (inner iterations) and residual
+ * (|r| / |g| at the end) may be NULL; returns VB_WARN_* like vb_ppe_solve. */
 vb_status vb_ppe_adjoint_solve(vb_ctx* ctx, int32_t n, const double* g, double* lambda, double tol,
                                int32_t itermax, int32_t* iters, double* residual);
 /* ---- the adjoint of a run of steps (no counterpart in the reference, which has no adjoint) ---- */
