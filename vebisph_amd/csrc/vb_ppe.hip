@@ -16,10 +16,14 @@
 // divisions and square root, ~1.8x slower.)
 //
 // Mixed precision: the same BiCGSTAB with FP64 vectors, dot products, scalars and solution update,
-// and FP32 matvecs on the compact rows: FP32 values applied to FP32 differences of the FP64 vector,
-// y_i = sum_k a32_k (x_j - x_i) + rowsum_i x_i (the FP64 row sum carries the diagonal), so smooth
-// vectors do not cancel catastrophically; every kRefresh iterations the recurrence residual is
-// replaced by the FP64 residual B - A x.
+// and matvecs that read FP32 values on the compact rows: y_i = sum_k a32_k (x_j - x_i) +
+// rowsum_i x_i, the differences and the sum in FP64 (the FP64 row sum carries the diagonal), so the
+// only FP32 rounding is the values', a fixed perturbation of A.  The recurrence residual then drifts
+// from B - A x by that perturbation times the iterate, so a convergence claim of the recurrence is
+// checked against the FP64 residual: it replaces the recurrence residual, and the solve stops only if
+// that meets the tolerance.  (Replacing it periodically instead, every 10 to 200 iterations, was
+// measured: at 1M particles, where the solve runs to its iteration limit, the final FP64 residual was
+// 3-13x worse than with no periodic replacement, which tracks the FP64 solve.)
 //
 // BiCGSTAB: all scalars (rho, alpha, omega, norms, means) stay on the device; the host only polls the
 // convergence flag every few iterations.  See the section comment below for the kernel sequence.
@@ -30,7 +34,7 @@
 namespace vb {
 
 constexpr double kOnePlusEps = 1.0 + 1.0e-15;   // M(i) + 1e-15 with M = 1, linear_solver.F90:55,318
-constexpr int kRefresh = 50;                     // mixed path: FP64 residual replacement period
+constexpr int kVerifyEvery = 8;                  // mixed path: iterations between convergence checks
 
 // SPMV_TRANSPOSE: y = A^T x through the transposed map (vb_operator.hip), for the adjoint solve
 enum SpmvMode { SPMV_FP64 = 0, SPMV_MIXED = 1, SPMV_TRANSPOSE = 2 };
@@ -402,21 +406,21 @@ __device__ inline double row_f32(const Mat& A, int i, int m, const double* __res
   if (ng > 1) { o1 = ld_nt(A.lnk4 + gs(1)); a1 = ld_nt(A.val4f + gs(1)); }
   double g0[4], g1[4] = {0.0, 0.0, 0.0, 0.0};
   g0[0] = v[i + o0.x]; g0[1] = v[i + o0.y]; g0[2] = v[i + o0.z]; g0[3] = v[i + o0.w];
-  float acc = 0.0f;
+  double acc = 0.0;
   for (int gi = 0; gi < ng; gi++) {
     if (gi + 2 < ng) {
       const size_t g2 = (size_t)(gi + 2);
       o2 = ld_nt(A.lnk4 + gs(g2)); a2 = ld_nt(A.val4f + gs(g2));
     }
     if (gi + 1 < ng) { g1[0] = v[i + o1.x]; g1[1] = v[i + o1.y]; g1[2] = v[i + o1.z]; g1[3] = v[i + o1.w]; }
-    acc = fmaf(a0.x, (float)(g0[0] - vi), acc);
-    acc = fmaf(a0.y, (float)(g0[1] - vi), acc);
-    acc = fmaf(a0.z, (float)(g0[2] - vi), acc);
-    acc = fmaf(a0.w, (float)(g0[3] - vi), acc);
+    acc = fma((double)a0.x, g0[0] - vi, acc);
+    acc = fma((double)a0.y, g0[1] - vi, acc);
+    acc = fma((double)a0.z, g0[2] - vi, acc);
+    acc = fma((double)a0.w, g0[3] - vi, acc);
     a0 = a1; a1 = a2; o1 = o2;
     g0[0] = g1[0]; g0[1] = g1[1]; g0[2] = g1[2]; g0[3] = g1[3];
   }
-  return (double)acc + A.rowsum[i] * xhat_i;
+  return acc + A.rowsum[i] * xhat_i;
 }
 
 __device__ inline double spmv_row_fp64(const Mat& A, int i, const double* __restrict__ x) {
@@ -527,7 +531,8 @@ __global__ void k_bi_init(Mat A, Vecs w, int x_given, RedBufs rb) {
 
 // small: the first-iteration norm threshold, 1.0d-6 (nonull, :367, :404) or the integer expression
 // 10-6 = 4 (open, :171, :207)
-__global__ void k_fin_init(RedBufs rb, int nblk, double tol, int itermax, double small, Scal* sc) {
+// (verify: check convergence claims against the FP64 residual, the mixed path)
+__global__ void k_fin_init(RedBufs rb, int nblk, double tol, int itermax, double small, int verify, Scal* sc) {
   double f[2];
   fold_n<2>(rb.P(ST_INIT), nblk, rb.R(ST_INIT), rb.n, f);
   const double rho = f[0];
@@ -537,6 +542,7 @@ __global__ void k_fin_init(RedBufs rb, int nblk, double tol, int itermax, double
     sc->iter = 1; sc->itermax = itermax; sc->exit_s = 0; sc->status = VB_OK;
     sc->mean_p = 0.0; sc->mean_s = 0.0; sc->sum_v = 0.0; sc->sum_p = 0.0;
     sc->done = 0;
+    sc->verify_claims = verify; sc->verify = 0; sc->true_at = -1;
     if (itermax < 1) { sc->done = 1; sc->status = VB_WARN_MAXITER; }
     else if (rho == 0.0) { sc->done = 1; sc->status = VB_WARN_BICGSTAB_BREAKDOWN; }
   }
@@ -630,7 +636,7 @@ __global__ void k_bi_sinv(int n, Vecs w, RedBufs rb, Scal* sc) {
   const bool first = sc->iter == 1;
   const double n2s0 = first ? ((n2s <= sc->n2_small) ? 1.0 : n2s) : sc->n2s0;
   const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
-  if (n2s <= sc->tol * sc->tol * n2s0) {
+  if (!sc->verify_claims && n2s <= sc->tol * sc->tol * n2s0) {   // (mixed: no unchecked exit on s)
     const double alpha = sc->alpha;
     for (int i = i0; i < rr.end; i += bd) w.x[i] = w.x[i] + alpha * w.pinv[i];
     if (lead) { sc->n2s0 = n2s0; sc->resid = sqrt(n2s / n2s0); sc->exit_s = 1; }
@@ -695,16 +701,18 @@ __global__ void k_bi_xr(int n, Vecs w, RedBufs rb, Scal* sc, int nspmv) {
   R.store(rb.P(ST_XR));
 }
 
-// Mixed path only: replace the recurrence residual by the FP64 residual B - A x.  The recurrence
-// residual differs from B - A x by a multiple of the constant vector (the DC projections of p and s
-// remove constants the recurrence never adds back), and A's columns need not sum to zero, so B - A x
-// carries a constant component the projected iteration cannot reduce.  The replacement therefore
-// keeps the recurrence residual's mean: r = (B - A x) - mean(B - A x) + mean(r).  Pass 1 writes
-// B - A x to t and the terms of sum(r), sum(B - A x); pass 2 shifts and forms |r|^2 and r_bar . r.
+// Mixed path only, when a convergence claim awaits its check (else both kernels return at once):
+// replace the recurrence residual by the FP64 residual B - A x.  The recurrence residual differs
+// from B - A x by a multiple of the constant vector (the DC projections of p and s remove constants
+// the recurrence never adds back), and A's columns need not sum to zero, so B - A x carries a
+// constant component the projected iteration cannot reduce.  The replacement therefore keeps the
+// recurrence residual's mean: r = (B - A x) - mean(B - A x) + mean(r).  Pass 1 writes B - A x to t
+// and the terms of sum(r), sum(B - A x); pass 2 shifts and forms |r|^2 and r_bar . r, and records
+// that the finish of iteration k sees the FP64 residual.
 // (MODE: SPMV_TRANSPOSE for the adjoint solve's A^T, else the FP64 rows of A)
 template <int MODE>
 __global__ void k_bi_true_residual(Mat A, Vecs w, RedBufs rb, const Scal* sc) {
-  if (finished(sc)) return;
+  if (finished(sc) || !sc->verify) return;
   const int n = A.n;
   RowRange rr = block_rows(n, gridDim.x);
   Red<2> R(rb.R(ST_TRUE), n);
@@ -720,8 +728,12 @@ __global__ void k_bi_true_residual(Mat A, Vecs w, RedBufs rb, const Scal* sc) {
 
 // (with_sum: also the terms of sum(r), which the five-kernel iteration folds after x,r)
 // (ntrue: workgroups of the pass-1 launch whose partials are folded)
-__global__ void k_bi_true_shift(int n, Vecs w, RedBufs rb, const Scal* sc, int with_sum, int ntrue) {
-  if (finished(sc)) return;
+// The iteration restarts from there, as a refinement step: r_bar = r and p = v = 0, so the next
+// direction is r itself (the Krylov space of the recurrence was built on the drifted residual).
+// (k: the iteration whose x, r this replaces)
+__global__ void k_bi_true_shift(int n, Vecs w, RedBufs rb, Scal* sc, int with_sum, int ntrue, int k) {
+  if (finished(sc) || !sc->verify) return;
+  if (blockIdx.x == 0 && threadIdx.x == 0) { sc->true_at = k; sc->sum_p = 0.0; sc->sum_v = 0.0; sc->mean_p = 0.0; }
   double f[2];
   fold_n<2>(rb.P(ST_TRUE), ntrue, rb.R(ST_TRUE), n, f);
   const double d = w.jac ? 0.0 : (f[0] - f[1]) / (double)n;   // open domain: no projections, r = B - A x
@@ -730,8 +742,11 @@ __global__ void k_bi_true_shift(int n, Vecs w, RedBufs rb, const Scal* sc, int w
   for (int i = rr.begin + threadIdx.x; i < rr.end; i += blockDim.x) {
     const double r = w.t[i] + d;
     w.r[i] = r;
+    w.rb[i] = r;
+    w.pn[i] = 0.0;
+    w.v[i] = 0.0;
     R.add(0, i, r * r);
-    R.add(1, i, w.rb[i] * r);
+    R.add(1, i, r * r);
     if (with_sum) R.add(2, i, r);
   }
   R.store(rb.P(ST_XR));
@@ -746,9 +761,13 @@ __global__ void k_fin_xr(RedBufs rb, int nblk, Scal* sc) {
   if (threadIdx.x == 0) {
     if (sc->iter == 1) sc->n2r0 = (n2r <= sc->n2_small) ? 1.0 : n2r;
     sc->resid = sqrt(n2r / sc->n2r0);
-    if (n2r <= sc->tol * sc->tol * sc->n2r0) {
+    const bool claimed = n2r <= sc->tol * sc->tol * sc->n2r0;
+    const bool checked = !sc->verify_claims || sc->true_at == sc->iter;
+    if (sc->true_at == sc->iter) sc->verify = 0;
+    if (claimed && checked) {
       sc->done = 1;
     } else {
+      if (claimed) sc->verify = 1;
       sc->iter = sc->iter + 1;
       sc->rho0 = sc->rho;
       sc->rho = rho;
@@ -796,7 +815,11 @@ __device__ inline FinState finish_iteration(const double (&f)[3], int k, Scal* s
   const double n2r0 = k == 1 ? ((n2r <= sc->n2_small) ? 1.0 : n2r) : sc->n2r0;
   FinState o{false, 0.0, f[2]};
   int status = VB_OK;
-  const bool converged = n2r <= sc->tol * sc->tol * n2r0;
+  // a claim of the recurrence residual converges only once checked (mixed path: the FP64 residual
+  // replaced it for this iteration)
+  const bool claimed = n2r <= sc->tol * sc->tol * n2r0;
+  const bool checked = !sc->verify_claims || sc->true_at == k;
+  const bool converged = claimed && checked;
   const double rho0 = sc->rho_pp[k & 1];
   if (converged) o.done = true;
   else if (k + 1 > sc->itermax) { o.done = true; status = VB_WARN_MAXITER; }
@@ -807,6 +830,8 @@ __device__ inline FinState finish_iteration(const double (&f)[3], int k, Scal* s
     sc->resid = sqrt(n2r / n2r0);
     if (!converged) { sc->iter = k + 1; sc->rho0 = rho0; sc->rho = rho; sc->rho_pp[(k + 1) & 1] = rho; }
     sc->beta = o.beta; sc->sum_r = o.sum_r;
+    if (sc->true_at == k) sc->verify = 0;
+    if (claimed && !converged) sc->verify = 1;
     if (o.done) { sc->done = 1; sc->status = status; }
   }
   return o;
@@ -902,7 +927,7 @@ __global__ void k_fu_xr(int n, Vecs w, RedBufs rb, Scal* sc, int nspmv, int k) {
     if (k == 1) sc->n2s0 = n2s0;
     sc->sum_p = fs[1];
   }
-  if (n2s <= sc->tol * sc->tol * n2s0) {                     // :374-381
+  if (!sc->verify_claims && n2s <= sc->tol * sc->tol * n2s0) {   // :374-381 (mixed: no unchecked exit)
     for (int i = i0; i < rr.end; i += bd) w.x[i] = w.x[i] + alpha * w.pinv[i];
     if (lead) { sc->resid = sqrt(n2s / n2s0); sc->exit_s = 1; }
     return;
@@ -961,12 +986,12 @@ static vb_status solve(vb_ctx* c, const Mat& A, Vecs w, int x_given, double tol,
   const bool fused = rb.rows == nullptr;   // serial mode: the reference's eight-kernel order
   const double small = w.jac ? (double)(10 - 6) : 1.0e-6;
   VB_LAUNCH(c, k_bi_init, nb, A, w, x_given, rb);
-  VB_FIN(c, k_fin_init, rb, nb, tol, itermax, small, sc);
+  VB_FIN(c, k_fin_init, rb, nb, tol, itermax, small, MODE == SPMV_MIXED ? 1 : 0, sc);
   int launched = 0, batch = 8;
   for (;;) {
     for (int b = 0; b < batch; b++) {
       const int k = launched + b + 1;
-      const bool refresh = MODE == SPMV_MIXED && k % kRefresh == 0;
+      const bool check = MODE == SPMV_MIXED && k % kVerifyEvery == 0;
       if (fused) {
         if (w.jac) VB_LAUNCH(c, k_fu_p<true>, nb, n, w, rb, sc, k, b > 0 ? 1 : 0);
         else VB_LAUNCH(c, k_fu_p<false>, nb, n, w, rb, sc, k, b > 0 ? 1 : 0);
@@ -975,9 +1000,9 @@ static vb_status solve(vb_ctx* c, const Mat& A, Vecs w, int x_given, double tol,
         else VB_LAUNCH(c, k_fu_s<false>, nb, n, w, rb, sc, ns, k);
         VB_LAUNCH_BS(c, (k_bi_spmv_s<MODE, true>), ns, kSpmvBlock, A, w, rb, sc);
         VB_LAUNCH(c, k_fu_xr, nb, n, w, rb, sc, ns, k);
-        if (refresh) {
+        if (check) {                 // returns at once unless a convergence claim is pending
           VB_LAUNCH_BS(c, k_bi_true_residual<MODE>, ns, kSpmvBlock, A, w, rb, sc);   // one row per thread
-          VB_LAUNCH(c, k_bi_true_shift, nb, n, w, rb, sc, 1, ns);
+          VB_LAUNCH(c, k_bi_true_shift, nb, n, w, rb, sc, 1, ns, k);
         }
         continue;
       }
@@ -988,9 +1013,9 @@ static vb_status solve(vb_ctx* c, const Mat& A, Vecs w, int x_given, double tol,
       VB_LAUNCH(c, k_bi_sinv, nb, n, w, rb, sc);
       VB_LAUNCH_BS(c, k_bi_spmv_s<MODE>, ns, kSpmvBlock, A, w, rb, sc);
       VB_LAUNCH(c, k_bi_xr, nb, n, w, rb, sc, ns);
-      if (refresh) {
+      if (check) {
         VB_LAUNCH_BS(c, k_bi_true_residual<MODE>, ns, kSpmvBlock, A, w, rb, sc);
-        VB_LAUNCH(c, k_bi_true_shift, nb, n, w, rb, sc, 0, ns);
+        VB_LAUNCH(c, k_bi_true_shift, nb, n, w, rb, sc, 0, ns, k);
       }
       VB_FIN(c, k_fin_xr, rb, nb, sc);
     }
