@@ -30,6 +30,13 @@ def have_ref():
     return refwrap.available()
 
 
+def mixed_floor(key):
+    """The mixed path's bar against the FP64 oracle where the oracle's own one-ulp spread is smaller:
+    its kernel weights and PPE matvec values are FP32 (6e-8 relative), so its fields are held to 1e-5,
+    its precision target against FP64, and positions to 1e-9."""
+    return 1e-9 if key in ("x", "y") else 1e-5
+
+
 @pytest.fixture(scope="session")
 def gpu_lib():
     """The HIP library; GPU tests fail (not skip) when it is missing or no device is visible."""

@@ -15,9 +15,10 @@ forward steps.
 * Central differences of a run that starts from the isotropic conformation A = I do not resolve a
   derivative (see prestretched), so the inversions start from rest with a pre-stretched conformation.
 
-The forward solves run to a relative residual of 1e-13, so that the finite differences (which see
-the solver's stopping error divided by 2 eps) resolve the derivative; the adjoint differentiates the
-exact solve.  Lists and mirror sets stay fixed under these perturbations (the pair test's kernel
+The forward solves run to a relative residual of 1e-12 (1e-13 is at the floor FP64 BiCGSTAB
+reaches on these systems: one of the runs stalled just above it), so that the finite differences
+(which see the solver's stopping error divided by 2 eps) resolve the derivative; the adjoint
+differentiates the exact solve.  Lists and mirror sets stay fixed under these perturbations (the pair test's kernel
 vanishes with its first two derivatives at the cutoff)."""
 import math
 
@@ -28,7 +29,7 @@ from vebisph_amd import _lib, adjoint, cases, solver
 
 pytestmark = pytest.mark.gpu
 
-TOL = 1e-13
+TOL = 1e-12
 ITMAX = 3000
 FIELDS = solver.Solver.ADJ_FIELDS
 
@@ -127,10 +128,11 @@ def test_adjoint_directional_derivatives(gpu_lib, name, nsteps):
         err = abs(adj - fd) / max(abs(fd), 1e-300)
         print(f"  {key}: adjoint {adj:.10e}  central FD {fd:.10e}  rel {err:.2e}", flush=True)
         assert err <= 1e-5, key
-    # parameters (step 1e-5 relative: at 1e-6 the smallest of these derivatives is within the
-    # functional's rounding / 2 eps; the truncation error at 1e-5 is ~1e-10 relative)
+    # parameters (step 1e-4 relative: the smallest of these derivatives, dL/dWi ~ 5e-4 against a
+    # functional of ~1e2, is within the solves' stopping error / 2 eps at smaller steps; the
+    # truncation error at 1e-4 is ~1e-8 relative)
     for q, pname in enumerate(("Wi", "beta", "Ra")):
-        h = 1e-5 * params[q]
+        h = 1e-4 * params[q]
         pp = list(params); pm = list(params)
         pp[q] += h; pm[q] -= h
         fd = (_apply(w, _final(case, pp, s0, nsteps)) - _apply(w, _final(case, pm, s0, nsteps))) / (2 * h)
@@ -197,11 +199,13 @@ def test_adjoint_reduced_convection_with_nusselt(gpu_lib):
     every 5; L = 1/2 |v(mid-plane) - v_target|^2 + (Nu - Nu_target)^2 with the target from the solver at
     Ra = 6e4.  Nu = 1 + sqrt(Ra Pr) <v theta> H / dT is formed in torch from the run's final v, theta,
     so its explicit dependence on Ra goes through autograd and the rest through the C ABI's adjoint.
-    Checked against central differences (eps = 1e-6 relative).  The forward is the mixed-precision
-    path here (FP32 matvecs, FP64-refined adjoint PPE), solved to 1e-11."""
+    Checked against central differences (eps = 1e-6 relative) of the FP64 run, solved to 1e-11; then
+    the same gradient through the mixed-precision run (FP32 kernel weights and matvec values) against
+    that FP64 adjoint.  (Central differences of the mixed run itself do not resolve a derivative at
+    this step: its weights are rounded to FP32, a 6e-8 quantisation of the forward map.)"""
     import torch
     case = cases.convection(nx=80, ny=40, seed=5, Ra=5e4, Wi=1.0, beta=0.5, steps=10, name="convection_c5_3k")
-    nsteps, every, prec, tol = 10, 5, _lib.PREC_MIXED, 1e-11
+    nsteps, every, tol = 10, 5, 1e-11
     nb = case.nbS
     start = prestretched(case)
     y = case.y[nb:]
@@ -211,23 +215,27 @@ def test_adjoint_reduced_convection_with_nusselt(gpu_lib):
     def nusselt_np(st, ra):
         return 1.0 + math.sqrt(ra * case.Pr) * np.mean(st["v"][nb:] * st["theta"][nb:]) * H / dT
 
-    tgt = adjoint.run_forward(case, 1.0, 0.5, 6e4, nsteps, tol=tol, itermax=ITMAX, precision=prec, state=start)
+    tgt = adjoint.run_forward(case, 1.0, 0.5, 6e4, nsteps, tol=tol, itermax=ITMAX, state=start)
     vt, nut = tgt["v"][nb:][mid], nusselt_np(tgt, 6e4)
 
     def loss_np(p):
-        st = adjoint.run_forward(case, *p, nsteps, tol=tol, itermax=ITMAX, precision=prec, state=start)
+        st = adjoint.run_forward(case, *p, nsteps, tol=tol, itermax=ITMAX, state=start)
         return 0.5 * np.sum((st["v"][nb:][mid] - vt) ** 2) + (nusselt_np(st, p[2]) - nut) ** 2
 
     Run = adjoint.make_run_function()
     p0 = (1.0, 0.5, 5e4)
-    params = torch.tensor(p0, dtype=torch.float64, requires_grad=True)
-    final = Run.apply(params, case, nsteps, every, tol, ITMAX, prec, start)
-    v, th = final[3, nb:], final[4, nb:]
-    nu = 1.0 + torch.sqrt(params[2] * case.Pr) * torch.mean(v * th) * H / dT
-    loss = 0.5 * ((v[torch.from_numpy(mid)] - torch.from_numpy(vt)) ** 2).sum() + (nu - nut) ** 2
-    loss.backward()
-    grads = params.grad.numpy()
-    print(f"loss {loss.item():.10e} (forward {loss_np(p0):.10e}), Nu {nu.item():.6f} target {nut:.6f}", flush=True)
+
+    def gradient(prec):
+        params = torch.tensor(p0, dtype=torch.float64, requires_grad=True)
+        final = Run.apply(params, case, nsteps, every, tol, ITMAX, prec, start)
+        v, th = final[3, nb:], final[4, nb:]
+        nu = 1.0 + torch.sqrt(params[2] * case.Pr) * torch.mean(v * th) * H / dT
+        loss = 0.5 * ((v[torch.from_numpy(mid)] - torch.from_numpy(vt)) ** 2).sum() + (nu - nut) ** 2
+        loss.backward()
+        return loss.item(), nu.item(), params.grad.numpy().copy()
+
+    loss, nu, grads = gradient(_lib.PREC_FP64)
+    print(f"loss {loss:.10e} (forward {loss_np(p0):.10e}), Nu {nu:.6f} target {nut:.6f}", flush=True)
     for q, name in enumerate(("Wi", "beta", "Ra")):
         h = 1e-6 * p0[q]
         pp = list(p0); pm = list(p0)
@@ -235,6 +243,11 @@ def test_adjoint_reduced_convection_with_nusselt(gpu_lib):
         fd = (loss_np(pp) - loss_np(pm)) / (2 * h)
         err = abs(grads[q] - fd) / max(abs(fd), 1e-300)
         print(f"dL/d{name}: adjoint {grads[q]:.10e}  central FD {fd:.10e}  rel {err:.2e}", flush=True)
+        assert err <= 1e-4, name
+    _, _, gmix = gradient(_lib.PREC_MIXED)
+    for q, name in enumerate(("Wi", "beta", "Ra")):
+        err = abs(gmix[q] - grads[q]) / abs(grads[q])
+        print(f"dL/d{name} through the mixed run: {gmix[q]:.10e}, against the FP64 adjoint {err:.2e}", flush=True)
         assert err <= 1e-4, name
 
 

@@ -7,8 +7,10 @@ steps the same start state; its one-ulp spread -- the reference against itself f
 moved by one ulp, two such runs -- is the scale.  At 1M every BiCGSTAB solve, the reference's
 included, stops at the 1000-iteration limit with a relative residual of 1e-6..1e-3 that depends on
 rounding, so p (and the velocity update through grad p) agrees between any two orders of summation
-only to that level; the fields are held to ten times the spread (1e-10 where the spread is below
-1e-10), and the relative errors are printed beside a 1e-5 mark for u, p, theta and tau.
+only to that level.  The FP64 fields are held to ten times the spread (1e-10 where the spread is
+below that); the mixed path's, whose kernel weights are rounded to FP32, to ten times the spread or
+1e-5 (its precision target against FP64; positions 1e-9), whichever is larger.  The relative errors
+are printed beside a 1e-5 mark for u, p, theta and tau.
 
 The reference's predictor runs on one thread (see _reference_run).  The comparison is taken after
 phase 11 of the last step (the pressure update; step 1 at 1M, where the reference takes ~25 s per
@@ -86,8 +88,18 @@ def _gpu_run(case, precision, nsteps):
     return rows, sc, force
 
 
+def _floor(name, key):
+    """The bar below which no spread is needed: FP64 1e-10; the mixed path (FP32 kernel weights and
+    matvec values) 1e-5 on the fields, its precision target against the FP64 reference, and 1e-9 on
+    positions."""
+    if name == "fp64":
+        return 1e-10
+    return 1e-9 if key in ("x", "y") else 1e-5
+
+
 def _compare(case, ref, spreads, runs, npfb):
-    """Prints each field's error against the reference beside the spread, then asserts the bar."""
+    """Prints each field's error against the reference beside the spread, then asserts the bar: ten
+    times the spread, or the run's floor where that is larger."""
     fails = []
     for key in FIELDS:
         spread = max(_rel(s[key][:npfb], ref[key][:npfb]) for s in spreads)
@@ -96,8 +108,7 @@ def _compare(case, ref, spreads, runs, npfb):
             err = _rel(rows[key][:npfb], ref[key][:npfb])
             mark = "" if key not in ("u", "p", "theta", "tau") else (" (<=1e-5)" if err <= 1e-5 else " (>1e-5)")
             line.append(f"{name} {err:.2e}{mark}")
-            bar = 1e-10 if spread < 1e-10 else 10.0 * spread
-            if err > bar:
+            if err > max(10.0 * spread, _floor(name, key)):
                 fails.append((name, key, err, spread))
         print("  " + ", ".join(line), flush=True)
     assert not fails, fails
@@ -138,8 +149,7 @@ def test_cylinder_1M_against_reference(gpu_lib):
           f"relative error FP64 {e64:.2e}, mixed {emx:.2e}, reference spread {fspread:.2e}", flush=True)
     # the GPU's own force kernel on its own rows agrees with the oracle's evaluation
     assert _rel(f64, forces["fp64"]) <= 1e-10 and _rel(fmx, forces["mixed"]) <= 1e-10
-    for e in (e64, emx):
-        assert e <= max(10.0 * fspread, 1e-10)
+    assert e64 <= max(10.0 * fspread, 1e-10) and emx <= max(10.0 * fspread, 1e-5)
 
 
 @pytest.mark.skipif(not have_ref(), reason="oracle/_ref (the reference build) is absent")

@@ -15,6 +15,7 @@ import os
 import numpy as np
 import pytest
 
+from tests.conftest import mixed_floor
 from vebisph_amd import cases, solver
 from oracle import oraclewrap as ow
 
@@ -69,7 +70,8 @@ def test_step_phases_against_oracle(gpu_lib, name):
                     else:
                         assert _rel(a, b) <= 1e-10
             if k == 7:
-                # pow() of the device library vs libm: ulps; later steps add the position differences
+                # the tensile term's fourth power (two squarings here, libm's pow in the oracle): ulps;
+                # later steps add the position differences
                 assert _rel(G.ushift(), O.ushift()) <= (1e-13 if first else 1e-8)
             if k == 9:
                 n = so["np"]
@@ -181,9 +183,9 @@ def test_bicgstab_parallel_iterates_follow_reference(gpu_lib):
 
 @pytest.mark.parametrize("name", list(SMALL))
 def test_mixed_precision_fields(gpu_lib, name):
-    """FP32 matvecs inside the FP64 BiCGSTAB: ten steps against the FP64 oracle, held to the same
-    reference-sensitivity scale as the FP64 path, with the FP64 residual of the final solve checked
-    on the assembled system."""
+    """The mixed path (FP32 kernel weights and PPE matvec values, FP64 BiCGSTAB vectors, sums and
+    refinement): ten steps against the FP64 oracle, held to ten times the oracle's one-ulp spread or
+    mixed_floor, with the FP64 residual of the final solve checked on the assembled system."""
     from vebisph_amd import _lib
     case = SMALL[name]()
     print(f"case {case.name}", flush=True)
@@ -196,8 +198,8 @@ def test_mixed_precision_fields(gpu_lib, name):
     for key in ("x", "y", "u", "v", "theta", "p", "tau", "trace"):
         spread = _rel(b2[key], b[key])
         err = _rel(a[key], b[key])
-        print(f"mixed {key}: vs oracle {err:.2e}, oracle one-ulp spread {spread:.2e}")
-        assert err <= max(10.0 * spread, 1e-12), key
+        print(f"mixed {key}: vs oracle {err:.2e}, oracle one-ulp spread {spread:.2e}", flush=True)
+        assert err <= max(10.0 * spread, mixed_floor(key)), key
     # the mixed solve meets the reference criterion in FP64 terms on its own system
     for k in range(1, 11):
         G.phase(k)
@@ -262,8 +264,8 @@ def test_whole_steps_serial_order_against_oracle(gpu_lib, name):
     """Ten whole steps with every global sum in index order (serial_reductions: the reference's
     single-thread loops; the SpMVs then use the stored rows in the reference's slot order, and the
     BiCGSTAB follows the reference's iterates bit for bit on identical systems).  What remains between
-    the GPU and the oracle is the device's exp/log (log-conformation) and pow (the shift's tensile
-    term) against the host's libm, ulps that the chaotic 1e-8-tolerance solve then carries.  Every
+    the GPU and the oracle is the device's exp/log (log-conformation) against the host's libm, and
+    the shift's tensile fourth power (two squarings against the host's pow), ulps that the chaotic 1e-8-tolerance solve then carries.  Every
     field whose oracle one-ulp spread (the oracle against itself from fluid positions moved by one
     ulp) is below 1e-10 is held to 1e-10; the others to ten times that spread."""
     case = SMALL[name]()
@@ -295,11 +297,12 @@ def test_whole_steps_serial_order_against_oracle(gpu_lib, name):
     assert G.scalars()["time"] == O.scalars()["time"]
 
 
-def test_newtonian_serial_step_differs_only_through_pow(gpu_lib):
+def test_newtonian_serial_step_differs_only_through_shift_power(gpu_lib):
     """Where the attribution can be made exact: the Newtonian program (no log-conformation, so no
     exp/log) in serial order.  The first step from rest then gives the oracle's velocities and
     pressure bit for bit (its right-hand side, matrix and BiCGSTAB iterates are the same operations),
-    and positions differ only through the shift's pow (ushift to ulps, positions to 1e-15)."""
+    and positions differ only through the shift's tensile fourth power, two squarings against the
+    host's pow (ushift to ulps, positions to 1e-15)."""
     case = SMALL["poiseuille"]()
     case.const_mod = 1
     O = ow.OracleSolver(case)

@@ -530,7 +530,8 @@ __global__ void k_adj_geometry(Geom g, Fwd F, Cot b, Coef cf, const double* lam,
       a21 = a21 - dv * gx * ry; a22 = a22 - dv * gy * ry;
       if (fluid) {
         const double w = wab(rad / h, g.ad7);
-        const double fab = 0.25 * pow(w / wab1, 4.0);
+        const double wr = w / wab1, wr2 = wr * wr;
+        const double fab = 0.25 * (wr2 * wr2);                       // as k_shift
         const double w1 = F.wg1[s], w2 = F.wg2[s];
         sfx = sfx + w1; sfy = sfy + w2; stx = stx + w1 * fab; sty = sty + w2 * fab;
       }
@@ -601,8 +602,8 @@ __global__ void k_adj_geometry(Geom g, Fwd F, Cot b, Coef cf, const double* lam,
       if (fluid) {                                                          // the shift
         const double qs = rad / h;
         const double w = wab(qs, g.ad7);
-        const double wr = w / wab1;
-        const double fab = 0.25 * pow(wr, 4.0);
+        const double wr = w / wab1, wr2 = wr * wr;
+        const double fab = 0.25 * (wr2 * wr2);                      // as k_shift
         wb1 += gcbx * (1.0 + fab); wb2 += gcby * (1.0 + fab);
         const double fabb = gcbx * w1 + gcby * w2;
         radb += fabb * (wr * wr * wr) * fac(qs, g.ad7) / (wab1 * h);      // d fab / d rad
@@ -840,6 +841,7 @@ struct AdjointCtx {
   std::vector<int> fwd_iters, adj_iters;
   std::vector<double> adj_resid;
   int adj_status = VB_OK;
+  double* w64 = nullptr;   // mixed path: the step's FP32 weight planes widened (w_G x, w_G y, w_L)
 };
 
 namespace vb {
@@ -849,7 +851,7 @@ void free_adjoint(vb_ctx* c) {
   if (!a) return;
   void* ptrs[] = {a->ck, a->ck_in, a->pst, a->seg, a->seg_in, a->fin, a->fin_in, a->rec.mjac, a->rec.msgn,
                   a->rec.ejac, a->rec.eflag, a->bcnt, a->boff, a->bfill, a->bkey, a->cot, a->b.S1, a->b.S7,
-                  a->part, a->as};
+                  a->part, a->as, a->w64};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (a->as_host) (void)hipHostFree(a->as_host);
@@ -915,6 +917,12 @@ static vb_status forward_step(vb_ctx* c, double tol, int itermax, const double* 
   return ls;
 }
 
+// b = a widened (grid-stride)
+__global__ void k_adj_widen(const float* a, double* b, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    b[i] = a[i];
+}
+
 static vb_status adjoint_alloc(vb_ctx* c, AdjointCtx* a) {
   const size_t n = c->npfb, nw = n * kNplink, mcap = c->mcap;
   const size_t sz = kStateFields * n;
@@ -938,6 +946,7 @@ static vb_status adjoint_alloc(vb_ctx* c, AdjointCtx* a) {
   VB_HIP(c, hipMalloc((void**)&a->b.S7, 7 * nw * 8));
   VB_HIP(c, hipMalloc((void**)&a->part, (size_t)AQ_N * kPartialStride * 8));
   VB_HIP(c, hipMalloc((void**)&a->as, sizeof(AdjScal)));
+  if (c->wf32) VB_HIP(c, hipMalloc((void**)&a->w64, 3 * nw * 8));
   VB_HIP(c, hipHostMalloc((void**)&a->as_host, sizeof(AdjScal)));
   double** f[kCotFields] = {&a->b.x, &a->b.y, &a->b.u, &a->b.v, &a->b.th, &a->b.t11, &a->b.t12, &a->b.t21,
                             &a->b.t22, &a->b.trc, &a->b.us, &a->b.vs, &a->b.u0, &a->b.v0, &a->b.x0, &a->b.y0,
@@ -1029,6 +1038,14 @@ static vb_status reverse_step(vb_ctx* c, int k, const double* tau0, double tol, 
   F.u0 = c->st.u0; F.v0 = c->st.v0; F.x0 = c->st.x0; F.y0 = c->st.y0;
   F.inbin = c->st.inbin; F.irel = c->st.irel; F.vrel = c->st.vrel;
   F.cnt = c->cnt; F.link = c->link; F.wg1 = c->wg1; F.wg2 = c->wg2; F.wl = c->wl;
+  if (c->wf32) {          // the reverse kernels read FP64 weights: widen the mixed path's FP32 planes
+    const size_t nw = (size_t)n * kNplink;
+    const int nwb = (int)std::min<size_t>((nw + kBlock - 1) / kBlock, 65535);
+    VB_LAUNCH(c, k_adj_widen, nwb, c->wg1f, a->w64, nw);
+    VB_LAUNCH(c, k_adj_widen, nwb, c->wg2f, a->w64 + nw, nw);
+    VB_LAUNCH(c, k_adj_widen, nwb, c->wlf, a->w64 + 2 * nw, nw);
+    F.wg1 = a->w64; F.wg2 = a->w64 + nw; F.wl = a->w64 + 2 * nw;
+  }
   F.k11 = c->k11; F.k12 = c->k12; F.k21 = c->k21; F.k22 = c->k22; F.kick = c->kick;
   Nbr N{c->cnt, c->link, c->op.rslot, a->boff, a->bkey, n};
   Cot b = a->b;
@@ -1053,7 +1070,7 @@ static vb_status reverse_step(vb_ctx* c, int k, const double* tau0, double tol, 
   VB_LAUNCH(c, k_adj_logconf, nb, c->g, F, b, cf, c->scal, part);
   VB_LAUNCH(c, k_adj_gather_u, ng, N, a->rec.msgn, b.S7, b.nw, b.ua, b.va);
   VB_LAUNCH(c, k_adj_geometry, nb, c->g, F, b, cf, lam, c->scal);
-  VB_LAUNCH(c, k_adj_gather_x, ng, N, a->rec, b.S1, b.S7, b.nw, lam, c->wl, b.x1, b.y1);
+  VB_LAUNCH(c, k_adj_gather_x, ng, N, a->rec, b.S1, b.S7, b.nw, lam, F.wl, b.x1, b.y1);
   VB_LAUNCH(c, k_adj_advect, nb, c->g, F, b, a->rec, c->scal, part);
   hipLaunchKernelGGL(k_adj_scalars, dim3(1), dim3(1024), 0, c->stream, c->g, cf, p.Ra, p.beta, part, nb,
                      c->st.u0, c->st.v0, c->scal, b, a->as);

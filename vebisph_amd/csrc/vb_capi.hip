@@ -64,7 +64,7 @@ static void free_all(vb_ctx* c) {
                   c->st.t22, c->st.trc, c->st.p, c->st.inbin, c->st.x0, c->st.y0, c->st.u0, c->st.v0,
                   c->st.irel, c->st.vrel, c->st.dP, c->u_alt, c->v_alt, c->th_alt, c->nsurf,
                   c->seg_counts, c->cellkey, c->cellslot, c->nc, c->ist, c->ip, c->scan_bsum, c->cnt, c->link,
-                  c->wg1, c->wg2, c->wl, c->k11, c->k12, c->k21, c->k22,
+                  c->wg1, c->wg2, c->wl, c->wg1f, c->wg2f, c->wlf, c->k11, c->k12, c->k21, c->k22,
                   c->usx, c->usy, c->kick, c->gpx, c->gpy, c->vort, c->conc, c->snx, c->sny, c->pscale, c->a_diag, c->a_val, c->a_rowsum, c->rhs,
                   c->a_col, c->a_len, c->rowflag, c->chunkg, c->choff, c->xflag, c->xscan, c->xrows,
                   c->lnk4, c->val4, c->val4f,
@@ -191,7 +191,11 @@ vb_status vb_create(const vb_params* prm, const vb_boundary* bnd, int32_t npfb, 
   A(c->seg_counts, c->seg_blocks);
   A(c->cellkey, cap); A(c->cellslot, cap); A(c->nc, c->g.nct + 1); A(c->ist, c->g.nct + 1); A(c->ip, cap);
   A(c->scan_bsum, (std::max<size_t>(cap, (size_t)c->g.nct) + 1023) / 1024 + 1);
-  A(c->cnt, n); A(c->link, nw); A(c->wg1, nw); A(c->wg2, nw); A(c->wl, nw);
+  A(c->cnt, n); A(c->link, nw);
+  // kernel weights: FP32 planes on the mixed path (the SPH sums read them widened), else FP64
+  c->wf32 = prm->precision == VB_PREC_MIXED;
+  if (c->wf32) { A(c->wg1f, nw); A(c->wg2f, nw); A(c->wlf, nw); }
+  else { A(c->wg1, nw); A(c->wg2, nw); A(c->wl, nw); }
   A(c->k11, n); A(c->k12, n); A(c->k21, n); A(c->k22, n);
   A(c->usx, n); A(c->usy, n); A(c->kick, n); A(c->gpx, n); A(c->gpy, n); A(c->vort, n); A(c->conc, n);
   if (prm->i_open_domain == 1) { A(c->snx, n); A(c->sny, n); A(c->pscale, n); }
@@ -472,13 +476,22 @@ vb_status vb_export_weights(vb_ctx* c, int32_t n, int32_t w, double* wG, double*
   std::vector<int32_t> cnt(n);
   const size_t nw = (size_t)n * w;
   VB_HIP(c, hipMemcpyAsync(cnt.data(), c->cnt, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
-  VB_HIP(c, hipMemcpyAsync(wG, c->wg1, nw * 8, hipMemcpyDeviceToHost, c->stream));
-  VB_HIP(c, hipMemcpyAsync(wG + nw, c->wg2, nw * 8, hipMemcpyDeviceToHost, c->stream));
-  VB_HIP(c, hipMemcpyAsync(wL, c->wl, nw * 8, hipMemcpyDeviceToHost, c->stream));
+  std::vector<float> wf(c->wf32 ? 3 * nw : 0);            // FP32 planes, widened below
+  if (c->wf32) {
+    VB_HIP(c, hipMemcpyAsync(wf.data(), c->wg1f, nw * 4, hipMemcpyDeviceToHost, c->stream));
+    VB_HIP(c, hipMemcpyAsync(wf.data() + nw, c->wg2f, nw * 4, hipMemcpyDeviceToHost, c->stream));
+    VB_HIP(c, hipMemcpyAsync(wf.data() + 2 * nw, c->wlf, nw * 4, hipMemcpyDeviceToHost, c->stream));
+  } else {
+    VB_HIP(c, hipMemcpyAsync(wG, c->wg1, nw * 8, hipMemcpyDeviceToHost, c->stream));
+    VB_HIP(c, hipMemcpyAsync(wG + nw, c->wg2, nw * 8, hipMemcpyDeviceToHost, c->stream));
+    VB_HIP(c, hipMemcpyAsync(wL, c->wl, nw * 8, hipMemcpyDeviceToHost, c->stream));
+  }
   double* kk[4] = {c->k11, c->k12, c->k21, c->k22};
   for (int q = 0; q < 4; q++)
     VB_HIP(c, hipMemcpyAsync(kg + (size_t)q * n, kk[q], (size_t)n * 8, hipMemcpyDeviceToHost, c->stream));
   VB_HIP(c, hipStreamSynchronize(c->stream));
+  if (c->wf32)
+    for (size_t s = 0; s < nw; s++) { wG[s] = wf[s]; wG[nw + s] = wf[nw + s]; wL[s] = wf[2 * nw + s]; }
   for (int k = 0; k < w; k++)
     for (int i = 0; i < n; i++)
       if (k >= cnt[i]) {

@@ -4,6 +4,8 @@
 
 #include <string>
 
+#include <type_traits>
+
 #include "vb_common.h"
 
 // The assembled PPE operator's transposed map (vb_operator.hip), built on first use after each
@@ -50,7 +52,9 @@ struct vb_ctx {
   int *cellkey, *cellslot, *nc, *ist, *ip;
   int* scan_bsum;                    // block sums of launch_exclusive_scan (one per 1024 elements)
   int *cnt, *link;                   // link[k*npfb + i], 0-based
-  double *wg1, *wg2, *wl;            // slot-major like link
+  double *wg1, *wg2, *wl;            // slot-major like link (FP64 path)
+  float *wg1f, *wg2f, *wlf;          // the same planes in FP32 (mixed path; then wg1.. are null)
+  bool wf32;                         // the weight planes are FP32 (precision = VB_PREC_MIXED)
   double *k11, *k12, *k21, *k22;
   double *usx, *usy;
   double* kick;                      // per-particle B perturbation (reference_kick)
@@ -133,6 +137,17 @@ vb_status sync_scalars(vb_ctx* c);
     vb_status _s = vb::hip_check((c), (call), #call);                       \
     if (_s != VB_OK) return _s;                                             \
   } while (0)
+
+// Calls f(wg1, wg2, wl) with the context's kernel-weight planes -- FP32 on the mixed path, else
+// FP64 -- and returns its status; kernels that read the planes take their element type as a template
+// argument (weight_t<decltype(w1)>), widening each weight to FP64 as they load it.
+template <class F>
+inline vb_status with_weights(vb_ctx* c, F&& f) {
+  if (c->wf32) return f(c->wg1f, c->wg2f, c->wlf);
+  return f(c->wg1, c->wg2, c->wl);
+}
+template <class P>
+using weight_t = std::remove_cv_t<std::remove_pointer_t<P>>;
 
 #define VB_LAUNCH(c, kern, grid, ...)                                       \
   do {                                                                      \

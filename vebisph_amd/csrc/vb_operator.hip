@@ -232,8 +232,9 @@ vb_status apply_ppe_operator(vb_ctx* c, bool transpose, const double* x, double*
 // particle field x (npfb): row i gives K_i sum_k (x_parent(j_k) - x_i) grad W_ik, mirrors taking
 // their parent's value (the pressure mirrors' offset dP is constant, so it is no part of the map).
 // Forward: y = [gx; gy] (2 npfb), the step's slot order.
-__global__ void k_grad_forward(int npfb, const int* cnt, const int* link, const int* irel, const double* wg1,
-                               const double* wg2, const double* k11, const double* k12, const double* k21,
+template <class WT>
+__global__ void k_grad_forward(int npfb, const int* cnt, const int* link, const int* irel, const WT* wg1,
+                               const WT* wg2, const double* k11, const double* k12, const double* k21,
                                const double* k22, const double* x, double* y) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= npfb) return;
@@ -264,8 +265,9 @@ __global__ void k_grad_coef(int npfb, const double* k11, const double* k12, cons
   b[i] = k12[i] * gx + k22[i] * gy;
 }
 
-__global__ void k_grad_transpose(int npfb, const int* cnt, const int* link, const int* irel, const double* wg1,
-                                 const double* wg2, const int* tcol, const int* rslot, const int* moff,
+template <class WT>
+__global__ void k_grad_transpose(int npfb, const int* cnt, const int* link, const int* irel, const WT* wg1,
+                                 const WT* wg2, const int* tcol, const int* rslot, const int* moff,
                                  const int* mkey, const double* a, const double* b, double* y) {
   const int c = blockIdx.x * kBlock + threadIdx.x;
   if (c >= npfb) return;
@@ -301,11 +303,12 @@ vb_status apply_grad_operator(vb_ctx* c, bool transpose, const double* x, double
     c->err = "the gradient operator needs the lists and weights of a step up to its vb_ppe_solve";
     return VB_ERR_ARG;
   }
-  if (!transpose) {
-    VB_LAUNCH(c, k_grad_forward, grid_1d(npfb), npfb, c->cnt, c->link, c->st.irel, c->wg1, c->wg2, c->k11,
-              c->k12, c->k21, c->k22, x, y);
-    return VB_OK;
-  }
+  if (!transpose)
+    return with_weights(c, [&](auto* w1, auto* w2, auto*) -> vb_status {
+      VB_LAUNCH(c, k_grad_forward<weight_t<decltype(w1)>>, grid_1d(npfb), npfb, c->cnt, c->link, c->st.irel, w1,
+                w2, c->k11, c->k12, c->k21, c->k22, x, y);
+      return VB_OK;
+    });
   if (!c->op.valid) {
     vb_status s = build_transpose(c);
     if (s != VB_OK) return s;
@@ -316,9 +319,11 @@ vb_status apply_grad_operator(vb_ctx* c, bool transpose, const double* x, double
     VB_HIP(c, hipMalloc((void**)&op.gb, (size_t)npfb * sizeof(double)));
   }
   VB_LAUNCH(c, k_grad_coef, grid_1d(npfb), npfb, c->k11, c->k12, c->k21, c->k22, x, op.ga, op.gb);
-  VB_LAUNCH(c, k_grad_transpose, grid_1d(npfb), npfb, c->cnt, c->link, c->st.irel, c->wg1, c->wg2, op.tcol,
-            op.rslot, op.moff, op.mkey, op.ga, op.gb, y);
-  return VB_OK;
+  return with_weights(c, [&](auto* w1, auto* w2, auto*) -> vb_status {
+    VB_LAUNCH(c, k_grad_transpose<weight_t<decltype(w1)>>, grid_1d(npfb), npfb, c->cnt, c->link, c->st.irel, w1,
+              w2, op.tcol, op.rslot, op.moff, op.mkey, op.ga, op.gb, y);
+    return VB_OK;
+  });
 }
 
 // v -= mean(v) over n rows: per-workgroup partial sums, then every workgroup of the second kernel

@@ -42,8 +42,9 @@ enum SpmvMode { SPMV_FP64 = 0, SPMV_MIXED = 1, SPMV_TRANSPOSE = 2 };
 // ------------------------------------------------------------------------------------------------
 // RHS, ppe_solve_in_house_simple_dirichlet.F90:27-53 (DC removal for the closed domain only)
 // ------------------------------------------------------------------------------------------------
-__global__ void k_ppe_rhs(Geom g, State st, const int* cnt, const int* link, const double* wg1,
-                          const double* wg2, const double* k11, const double* k12, const double* k21,
+template <class WT>
+__global__ void k_ppe_rhs(Geom g, State st, const int* cnt, const int* link, const WT* wg1,
+                          const WT* wg2, const double* k11, const double* k12, const double* k21,
                           const double* k22, double* rhs, double* partials, double* red, const Scal* sc) {
   const int npfb = g.npfb;
   RowRange rr = block_rows(npfb, gridDim.x);
@@ -89,10 +90,11 @@ constexpr int kChunk = 64;
 // stored rows, the diagonal, row sum and right-hand side, marks the rows the compact SpMV can take
 // (rowflag 0: no mirror neighbour, every neighbour within 32767 rows) and records per chunk the
 // groups its compact rows need (chunkg); k_assemble_sliced then writes the compact rows.
-__global__ void k_assemble(Geom g, State st, const int* cnt, const int* link, const double* wl,
+template <class WT>
+__global__ void k_assemble(Geom g, State st, const int* cnt, const int* link, const WT* wl,
                            const int* nsurf, const double* pscale, double* rhs, double* a_diag, int* a_col,
                            double* a_val, uint8_t* rowflag, double* rowsum, int* chunkg, const Scal* sc) {
-  const int i = blockIdx.x * kBlock + threadIdx.x;
+  const int i = xcd_row(g.npfb);
   const int npfb = g.npfb;
   int ngroups = 0;
   if (i < npfb) {
@@ -153,9 +155,10 @@ __global__ void k_assemble(Geom g, State st, const int* cnt, const int* link, co
 // (it went into the diagonal).  The columns and -w_L sit in LDS, so each thread's scans read LDS.
 constexpr int kExplicitThreads = 128;
 static_assert(kExplicitThreads >= kNplink, "one thread per slot");
+template <class WT>
 __global__ void __launch_bounds__(kExplicitThreads)
 k_assemble_explicit(int npfb, const int* rows, int nrows, const int* cnt, const int* link,
-                    const double* wl, const int* irel, const double* pscale, const double* a_diag, int* a_col,
+                    const WT* wl, const int* irel, const double* pscale, const double* a_diag, int* a_col,
                     double* a_val, double* rowsum) {
   if ((int)blockIdx.x >= nrows) return;
   __shared__ int cols[kExplicitThreads];
@@ -988,10 +991,13 @@ static vb_status solve(vb_ctx* c, const Mat& A, Vecs w, int x_given, double tol,
   VB_LAUNCH(c, k_bi_init, nb, A, w, x_given, rb);
   VB_FIN(c, k_fin_init, rb, nb, tol, itermax, small, MODE == SPMV_MIXED ? 1 : 0, sc);
   int launched = 0, batch = 8;
+  // mixed: the check slots are launched once the polled residual is within 1e3 of the tolerance (a
+  // claim cannot come earlier; at 1M particles, where the solve runs to its limit, never)
+  double polled = 1.0;
   for (;;) {
     for (int b = 0; b < batch; b++) {
       const int k = launched + b + 1;
-      const bool check = MODE == SPMV_MIXED && k % kVerifyEvery == 0;
+      const bool check = MODE == SPMV_MIXED && polled <= 1.0e3 * tol && k % kVerifyEvery == 0;
       if (fused) {
         if (w.jac) VB_LAUNCH(c, k_fu_p<true>, nb, n, w, rb, sc, k, b > 0 ? 1 : 0);
         else VB_LAUNCH(c, k_fu_p<false>, nb, n, w, rb, sc, k, b > 0 ? 1 : 0);
@@ -1025,6 +1031,7 @@ static vb_status solve(vb_ctx* c, const Mat& A, Vecs w, int x_given, double tol,
     VB_HIP(c, hipStreamSynchronize(c->stream));
     if (c->scal_host->done || c->scal_host->exit_s) break;
     if (launched > itermax + 2) break;
+    polled = c->scal_host->resid;
     batch = launched < 32 ? 8 : 32;
   }
   return VB_OK;
@@ -1039,11 +1046,20 @@ static Vecs step_vecs(vb_ctx* c, double* x, bool open) {
 // rows, and the transposed map marked stale.
 vb_status assemble_ppe(vb_ctx* c) {
   const int nb = spmv_grid(c->npfb);    // a neighbour sum: one row per thread, like the SpMV
-  VB_LAUNCH_BS(c, k_ppe_rhs, nb, kSpmvBlock, c->g, c->st, c->cnt, c->link, c->wg1, c->wg2, c->k11, c->k12, c->k21,
-            c->k22, c->rhs, c->partials, c->red, c->scal);
+  vb_status sw = with_weights(c, [&](auto* w1, auto* w2, auto*) -> vb_status {
+    VB_LAUNCH_BS(c, k_ppe_rhs<weight_t<decltype(w1)>>, nb, kSpmvBlock, c->g, c->st, c->cnt, c->link, w1, w2,
+                 c->k11, c->k12, c->k21, c->k22, c->rhs, c->partials, c->red, c->scal);
+    return VB_OK;
+  });
+  if (sw != VB_OK) return sw;
   VB_FIN(c, k_fin_ppe_rhs, c->partials, nb, c->red, c->npfb, c->g.open, c->scal);
-  VB_LAUNCH(c, k_assemble, grid_1d(c->npfb), c->g, c->st, c->cnt, c->link, c->wl, c->nsurf, c->pscale,
-            c->rhs, c->a_diag, c->a_col, c->a_val, c->rowflag, c->a_rowsum, c->chunkg, c->scal);
+  sw = with_weights(c, [&](auto*, auto*, auto* wl) -> vb_status {
+    VB_LAUNCH(c, k_assemble<weight_t<decltype(wl)>>, xcd_grid(c->npfb), c->g, c->st, c->cnt, c->link, wl,
+              c->nsurf, c->pscale, c->rhs, c->a_diag, c->a_col, c->a_val, c->rowflag, c->a_rowsum, c->chunkg,
+              c->scal);
+    return VB_OK;
+  });
+  if (sw != VB_OK) return sw;
   const int nch = (c->npfb + kChunk - 1) / kChunk;
   vb_status ss = launch_exclusive_scan(c, c->chunkg, c->choff, nch);
   if (ss != VB_OK) return ss;
@@ -1056,10 +1072,14 @@ vb_status assemble_ppe(vb_ctx* c) {
   if ((ss = sync_scalars(c)) != VB_OK) return ss;
   const int nx = c->scal_host->nexplicit;
   if (nx > 0) {
-    hipLaunchKernelGGL(k_assemble_explicit, dim3(nx), dim3(kExplicitThreads), 0, c->stream, c->npfb, c->xrows,
-                       nx, c->cnt, c->link, c->wl, c->st.irel, c->pscale, c->a_diag, c->a_col, c->a_val,
-                       c->a_rowsum);
-    VB_HIP(c, hipGetLastError());
+    sw = with_weights(c, [&](auto*, auto*, auto* wl) -> vb_status {
+      hipLaunchKernelGGL(k_assemble_explicit<weight_t<decltype(wl)>>, dim3(nx), dim3(kExplicitThreads), 0,
+                         c->stream, c->npfb, c->xrows, nx, c->cnt, c->link, wl, c->st.irel, c->pscale, c->a_diag,
+                         c->a_col, c->a_val, c->a_rowsum);
+      VB_HIP(c, hipGetLastError());
+      return VB_OK;
+    });
+    if (sw != VB_OK) return sw;
   }
   c->op.valid = false; c->op.step_matrix = true;        // a new matrix: the transposed map is rebuilt on use
   return VB_OK;

@@ -556,7 +556,7 @@ __global__ void k_cell_sort(const int* ist, int* ip, int nct) {
 
 __global__ void k_neighbours(Geom g, State st, const int* cellkey, const int* ist, const int* ip,
                              int* cnt, int* link, Scal* sc) {
-  int i = blockIdx.x * kBlock + threadIdx.x;
+  const int i = xcd_row(g.npfb);
   if (i >= g.npfb) return;
   int n = 0;
   if (!st.inbin[i]) {
@@ -608,11 +608,12 @@ __global__ void k_neighbours(Geom g, State st, const int* cellkey, const int* is
 //   1 on surface rows (their Dirichlet row is not smoothed) and where div r > 1.2 beta, else
 //   0.5 (1 - cos((div r - beta) pi / (0.2 beta))).
 // ------------------------------------------------------------------------------------------------
-template <bool OPEN>
-__global__ void k_weights(Geom g, State st, const int* cnt, const int* link, double* wg1, double* wg2,
-                          double* wl, double* k11, double* k12, double* k21, double* k22, double* conc,
+// (WT: the weight planes' type, float on the mixed path: the stores round to it)
+template <bool OPEN, class WT>
+__global__ void k_weights(Geom g, State st, const int* cnt, const int* link, WT* wg1, WT* wg2,
+                          WT* wl, double* k11, double* k12, double* k21, double* k22, double* conc,
                           double* snx, double* sny, int* nsurf, double* pscale, Scal* sc) {
-  int i = blockIdx.x * kBlock + threadIdx.x;
+  const int i = xcd_row(g.npfb);
   int mx = 0;
   if (i < g.npfb) {
     const int npfb = g.npfb, n = cnt[i];
@@ -676,10 +677,11 @@ __global__ void k_freesurf(int* nsurf, const int* irel, int npfb, int np, int op
 //      normal Shepard-filtered over the real neighbours, smooth_normals (freesurf_mod.F90:114-127):
 //      dv (sum_{j real} W(|r_ij|/h) sn_j + W(0) sn_i), summed here in the same slot loop.
 // ------------------------------------------------------------------------------------------------
-__global__ void k_shift(Geom g, State st, const int* cnt, const int* link, const double* wg1,
-                        const double* wg2, const int* nsurf, const double* snx, const double* sny,
+template <class WT>
+__global__ void k_shift(Geom g, State st, const int* cnt, const int* link, const WT* wg1,
+                        const WT* wg2, const int* nsurf, const double* snx, const double* sny,
                         double* usx, double* usy, const Scal* sc) {
-  int i = blockIdx.x * kBlock + threadIdx.x;
+  const int i = xcd_row(g.npfb);
   if (i >= g.npfb) return;
   const int npfb = g.npfb, n = cnt[i];
   const double xi = st.x[i], yi = st.y[i], wab1 = sc->wab1, h = g.h;
@@ -691,7 +693,10 @@ __global__ void k_shift(Geom g, State st, const int* cnt, const int* link, const
     double rad = sqrt(rx * rx + ry * ry);
     double qq = rad / h;
     double w = wab(qq, g.ad7);
-    double fab = 0.25 * pow(w / wab1, 4.0);
+    // Rt (W/W1)^nt with the real exponent nt = 4 (:147): two squarings instead of pow (within an ulp
+    // or two of the host's pow; pow's log/exp was the kernel's largest cost)
+    const double wr = w / wab1, wr2 = wr * wr;
+    double fab = 0.25 * (wr2 * wr2);
     double w1 = wg1[s], w2 = wg2[s];
     sfx = sfx + w1; sfy = sfy + w2;
     stx = stx + w1 * fab; sty = sty + w2 * fab;
@@ -734,11 +739,12 @@ __global__ void k_shift(Geom g, State st, const int* cnt, const int* link, const
 // ------------------------------------------------------------------------------------------------
 // grad u, grad v (grad_operator, gradient_calculation.F90:61-71) fused with the per-particle
 // log-conformation update (walls included, predictor_mod.F90:166)
-__global__ void k_grad_logconf(Geom g, State st, const int* cnt, const int* link, const double* wg1,
-                               const double* wg2, const double* k11, const double* k12,
+template <class WT>
+__global__ void k_grad_logconf(Geom g, State st, const int* cnt, const int* link, const WT* wg1,
+                               const WT* wg2, const double* k11, const double* k12,
                                const double* k21, const double* k22, const double* kick, int use_kick,
                                double wi, double eps_ptt, const Scal* sc) {
-  int i = blockIdx.x * kBlock + threadIdx.x;
+  const int i = xcd_row(g.npfb);
   if (i >= g.npfb) return;
   const int npfb = g.npfb, n = cnt[i];
   const double ui = st.u[i], vi = st.v[i];
@@ -809,12 +815,12 @@ __global__ void k_mirror_stress(Geom g, State st, const Scal* sc) {
 }
 
 // RHS, buoyancy, u* and theta* (predictor_mod.F90:203-282), written to the alternate buffers
-template <bool NEWTONIAN>
-__global__ void k_rhs(Geom g, State st, const int* cnt, const int* link, const double* wg1,
-                      const double* wg2, const double* wl, const double* k11, const double* k12,
+template <bool NEWTONIAN, class WT>
+__global__ void k_rhs(Geom g, State st, const int* cnt, const int* link, const WT* wg1,
+                      const WT* wg2, const WT* wl, const double* k11, const double* k12,
                       const double* k21, const double* k22, double cs, double cp, double ct,
                       double* un, double* vn, double* thn, const Scal* sc) {
-  int i = blockIdx.x * kBlock + threadIdx.x;
+  const int i = xcd_row(g.npfb);
   if (i >= g.npfb) return;
   const int npfb = g.npfb, n = cnt[i];
   const double ui = st.u[i], vi = st.v[i], thi = st.th[i];
@@ -926,11 +932,12 @@ __global__ void k_mirror_p(Geom g, State st, const Scal* sc) {
   st.p[j] = st.p[st.irel[j]] + st.dP[j];
 }
 
-__global__ void k_pressure_update(Geom g, State st, const int* cnt, const int* link, const double* wg1,
-                                  const double* wg2, const double* k11, const double* k12,
+template <class WT>
+__global__ void k_pressure_update(Geom g, State st, const int* cnt, const int* link, const WT* wg1,
+                                  const WT* wg2, const double* k11, const double* k12,
                                   const double* k21, const double* k22, double* gpx, double* gpy,
                                   const Scal* sc) {
-  int i = blockIdx.x * kBlock + threadIdx.x;
+  const int i = xcd_row(g.npfb);
   if (i >= g.npfb) return;
   const int npfb = g.npfb, n = cnt[i];
   const double pi = st.p[i];
@@ -961,8 +968,9 @@ __global__ void k_final(Geom g, State st, const double* usx, const double* usy, 
 // in slot order (binned rows have no neighbours: 0), and the terms of sum_i u_i.u_i for the RMS
 // velocity (:171-181).  The reference runs it between the velocity update and the final advection,
 // which changes neither u nor the lists and weights, so it may run after the step.
-__global__ void k_vorticity(Geom g, State st, const int* cnt, const int* link, const double* wg1,
-                            const double* wg2, double* vort, double* partials, double* red) {
+template <class WT>
+__global__ void k_vorticity(Geom g, State st, const int* cnt, const int* link, const WT* wg1,
+                            const WT* wg2, double* vort, double* partials, double* red) {
   const int npfb = g.npfb;
   RowRange rr = block_rows(npfb, gridDim.x);
   Red<1> R(red, npfb);
@@ -1012,8 +1020,9 @@ __device__ inline double block_sum_fixed(double v, double* sh) {
   return s;
 }
 
+template <class WT>
 __global__ void __launch_bounds__(kDiagThreads)
-k_circle_forces(Geom g, State st, const int* cnt, const int* link, const double* wg1, const double* wg2,
+k_circle_forces(Geom g, State st, const int* cnt, const int* link, const WT* wg1, const WT* wg2,
                 const double* k11, const double* k12, const double* k21, const double* k22, double cs,
                 double cp, double* out) {
   __shared__ double sh[kDiagThreads / 64];
@@ -1215,19 +1224,22 @@ vb_status launch_neighbours(vb_ctx* c) {
   if (s != VB_OK) return s;
   VB_LAUNCH(c, k_cell_scatter, grid_1d(np), np, c->cellkey, c->cellslot, c->ist, c->ip);
   VB_LAUNCH(c, k_cell_sort, grid_1d(g.nct), c->ist, c->ip, g.nct);
-  VB_LAUNCH(c, k_neighbours, grid_1d(c->npfb), c->g, c->st, c->cellkey, c->ist, c->ip, c->cnt, c->link,
+  VB_LAUNCH(c, k_neighbours, xcd_grid(c->npfb), c->g, c->st, c->cellkey, c->ist, c->ip, c->cnt, c->link,
             c->scal);
   return VB_OK;
 }
 
 vb_status launch_weights(vb_ctx* c) {
-  if (c->g.open)
-    VB_LAUNCH(c, k_weights<true>, grid_1d(c->npfb), c->g, c->st, c->cnt, c->link, c->wg1, c->wg2, c->wl,
-              c->k11, c->k12, c->k21, c->k22, c->conc, c->snx, c->sny, c->nsurf, c->pscale, c->scal);
-  else
-    VB_LAUNCH(c, k_weights<false>, grid_1d(c->npfb), c->g, c->st, c->cnt, c->link, c->wg1, c->wg2, c->wl,
-              c->k11, c->k12, c->k21, c->k22, nullptr, nullptr, nullptr, nullptr, nullptr, c->scal);
-  return VB_OK;
+  return with_weights(c, [&](auto* w1, auto* w2, auto* wl) -> vb_status {
+    using WT = weight_t<decltype(w1)>;
+    if (c->g.open)
+      VB_LAUNCH(c, (k_weights<true, WT>), xcd_grid(c->npfb), c->g, c->st, c->cnt, c->link, w1, w2, wl,
+                c->k11, c->k12, c->k21, c->k22, c->conc, c->snx, c->sny, c->nsurf, c->pscale, c->scal);
+    else
+      VB_LAUNCH(c, (k_weights<false, WT>), xcd_grid(c->npfb), c->g, c->st, c->cnt, c->link, w1, w2, wl,
+                c->k11, c->k12, c->k21, c->k22, nullptr, nullptr, nullptr, nullptr, nullptr, c->scal);
+    return VB_OK;
+  });
 }
 
 vb_status launch_freesurf(vb_ctx* c) {
@@ -1236,9 +1248,11 @@ vb_status launch_freesurf(vb_ctx* c) {
 }
 
 vb_status launch_shift(vb_ctx* c) {
-  VB_LAUNCH(c, k_shift, grid_1d(c->npfb), c->g, c->st, c->cnt, c->link, c->wg1, c->wg2, c->nsurf, c->snx,
-            c->sny, c->usx, c->usy, c->scal);
-  return VB_OK;
+  return with_weights(c, [&](auto* w1, auto* w2, auto*) -> vb_status {
+    VB_LAUNCH(c, k_shift<weight_t<decltype(w1)>>, xcd_grid(c->npfb), c->g, c->st, c->cnt, c->link, w1, w2,
+              c->nsurf, c->snx, c->sny, c->usx, c->usy, c->scal);
+    return VB_OK;
+  });
 }
 
 vb_status launch_predict(vb_ctx* c) {
@@ -1248,21 +1262,24 @@ vb_status launch_predict(vb_ctx* c) {
   const bool newtonian = p.fluid_model == 1;
   // the velocity gradient feeds only constitutive_eqn_evolve (and its mirror stresses): the
   // Newtonian build computes it and never reads it (predictor_mod.F90:35-46), so it is skipped
-  if (!newtonian) {
-    VB_LAUNCH(c, k_grad_logconf, grid_1d(c->npfb), c->g, c->st, c->cnt, c->link, c->wg1, c->wg2, c->k11,
-              c->k12, c->k21, c->k22, c->kick, use_kick, wi, p.ve_nonlinearity, c->scal);
-    VB_LAUNCH(c, k_mirror_stress, grid_1d(c->nmirror_host), c->g, c->st, c->scal);
-  }
   double cs = (p.beta) * sqrt(p.Pr / p.Ra);
   double cp = (1.0 - p.beta) * p.Pr / (p.Ra * p.El);
   double ct = 1.0 / sqrt(p.Ra * p.Pr);
-  if (newtonian)
-    VB_LAUNCH(c, k_rhs<true>, grid_1d(c->npfb), c->g, c->st, c->cnt, c->link, c->wg1, c->wg2, c->wl, c->k11,
-              c->k12, c->k21, c->k22, cs, cp, ct, c->u_alt, c->v_alt, c->th_alt, c->scal);
-  else
-    VB_LAUNCH(c, k_rhs<false>, grid_1d(c->npfb), c->g, c->st, c->cnt, c->link, c->wg1, c->wg2, c->wl, c->k11,
-              c->k12, c->k21, c->k22, cs, cp, ct, c->u_alt, c->v_alt, c->th_alt, c->scal);
-  return VB_OK;
+  return with_weights(c, [&](auto* w1, auto* w2, auto* wl) -> vb_status {
+    using WT = weight_t<decltype(w1)>;
+    if (!newtonian) {
+      VB_LAUNCH(c, k_grad_logconf<WT>, xcd_grid(c->npfb), c->g, c->st, c->cnt, c->link, w1, w2, c->k11,
+                c->k12, c->k21, c->k22, c->kick, use_kick, wi, p.ve_nonlinearity, c->scal);
+      VB_LAUNCH(c, k_mirror_stress, grid_1d(c->nmirror_host), c->g, c->st, c->scal);
+    }
+    if (newtonian)
+      VB_LAUNCH(c, (k_rhs<true, WT>), xcd_grid(c->npfb), c->g, c->st, c->cnt, c->link, w1, w2, wl, c->k11,
+                c->k12, c->k21, c->k22, cs, cp, ct, c->u_alt, c->v_alt, c->th_alt, c->scal);
+    else
+      VB_LAUNCH(c, (k_rhs<false, WT>), xcd_grid(c->npfb), c->g, c->st, c->cnt, c->link, w1, w2, wl, c->k11,
+                c->k12, c->k21, c->k22, cs, cp, ct, c->u_alt, c->v_alt, c->th_alt, c->scal);
+    return VB_OK;
+  });
 }
 
 vb_status launch_mirror_vel(vb_ctx* c) {
@@ -1278,9 +1295,11 @@ vb_status launch_mirror_vel(vb_ctx* c) {
 
 vb_status launch_pressure_update(vb_ctx* c) {
   if (c->nmirror_host > 0) VB_LAUNCH(c, k_mirror_p, grid_1d(c->nmirror_host), c->g, c->st, c->scal);
-  VB_LAUNCH(c, k_pressure_update, grid_1d(c->npfb), c->g, c->st, c->cnt, c->link, c->wg1, c->wg2, c->k11,
-            c->k12, c->k21, c->k22, c->gpx, c->gpy, c->scal);
-  return VB_OK;
+  return with_weights(c, [&](auto* w1, auto* w2, auto*) -> vb_status {
+    VB_LAUNCH(c, k_pressure_update<weight_t<decltype(w1)>>, xcd_grid(c->npfb), c->g, c->st, c->cnt, c->link,
+              w1, w2, c->k11, c->k12, c->k21, c->k22, c->gpx, c->gpy, c->scal);
+    return VB_OK;
+  });
 }
 
 vb_status launch_mirror_p(vb_ctx* c) {
@@ -1296,7 +1315,12 @@ vb_status launch_final(vb_ctx* c) {
 
 vb_status launch_secondary_properties(vb_ctx* c) {
   const int nb = spmv_grid(c->npfb);    // a neighbour sum: one row per thread
-  VB_LAUNCH_BS(c, k_vorticity, nb, kSpmvBlock, c->g, c->st, c->cnt, c->link, c->wg1, c->wg2, c->vort, c->partials, c->red);
+  vb_status s = with_weights(c, [&](auto* w1, auto* w2, auto*) -> vb_status {
+    VB_LAUNCH_BS(c, k_vorticity<weight_t<decltype(w1)>>, nb, kSpmvBlock, c->g, c->st, c->cnt, c->link, w1, w2,
+                 c->vort, c->partials, c->red);
+    return VB_OK;
+  });
+  if (s != VB_OK) return s;
   hipLaunchKernelGGL(k_fin_vorticity, dim3(1), dim3(kBlock), 0, c->stream, c->partials, nb, c->red, c->npfb,
                      c->scal);
   VB_HIP(c, hipGetLastError());
@@ -1311,10 +1335,13 @@ vb_status launch_circle_forces(vb_ctx* c) {
   const double cs = (p.beta) * sqrt(p.Pr / p.Ra);            // the coefficients of launch_predict
   // the Newtonian build has no polymeric stress (its c_p may be 0/0 there)
   const double cp = p.fluid_model == 1 ? 0.0 : (1.0 - p.beta) * p.Pr / (p.Ra * p.El);
-  hipLaunchKernelGGL(k_circle_forces, dim3(1), dim3(kDiagThreads), 0, c->stream, c->g, c->st, c->cnt, c->link,
-                     c->wg1, c->wg2, c->k11, c->k12, c->k21, c->k22, cs, cp, scal_field(c, offsetof(Scal, circ)));
-  VB_HIP(c, hipGetLastError());
-  return VB_OK;
+  return with_weights(c, [&](auto* w1, auto* w2, auto*) -> vb_status {
+    hipLaunchKernelGGL(k_circle_forces<weight_t<decltype(w1)>>, dim3(1), dim3(kDiagThreads), 0, c->stream, c->g,
+                       c->st, c->cnt, c->link, w1, w2, c->k11, c->k12, c->k21, c->k22, cs, cp,
+                       scal_field(c, offsetof(Scal, circ)));
+    VB_HIP(c, hipGetLastError());
+    return VB_OK;
+  });
 }
 
 vb_status launch_nusselt(vb_ctx* c, double scale) {
