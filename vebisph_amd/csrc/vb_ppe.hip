@@ -34,6 +34,11 @@
 namespace vb {
 
 constexpr double kOnePlusEps = 1.0 + 1.0e-15;   // M(i) + 1e-15 with M = 1, linear_solver.F90:55,318
+// 1 / (1 + 1e-15): with M = 1 the preconditioned directions p_inv = p / (1 + 1e-15) and s_inv are p and
+// s times this constant, so the five-kernel closed-domain iteration keeps no p_inv, s_inv vectors: it
+// gathers p and s and applies the constant to the SpMV output and the solution update (the same
+// quantities up to rounding; 24 bytes less traffic per row and iteration)
+constexpr double kInvOnePlusEps = 1.0 / kOnePlusEps;
 constexpr int kVerifyEvery = 8;                  // mixed path: iterations between convergence checks
 
 // SPMV_TRANSPOSE: y = A^T x through the transposed map (vb_operator.hip), for the adjoint solve
@@ -590,8 +595,11 @@ __global__ void k_bi_spmv_p(Mat A, Vecs w, RedBufs rb, const Scal* sc) {
   const int n = A.n;
   RowRange rr = block_rows(n, gridDim.x);
   Red<FUSED ? 2 : 1> R(rb.R(ST_SPMV_P), n);
+  const bool scaled = FUSED && !w.jac;           // closed five-kernel form: A p_inv = c A p
+  const double* x = scaled ? w.pn : w.pinv;
   for (int i = rr.begin + threadIdx.x; i < rr.end; i += blockDim.x) {
-    double bb = spmv_row<MODE>(A, i, w.pinv);
+    double bb = spmv_row<MODE>(A, i, x);
+    if (scaled) bb = bb * kInvOnePlusEps;
     w.v[i] = bb;
     R.add(0, i, w.rb[i] * bb);
     if (FUSED) R.add(FUSED ? 1 : 0, i, bb);
@@ -661,8 +669,11 @@ __global__ void k_bi_spmv_s(Mat A, Vecs w, RedBufs rb, const Scal* sc) {
   RowRange rr = block_rows(n, gridDim.x);
   const double mean = FUSED ? 0.0 : sc->mean_s;
   Red<2> R(rb.R(ST_SPMV_S), n);
+  const bool scaled = FUSED && !w.jac;           // closed five-kernel form: A s_inv = c A s
+  const double* x = scaled ? w.s : w.sinv;
   for (int i = rr.begin + threadIdx.x; i < rr.end; i += blockDim.x) {
-    double bb = spmv_row<MODE>(A, i, w.sinv);
+    double bb = spmv_row<MODE>(A, i, x);
+    if (scaled) bb = bb * kInvOnePlusEps;
     w.t[i] = bb;
     double si = w.s[i] - mean;
     R.add(0, i, bb * si);
@@ -786,13 +797,16 @@ __global__ void k_fin_xr(RedBufs rb, int nblk, Scal* sc) {
 // which reproduce the reference's operation order):
 //
 //   k_fu_p      [finish iteration k-1: convergence, rho, beta]  p = r + beta (p - omega v),
-//               p -= mean(p), p_inv = p/(1+1e-15)                              -> sum(p) terms
-//   k_bi_spmv_p v = A p_inv                                                    -> r_bar.v, sum(v)
-//   k_fu_s      alpha; s = r - alpha v, s -= mean(s), s_inv = s/(1+1e-15)     -> |s|^2 (before
+//               p -= mean(p)                                                   -> sum(p) terms
+//   k_bi_spmv_p v = A p_inv = c A p                                            -> r_bar.v, sum(v)
+//   k_fu_s      alpha; s = r - alpha v, s -= mean(s)                           -> |s|^2 (before
 //               remove_dc, as the reference's early-exit test)
-//   k_bi_spmv_s t = A s_inv                                                    -> t.s, t.t
-//   k_fu_xr     early exit on |s|, omega; x += alpha p_inv + omega s_inv; r = s - omega t
+//   k_bi_spmv_s t = A s_inv = c A s                                            -> t.s, t.t
+//   k_fu_xr     early exit on |s|, omega; x += alpha c p + omega c s; r = s - omega t
 //                                                                              -> |r|^2, r_bar.r, sum(r)
+//
+// (c = 1 / (1 + 1e-15), the closed domain's inverseM; the open domain's Jacobi directions
+// p_inv = p / (diag + 1e-15) and s_inv are stored by k_fu_p and k_fu_s and read in their place.)
 //
 // The two remove_dc means are not folded from the new vectors (that takes a kernel boundary each)
 // but follow from sums already folded: sum(p_k) = sum(r) + beta (sum(p_{k-1}) - omega sum(v)) and
@@ -872,7 +886,7 @@ __global__ void k_fu_p(int n, Vecs w, RedBufs rb, Scal* sc, int k, int fin) {
     const double pn = first ? r : r + beta * (p - omega * v);
     const double pp = pn - mean;
     w.pn[i] = pp;
-    w.pinv[i] = inv_m_t<OPEN>(w, i, pp);
+    if (OPEN) w.pinv[i] = inv_m_t<OPEN>(w, i, pp);   // closed: p_inv = c p is never stored
     R.add(0, i, pp);
   };
 #pragma unroll
@@ -901,7 +915,7 @@ __global__ void k_fu_s(int n, Vecs w, RedBufs rb, Scal* sc, int nspmv, int k) {
     R.add(0, i, s * s);
     const double sp = s - mean;
     w.s[i] = sp;
-    w.sinv[i] = inv_m_t<OPEN>(w, i, sp);
+    if (OPEN) w.sinv[i] = inv_m_t<OPEN>(w, i, sp);   // closed: s_inv = c s is never stored
   };
 #pragma unroll
   for (int u = 0; u < kPre; u++) if (i0 + u * bd < rr.end) row(i0 + u * bd, rv[u], vv[u]);
@@ -909,15 +923,21 @@ __global__ void k_fu_s(int n, Vecs w, RedBufs rb, Scal* sc, int nspmv, int k) {
   R.store(rb.P(ST_S));
 }
 
+// (OPEN: the directions p_inv, s_inv are stored; closed: they are c p and c s, c = 1 / (1 + 1e-15))
+template <bool OPEN>
 __global__ void k_fu_xr(int n, Vecs w, RedBufs rb, Scal* sc, int nspmv, int k) {
   if (finished(sc)) return;
   RowRange rr = block_rows(n, gridDim.x);
   const int i0 = rr.begin + threadIdx.x, bd = blockDim.x;
+  const double* pd = OPEN ? w.pinv : w.pn;
   double xv[kPre], pv[kPre], qv[kPre], sv[kPre], tv[kPre], bv[kPre];
 #pragma unroll
   for (int u = 0; u < kPre; u++) {
     const int i = i0 + u * bd;
-    if (i < rr.end) { xv[u] = w.x[i]; pv[u] = w.pinv[i]; qv[u] = w.sinv[i]; sv[u] = w.s[i]; tv[u] = w.t[i]; bv[u] = w.rb[i]; }
+    if (i < rr.end) {
+      xv[u] = w.x[i]; pv[u] = pd[i]; sv[u] = w.s[i]; tv[u] = w.t[i]; bv[u] = w.rb[i];
+      qv[u] = OPEN ? w.sinv[i] : sv[u];
+    }
   }
   double fs[2], ft[2];
   fold_n<2>(rb.P(ST_S), gridDim.x, nullptr, n, fs);          // |s|^2, sum(p)
@@ -926,20 +946,22 @@ __global__ void k_fu_xr(int n, Vecs w, RedBufs rb, Scal* sc, int nspmv, int k) {
   const double n2s0 = k == 1 ? ((n2s <= sc->n2_small) ? 1.0 : n2s) : sc->n2s0;
   const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
   const double alpha = sc->alpha;
+  const double ca = OPEN ? alpha : alpha * kInvOnePlusEps;   // the factor of the stored direction
   if (lead) {
     if (k == 1) sc->n2s0 = n2s0;
     sc->sum_p = fs[1];
   }
   if (!sc->verify_claims && n2s <= sc->tol * sc->tol * n2s0) {   // :374-381 (mixed: no unchecked exit)
-    for (int i = i0; i < rr.end; i += bd) w.x[i] = w.x[i] + alpha * w.pinv[i];
+    for (int i = i0; i < rr.end; i += bd) w.x[i] = w.x[i] + ca * pd[i];
     if (lead) { sc->resid = sqrt(n2s / n2s0); sc->exit_s = 1; }
     return;
   }
   const double omega = ft[0] / (ft[1] + 1.0e-15);
+  const double co = OPEN ? omega : omega * kInvOnePlusEps;
   if (lead) sc->omega = omega;
   Red<3> R(nullptr, n);
-  auto row = [&](int i, double x, double pinv, double sinv, double s, double t, double rbi) {
-    w.x[i] = x + alpha * pinv + omega * sinv;
+  auto row = [&](int i, double x, double pdir, double sdir, double s, double t, double rbi) {
+    w.x[i] = x + ca * pdir + co * sdir;
     const double r = s - omega * t;
     w.r[i] = r;
     R.add(0, i, r * r);
@@ -949,7 +971,8 @@ __global__ void k_fu_xr(int n, Vecs w, RedBufs rb, Scal* sc, int nspmv, int k) {
 #pragma unroll
   for (int u = 0; u < kPre; u++)
     if (i0 + u * bd < rr.end) row(i0 + u * bd, xv[u], pv[u], qv[u], sv[u], tv[u], bv[u]);
-  for (int i = i0 + kPre * bd; i < rr.end; i += bd) row(i, w.x[i], w.pinv[i], w.sinv[i], w.s[i], w.t[i], w.rb[i]);
+  for (int i = i0 + kPre * bd; i < rr.end; i += bd)
+    row(i, w.x[i], pd[i], OPEN ? w.sinv[i] : w.s[i], w.s[i], w.t[i], w.rb[i]);
   R.store(rb.P(ST_XR));
 }
 
@@ -1005,7 +1028,8 @@ static vb_status solve(vb_ctx* c, const Mat& A, Vecs w, int x_given, double tol,
         if (w.jac) VB_LAUNCH(c, k_fu_s<true>, nb, n, w, rb, sc, ns, k);
         else VB_LAUNCH(c, k_fu_s<false>, nb, n, w, rb, sc, ns, k);
         VB_LAUNCH_BS(c, (k_bi_spmv_s<MODE, true>), ns, kSpmvBlock, A, w, rb, sc);
-        VB_LAUNCH(c, k_fu_xr, nb, n, w, rb, sc, ns, k);
+        if (w.jac) VB_LAUNCH(c, k_fu_xr<true>, nb, n, w, rb, sc, ns, k);
+        else VB_LAUNCH(c, k_fu_xr<false>, nb, n, w, rb, sc, ns, k);
         if (check) {                 // returns at once unless a convergence claim is pending
           VB_LAUNCH_BS(c, k_bi_true_residual<MODE>, ns, kSpmvBlock, A, w, rb, sc);   // one row per thread
           VB_LAUNCH(c, k_bi_true_shift, nb, n, w, rb, sc, 1, ns, k);
