@@ -248,6 +248,12 @@ vb_status vb_nusselt(vb_ctx* ctx, double* nu);
 vb_status vb_get_scalars(vb_ctx* ctx, int32_t* ints12, double* dbls8);
 /* irelation, vrelation, dP_mp of the m mirrors (rows npfb+1..np), 1-based parents */
 vb_status vb_export_mirrors(vb_ctx* ctx, int32_t m, int32_t* irelation, int32_t* vrelation, double* dP_mp);
+/* The cell lists of the step's divide (neighbour_finding_mod.F90:56-127), as the reference holds them,
+ * 1-based: cellpart(1..np) the cell of each particle (0 for binned particles), ist(1..nct+1) the first
+ * position of each cell in ip (ist(nct+1) - 1 particles binned into cells), ip(1..np) the particles in
+ * cell order, stable within a cell (0 past the binned count).  np and nct1 = nct + 1 must equal the
+ * step's (vb_get_scalars). */
+vb_status vb_export_cells(vb_ctx* ctx, int32_t np, int32_t* cellpart, int32_t nct1, int32_t* ist, int32_t* ip);
 /* ij_count(1..n) and ij_link(1..n, 1..w), slot-major with leading dimension n, 1-based, 0-padded */
 vb_status vb_export_neighbours(vb_ctx* ctx, int32_t n, int32_t w, int32_t* ij_count, int32_t* ij_link);
 /* ij_w_G(n,w,2), ij_w_L(n,w) slot-major; kgcm as four columns K11,K12,K21,K22 (n each) */

@@ -59,6 +59,12 @@ def test_step_phases_against_oracle(gpu_lib, name):
                     else:
                         assert _rel(a, b) <= (0 if first else 1e-12)
             if k == 4:
+                # the cell lists of divide (the counting sort, stable within a cell) and the lists
+                cpg, istg, ipg = G.cells(); cpo, isto, ipo = O.cells()
+                nb = istg[-1] - 1
+                inb = G.get_state(len(cpg))["inbin"] != 0
+                _eq(istg, isto); _eq(ipg[:nb], ipo[:nb])
+                _eq(cpg[~inb], cpo[~inb])
                 cg, lg = G.neighbours(); co, lo = O.neighbours()
                 _eq(cg, co); _eq(lg, lo)
             if k == 5:

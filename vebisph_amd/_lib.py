@@ -25,7 +25,7 @@ EXPORTS = [
     "vb_mirror_velocities", "vb_ppe_solve", "vb_sparse_pbicgstab_nonull", "vb_sparse_pbicgstab",
     "vb_pressure_update",
     "vb_final_advection", "vb_div_free",
-    "vb_get_scalars", "vb_export_mirrors", "vb_export_neighbours", "vb_export_weights",
+    "vb_get_scalars", "vb_export_mirrors", "vb_export_cells", "vb_export_neighbours", "vb_export_weights",
     "vb_export_ushift", "vb_export_grad_p", "vb_export_ppe", "vb_secondary_properties",
     "vb_ppe_operator", "vb_grad_operator", "vb_ppe_adjoint_solve", "vb_adjoint_record",
     "vb_adjoint_backward", "vb_free_surface", "vb_field_maxima", "vb_circle_forces", "vb_nusselt", "vb_profile_spmv", "vb_profile_copy",
@@ -94,6 +94,7 @@ def lib():
         L.vb_sparse_pbicgstab.argtypes = L.vb_sparse_pbicgstab_nonull.argtypes
         L.vb_get_scalars.argtypes = [vp, ip, dp]
         L.vb_export_mirrors.argtypes = [vp, C.c_int32, ip, ip, dp]
+        L.vb_export_cells.argtypes = [vp, C.c_int32, ip, C.c_int32, ip, ip]
         L.vb_export_neighbours.argtypes = [vp, C.c_int32, C.c_int32, ip, ip]
         L.vb_export_weights.argtypes = [vp, C.c_int32, C.c_int32, dp, dp, dp]
         L.vb_export_ushift.argtypes = [vp, C.c_int32, dp]

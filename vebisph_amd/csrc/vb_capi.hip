@@ -457,6 +457,21 @@ vb_status vb_export_mirrors(vb_ctx* c, int32_t m, int32_t* irel, int32_t* vrel, 
   return VB_OK;
 }
 
+vb_status vb_export_cells(vb_ctx* c, int32_t np, int32_t* cellpart, int32_t nct1, int32_t* ist, int32_t* ip) {
+  if (!c || np != c->np_host || nct1 != c->g.nct + 1 || !cellpart || !ist || !ip) return VB_ERR_ARG;
+  VB_HIP(c, hipMemcpyAsync(cellpart, c->cellkey, (size_t)np * 4, hipMemcpyDeviceToHost, c->stream));
+  VB_HIP(c, hipMemcpyAsync(ist, c->ist, (size_t)nct1 * 4, hipMemcpyDeviceToHost, c->stream));
+  VB_HIP(c, hipMemcpyAsync(ip, c->ip, (size_t)np * 4, hipMemcpyDeviceToHost, c->stream));
+  VB_HIP(c, hipStreamSynchronize(c->stream));
+  const int32_t binned = ist[nct1 - 1];
+  for (int32_t k = 0; k < np; k++) {
+    cellpart[k] = cellpart[k] < 1 ? 0 : cellpart[k];
+    ip[k] = k < binned ? ip[k] + 1 : 0;
+  }
+  for (int32_t q = 0; q < nct1; q++) ist[q] += 1;
+  return VB_OK;
+}
+
 vb_status vb_export_neighbours(vb_ctx* c, int32_t n, int32_t w, int32_t* cnt, int32_t* link) {
   if (!c || n != c->npfb || w < 1 || w > kNplink) return VB_ERR_ARG;
   std::vector<int32_t> l((size_t)n * w);

@@ -201,6 +201,13 @@ module vebisph_gpu
        real(c_double), intent(out) :: dp_mp(*)
        integer(c_int) :: s
      end function vb_export_mirrors
+     function vb_export_cells(ctx, np, cellpart, nct1, ist, ip) bind(C, name="vb_export_cells") result(s)
+       import :: c_int, c_int32_t, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: np, nct1
+       integer(c_int32_t), intent(out) :: cellpart(*), ist(*), ip(*)
+       integer(c_int) :: s
+     end function vb_export_cells
      function vb_export_neighbours(ctx, n, w, ij_count, ij_link) bind(C, name="vb_export_neighbours") result(s)
        import :: c_int, c_int32_t, c_ptr
        type(c_ptr), value :: ctx

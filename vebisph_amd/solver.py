@@ -265,6 +265,15 @@ class Solver:
         self._chk(_lib.lib().vb_export_mirrors(self.h, m, _i(irel), _i(vrel), _d(dp)))
         return irel, vrel, dp
 
+    def cells(self):
+        """The step's cell lists, 1-based as the reference's divide leaves them: cellpart (np), ist
+        (nct + 1), ip (np; 0 past the binned count)."""
+        s = self.scalars()
+        np_, nct = s["np"], s["nct"]
+        cp = np.zeros(np_, np.int32); ist = np.zeros(nct + 1, np.int32); ip = np.zeros(np_, np.int32)
+        self._chk(_lib.lib().vb_export_cells(self.h, np_, _i(cp), nct + 1, _i(ist), _i(ip)))
+        return cp, ist, ip
+
     def neighbours(self, w=NPLINK):
         n = self.case.npfb
         cnt = np.zeros(n, np.int32); link = np.zeros(n * w, np.int32)
