@@ -1,6 +1,7 @@
 """The benchmark configurations at their own sizes against the reference itself: the flow past a
 cylinder at 1,003,417 particles (FP64 and mixed precision; fields and the force on the cylinder) and
-the Boussinesq convection box at 80,000 particles (400 x 200).
+the Boussinesq convection box at 80,000 particles (400 x 200); and the Poiseuille configuration over
+its whole 1000-step run against the oracle.
 
 The reference (oracle/_ref, its Fortran compiled from its own sources, OpenMP on the host cores)
 steps the same start state; its one-ulp spread -- the reference against itself from fluid positions
@@ -167,3 +168,49 @@ def test_convection_80k_against_reference(gpu_lib):
           f"mixed LS_iters {scmx['LS_iters']} residual {scmx['LS_residual']:.2e}", flush=True)
     assert sc64["np"] == rsc["np"] and scmx["np"] == rsc["np"]
     _compare(case, ref, spreads, {"fp64": g64, "mixed": gmx}, n)
+
+
+def _oracle_run(args):
+    """Worker: the oracle over nsteps from the case's start state, or from fluid positions moved by
+    one ulp (seed); returns the final state and the scalars."""
+    name, nsteps, seed = args
+    from oracle import oraclewrap as ow
+    case = cases.make(name)
+    O = ow.OracleSolver(case, state=None if seed is None else _ulp_start(case, seed))
+    for _ in range(nsteps):
+        O.step()
+    return O.get_state(), O.scalars()
+
+
+def test_poiseuille_2500_whole_run_against_oracle(gpu_lib):
+    """The Poiseuille configuration over its whole run, 1000 steps (2.6k particles, Oldroyd-B, Wi = 1,
+    beta = 0.1, Re = 0.1): the GPU in FP64 and in mixed precision against the oracle, with the oracle's
+    own one-ulp spread as the scale (both oracle runs in CPU worker processes, started before this
+    process uses the GPU)."""
+    import concurrent.futures as cf
+    import multiprocessing as mp
+    name, nsteps = "poiseuille_2500", 1000
+    with cf.ProcessPoolExecutor(max_workers=2, mp_context=mp.get_context("spawn")) as ex:
+        fut = [ex.submit(_oracle_run, (name, nsteps, seed)) for seed in (None, 1)]
+        case = cases.make(name)
+        runs, scs = {}, {}
+        for prec, label in ((_lib.PREC_FP64, "fp64"), (_lib.PREC_MIXED, "mixed")):
+            G = solver.Solver(case, precision=prec)
+            worst = 0
+            for _ in range(nsteps):
+                G.div_free()
+                st = G.scalars()["status"]
+                assert st in (0, 6), st
+                worst = max(worst, G.scalars()["LS_iters"])
+            runs[label], scs[label] = G.get_state(), G.scalars()
+            print(f"GPU {label}: time {scs[label]['time']:.6f}, n_inbin {scs[label]['n_inbin']}, "
+                  f"most BiCGSTAB iterations {worst}", flush=True)
+            G.close()
+        (ref, rsc), (twin, _) = fut[0].result(), fut[1].result()
+    print(f"oracle: time {rsc['time']:.6f}, LS_iters (last) {rsc['LS_iters']}", flush=True)
+    for label in runs:
+        assert scs[label]["n_inbin"] == rsc["n_inbin"] == 0
+        assert abs(scs[label]["time"] - rsc["time"]) <= 1e-10 * rsc["time"]      # dt: CFL or viscous limit
+    umax = np.max(np.abs(ref["u"][case.nbS:]))
+    print(f"centreline-scale velocity after {nsteps} steps: {umax:.6f}", flush=True)
+    _compare(case, ref, [twin], runs, case.npfb)

@@ -190,7 +190,7 @@ def test_bicgstab_parallel_iterates_follow_reference(gpu_lib):
 @pytest.mark.parametrize("name", list(SMALL))
 def test_mixed_precision_fields(gpu_lib, name):
     """The mixed path (FP32 kernel weights and PPE matvec values, FP64 BiCGSTAB vectors, sums and
-    refinement): ten steps against the FP64 oracle, held to ten times the oracle's one-ulp spread or
+    solution): ten steps against the FP64 oracle, held to ten times the oracle's one-ulp spread or
     mixed_floor, with the FP64 residual of the final solve checked on the assembled system."""
     from vebisph_amd import _lib
     case = SMALL[name]()

@@ -34,7 +34,7 @@ constexpr int kMaxBlocks = VB_MAX_BLOCKS;
 constexpr int kPartialStride = kSpmvMaxBlocks > kMaxBlocks ? kSpmvMaxBlocks : kMaxBlocks;
 static_assert(kSpmvBlock % 64 == 0 && kSpmvBlock <= 1024, "SpMV block");
 static_assert(kMaxBlocks % 8 == 0 && kSpmvMaxBlocks % 8 == 0, "grid caps");
-constexpr int kSolverStages = 7;           // BiCGSTAB reduction stages with their own buffers
+constexpr int kSolverStages = 6;           // BiCGSTAB reduction stages with their own buffers
 constexpr int kNplink = VB_NPLINK;
 constexpr double kEps = 2.220446049250313e-16;   // epsilon(1d0)
 
@@ -88,10 +88,6 @@ struct Scal {
   double rho, rho0, alpha, omega, beta, n2s0, n2r0, resid, mean_p, mean_s, tol;
   double n2_small;                         // first-iteration norm threshold (1e-6, or 4 when open)
   int iter, itermax, done, exit_s, status;
-  // mixed path: a convergence claim of the recurrence residual is checked against the FP64 residual
-  // (verify_claims: the solve does so; verify: a claim awaits the check; true_at: the iteration whose
-  // finish sees the FP64 residual)
-  int verify_claims, verify, true_at;
   // BiCGSTAB in five kernels per iteration: rho of iteration k in rho_pp[k & 1]; sums of r, v and
   // of the projected p, each recorded by the kernel that folds it
   double rho_pp[2], sum_r, sum_v, sum_p;

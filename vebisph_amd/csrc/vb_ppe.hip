@@ -17,13 +17,10 @@
 //
 // Mixed precision: the same BiCGSTAB with FP64 vectors, dot products, scalars and solution update,
 // and matvecs that read FP32 values on the compact rows: y_i = sum_k a32_k (x_j - x_i) +
-// rowsum_i x_i, the differences and the sum in FP64 (the FP64 row sum carries the diagonal), so the
-// only FP32 rounding is the values', a fixed perturbation of A.  The recurrence residual then drifts
-// from B - A x by that perturbation times the iterate, so a convergence claim of the recurrence is
-// checked against the FP64 residual: it replaces the recurrence residual, and the solve stops only if
-// that meets the tolerance.  (Replacing it periodically instead, every 10 to 200 iterations, was
-// measured: at 1M particles, where the solve runs to its iteration limit, the final FP64 residual was
-// 3-13x worse than with no periodic replacement, which tracks the FP64 solve.)
+// rowsum_i x_i, the differences and the sum in FP64 (the FP64 row sum carries the diagonal).  The
+// mixed path's kernel weights are FP32 (k_weights), so the compact values -w_L are exact in FP32: the
+// FP32 value planes are the assembled matrix itself, the SpMV differs from the FP64 rows' only in
+// rounding, and the recurrence residual tracks B - A x as in the FP64 solve.
 //
 // BiCGSTAB: all scalars (rho, alpha, omega, norms, means) stay on the device; the host only polls the
 // convergence flag every few iterations.  See the section comment below for the kernel sequence.
@@ -39,7 +36,6 @@ constexpr double kOnePlusEps = 1.0 + 1.0e-15;   // M(i) + 1e-15 with M = 1, line
 // gathers p and s and applies the constant to the SpMV output and the solution update (the same
 // quantities up to rounding; 24 bytes less traffic per row and iteration)
 constexpr double kInvOnePlusEps = 1.0 / kOnePlusEps;
-constexpr int kVerifyEvery = 8;                  // mixed path: iterations between convergence checks
 
 // SPMV_TRANSPOSE: y = A^T x through the transposed map (vb_operator.hip), for the adjoint solve
 enum SpmvMode { SPMV_FP64 = 0, SPMV_MIXED = 1, SPMV_TRANSPOSE = 2 };
@@ -498,8 +494,8 @@ __device__ inline double inv_m_t(const Vecs& w, int i, double x) {
   return OPEN ? x / (w.jac[i] + 1.0e-15) : x / kOnePlusEps;
 }
 
-enum { ST_P = 0, ST_SPMV_P = 1, ST_S = 2, ST_SPMV_S = 3, ST_XR = 4, ST_INIT = 5, ST_TRUE = 6 };
-static_assert(ST_TRUE < kSolverStages, "stage buffers");
+enum { ST_P = 0, ST_SPMV_P = 1, ST_S = 2, ST_SPMV_S = 3, ST_XR = 4, ST_INIT = 5 };
+static_assert(ST_INIT < kSolverStages, "stage buffers");
 
 struct RedBufs {
   double* partials;     // kStages * 2 * kPartialStride
@@ -539,8 +535,7 @@ __global__ void k_bi_init(Mat A, Vecs w, int x_given, RedBufs rb) {
 
 // small: the first-iteration norm threshold, 1.0d-6 (nonull, :367, :404) or the integer expression
 // 10-6 = 4 (open, :171, :207)
-// (verify: check convergence claims against the FP64 residual, the mixed path)
-__global__ void k_fin_init(RedBufs rb, int nblk, double tol, int itermax, double small, int verify, Scal* sc) {
+__global__ void k_fin_init(RedBufs rb, int nblk, double tol, int itermax, double small, Scal* sc) {
   double f[2];
   fold_n<2>(rb.P(ST_INIT), nblk, rb.R(ST_INIT), rb.n, f);
   const double rho = f[0];
@@ -550,7 +545,6 @@ __global__ void k_fin_init(RedBufs rb, int nblk, double tol, int itermax, double
     sc->iter = 1; sc->itermax = itermax; sc->exit_s = 0; sc->status = VB_OK;
     sc->mean_p = 0.0; sc->mean_s = 0.0; sc->sum_v = 0.0; sc->sum_p = 0.0;
     sc->done = 0;
-    sc->verify_claims = verify; sc->verify = 0; sc->true_at = -1;
     if (itermax < 1) { sc->done = 1; sc->status = VB_WARN_MAXITER; }
     else if (rho == 0.0) { sc->done = 1; sc->status = VB_WARN_BICGSTAB_BREAKDOWN; }
   }
@@ -647,7 +641,7 @@ __global__ void k_bi_sinv(int n, Vecs w, RedBufs rb, Scal* sc) {
   const bool first = sc->iter == 1;
   const double n2s0 = first ? ((n2s <= sc->n2_small) ? 1.0 : n2s) : sc->n2s0;
   const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
-  if (!sc->verify_claims && n2s <= sc->tol * sc->tol * n2s0) {   // (mixed: no unchecked exit on s)
+  if (n2s <= sc->tol * sc->tol * n2s0) {
     const double alpha = sc->alpha;
     for (int i = i0; i < rr.end; i += bd) w.x[i] = w.x[i] + alpha * w.pinv[i];
     if (lead) { sc->n2s0 = n2s0; sc->resid = sqrt(n2s / n2s0); sc->exit_s = 1; }
@@ -715,57 +709,6 @@ __global__ void k_bi_xr(int n, Vecs w, RedBufs rb, Scal* sc, int nspmv) {
   R.store(rb.P(ST_XR));
 }
 
-// Mixed path only, when a convergence claim awaits its check (else both kernels return at once):
-// replace the recurrence residual by the FP64 residual B - A x.  The recurrence residual differs
-// from B - A x by a multiple of the constant vector (the DC projections of p and s remove constants
-// the recurrence never adds back), and A's columns need not sum to zero, so B - A x carries a
-// constant component the projected iteration cannot reduce.  The replacement therefore keeps the
-// recurrence residual's mean: r = (B - A x) - mean(B - A x) + mean(r).  Pass 1 writes B - A x to t
-// and the terms of sum(r), sum(B - A x); pass 2 shifts and forms |r|^2 and r_bar . r, and records
-// that the finish of iteration k sees the FP64 residual.
-// (MODE: SPMV_TRANSPOSE for the adjoint solve's A^T, else the FP64 rows of A)
-template <int MODE>
-__global__ void k_bi_true_residual(Mat A, Vecs w, RedBufs rb, const Scal* sc) {
-  if (finished(sc) || !sc->verify) return;
-  const int n = A.n;
-  RowRange rr = block_rows(n, gridDim.x);
-  Red<2> R(rb.R(ST_TRUE), n);
-  for (int i = rr.begin + threadIdx.x; i < rr.end; i += blockDim.x) {
-    const double ax = MODE == SPMV_TRANSPOSE ? spmv_row_transpose(A, i, w.x) : spmv_row_fp64(A, i, w.x);
-    const double rt = w.b[i] - ax;
-    R.add(0, i, w.r[i]);
-    R.add(1, i, rt);
-    w.t[i] = rt;
-  }
-  R.store(rb.P(ST_TRUE));
-}
-
-// (with_sum: also the terms of sum(r), which the five-kernel iteration folds after x,r)
-// (ntrue: workgroups of the pass-1 launch whose partials are folded)
-// The iteration restarts from there, as a refinement step: r_bar = r and p = v = 0, so the next
-// direction is r itself (the Krylov space of the recurrence was built on the drifted residual).
-// (k: the iteration whose x, r this replaces)
-__global__ void k_bi_true_shift(int n, Vecs w, RedBufs rb, Scal* sc, int with_sum, int ntrue, int k) {
-  if (finished(sc) || !sc->verify) return;
-  if (blockIdx.x == 0 && threadIdx.x == 0) { sc->true_at = k; sc->sum_p = 0.0; sc->sum_v = 0.0; sc->mean_p = 0.0; }
-  double f[2];
-  fold_n<2>(rb.P(ST_TRUE), ntrue, rb.R(ST_TRUE), n, f);
-  const double d = w.jac ? 0.0 : (f[0] - f[1]) / (double)n;   // open domain: no projections, r = B - A x
-  RowRange rr = block_rows(n, gridDim.x);
-  Red<3> R(rb.R(ST_XR), n);
-  for (int i = rr.begin + threadIdx.x; i < rr.end; i += blockDim.x) {
-    const double r = w.t[i] + d;
-    w.r[i] = r;
-    w.rb[i] = r;
-    w.pn[i] = 0.0;
-    w.v[i] = 0.0;
-    R.add(0, i, r * r);
-    R.add(1, i, r * r);
-    if (with_sum) R.add(2, i, r);
-  }
-  R.store(rb.P(ST_XR));
-}
-
 // :403-419 convergence; next rho (:330) and beta (:342-343)
 __global__ void k_fin_xr(RedBufs rb, int nblk, Scal* sc) {
   if (finished(sc)) return;
@@ -775,13 +718,9 @@ __global__ void k_fin_xr(RedBufs rb, int nblk, Scal* sc) {
   if (threadIdx.x == 0) {
     if (sc->iter == 1) sc->n2r0 = (n2r <= sc->n2_small) ? 1.0 : n2r;
     sc->resid = sqrt(n2r / sc->n2r0);
-    const bool claimed = n2r <= sc->tol * sc->tol * sc->n2r0;
-    const bool checked = !sc->verify_claims || sc->true_at == sc->iter;
-    if (sc->true_at == sc->iter) sc->verify = 0;
-    if (claimed && checked) {
+    if (n2r <= sc->tol * sc->tol * sc->n2r0) {
       sc->done = 1;
     } else {
-      if (claimed) sc->verify = 1;
       sc->iter = sc->iter + 1;
       sc->rho0 = sc->rho;
       sc->rho = rho;
@@ -832,11 +771,7 @@ __device__ inline FinState finish_iteration(const double (&f)[3], int k, Scal* s
   const double n2r0 = k == 1 ? ((n2r <= sc->n2_small) ? 1.0 : n2r) : sc->n2r0;
   FinState o{false, 0.0, f[2]};
   int status = VB_OK;
-  // a claim of the recurrence residual converges only once checked (mixed path: the FP64 residual
-  // replaced it for this iteration)
-  const bool claimed = n2r <= sc->tol * sc->tol * n2r0;
-  const bool checked = !sc->verify_claims || sc->true_at == k;
-  const bool converged = claimed && checked;
+  const bool converged = n2r <= sc->tol * sc->tol * n2r0;
   const double rho0 = sc->rho_pp[k & 1];
   if (converged) o.done = true;
   else if (k + 1 > sc->itermax) { o.done = true; status = VB_WARN_MAXITER; }
@@ -847,8 +782,6 @@ __device__ inline FinState finish_iteration(const double (&f)[3], int k, Scal* s
     sc->resid = sqrt(n2r / n2r0);
     if (!converged) { sc->iter = k + 1; sc->rho0 = rho0; sc->rho = rho; sc->rho_pp[(k + 1) & 1] = rho; }
     sc->beta = o.beta; sc->sum_r = o.sum_r;
-    if (sc->true_at == k) sc->verify = 0;
-    if (claimed && !converged) sc->verify = 1;
     if (o.done) { sc->done = 1; sc->status = status; }
   }
   return o;
@@ -951,7 +884,7 @@ __global__ void k_fu_xr(int n, Vecs w, RedBufs rb, Scal* sc, int nspmv, int k) {
     if (k == 1) sc->n2s0 = n2s0;
     sc->sum_p = fs[1];
   }
-  if (!sc->verify_claims && n2s <= sc->tol * sc->tol * n2s0) {   // :374-381 (mixed: no unchecked exit)
+  if (n2s <= sc->tol * sc->tol * n2s0) {                     // :374-381
     for (int i = i0; i < rr.end; i += bd) w.x[i] = w.x[i] + ca * pd[i];
     if (lead) { sc->resid = sqrt(n2s / n2s0); sc->exit_s = 1; }
     return;
@@ -1012,15 +945,11 @@ static vb_status solve(vb_ctx* c, const Mat& A, Vecs w, int x_given, double tol,
   const bool fused = rb.rows == nullptr;   // serial mode: the reference's eight-kernel order
   const double small = w.jac ? (double)(10 - 6) : 1.0e-6;
   VB_LAUNCH(c, k_bi_init, nb, A, w, x_given, rb);
-  VB_FIN(c, k_fin_init, rb, nb, tol, itermax, small, MODE == SPMV_MIXED ? 1 : 0, sc);
+  VB_FIN(c, k_fin_init, rb, nb, tol, itermax, small, sc);
   int launched = 0, batch = 8;
-  // mixed: the check slots are launched once the polled residual is within 1e3 of the tolerance (a
-  // claim cannot come earlier; at 1M particles, where the solve runs to its limit, never)
-  double polled = 1.0;
   for (;;) {
     for (int b = 0; b < batch; b++) {
       const int k = launched + b + 1;
-      const bool check = MODE == SPMV_MIXED && polled <= 1.0e3 * tol && k % kVerifyEvery == 0;
       if (fused) {
         if (w.jac) VB_LAUNCH(c, k_fu_p<true>, nb, n, w, rb, sc, k, b > 0 ? 1 : 0);
         else VB_LAUNCH(c, k_fu_p<false>, nb, n, w, rb, sc, k, b > 0 ? 1 : 0);
@@ -1030,10 +959,6 @@ static vb_status solve(vb_ctx* c, const Mat& A, Vecs w, int x_given, double tol,
         VB_LAUNCH_BS(c, (k_bi_spmv_s<MODE, true>), ns, kSpmvBlock, A, w, rb, sc);
         if (w.jac) VB_LAUNCH(c, k_fu_xr<true>, nb, n, w, rb, sc, ns, k);
         else VB_LAUNCH(c, k_fu_xr<false>, nb, n, w, rb, sc, ns, k);
-        if (check) {                 // returns at once unless a convergence claim is pending
-          VB_LAUNCH_BS(c, k_bi_true_residual<MODE>, ns, kSpmvBlock, A, w, rb, sc);   // one row per thread
-          VB_LAUNCH(c, k_bi_true_shift, nb, n, w, rb, sc, 1, ns, k);
-        }
         continue;
       }
       VB_LAUNCH(c, k_bi_p, nb, n, w, rb, sc);
@@ -1043,10 +968,6 @@ static vb_status solve(vb_ctx* c, const Mat& A, Vecs w, int x_given, double tol,
       VB_LAUNCH(c, k_bi_sinv, nb, n, w, rb, sc);
       VB_LAUNCH_BS(c, k_bi_spmv_s<MODE>, ns, kSpmvBlock, A, w, rb, sc);
       VB_LAUNCH(c, k_bi_xr, nb, n, w, rb, sc, ns);
-      if (check) {
-        VB_LAUNCH_BS(c, k_bi_true_residual<MODE>, ns, kSpmvBlock, A, w, rb, sc);
-        VB_LAUNCH(c, k_bi_true_shift, nb, n, w, rb, sc, 0, ns, k);
-      }
       VB_FIN(c, k_fin_xr, rb, nb, sc);
     }
     if (fused) VB_FIN(c, k_fu_fin, rb, nb, sc, launched + batch);
@@ -1055,7 +976,6 @@ static vb_status solve(vb_ctx* c, const Mat& A, Vecs w, int x_given, double tol,
     VB_HIP(c, hipStreamSynchronize(c->stream));
     if (c->scal_host->done || c->scal_host->exit_s) break;
     if (launched > itermax + 2) break;
-    polled = c->scal_host->resid;
     batch = launched < 32 ? 8 : 32;
   }
   return VB_OK;
