@@ -65,7 +65,7 @@ static void free_all(vb_ctx* c) {
                   c->st.irel, c->st.vrel, c->st.dP, c->u_alt, c->v_alt, c->th_alt, c->nsurf,
                   c->seg_counts, c->cellkey, c->cellslot, c->nc, c->ist, c->ip, c->scan_bsum, c->cnt, c->link,
                   c->wg1, c->wg2, c->wl, c->wg1f, c->wg2f, c->wlf, c->k11, c->k12, c->k21, c->k22,
-                  c->usx, c->usy, c->kick, c->gpx, c->gpy, c->vort, c->conc, c->snx, c->sny, c->pscale, c->a_diag, c->a_val, c->a_rowsum, c->rhs,
+                  c->usx, c->usy, c->kick, c->gpx, c->gpy, c->vort, c->conc, c->snx, c->sny, c->pscale, c->a_diag, c->a_val, c->rhs,
                   c->a_col, c->a_len, c->rowflag, c->chunkg, c->choff, c->xflag, c->xscan, c->xrows,
                   c->lnk4, c->val4, c->val4f,
                   c->r, c->rb, c->vv,
@@ -199,7 +199,7 @@ vb_status vb_create(const vb_params* prm, const vb_boundary* bnd, int32_t npfb, 
   A(c->k11, n); A(c->k12, n); A(c->k21, n); A(c->k22, n);
   A(c->usx, n); A(c->usy, n); A(c->kick, n); A(c->gpx, n); A(c->gpy, n); A(c->vort, n); A(c->conc, n);
   if (prm->i_open_domain == 1) { A(c->snx, n); A(c->sny, n); A(c->pscale, n); }
-  A(c->a_diag, n); A(c->a_val, nw); A(c->a_rowsum, n); A(c->rhs, n); A(c->a_col, nw); A(c->a_len, n);
+  A(c->a_diag, n); A(c->a_val, nw); A(c->rhs, n); A(c->a_col, nw); A(c->a_len, n);
   // compact grouped rows: up to VB_NPLINK slots rounded up to a multiple of 4
   // compact rows: at most VB_NPLINK slots rounded up to groups of 4, for every row of every chunk
   const size_t nch = ((size_t)npfb + 63) / 64;

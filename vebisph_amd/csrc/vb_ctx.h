@@ -65,7 +65,7 @@ struct vb_ctx {
   double* pscale;                    // open domain: PPE row scale, ppe_smooth or 1 (surface rows)
 
   // PPE (ELL, slot-major) and BiCGSTAB vectors
-  double *a_diag, *a_val, *a_rowsum, *rhs;
+  double *a_diag, *a_val, *rhs;
   int *a_col, *a_len;                // a_len: row lengths of a caller-supplied system
   uint8_t* rowflag;                  // 1: SpMV uses the full stored row (mirror neighbours)
   int *xflag, *xscan, *xrows;        // the rows with rowflag 1, listed for their assembly

@@ -16,11 +16,10 @@
 // divisions and square root, ~1.8x slower.)
 //
 // Mixed precision: the same BiCGSTAB with FP64 vectors, dot products, scalars and solution update,
-// and matvecs that read FP32 values on the compact rows: y_i = sum_k a32_k (x_j - x_i) +
-// rowsum_i x_i, the differences and the sum in FP64 (the FP64 row sum carries the diagonal).  The
-// mixed path's kernel weights are FP32 (k_weights), so the compact values -w_L are exact in FP32: the
-// FP32 value planes are the assembled matrix itself, the SpMV differs from the FP64 rows' only in
-// rounding, and the recurrence residual tracks B - A x as in the FP64 solve.
+// and matvecs that read FP32 values on the compact rows and sum in FP64.  The mixed path's kernel
+// weights are FP32 (k_weights), so the compact values -w_L are exact in FP32: the FP32 value planes
+// are the assembled matrix itself, the SpMV differs from the FP64 rows' only in rounding, and the
+// recurrence residual tracks B - A x as in the FP64 solve.
 //
 // BiCGSTAB: all scalars (rho, alpha, omega, norms, means) stay on the device; the host only polls the
 // convergence flag every few iterations.  See the section comment below for the kernel sequence.
@@ -88,13 +87,13 @@ __host__ __device__ inline size_t grouped(int k, int i, int n) { return ((size_t
 constexpr int kChunk = 64;
 
 // Assembly, :60-122.  One thread per row (a wavefront = one chunk of 64 rows).  Writes the full
-// stored rows, the diagonal, row sum and right-hand side, marks the rows the compact SpMV can take
+// stored rows, the diagonal and right-hand side, marks the rows the compact SpMV can take
 // (rowflag 0: no mirror neighbour, every neighbour within 32767 rows) and records per chunk the
 // groups its compact rows need (chunkg); k_assemble_sliced then writes the compact rows.
 template <class WT>
 __global__ void k_assemble(Geom g, State st, const int* cnt, const int* link, const WT* wl,
                            const int* nsurf, const double* pscale, double* rhs, double* a_diag, int* a_col,
-                           double* a_val, uint8_t* rowflag, double* rowsum, int* chunkg, const Scal* sc) {
+                           double* a_val, uint8_t* rowflag, int* chunkg, const Scal* sc) {
   const int i = xcd_row(g.npfb);
   const int npfb = g.npfb;
   int ngroups = 0;
@@ -126,19 +125,16 @@ __global__ void k_assemble(Geom g, State st, const int* cnt, const int* link, co
   if (ps != 1.0) explicit_row = true;
   bi = bi * ps;
   double d = 0.0 + adiag;
-  double rs = d;
   if (!explicit_row) {
     for (int k = 0; k < n; k++) {
       int s = k * npfb + i;
       double val = 0.0 + (-wl[s]);
       a_col[s] = link[s];
       a_val[s] = val;
-      rs = rs + val;
     }
     ngroups = (n + 3) >> 2;
-    rowsum[i] = rs;
   }
-  // rows with mirror neighbours: k_assemble_explicit writes their slots and row sum
+  // rows with mirror neighbours: k_assemble_explicit writes their slots
   a_diag[i] = d;
   rowflag[i] = explicit_row ? 1 : 0;
   rhs[i] = bi;
@@ -160,10 +156,10 @@ template <class WT>
 __global__ void __launch_bounds__(kExplicitThreads)
 k_assemble_explicit(int npfb, const int* rows, int nrows, const int* cnt, const int* link,
                     const WT* wl, const int* irel, const double* pscale, const double* a_diag, int* a_col,
-                    double* a_val, double* rowsum) {
+                    double* a_val) {
   if ((int)blockIdx.x >= nrows) return;
   __shared__ int cols[kExplicitThreads];
-  __shared__ double mw[kExplicitThreads], vals[kExplicitThreads];
+  __shared__ double mw[kExplicitThreads];
   const int i = rows[blockIdx.x], n = cnt[i], k = threadIdx.x;
   int ck = -1;
   if (k < n) {
@@ -187,13 +183,6 @@ k_assemble_explicit(int npfb, const int* rows, int nrows, const int* cnt, const 
     }
     a_col[k * npfb + i] = ck;
     a_val[k * npfb + i] = val;
-    vals[k] = val;
-  }
-  __syncthreads();
-  if (k == 0) {
-    double rs = a_diag[i];
-    for (int k2 = 0; k2 < n; k2++) rs = rs + vals[k2];
-    rowsum[i] = rs;
   }
 }
 
@@ -283,7 +272,7 @@ vb_status launch_transpose_sliced(vb_ctx* c) {
 // SpMV rows.  x_j = (src_j - mean) / (1 + 1e-15) reproduces remove_dc + inverseM exactly.
 // ------------------------------------------------------------------------------------------------
 struct Mat {
-  const double *diag, *val, *rowsum;
+  const double *diag, *val;
   const int *col, *len;
   const uint8_t* rowflag;          // null: every row uses the full stored row
   // compact rows, sliced by chunks of 64 rows (layout at kChunk)
@@ -388,20 +377,19 @@ __device__ inline double spmv_row_fp64_v(const Mat& A, const double2* __restrict
 }
 
 // ------------------------------------------------------------------------------------------------
-// Mixed path SpMV rows.  Compact rows: the FP32 values applied to FP32 differences of the FP64
-// vector, y_i = sum_k a32_k (x_j - x_i) + rowsum_i x_i with the FP64 row sum carrying the diagonal,
-// so the vector's DC part and smooth components do not cancel in FP32.  Same three-stage pipeline
-// as the FP64 row; padded slots hold offset 0 and value 0.  Rows with mirror neighbours use the
-// full FP64 row.  (Measured alternatives: recomputing the FP32 entries from the pair distance,
-// three FP64 gathers per slot, was 1.5x slower than reading 4 bytes per slot; folding the p and s
-// projections into the SpMV, which then folds the partial sums in every one of its workgroups,
-// made the SpMV 1.5x slower than the separate projection kernels.)
+// Mixed path SpMV rows.  Compact rows: y_i = diag_i x_i + sum_k a32_k x_j in FP64, the values read
+// as FP32 (4 bytes per slot instead of 8).  The mixed path's kernel weights are FP32, so the values
+// -w_L are exact in FP32 and this is the FP64 compact row's sum on the same numbers (with fused
+// multiply-adds).  Same three-stage pipeline as the FP64 row; padded slots hold offset 0 and value 0.
+// Rows with mirror neighbours use the full FP64 row.  (Measured alternatives: recomputing the FP32
+// entries from the pair distance, three FP64 gathers per slot, was 1.5x slower than reading 4 bytes
+// per slot; folding the p and s projections into the SpMV, which then folds the partial sums in
+// every one of its workgroups, made the SpMV 1.5x slower than the separate projection kernels.)
 // ------------------------------------------------------------------------------------------------
 __device__ inline double row_f32(const Mat& A, int i, int m, const double* __restrict__ v, double xhat_i) {
   const int ng = (m + 3) >> 2;
   const size_t P = (size_t)A.choff[i / kChunk], lane = (size_t)(i % kChunk);
   auto gs = [&](size_t g) { return (P + g) * kChunk + lane; };
-  const double vi = v[i];
   const short4 zo = make_short4(0, 0, 0, 0);
   const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
   const short4 o0 = ld_nt(A.lnk4 + gs(0));
@@ -410,21 +398,21 @@ __device__ inline double row_f32(const Mat& A, int i, int m, const double* __res
   if (ng > 1) { o1 = ld_nt(A.lnk4 + gs(1)); a1 = ld_nt(A.val4f + gs(1)); }
   double g0[4], g1[4] = {0.0, 0.0, 0.0, 0.0};
   g0[0] = v[i + o0.x]; g0[1] = v[i + o0.y]; g0[2] = v[i + o0.z]; g0[3] = v[i + o0.w];
-  double acc = 0.0;
+  double acc = A.diag[i] * xhat_i;
   for (int gi = 0; gi < ng; gi++) {
     if (gi + 2 < ng) {
       const size_t g2 = (size_t)(gi + 2);
       o2 = ld_nt(A.lnk4 + gs(g2)); a2 = ld_nt(A.val4f + gs(g2));
     }
     if (gi + 1 < ng) { g1[0] = v[i + o1.x]; g1[1] = v[i + o1.y]; g1[2] = v[i + o1.z]; g1[3] = v[i + o1.w]; }
-    acc = fma((double)a0.x, g0[0] - vi, acc);
-    acc = fma((double)a0.y, g0[1] - vi, acc);
-    acc = fma((double)a0.z, g0[2] - vi, acc);
-    acc = fma((double)a0.w, g0[3] - vi, acc);
+    acc = fma((double)a0.x, g0[0], acc);
+    acc = fma((double)a0.y, g0[1], acc);
+    acc = fma((double)a0.z, g0[2], acc);
+    acc = fma((double)a0.w, g0[3], acc);
     a0 = a1; a1 = a2; o1 = o2;
     g0[0] = g1[0]; g0[1] = g1[1]; g0[2] = g1[2]; g0[3] = g1[3];
   }
-  return acc + A.rowsum[i] * xhat_i;
+  return acc;
 }
 
 __device__ inline double spmv_row_fp64(const Mat& A, int i, const double* __restrict__ x) {
@@ -925,7 +913,7 @@ __global__ void k_fu_fin(RedBufs rb, int nblk, Scal* sc, int k) {
 
 static Mat step_matrix(vb_ctx* c) {
   Mat A;
-  A.diag = c->a_diag; A.val = c->a_val; A.rowsum = c->a_rowsum; A.col = c->a_col; A.len = c->cnt;
+  A.diag = c->a_diag; A.val = c->a_val; A.col = c->a_col; A.len = c->cnt;
   A.rowflag = c->rowflag; A.lnk4 = (const short4*)c->lnk4; A.val4 = (const double2*)c->val4;
   A.val4f = (const float4*)c->val4f; A.choff = c->choff; A.n = c->npfb;
   A.tval = c->op.tval; A.mval = c->op.mval; A.tcol = c->op.tcol; A.moff = c->op.moff; A.mrow = c->op.mrow;
@@ -999,7 +987,7 @@ vb_status assemble_ppe(vb_ctx* c) {
   VB_FIN(c, k_fin_ppe_rhs, c->partials, nb, c->red, c->npfb, c->g.open, c->scal);
   sw = with_weights(c, [&](auto*, auto*, auto* wl) -> vb_status {
     VB_LAUNCH(c, k_assemble<weight_t<decltype(wl)>>, xcd_grid(c->npfb), c->g, c->st, c->cnt, c->link, wl,
-              c->nsurf, c->pscale, c->rhs, c->a_diag, c->a_col, c->a_val, c->rowflag, c->a_rowsum, c->chunkg,
+              c->nsurf, c->pscale, c->rhs, c->a_diag, c->a_col, c->a_val, c->rowflag, c->chunkg,
               c->scal);
     return VB_OK;
   });
@@ -1019,7 +1007,7 @@ vb_status assemble_ppe(vb_ctx* c) {
     sw = with_weights(c, [&](auto*, auto*, auto* wl) -> vb_status {
       hipLaunchKernelGGL(k_assemble_explicit<weight_t<decltype(wl)>>, dim3(nx), dim3(kExplicitThreads), 0,
                          c->stream, c->npfb, c->xrows, nx, c->cnt, c->link, wl, c->st.irel, c->pscale, c->a_diag,
-                         c->a_col, c->a_val, c->a_rowsum);
+                         c->a_col, c->a_val);
       VB_HIP(c, hipGetLastError());
       return VB_OK;
     });
