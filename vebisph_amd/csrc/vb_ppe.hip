@@ -429,6 +429,26 @@ __device__ inline double spmv_row_f32(const Mat& A, int i, const double* __restr
 // Row c of A^T: rows without mirror neighbours as compact FP64 rows (the forward row's offsets, the
 // A^T value plane, column order), the others from the transposed map in c's slot order; then c's
 // mirror bucket.
+// acc + sum over q in [0, m) of val(q) x[idx(q)], in q order, with eight entries' loads (value and
+// index, then the gathers) in flight before they are summed: each entry is otherwise a dependent
+// chain of two L2 round trips, and a wall row's mirror bucket holds tens of entries
+template <class V, class I>
+__device__ inline double gather_sum(double acc, int m, V val, I idx, const double* __restrict__ x) {
+  int q = 0;
+  for (; q + 8 <= m; q += 8) {
+    double v[8], g[8];
+    int j[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) { v[u] = val(q + u); j[u] = idx(q + u); }
+#pragma unroll
+    for (int u = 0; u < 8; u++) g[u] = x[j[u]];
+#pragma unroll
+    for (int u = 0; u < 8; u++) acc = acc + v[u] * g[u];
+  }
+  for (; q < m; q++) acc = acc + val(q) * x[idx(q)];
+  return acc;
+}
+
 __device__ inline double spmv_row_transpose(const Mat& A, int c, const double* __restrict__ x) {
   // the bucket bounds first: their loads overlap the row's chain instead of following it
   const int pb = A.moff[c], pe = A.moff[c + 1];
@@ -436,15 +456,11 @@ __device__ inline double spmv_row_transpose(const Mat& A, int c, const double* _
   if (A.rowflag && !A.rowflag[c]) {
     acc = spmv_row_fp64_v(A, A.val4T, c, x);
   } else {
-    const int n = A.n, m = A.len[c];
-    acc = A.diag[c] * x[c];
-    for (int k = 0; k < m; k++) {
-      const size_t s = (size_t)k * n + c;
-      acc = acc + A.tval[s] * x[A.tcol[s]];
-    }
+    const size_t n = A.n;
+    acc = gather_sum(A.diag[c] * x[c], A.len[c], [&](int k) { return A.tval[(size_t)k * n + c]; },
+                     [&](int k) { return A.tcol[(size_t)k * n + c]; }, x);
   }
-  for (int p = pb; p < pe; p++) acc = acc + A.mval[p] * x[A.mrow[p]];
-  return acc;
+  return gather_sum(acc, pe - pb, [&](int q) { return A.mval[pb + q]; }, [&](int q) { return A.mrow[pb + q]; }, x);
 }
 
 template <int MODE>
