@@ -289,22 +289,30 @@ struct Mat {
 };
 
 // The full stored row, in the reference's slot order (sparse_atime, linear_solver.F90:264-272).
+// acc + sum over q in [0, m) of val(q) x[idx(q)], in q order, with eight entries' loads (value and
+// index, then the gathers) in flight before they are summed: each entry is otherwise a dependent
+// chain of two L2 round trips, and a wall row's mirror bucket holds tens of entries
+template <class V, class I>
+__device__ inline double gather_sum(double acc, int m, V val, I idx, const double* __restrict__ x) {
+  int q = 0;
+  for (; q + 8 <= m; q += 8) {
+    double v[8], g[8];
+    int j[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) { v[u] = val(q + u); j[u] = idx(q + u); }
+#pragma unroll
+    for (int u = 0; u < 8; u++) g[u] = x[j[u]];
+#pragma unroll
+    for (int u = 0; u < 8; u++) acc = acc + v[u] * g[u];
+  }
+  for (; q < m; q++) acc = acc + val(q) * x[idx(q)];
+  return acc;
+}
+
 __device__ inline double row_full(const Mat& A, int i, int m, double xi, const double* __restrict__ x) {
-  const int n = A.n;
-  double bb = A.diag[i] * xi;
-  int k = 0;
-  for (; k + 4 <= m; k += 4) {
-    int s0 = k * n + i;
-    int c0 = A.col[s0], c1 = A.col[s0 + n], c2 = A.col[s0 + 2 * n], c3 = A.col[s0 + 3 * n];
-    double v0 = A.val[s0], v1 = A.val[s0 + n], v2 = A.val[s0 + 2 * n], v3 = A.val[s0 + 3 * n];
-    double x0 = x[c0], x1 = x[c1], x2 = x[c2], x3 = x[c3];
-    bb = bb + v0 * x0; bb = bb + v1 * x1; bb = bb + v2 * x2; bb = bb + v3 * x3;
-  }
-  for (; k < m; k++) {
-    int s0 = k * n + i;
-    bb = bb + A.val[s0] * x[A.col[s0]];
-  }
-  return bb;
+  const size_t n = A.n;
+  return gather_sum(A.diag[i] * xi, m, [&](int k) { return A.val[(size_t)k * n + i]; },
+                    [&](int k) { return A.col[(size_t)k * n + i]; }, x);
 }
 
 // The compact matrix arrays are read once per SpMV: non-temporal loads keep them from displacing the
@@ -429,26 +437,6 @@ __device__ inline double spmv_row_f32(const Mat& A, int i, const double* __restr
 // Row c of A^T: rows without mirror neighbours as compact FP64 rows (the forward row's offsets, the
 // A^T value plane, column order), the others from the transposed map in c's slot order; then c's
 // mirror bucket.
-// acc + sum over q in [0, m) of val(q) x[idx(q)], in q order, with eight entries' loads (value and
-// index, then the gathers) in flight before they are summed: each entry is otherwise a dependent
-// chain of two L2 round trips, and a wall row's mirror bucket holds tens of entries
-template <class V, class I>
-__device__ inline double gather_sum(double acc, int m, V val, I idx, const double* __restrict__ x) {
-  int q = 0;
-  for (; q + 8 <= m; q += 8) {
-    double v[8], g[8];
-    int j[8];
-#pragma unroll
-    for (int u = 0; u < 8; u++) { v[u] = val(q + u); j[u] = idx(q + u); }
-#pragma unroll
-    for (int u = 0; u < 8; u++) g[u] = x[j[u]];
-#pragma unroll
-    for (int u = 0; u < 8; u++) acc = acc + v[u] * g[u];
-  }
-  for (; q < m; q++) acc = acc + val(q) * x[idx(q)];
-  return acc;
-}
-
 __device__ inline double spmv_row_transpose(const Mat& A, int c, const double* __restrict__ x) {
   // the bucket bounds first: their loads overlap the row's chain instead of following it
   const int pb = A.moff[c], pe = A.moff[c + 1];
