@@ -214,7 +214,7 @@ def adjoint_cost(s, case):
 
 
 def adjoint_run_cost(workload, nsteps, every, precision="fp64"):
-    """The adjoint's cost at the adjoint Poiseuille inversion's size: a plain forward run of nsteps
+    """The adjoint's cost at an adjoint inversion's size: a plain forward run of nsteps
     steps (vb_div_free), the same run recorded for the adjoint (vb_adjoint_record: checkpoints every
     `every` steps, every step's pressure kept), and the reverse sweep (vb_adjoint_backward) for the
     velocity-mismatch loss 1/2 |u - u_target|^2 with u_target = 0.9 u_final.  The start is the fluid at
@@ -272,6 +272,10 @@ def main():
                     help="the adjoint cost's workload ('' skips it)")
     ap.add_argument("--adjoint-steps", type=int, default=200)
     ap.add_argument("--adjoint-every", type=int, default=20)
+    ap.add_argument("--adjoint-convection-workload", default="convection_1M",
+                    help="the convection inversion's adjoint cost ('' skips it): mixed-precision forward")
+    ap.add_argument("--adjoint-convection-steps", type=int, default=20)
+    ap.add_argument("--adjoint-convection-every", type=int, default=10)
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: the step has no multi-GPU path in this library (the box "
@@ -304,6 +308,10 @@ def main():
     adj_run = None
     if a.adjoint_workload:
         adj_run = adjoint_run_cost(a.adjoint_workload, a.adjoint_steps, a.adjoint_every)
+    adj_conv = None
+    if a.adjoint_convection_workload:
+        adj_conv = adjoint_run_cost(a.adjoint_convection_workload, a.adjoint_convection_steps,
+                                    a.adjoint_convection_every, precision="mixed")
 
     cpu = None
     if not a.no_cpu_baseline:
@@ -356,6 +364,8 @@ def main():
         "adjoint_ppe": main_res["adjoint_ppe"],
         # adjoint/forward cost of a whole run at the adjoint inversion's size (checkpointed sweep)
         "adjoint": adj_run,
+        # the same at the convection inversion's size, 1M particles, mixed-precision forward
+        "adjoint_convection": adj_conv,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
