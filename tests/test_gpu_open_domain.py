@@ -84,7 +84,10 @@ def test_open_step_phases_against_oracle(gpu_lib, name):
                 if first:
                     _eq(conc_g[:m], so_["conc"][:m])
                 else:
-                    assert _rel(conc_g[:m], so_["conc"][:m]) <= 1e-12
+                    # after a step whose parallel-order solve differs from the oracle's serial one by
+                    # rounding, positions differ by ~1e-13 and the concentration (a kernel sum, ~30x
+                    # that sensitivity) by a few 1e-12 (2.5e-12 measured on the dam break)
+                    assert _rel(conc_g[:m], so_["conc"][:m]) <= 1e-11
                 ramp = int(((so_["psmooth"] > 0) & (so_["psmooth"] < 1)).sum())
                 print(f"step {step + 1}: {int(so_['nsurf'][:m].sum())} surface rows, {ramp} ramp rows",
                       flush=True)

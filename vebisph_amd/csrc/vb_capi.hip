@@ -66,8 +66,8 @@ static void free_all(vb_ctx* c) {
                   c->seg_counts, c->cellkey, c->cellslot, c->nc, c->ist, c->ip, c->scan_bsum, c->cnt, c->link,
                   c->wg1, c->wg2, c->wl, c->wg1f, c->wg2f, c->wlf, c->k11, c->k12, c->k21, c->k22,
                   c->usx, c->usy, c->kick, c->gpx, c->gpy, c->vort, c->conc, c->snx, c->sny, c->pscale, c->a_diag, c->a_val, c->rhs,
-                  c->a_col, c->a_len, c->rowflag, c->chunkg, c->choff, c->xflag, c->xscan, c->xrows,
-                  c->lnk4, c->val4, c->val4f,
+                  c->a_col, c->a_len, c->rowflag, c->chunkb, c->xflag, c->xscan, c->xrows, c->xcol, c->xval,
+                  c->boff, c->bv64, c->bv32, c->bpos,
                   c->r, c->rb, c->vv,
                   c->pn, c->pinv, c->ss, c->sinv, c->tt,
                   c->xsys, c->partials, c->red, c->scal};
@@ -200,17 +200,16 @@ vb_status vb_create(const vb_params* prm, const vb_boundary* bnd, int32_t npfb, 
   A(c->usx, n); A(c->usy, n); A(c->kick, n); A(c->gpx, n); A(c->gpy, n); A(c->vort, n); A(c->conc, n);
   if (prm->i_open_domain == 1) { A(c->snx, n); A(c->sny, n); A(c->pscale, n); }
   A(c->a_diag, n); A(c->a_val, nw); A(c->rhs, n); A(c->a_col, nw); A(c->a_len, n);
-  // compact grouped rows: up to VB_NPLINK slots rounded up to a multiple of 4
-  // compact rows: at most VB_NPLINK slots rounded up to groups of 4, for every row of every chunk
-  const size_t nch = ((size_t)npfb + 63) / 64;
-  const size_t nw4 = nch * 64 * ((kNplink + 3) & ~3);
-  A(c->chunkg, nch + 1); A(c->choff, nch + 1);
+  // compact rows: kMaxBlk column blocks for every row of every chunk; the full rows' list-order copy
+  // for every row
+  A(c->chunkb, ppe_chunks(npfb) + 1);
   A(c->xflag, n + 1); A(c->xscan, n + 1); A(c->xrows, n);
-  A(c->rowflag, n); A(c->lnk4, nw4); A(c->val4, nw4);
+  A(c->xcol, ppe_list_slots(npfb)); A(c->xval, ppe_list_slots(npfb));
+  A(c->rowflag, n); A(c->boff, ppe_block_offsets(npfb)); A(c->bv64, ppe_block_values(npfb)); A(c->bpos, nw);
   // VB_SPMV=stored: every FP64 SpMV row from the stored matrix (measurement switch; same results).
   const char* sp = getenv("VB_SPMV");
   c->spmv_stored = sp && strcmp(sp, "stored") == 0;
-  if (prm->precision == VB_PREC_MIXED) A(c->val4f, nw4);
+  if (prm->precision == VB_PREC_MIXED) A(c->bv32, ppe_block_values(npfb));
   A(c->r, n); A(c->rb, n); A(c->vv, n); A(c->pn, n); A(c->pinv, n); A(c->ss, n); A(c->sinv, n);
   A(c->tt, n); A(c->xsys, n);
   A(c->partials, (size_t)kSolverStages * 2 * kPartialStride); A(c->scal, 1);

@@ -31,11 +31,24 @@ constexpr int kSpmvMaxBlocks = 16384;
 // cap also bounds the fold traffic (workgroups x partials).  Measured on the 1M-row PPE (ms per
 // step, mixed): 256 -> 224.7, 512 -> 219.9, 1024 -> 223.9, 2048 -> 238.3.
 constexpr int kMaxBlocks = VB_MAX_BLOCKS;
-constexpr int kPartialStride = kSpmvMaxBlocks > kMaxBlocks ? kSpmvMaxBlocks : kMaxBlocks;
+// Workgroups of an SpMV launch that take the PPE's listed full rows (vb_ppe.hip: spmv_rows), at most
+constexpr int kSpmvListBlocks = 1024;
+constexpr int kPartialStride = kSpmvMaxBlocks + kSpmvListBlocks > kMaxBlocks ? kSpmvMaxBlocks + kSpmvListBlocks
+                                                                            : kMaxBlocks;
 static_assert(kSpmvBlock % 64 == 0 && kSpmvBlock <= 1024, "SpMV block");
 static_assert(kMaxBlocks % 8 == 0 && kSpmvMaxBlocks % 8 == 0, "grid caps");
 constexpr int kSolverStages = 6;           // BiCGSTAB reduction stages with their own buffers
 constexpr int kNplink = VB_NPLINK;
+// The PPE's compact rows as column blocks (layout in vb_ppe.hip): rows in chunks of 64 (a
+// wavefront), blocks of kBlkCols columns, at most kMaxBlk blocks per row.
+constexpr int kChunk = 64;
+constexpr int kBlkCols = 8;
+constexpr int kMaxBlk = 12;
+constexpr int kOffGroups = kMaxBlk / 4;
+inline size_t ppe_chunks(int npfb) { return ((size_t)npfb + kChunk - 1) / kChunk; }
+inline size_t ppe_block_values(int npfb) { return ppe_chunks(npfb) * kChunk * kMaxBlk * kBlkCols; }
+inline size_t ppe_block_offsets(int npfb) { return ppe_chunks(npfb) * kChunk * kMaxBlk; }
+inline size_t ppe_list_slots(int npfb) { return ppe_chunks(npfb) * kChunk * kNplink; }
 constexpr double kEps = 2.220446049250313e-16;   // epsilon(1d0)
 
 // Boundary description and derived constants, passed by value to kernels (< 4 KiB).
@@ -298,8 +311,7 @@ __device__ inline double fold(const double* partials, int q, int nblk, const dou
 // (workgroups are dealt round-robin to the 8 XCDs, each with its own L2), so a block's neighbour
 // gathers mostly hit rows its XCD already holds.
 struct RowRange { int begin, end; };
-__device__ inline RowRange block_rows(int n, int nblk) {
-  int b = blockIdx.x;
+__device__ inline RowRange block_rows(int n, int b, int nblk) {
   int lb = b;
   if ((nblk & 7) == 0) lb = (b & 7) * (nblk >> 3) + (b >> 3);
   int chunk = (n + nblk - 1) / nblk;
@@ -308,6 +320,7 @@ __device__ inline RowRange block_rows(int n, int nblk) {
   int r1 = min(n, r0 + chunk);
   return {r0, r1};
 }
+__device__ inline RowRange block_rows(int n, int nblk) { return block_rows(n, (int)blockIdx.x, nblk); }
 
 inline int grid_for(int n) {
   int b = (n + kBlock - 1) / kBlock;

@@ -18,7 +18,7 @@ struct OperatorMap {
   double* mval = nullptr;
   size_t cap_m = 0;
   double *g = nullptr, *lam = nullptr;   // adjoint solve: right-hand side and solution (npfb)
-  double* val4T = nullptr;            // A^T values of the compact rows (layout of vb_ctx::val4)
+  double* bv64T = nullptr;            // A^T values of the compact rows (layout of vb_ctx::bv64)
   int* rslot = nullptr;               // reverse slot: the slot of c in i's list, beside tcol
   double *rref = nullptr, *bref = nullptr, *dref = nullptr;   // adjoint refinement: residual,
   double* refs = nullptr;                                       // scaled residual, correction, scalars
@@ -69,10 +69,13 @@ struct vb_ctx {
   int *a_col, *a_len;                // a_len: row lengths of a caller-supplied system
   uint8_t* rowflag;                  // 1: SpMV uses the full stored row (mirror neighbours)
   int *xflag, *xscan, *xrows;        // the rows with rowflag 1, listed for their assembly
-  int *chunkg, *choff;               // compact rows: groups per 64-row chunk, first group slot
-  int16_t* lnk4;                     // compact rows, sliced groups of 4 slots: column offsets
-  double* val4;                      //   FP64 values
-  float* val4f;                      //   FP32 values (mixed path, stored-value SpMV)
+  int* xcol;                         // the listed rows' columns and values in list order (xslot)
+  double* xval;
+  int* chunkb;                       // compact rows: column blocks per 64-row chunk
+  int16_t* boff;                     // compact rows, column blocks: start offsets
+  double* bv64;                      //   FP64 values
+  float* bv32;                       //   FP32 values (mixed path)
+  uint8_t* bpos;                     //   block position of each neighbour slot (for A^T)
 
   bool spmv_stored;                  // FP64 path: full stored rows everywhere (VB_SPMV=stored)
   double *r, *rb, *vv, *pn, *pinv, *ss, *sinv, *tt, *xsys;
@@ -117,7 +120,7 @@ vb_status apply_ppe_operator(vb_ctx* c, bool transpose, const double* x, double*
 vb_status apply_solver_spmv(vb_ctx* c, bool transpose, const double* x, double* y);
 vb_status build_transpose(vb_ctx* c);
 vb_status alloc_operator(vb_ctx* c, size_t bucket_cap);
-vb_status launch_transpose_sliced(vb_ctx* c);
+vb_status launch_transpose_blocks(vb_ctx* c);
 vb_status apply_grad_operator(vb_ctx* c, bool transpose, const double* x, double* y);
 // A^T lambda = g, FP64 iterative refinement around the reference's BiCGSTAB on the transposed map
 // (vb_ppe.hip); g, lambda in op; total inner iterations, |g - A^T lambda| / |g| and a vb_status

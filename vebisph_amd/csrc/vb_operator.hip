@@ -147,8 +147,7 @@ vb_status alloc_operator(vb_ctx* c, size_t bucket_cap) {
     VB_HIP(c, hipMalloc((void**)&op.moff, ((size_t)npfb + 1) * sizeof(int)));
     VB_HIP(c, hipMalloc((void**)&op.mfill, ((size_t)npfb + 1) * sizeof(int)));
     VB_HIP(c, hipMalloc((void**)&op.nmiss, sizeof(int)));
-    const size_t nw4 = ((size_t)npfb + 63) / 64 * 64 * ((kNplink + 3) & ~3);     // as vb_ctx::val4
-    VB_HIP(c, hipMalloc((void**)&op.val4T, nw4 * sizeof(double)));
+    VB_HIP(c, hipMalloc((void**)&op.bv64T, ppe_block_values(npfb) * sizeof(double)));   // as vb_ctx::bv64
   }
   if (!op.g) {
     VB_HIP(c, hipMalloc((void**)&op.g, (size_t)npfb * sizeof(double)));
@@ -183,7 +182,7 @@ vb_status build_transpose(vb_ctx* c) {
   VB_LAUNCH(c, k_op_zero, 1, op.nmiss, 1);
   VB_LAUNCH(c, k_transpose_real, grid_1d(npfb), npfb, c->cnt, c->link, c->a_val, op.tval, op.tcol, op.rslot,
             op.nmiss);
-  if ((s0 = launch_transpose_sliced(c)) != VB_OK) return s0;
+  if ((s0 = launch_transpose_blocks(c)) != VB_OK) return s0;
   // the explicit rows listed by the last assembly (the rows with mirror neighbours)
   const int nx = c->scal_host->nexplicit;
   if (nx > 0)
@@ -350,7 +349,7 @@ vb_status remove_mean(vb_ctx* c, double* v, int n) {
 
 void free_operator(vb_ctx* c) {
   void* ptrs[] = {c->op.tval, c->op.tcol, c->op.mcount, c->op.moff, c->op.mfill, c->op.mkey, c->op.mrow,
-                  c->op.mval, c->op.nmiss, c->op.g, c->op.lam, c->op.val4T, c->op.rslot, c->op.ga,
+                  c->op.mval, c->op.nmiss, c->op.g, c->op.lam, c->op.bv64T, c->op.rslot, c->op.ga,
                   c->op.rref, c->op.bref, c->op.dref, c->op.refs,
                   c->op.gb, c->op.gin, c->op.gout};
   for (void* p : ptrs)

@@ -74,29 +74,54 @@ __global__ void k_fin_ppe_rhs(const double* partials, int nblk, const double* re
   if (threadIdx.x == 0) sc->mean = open ? 0.0 : s / (double)n;     // b - 0.0 == b bit for bit
 }
 
-__host__ __device__ inline size_t grouped(int k, int i, int n) { return ((size_t)(k >> 2) * n + i) * 4 + (k & 3); }
+// Compact rows are stored as column blocks: the particles are numbered lattice-fashion (one lattice
+// column of consecutive indices after another), so a row's neighbours come in runs of consecutive
+// indices, one run per lattice column the kernel support crosses (seven at h = 1.3 dx).  A block
+// is kBlkCols = 8 consecutive columns starting at an even column s; a row is covered greedily in
+// column order (a new block starts at the first column not yet covered), and its block values hold
+// A_ij at column s + e, zero where j is not a neighbour or is the row itself (the diagonal is
+// applied separately, as before).  One block costs one 2-byte offset and four aligned 16-byte
+// gathers of x instead of eight 2-byte offsets and eight 8-byte gathers, and the row is summed in
+// the same column order as before with exact zeros between (the same bits up to the sign of a zero
+// sum).  At the 1M-particle cylinder every compact row needs 7 or 8 blocks (56 slots for 44
+// neighbours).  A row that would need more than kMaxBlk blocks, or whose last block would reach past
+// the last row, is stored full instead (rowflag 1).
+//
+// The blocks are sliced by chunks of 64 rows (a wavefront): every row of chunk c has chunkb[c]
+// blocks (its rows' maximum; padding blocks repeat the row's last start with zero values), stored at
+// a fixed stride of kMaxBlk blocks per chunk; block b of row i (lane = i % 64):
+//   start offset s - i   short4 boff[(c * kOffGroups + b / 4) * 64 + lane], component b % 4
+//   FP64 values          double2 bv64[((c * kMaxBlk + b) * 4 + h) * 64 + lane], columns s+2h, s+2h+1
+//   FP32 values          float4 bv32[((c * kMaxBlk + b) * 2 + h) * 64 + lane], columns s+4h .. s+4h+3
+// so each load of a wavefront is one contiguous 512-byte or 1-KiB run.  bpos[k * npfb + i] records
+// where slot k of the neighbour list went (b * 8 + e), for the transposed values.
+static_assert(kMaxBlk % 4 == 0 && kMaxBlk * kBlkCols < 256, "block offsets in short4 groups; bpos fits a byte");
+__host__ __device__ inline size_t blk64(size_t c, int b, int e, int lane) {     // element of bv64
+  return (((c * kMaxBlk + b) * 4 + (e >> 1)) * kChunk + lane) * 2 + (e & 1);
+}
+__host__ __device__ inline size_t blk32(size_t c, int b, int e, int lane) {     // element of bv32
+  return (((c * kMaxBlk + b) * 2 + (e >> 2)) * kChunk + lane) * 4 + (e & 3);
+}
+__host__ __device__ inline size_t blkoff(size_t c, int b, int lane) {           // element of boff
+  return ((c * kOffGroups + (b >> 2)) * kChunk + lane) * 4 + (b & 3);
+}
+// Rows stored full (mirror neighbours, open-domain surface and scaled rows, rows the blocks do not
+// fit) are also kept sliced in list order for the solver's SpMV: entry k of the t-th listed row at
+// xcol/xval[((t / 64) * kNplink + k) * 64 + t % 64], so that a wavefront of listed rows reads one
+// contiguous run per slot instead of 64 rows kNplink apart.
+__host__ __device__ inline size_t xslot(int t, int k) {
+  return ((size_t)(t / kChunk) * kNplink + k) * kChunk + (t % kChunk);
+}
 
-// Compact rows are stored sliced: the 64 rows of a wavefront (a chunk) keep each group of four slots
-// contiguous, chunk after chunk, so that a wavefront streams one contiguous region instead of one
-// region per group.  Chunk c holds G_c groups (the most any of its compact rows needs), starting at
-// group slot P = choff[c]; group g of row i (lane = i % 64) is group slot (P + g) * 64 + lane:
-//   column offsets   short4 lnk4[(P + g) * 64 + lane]
-//   FP64 values      double2 val4[(2 (P + g)) * 64 + lane] (slots 4g, 4g+1),
-//                    double2 val4[(2 (P + g) + 1) * 64 + lane] (slots 4g+2, 4g+3)
-//   FP32 values      float4 val4f[(P + g) * 64 + lane]
-constexpr int kChunk = 64;
-
-// Assembly, :60-122.  One thread per row (a wavefront = one chunk of 64 rows).  Writes the full
-// stored rows, the diagonal and right-hand side, marks the rows the compact SpMV can take
-// (rowflag 0: no mirror neighbour, every neighbour within 32767 rows) and records per chunk the
-// groups its compact rows need (chunkg); k_assemble_sliced then writes the compact rows.
+// Assembly, :60-122.  One thread per row.  Writes the full stored rows, the diagonal and right-hand
+// side, and marks the rows the compact SpMV may take (rowflag 0: no mirror neighbour, every
+// neighbour within 32767 rows); k_assemble_blocks then writes those rows' blocks.
 template <class WT>
 __global__ void k_assemble(Geom g, State st, const int* cnt, const int* link, const WT* wl,
                            const int* nsurf, const double* pscale, double* rhs, double* a_diag, int* a_col,
-                           double* a_val, uint8_t* rowflag, int* chunkg, const Scal* sc) {
+                           double* a_val, uint8_t* rowflag, const Scal* sc) {
   const int i = xcd_row(g.npfb);
   const int npfb = g.npfb;
-  int ngroups = 0;
   if (i < npfb) {
   const int n = cnt[i];
   double bi = rhs[i] - sc->mean;
@@ -132,20 +157,15 @@ __global__ void k_assemble(Geom g, State st, const int* cnt, const int* link, co
       a_col[s] = link[s];
       a_val[s] = val;
     }
-    ngroups = (n + 3) >> 2;
   }
   // rows with mirror neighbours: k_assemble_explicit writes their slots
   a_diag[i] = d;
   rowflag[i] = explicit_row ? 1 : 0;
   rhs[i] = bi;
   }
-  // groups of the chunk: the wavefront's maximum
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) { const int t = __shfl_down(ngroups, o, 64); ngroups = t > ngroups ? t : ngroups; }
-  if ((threadIdx.x & 63) == 0 && i < npfb) chunkg[i / kChunk] = ngroups;
 }
 
-// Rows with mirror neighbours (:76-115), one workgroup per row (rows = the explicit rows listed by
+// Rows stored full (:76-115), one workgroup per row (rows = the explicit rows listed by
 // k_list_explicit), one thread per slot.  Slot k's column is the neighbour or the mirror's parent;
 // a column's value goes to its last slot and is the sum of -w_L over its slots in slot order
 // (earlier slots of the column hold 0); a self-mirror's column is the row itself and its value 0
@@ -156,7 +176,7 @@ template <class WT>
 __global__ void __launch_bounds__(kExplicitThreads)
 k_assemble_explicit(int npfb, const int* rows, int nrows, const int* cnt, const int* link,
                     const WT* wl, const int* irel, const double* pscale, const double* a_diag, int* a_col,
-                    double* a_val) {
+                    double* a_val, int* xcol, double* xval) {
   if ((int)blockIdx.x >= nrows) return;
   __shared__ int cols[kExplicitThreads];
   __shared__ double mw[kExplicitThreads];
@@ -183,6 +203,8 @@ k_assemble_explicit(int npfb, const int* rows, int nrows, const int* cnt, const 
     }
     a_col[k * npfb + i] = ck;
     a_val[k * npfb + i] = val;
+    xcol[xslot(blockIdx.x, k)] = ck;                     // the same row in list order (xslot)
+    xval[xslot(blockIdx.x, k)] = val;
   }
 }
 
@@ -198,73 +220,87 @@ __global__ void k_flag_to_int(int n, const uint8_t* f, int* out) {
   if (i < n) out[i] = f[i];
 }
 
-// The compact rows in the sliced layout (choff = exclusive scan of chunkg); padding slots of a row's
-// last group hold offset 0 (the row itself) and value 0.  The slots are placed in increasing column
-// order (slot k goes to its rank among the row's column offsets, kept in LDS): then slot k of 64
-// consecutive rows gathers from nearly consecutive columns, 5.3 instead of 9.1 distinct 128-byte
-// lines per wavefront gather on the 1M-particle cylinder.  The row sum is then taken in column
-// order, not in the reference's slot order; the full stored rows keep the reference's order.
-__global__ void k_assemble_sliced(int npfb, const int* cnt, const int* link, const double* a_val,
-                                  const uint8_t* rowflag, const int* choff, int16_t* lnk4, double* val4,
-                                  float* val4f) {
-  __shared__ int16_t off[kNplink][kBlock];     // each thread uses only its own column
-  const int t = threadIdx.x, i = blockIdx.x * kBlock + t;
-  if (i >= npfb || rowflag[i]) return;
-  const int n = cnt[i], padded = (n + 3) & ~3;
-  const size_t P = (size_t)choff[i / kChunk], lane = i % kChunk;
-  for (int k = 0; k < n; k++) off[k][t] = (int16_t)(link[k * npfb + i] - i);
-  auto put = [&](int k, int16_t o, double val) {
-    const size_t gs = (P + (k >> 2)) * kChunk + lane;                 // group slot
-    const size_t e = gs * 4 + (k & 3);
-    const size_t e2 = ((2 * (P + (k >> 2)) + ((k & 3) >> 1)) * kChunk + lane) * 2 + (k & 1);
-    lnk4[e] = o;
-    val4[e2] = val;
-    if (val4f) val4f[e] = (float)val;
-  };
-  // (~0.7 ms per step at 1M, against 0.28 ms unsorted: the stores of a wavefront now scatter over
-  // the row's groups; the SpMVs gain ~5 ms per step)
-  for (int k = 0; k < n; k++) {
-    const int16_t ok = off[k][t];
-    int pos = 0;                                  // rank: smaller offsets, ties broken by slot
-    for (int j = 0; j < n; j++) {
-      const int16_t oj = off[j][t];
-      pos += (oj < ok) || (oj == ok && j < k);
+// The compact rows' column blocks (layout above).  One thread per row, rows in index order, so a
+// wavefront is one chunk; each row's column offsets and their column order sit in LDS.  Pass 1 ranks
+// the offsets (a compact row has no repeated column), walks them in column order and counts blocks;
+// a row with no neighbour, one needing more than kMaxBlk blocks, or one whose last block would
+// reach past row npfb - 1, becomes a full row (rowflag 1).  The wavefront's maximum is the chunk's
+// block count.  Pass 2 zero-fills the row's chunkb[c] blocks, writes their starts (padding blocks
+// repeat the last) and puts each value at its column.  (~0.2 ms more per step at 1M than the
+// four-slot groups it replaced; the SpMVs gain ~25 us each.)
+__global__ void __launch_bounds__(kBlock)
+k_assemble_blocks(int npfb, const int* cnt, const int* link, const double* a_val, uint8_t* rowflag,
+                  int* chunkb, int16_t* boff, double* bv64, float* bv32, uint8_t* bpos) {
+  __shared__ int16_t off[kNplink][kBlock];
+  __shared__ uint8_t ord[kNplink][kBlock];
+  const int t = threadIdx.x, i = blockIdx.x * kBlock + t, lane = t & (kChunk - 1);
+  bool live = i < npfb && !rowflag[i];
+  const int n = live ? cnt[i] : 0;
+  int nb = 0;
+  if (live) {
+    for (int k = 0; k < n; k++) off[k][t] = (int16_t)(link[k * npfb + i] - i);
+    for (int k = 0; k < n; k++) {
+      const int16_t ok = off[k][t];
+      int pos = 0;
+      for (int j = 0; j < n; j++) pos += off[j][t] < ok;
+      ord[pos][t] = (uint8_t)k;
     }
-    put(pos, ok, a_val[k * npfb + i]);
+    int end = INT_MIN;                             // last column the blocks cover
+    for (int q = 0; q < n; q++) {
+      const int col = i + off[ord[q][t]][t];
+      if (col > end) { nb++; end = (col & ~1) + kBlkCols - 1; }
+    }
+    if (n == 0 || nb > kMaxBlk || end > npfb - 1) { rowflag[i] = 1; live = false; nb = 0; }
   }
-  for (int k = n; k < padded; k++) put(k, 0, 0.0);
+  int nbc = nb;                                    // every lane takes part in the maximum
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) { const int v = __shfl_xor(nbc, o, 64); nbc = v > nbc ? v : nbc; }
+  const size_t c = (size_t)(blockIdx.x * kBlock + (t & ~(kChunk - 1))) / kChunk;
+  if (lane == 0 && c * kChunk < (size_t)npfb) chunkb[c] = nbc;
+  if (!live) return;
+  for (int b = 0; b < nbc; b++)
+    for (int e = 0; e < kBlkCols; e++) {
+      bv64[blk64(c, b, e, lane)] = 0.0;
+      if (bv32) bv32[blk32(c, b, e, lane)] = 0.f;
+    }
+  int end = INT_MIN, b = -1, s = 0;
+  for (int q = 0; q < n; q++) {
+    const int k = ord[q][t];
+    const int col = i + off[k][t];
+    if (col > end) {
+      s = col & ~1; b++; end = s + kBlkCols - 1;
+      boff[blkoff(c, b, lane)] = (int16_t)(s - i);
+    }
+    const int e = col - s;
+    const double v = a_val[(size_t)k * npfb + i];
+    bv64[blk64(c, b, e, lane)] = v;
+    if (bv32) bv32[blk32(c, b, e, lane)] = (float)v;       // exact: the mixed path's weights are FP32
+    bpos[(size_t)k * npfb + i] = (uint8_t)(b * kBlkCols + e);
+  }
+  for (int bb = b + 1; bb < nbc; bb++) boff[blkoff(c, bb, lane)] = (int16_t)(s - i);
 }
 
 // The compact rows of A^T (the adjoint solve): a row without mirror neighbours has the same columns in
-// A and A^T (the real neighbour relation is symmetric), so A^T reuses the column offsets written
-// above and takes a second FP64 value plane, val4T, filled from the transposed map's real part in the
-// same rank order.
-__global__ void k_transpose_sliced(int npfb, const int* cnt, const int* link, const double* tval,
-                                   const uint8_t* rowflag, const int* choff, double* val4T) {
-  __shared__ int16_t off[kNplink][kBlock];
-  const int t = threadIdx.x, i = blockIdx.x * kBlock + t;
+// A and A^T (the real neighbour relation is symmetric), so A^T reuses the block starts written above
+// and takes a second FP64 value plane, bv64T, filled from the transposed map's real part at the
+// positions bpos recorded.
+__global__ void k_transpose_blocks(int npfb, const int* cnt, const double* tval, const uint8_t* rowflag,
+                                   const int* chunkb, const uint8_t* bpos, double* bv64T) {
+  const int t = threadIdx.x, i = blockIdx.x * kBlock + t, lane = t & (kChunk - 1);
   if (i >= npfb || rowflag[i]) return;
-  const int n = cnt[i], padded = (n + 3) & ~3;
-  const size_t P = (size_t)choff[i / kChunk], lane = i % kChunk;
-  for (int k = 0; k < n; k++) off[k][t] = (int16_t)(link[k * npfb + i] - i);
-  auto put = [&](int k, double val) {
-    val4T[((2 * (P + (k >> 2)) + ((k & 3) >> 1)) * kChunk + lane) * 2 + (k & 1)] = val;
-  };
+  const size_t c = (size_t)i / kChunk;
+  const int nbc = chunkb[c], n = cnt[i];
+  for (int b = 0; b < nbc; b++)
+    for (int e = 0; e < kBlkCols; e++) bv64T[blk64(c, b, e, lane)] = 0.0;
   for (int k = 0; k < n; k++) {
-    const int16_t ok = off[k][t];
-    int pos = 0;
-    for (int j = 0; j < n; j++) {
-      const int16_t oj = off[j][t];
-      pos += (oj < ok) || (oj == ok && j < k);
-    }
-    put(pos, tval[(size_t)k * npfb + i]);
+    const int p = bpos[(size_t)k * npfb + i];
+    bv64T[blk64(c, p / kBlkCols, p % kBlkCols, lane)] = tval[(size_t)k * npfb + i];
   }
-  for (int k = n; k < padded; k++) put(k, 0.0);
 }
 
-vb_status launch_transpose_sliced(vb_ctx* c) {
-  VB_LAUNCH(c, k_transpose_sliced, grid_1d(c->npfb), c->npfb, c->cnt, c->link, c->op.tval, c->rowflag, c->choff,
-            c->op.val4T);
+vb_status launch_transpose_blocks(vb_ctx* c) {
+  VB_LAUNCH(c, k_transpose_blocks, grid_1d(c->npfb), c->npfb, c->cnt, c->op.tval, c->rowflag, c->chunkb,
+            c->bpos, c->op.bv64T);
   return VB_OK;
 }
 
@@ -275,14 +311,19 @@ struct Mat {
   const double *diag, *val;
   const int *col, *len;
   const uint8_t* rowflag;          // null: every row uses the full stored row
-  // compact rows, sliced by chunks of 64 rows (layout at kChunk)
-  const int* choff;                // first group slot of each chunk
-  const short4* lnk4;              // column offsets (column = row + offset)
-  const double2* val4;             // FP64 values, two double2 planes per group
-  const float4* val4f;             // FP32 values (mixed path), one float4 per group
+  // compact rows as column blocks, sliced by chunks of 64 rows (layout at kChunk)
+  const int* chunkb;               // blocks of each chunk's rows
+  const short4* boff;              // block start offsets (start column = row + offset, even)
+  const double2* bv64;             // FP64 values, four double2 per block
+  const float4* bv32;              // FP32 values (mixed path), two float4 per block
+  // full rows in list order (xslot): the solver's SpMV takes them in nxb workgroups of their own
+  const int* xrows;                // the listed rows
+  const int* xcol;
+  const double* xval;
+  int nx, nxb;                     // listed rows; their workgroups (0: full rows inline)
   // the transposed map (SPMV_TRANSPOSE): real part slot-major like the lists, mirror buckets per column,
-  // and the compact rows' A^T values (same offsets as lnk4)
-  const double2* val4T;
+  // and the compact rows' A^T values (the forward blocks' starts)
+  const double2* bv64T;
   const double *tval, *mval;
   const int *tcol, *moff, *mrow;
   int n;
@@ -334,107 +375,111 @@ __device__ inline float4 ld_nt(const float4* p) {
   return make_float4(v.x, v.y, v.z, v.w);
 }
 
-// One FP64 SpMV row y_i = A(i) x_i + sum_k a_k x_col(k).  Rows with mirror neighbours use the full
-// stored row; the others (~98%) load four slots per instruction from the grouped compact arrays: a
-// 2-byte column offset and the FP64 value, slots in increasing column order.
-// (val4: the value plane, A.val4 for A and A.val4T for the compact rows of A^T)
-__device__ inline double spmv_row_fp64_v(const Mat& A, const double2* __restrict__ val4, int i,
-                                         const double* __restrict__ x) {
-  const int m = A.len[i];
-  const double xi = x[i];
-  if (!A.rowflag || A.rowflag[i]) return row_full(A, i, m, xi, x);
-  const int ng = (m + 3) >> 2;
-  // group g of this row: group slot (P + g) * 64 + lane (sliced layout, see kChunk)
-  const size_t P = (size_t)A.choff[i / kChunk], lane = (size_t)(i % kChunk);
-  auto gs = [&](size_t g) { return (P + g) * kChunk + lane; };
-  auto vs = [&](size_t g, size_t h) { return (2 * (P + g) + h) * kChunk + lane; };
-  // Three-stage software pipeline over the row's groups g (a row is a dependent chain of ~11
-  // groups: offsets -> gathers -> accumulate): while group g is accumulated, the gathers of g+1 and
-  // the offset/value loads of g+2 are in flight, so each step overlaps one HBM round trip and one
-  // L2 gather round trip instead of waiting for both.
-  const short4 zo = make_short4(0, 0, 0, 0);
-  short4 o1 = zo, o2 = zo;
-  double g0[4], g1[4] = {0.0, 0.0, 0.0, 0.0};
-  const short4 o0 = ld_nt(A.lnk4 + gs(0));
-  if (ng > 1) o1 = ld_nt(A.lnk4 + gs(1));
-  {
-    // FP64 values: double2 plane 2g holds slots 4g, 4g+1; plane 2g+1 slots 4g+2, 4g+3
-    const double2 z2 = make_double2(0.0, 0.0);
-    double2 a0 = ld_nt(val4 + vs(0, 0)), b0 = ld_nt(val4 + vs(0, 1)), a1 = z2, b1 = z2, a2 = z2, b2 = z2;
-    if (ng > 1) { a1 = ld_nt(val4 + vs(1, 0)); b1 = ld_nt(val4 + vs(1, 1)); }
-    g0[0] = x[i + o0.x]; g0[1] = x[i + o0.y]; g0[2] = x[i + o0.z]; g0[3] = x[i + o0.w];
-    double bb = A.diag[i] * xi;
-    for (int gi = 0; gi < ng; gi++) {
-      if (gi + 2 < ng) {
-        const size_t g2 = (size_t)(gi + 2);
-        o2 = ld_nt(A.lnk4 + gs(g2));
-        a2 = ld_nt(val4 + vs(g2, 0));
-        b2 = ld_nt(val4 + vs(g2, 1));
-      }
-      if (gi + 1 < ng) { g1[0] = x[i + o1.x]; g1[1] = x[i + o1.y]; g1[2] = x[i + o1.z]; g1[3] = x[i + o1.w]; }
-      const int k = 4 * gi;
-      bb = bb + a0.x * g0[0];
-      if (k + 1 < m) bb = bb + a0.y * g0[1];
-      if (k + 2 < m) bb = bb + b0.x * g0[2];
-      if (k + 3 < m) bb = bb + b0.y * g0[3];
-      a0 = a1; b0 = b1; a1 = a2; b1 = b2; o1 = o2;
-      g0[0] = g1[0]; g0[1] = g1[1]; g0[2] = g1[2]; g0[3] = g1[3];
+// The row's block starts: up to kMaxBlk offsets in short4 groups, the groups the chunk uses.
+__device__ inline void block_offsets(const Mat& A, size_t c, int lane, int nb, short (&o)[kMaxBlk]) {
+#pragma unroll
+  for (int g = 0; g < kOffGroups; g++)
+    if (4 * g < nb) {
+      const short4 t = ld_nt(A.boff + (c * kOffGroups + g) * kChunk + lane);
+      o[4 * g] = t.x; o[4 * g + 1] = t.y; o[4 * g + 2] = t.z; o[4 * g + 3] = t.w;
     }
-    return bb;
-  }
 }
+typedef double xd2 __attribute__((ext_vector_type(2)));    // two gathered x, 16-byte aligned
 
-// ------------------------------------------------------------------------------------------------
-// Mixed path SpMV rows.  Compact rows: y_i = diag_i x_i + sum_k a32_k x_j in FP64, the values read
-// as FP32 (4 bytes per slot instead of 8).  The mixed path's kernel weights are FP32, so the values
-// -w_L are exact in FP32 and this is the FP64 compact row's sum on the same numbers (with fused
-// multiply-adds).  Same three-stage pipeline as the FP64 row; padded slots hold offset 0 and value 0.
-// Rows with mirror neighbours use the full FP64 row.  (Measured alternatives: recomputing the FP32
-// entries from the pair distance, three FP64 gathers per slot, was 1.5x slower than reading 4 bytes
-// per slot; folding the p and s projections into the SpMV, which then folds the partial sums in
-// every one of its workgroups, made the SpMV 1.5x slower than the separate projection kernels.)
-// ------------------------------------------------------------------------------------------------
-__device__ inline double row_f32(const Mat& A, int i, int m, const double* __restrict__ v, double xhat_i) {
-  const int ng = (m + 3) >> 2;
-  const size_t P = (size_t)A.choff[i / kChunk], lane = (size_t)(i % kChunk);
-  auto gs = [&](size_t g) { return (P + g) * kChunk + lane; };
-  const short4 zo = make_short4(0, 0, 0, 0);
-  const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
-  const short4 o0 = ld_nt(A.lnk4 + gs(0));
-  short4 o1 = zo, o2 = zo;
-  float4 a0 = ld_nt(A.val4f + gs(0)), a1 = z4, a2 = z4;
-  if (ng > 1) { o1 = ld_nt(A.lnk4 + gs(1)); a1 = ld_nt(A.val4f + gs(1)); }
-  double g0[4], g1[4] = {0.0, 0.0, 0.0, 0.0};
-  g0[0] = v[i + o0.x]; g0[1] = v[i + o0.y]; g0[2] = v[i + o0.z]; g0[3] = v[i + o0.w];
-  double acc = A.diag[i] * xhat_i;
-  for (int gi = 0; gi < ng; gi++) {
-    if (gi + 2 < ng) {
-      const size_t g2 = (size_t)(gi + 2);
-      o2 = ld_nt(A.lnk4 + gs(g2)); a2 = ld_nt(A.val4f + gs(g2));
+// A compact FP64 row from its blocks (val: bv64 for A, bv64T for the compact rows of A^T):
+// acc + sum over blocks and columns, in column order, of a * x (no fused multiply-add, as the FP64
+// path everywhere).  Two-stage pipeline: block b+1's values and gathers are in flight while block b
+// is summed.  The block loop is unrolled to kMaxBlk with a wavefront-uniform exit, so every block's
+// start comes from a register chosen at compile time (measured at the 1M cylinder, mixed row: 59.5 us
+// per SpMV against 76.5 us for the rolled loop, which picks o[b + 1] by compares).
+__device__ inline double row_blk64(const Mat& A, const double2* __restrict__ bv, int i,
+                                   const double* __restrict__ x, double acc) {
+  const size_t c = (size_t)i / kChunk;
+  const int lane = i % kChunk, nb = A.chunkb[c];
+  short o[kMaxBlk];
+  block_offsets(A, c, lane, nb, o);
+  auto vs = [&](int b, int h) { return ((c * kMaxBlk + b) * 4 + h) * kChunk + lane; };
+  double2 a0 = ld_nt(bv + vs(0, 0)), a1 = ld_nt(bv + vs(0, 1)), a2 = ld_nt(bv + vs(0, 2)), a3 = ld_nt(bv + vs(0, 3));
+  const xd2* xb = reinterpret_cast<const xd2*>(x + i + o[0]);
+  xd2 x0 = xb[0], x1 = xb[1], x2 = xb[2], x3 = xb[3];
+#pragma unroll
+  for (int b = 0; b < kMaxBlk; b++) {
+    if (b >= nb) break;
+    double2 n0, n1, n2, n3;
+    xd2 y0, y1, y2, y3;
+    if (b + 1 < nb && b + 1 < kMaxBlk) {
+      n0 = ld_nt(bv + vs(b + 1, 0)); n1 = ld_nt(bv + vs(b + 1, 1));
+      n2 = ld_nt(bv + vs(b + 1, 2)); n3 = ld_nt(bv + vs(b + 1, 3));
+      const xd2* xn = reinterpret_cast<const xd2*>(x + i + o[b + 1 < kMaxBlk ? b + 1 : 0]);
+      y0 = xn[0]; y1 = xn[1]; y2 = xn[2]; y3 = xn[3];
     }
-    if (gi + 1 < ng) { g1[0] = v[i + o1.x]; g1[1] = v[i + o1.y]; g1[2] = v[i + o1.z]; g1[3] = v[i + o1.w]; }
-    acc = fma((double)a0.x, g0[0], acc);
-    acc = fma((double)a0.y, g0[1], acc);
-    acc = fma((double)a0.z, g0[2], acc);
-    acc = fma((double)a0.w, g0[3], acc);
-    a0 = a1; a1 = a2; o1 = o2;
-    g0[0] = g1[0]; g0[1] = g1[1]; g0[2] = g1[2]; g0[3] = g1[3];
+    acc = acc + a0.x * x0.x; acc = acc + a0.y * x0.y;
+    acc = acc + a1.x * x1.x; acc = acc + a1.y * x1.y;
+    acc = acc + a2.x * x2.x; acc = acc + a2.y * x2.y;
+    acc = acc + a3.x * x3.x; acc = acc + a3.y * x3.y;
+    a0 = n0; a1 = n1; a2 = n2; a3 = n3; x0 = y0; x1 = y1; x2 = y2; x3 = y3;
   }
   return acc;
 }
 
+// ------------------------------------------------------------------------------------------------
+// Mixed path compact rows: y_i = diag_i x_i + sum a32 x_j in FP64, the values read as FP32 (4 bytes
+// per block column instead of 8).  The mixed path's kernel weights are FP32, so the values -w_L are
+// exact in FP32 and this is the FP64 compact row's sum on the same numbers (with fused multiply-adds).
+// Same two-stage pipeline as the FP64 row.  (Measured alternatives: recomputing the FP32 entries
+// from the pair distance, three FP64 gathers per slot, was 1.5x slower than reading the stored
+// values; folding the p and s projections into the SpMV, which then folds the partial sums in every
+// one of its workgroups, made the SpMV 1.5x slower than the separate projection kernels.)
+// ------------------------------------------------------------------------------------------------
+__device__ inline double row_blk32(const Mat& A, int i, const double* __restrict__ x, double acc) {
+  const size_t c = (size_t)i / kChunk;
+  const int lane = i % kChunk, nb = A.chunkb[c];
+  short o[kMaxBlk];
+  block_offsets(A, c, lane, nb, o);
+  auto vs = [&](int b, int h) { return ((c * kMaxBlk + b) * 2 + h) * kChunk + lane; };
+  float4 a0 = ld_nt(A.bv32 + vs(0, 0)), b0 = ld_nt(A.bv32 + vs(0, 1));
+  const xd2* xb = reinterpret_cast<const xd2*>(x + i + o[0]);
+  xd2 x0 = xb[0], x1 = xb[1], x2 = xb[2], x3 = xb[3];
+#pragma unroll
+  for (int b = 0; b < kMaxBlk; b++) {
+    if (b >= nb) break;
+    float4 a1, b1;
+    xd2 y0, y1, y2, y3;
+    if (b + 1 < nb && b + 1 < kMaxBlk) {
+      a1 = ld_nt(A.bv32 + vs(b + 1, 0)); b1 = ld_nt(A.bv32 + vs(b + 1, 1));
+      const xd2* xn = reinterpret_cast<const xd2*>(x + i + o[b + 1 < kMaxBlk ? b + 1 : 0]);
+      y0 = xn[0]; y1 = xn[1]; y2 = xn[2]; y3 = xn[3];
+    }
+    acc = fma((double)a0.x, x0.x, acc); acc = fma((double)a0.y, x0.y, acc);
+    acc = fma((double)a0.z, x1.x, acc); acc = fma((double)a0.w, x1.y, acc);
+    acc = fma((double)b0.x, x2.x, acc); acc = fma((double)b0.y, x2.y, acc);
+    acc = fma((double)b0.z, x3.x, acc); acc = fma((double)b0.w, x3.y, acc);
+    a0 = a1; b0 = b1; x0 = y0; x1 = y1; x2 = y2; x3 = y3;
+  }
+  return acc;
+}
+
+// A full row taken from its list-order copy (t-th listed row i), in the reference's slot order:
+// the same sum as row_full.
+__device__ inline double row_listed(const Mat& A, int t, int i, const double* __restrict__ x) {
+  return gather_sum(A.diag[i] * x[i], A.len[i], [&](int k) { return A.xval[xslot(t, k)]; },
+                    [&](int k) { return A.xcol[xslot(t, k)]; }, x);
+}
+
 __device__ inline double spmv_row_fp64(const Mat& A, int i, const double* __restrict__ x) {
-  return spmv_row_fp64_v(A, A.val4, i, x);
+  const int m = A.len[i];
+  const double xi = x[i];
+  if (!A.rowflag || A.rowflag[i]) return row_full(A, i, m, xi, x);
+  return row_blk64(A, A.bv64, i, x, A.diag[i] * xi);
 }
 
 __device__ inline double spmv_row_f32(const Mat& A, int i, const double* __restrict__ x) {
-  const int m = A.len[i];
   const double xi = x[i];
-  if (A.rowflag[i]) return row_full(A, i, m, xi, x);
-  return row_f32(A, i, m, x, xi);
+  if (A.rowflag[i]) return row_full(A, i, A.len[i], xi, x);
+  return row_blk32(A, i, x, A.diag[i] * xi);
 }
 
-// Row c of A^T: rows without mirror neighbours as compact FP64 rows (the forward row's offsets, the
+// Row c of A^T: rows without mirror neighbours as compact FP64 rows (the forward row's blocks, the
 // A^T value plane, column order), the others from the transposed map in c's slot order; then c's
 // mirror bucket.
 __device__ inline double spmv_row_transpose(const Mat& A, int c, const double* __restrict__ x) {
@@ -442,7 +487,7 @@ __device__ inline double spmv_row_transpose(const Mat& A, int c, const double* _
   const int pb = A.moff[c], pe = A.moff[c + 1];
   double acc;
   if (A.rowflag && !A.rowflag[c]) {
-    acc = spmv_row_fp64_v(A, A.val4T, c, x);
+    acc = row_blk64(A, A.bv64T, c, x, A.diag[c] * x[c]);
   } else {
     const size_t n = A.n;
     acc = gather_sum(A.diag[c] * x[c], A.len[c], [&](int k) { return A.tval[(size_t)k * n + c]; },
@@ -455,6 +500,32 @@ template <int MODE>
 __device__ inline double spmv_row(const Mat& A, int i, const double* __restrict__ x) {
   if (MODE == SPMV_TRANSPOSE) return spmv_row_transpose(A, i, x);
   return MODE == SPMV_MIXED ? spmv_row_f32(A, i, x) : spmv_row_fp64(A, i, x);
+}
+
+// The rows of one SpMV launch, out(i, (A x)_i) for each.  With A.nxb > 0 the first nxb workgroups
+// (a multiple of 8, so the others keep their XCD placement) take the listed full rows from their
+// list-order copy, and the other workgroups the compact rows, skipping the listed ones: a full row
+// is a chain of ~6 dependent batches of uncoalesced loads, and inside a wavefront of compact rows
+// it held the whole wavefront (measured at the 1M cylinder, 2% of the rows: ~19 us of an 80 us
+// SpMV).  With A.nxb = 0 every row is taken in place.
+template <int MODE, class Out>
+__device__ inline void spmv_rows(const Mat& A, const double* __restrict__ x, Out&& out) {
+  if (A.nxb > 0) {
+    if ((int)blockIdx.x < A.nxb) {
+      for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < A.nx; t += A.nxb * blockDim.x)
+        out(A.xrows[t], row_listed(A, t, A.xrows[t], x));
+      return;
+    }
+    const RowRange rr = block_rows(A.n, (int)blockIdx.x - A.nxb, (int)gridDim.x - A.nxb);
+    for (int i = rr.begin + threadIdx.x; i < rr.end; i += blockDim.x) {
+      if (A.rowflag[i]) continue;
+      const double d = A.diag[i] * x[i];
+      out(i, MODE == SPMV_MIXED ? row_blk32(A, i, x, d) : row_blk64(A, A.bv64, i, x, d));
+    }
+    return;
+  }
+  const RowRange rr = block_rows(A.n, gridDim.x);
+  for (int i = rr.begin + threadIdx.x; i < rr.end; i += blockDim.x) out(i, spmv_row<MODE>(A, i, x));
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -579,17 +650,16 @@ template <int MODE, bool FUSED = false>
 __global__ void k_bi_spmv_p(Mat A, Vecs w, RedBufs rb, const Scal* sc) {
   if (finished(sc)) return;
   const int n = A.n;
-  RowRange rr = block_rows(n, gridDim.x);
   Red<FUSED ? 2 : 1> R(rb.R(ST_SPMV_P), n);
   const bool scaled = FUSED && !w.jac;           // closed five-kernel form: A p_inv = c A p
   const double* x = scaled ? w.pn : w.pinv;
-  for (int i = rr.begin + threadIdx.x; i < rr.end; i += blockDim.x) {
-    double bb = spmv_row<MODE>(A, i, x);
+  auto out = [&](int i, double bb) {
     if (scaled) bb = bb * kInvOnePlusEps;
     w.v[i] = bb;
     R.add(0, i, w.rb[i] * bb);
     if (FUSED) R.add(FUSED ? 1 : 0, i, bb);
-  }
+  };
+  spmv_rows<MODE>(A, x, out);
   R.store(rb.P(ST_SPMV_P));
 }
 
@@ -652,19 +722,18 @@ template <int MODE, bool FUSED = false>
 __global__ void k_bi_spmv_s(Mat A, Vecs w, RedBufs rb, const Scal* sc) {
   if (finished(sc)) return;
   const int n = A.n;
-  RowRange rr = block_rows(n, gridDim.x);
   const double mean = FUSED ? 0.0 : sc->mean_s;
   Red<2> R(rb.R(ST_SPMV_S), n);
   const bool scaled = FUSED && !w.jac;           // closed five-kernel form: A s_inv = c A s
   const double* x = scaled ? w.s : w.sinv;
-  for (int i = rr.begin + threadIdx.x; i < rr.end; i += blockDim.x) {
-    double bb = spmv_row<MODE>(A, i, x);
+  auto out = [&](int i, double bb) {
     if (scaled) bb = bb * kInvOnePlusEps;
     w.t[i] = bb;
     double si = w.s[i] - mean;
     R.add(0, i, bb * si);
     R.add(1, i, bb * bb);
-  }
+  };
+  spmv_rows<MODE>(A, x, out);
   R.store(rb.P(ST_SPMV_S));
 }
 
@@ -918,11 +987,22 @@ __global__ void k_fu_fin(RedBufs rb, int nblk, Scal* sc, int k) {
 static Mat step_matrix(vb_ctx* c) {
   Mat A;
   A.diag = c->a_diag; A.val = c->a_val; A.col = c->a_col; A.len = c->cnt;
-  A.rowflag = c->rowflag; A.lnk4 = (const short4*)c->lnk4; A.val4 = (const double2*)c->val4;
-  A.val4f = (const float4*)c->val4f; A.choff = c->choff; A.n = c->npfb;
+  A.rowflag = c->rowflag; A.chunkb = c->chunkb; A.boff = (const short4*)c->boff;
+  A.bv64 = (const double2*)c->bv64; A.bv32 = (const float4*)c->bv32; A.n = c->npfb;
+  A.xrows = c->xrows; A.xcol = c->xcol; A.xval = c->xval; A.nx = 0; A.nxb = 0;
   A.tval = c->op.tval; A.mval = c->op.mval; A.tcol = c->op.tcol; A.moff = c->op.moff; A.mrow = c->op.mrow;
-  A.val4T = (const double2*)c->op.val4T;
+  A.bv64T = (const double2*)c->op.bv64T;
   return A;
+}
+
+// The forward solver's SpMVs take the assembled step matrix's listed full rows in workgroups of their
+// own (spmv_rows); nexplicit is the count the assembly synchronised.
+static void split_listed_rows(vb_ctx* c, Mat& A) {
+  if (!A.rowflag) return;
+  A.nx = c->scal_host->nexplicit;
+  int b = (A.nx + kSpmvBlock - 1) / kSpmvBlock;
+  if (b > kSpmvListBlocks) b = kSpmvListBlocks;
+  A.nxb = (b + 7) / 8 * 8;
 }
 
 static RedBufs red_bufs(vb_ctx* c, int n) { return RedBufs{c->partials, c->red, n}; }
@@ -931,7 +1011,7 @@ template <int MODE>
 static vb_status solve(vb_ctx* c, const Mat& A, Vecs w, int x_given, double tol, int itermax) {
   const int n = A.n;
   const int nb = grid_for(n);          // streaming vector kernels
-  const int ns = spmv_grid(n);         // SpMV kernels, one row per thread
+  const int ns = spmv_grid(n) + A.nxb; // SpMV kernels, one row per thread (+ the listed rows' workgroups)
   RedBufs rb = red_bufs(c, n);
   Scal* sc = c->scal;
   const bool fused = rb.rows == nullptr;   // serial mode: the reference's eight-kernel order
@@ -991,18 +1071,15 @@ vb_status assemble_ppe(vb_ctx* c) {
   VB_FIN(c, k_fin_ppe_rhs, c->partials, nb, c->red, c->npfb, c->g.open, c->scal);
   sw = with_weights(c, [&](auto*, auto*, auto* wl) -> vb_status {
     VB_LAUNCH(c, k_assemble<weight_t<decltype(wl)>>, xcd_grid(c->npfb), c->g, c->st, c->cnt, c->link, wl,
-              c->nsurf, c->pscale, c->rhs, c->a_diag, c->a_col, c->a_val, c->rowflag, c->chunkg,
-              c->scal);
+              c->nsurf, c->pscale, c->rhs, c->a_diag, c->a_col, c->a_val, c->rowflag, c->scal);
     return VB_OK;
   });
   if (sw != VB_OK) return sw;
-  const int nch = (c->npfb + kChunk - 1) / kChunk;
-  vb_status ss = launch_exclusive_scan(c, c->chunkg, c->choff, nch);
-  if (ss != VB_OK) return ss;
-  VB_LAUNCH(c, k_assemble_sliced, grid_1d(c->npfb), c->npfb, c->cnt, c->link, c->a_val, c->rowflag,
-            c->choff, c->lnk4, c->val4, c->prm.precision == VB_PREC_MIXED ? c->val4f : nullptr);
-  // rows with mirror neighbours: listed, counted (one synchronisation), one workgroup each
+  VB_LAUNCH(c, k_assemble_blocks, grid_1d(c->npfb), c->npfb, c->cnt, c->link, c->a_val, c->rowflag,
+            c->chunkb, c->boff, c->bv64, c->prm.precision == VB_PREC_MIXED ? c->bv32 : nullptr, c->bpos);
+  // the rows stored full (rowflag 1): listed, counted (one synchronisation), one workgroup each
   VB_LAUNCH(c, k_flag_to_int, grid_1d(c->npfb), c->npfb, c->rowflag, c->xflag);
+  vb_status ss;
   if ((ss = launch_exclusive_scan(c, c->xflag, c->xscan, c->npfb)) != VB_OK) return ss;
   VB_LAUNCH(c, k_list_explicit, grid_1d(c->npfb), c->npfb, c->rowflag, c->xscan, c->xrows, c->scal);
   if ((ss = sync_scalars(c)) != VB_OK) return ss;
@@ -1011,7 +1088,7 @@ vb_status assemble_ppe(vb_ctx* c) {
     sw = with_weights(c, [&](auto*, auto*, auto* wl) -> vb_status {
       hipLaunchKernelGGL(k_assemble_explicit<weight_t<decltype(wl)>>, dim3(nx), dim3(kExplicitThreads), 0,
                          c->stream, c->npfb, c->xrows, nx, c->cnt, c->link, wl, c->st.irel, c->pscale, c->a_diag,
-                         c->a_col, c->a_val);
+                         c->a_col, c->a_val, c->xcol, c->xval);
       VB_HIP(c, hipGetLastError());
       return VB_OK;
     });
@@ -1026,9 +1103,10 @@ vb_status launch_ppe(vb_ctx* c, double tol, int itermax) {
   if (sa != VB_OK) return sa;
   Mat A = step_matrix(c);
   Vecs w = step_vecs(c, c->st.p, c->g.open != 0);
-  if (c->prm.precision == VB_PREC_MIXED) return solve<SPMV_MIXED>(c, A, w, 0, tol, itermax);
   // FP64 from the stored rows in the reference's slot order: on request, and in serial mode
-  if (c->spmv_stored || c->red) A.rowflag = nullptr;
+  if ((c->spmv_stored || c->red) && c->prm.precision != VB_PREC_MIXED) A.rowflag = nullptr;
+  if (!c->red) split_listed_rows(c, A);
+  if (c->prm.precision == VB_PREC_MIXED) return solve<SPMV_MIXED>(c, A, w, 0, tol, itermax);
   return solve<SPMV_FP64>(c, A, w, 0, tol, itermax);
 }
 
@@ -1209,6 +1287,11 @@ vb_status apply_solver_spmv(vb_ctx* c, bool transpose, const double* x, double* 
     vb_status s = build_transpose(c);
     if (s != VB_OK) return s;
   }
+  // the compact rows gather x two elements at a time from even columns
+  if (reinterpret_cast<uintptr_t>(x) % 16 != 0) {
+    c->err = "the solver's SpMV needs x aligned to 16 bytes";
+    return VB_ERR_ARG;
+  }
   Mat A = step_matrix(c);
   if (c->spmv_stored || c->red) A.rowflag = nullptr;
   if (transpose) VB_LAUNCH(c, k_apply_solver_row<SPMV_TRANSPOSE>, grid_1d(c->npfb), A, x, y);
@@ -1236,10 +1319,12 @@ __global__ void k_bi_set_live(Scal* sc, int* counts_out, const int* cnt, const u
 }
 
 vb_status profile_spmv(vb_ctx* c, int reps, double* avg_us, double* alg_bytes) {
-  const int n = c->npfb, nb = spmv_grid(n);
+  const int n = c->npfb;
   const bool mixed = c->prm.precision == VB_PREC_MIXED;
   Mat A = step_matrix(c);
   if ((c->spmv_stored || c->red) && !mixed) A.rowflag = nullptr;
+  if (!c->red) split_listed_rows(c, A);
+  const int nb = spmv_grid(n) + A.nxb;           // the launch the solve makes
   Vecs w = step_vecs(c, c->st.p, c->g.open != 0);
   int* d_counts = c->seg_counts;
   hipLaunchKernelGGL(k_bi_set_live, dim3(1), dim3(kBlock), 0, c->stream, c->scal, d_counts, c->cnt,
@@ -1269,10 +1354,11 @@ vb_status profile_spmv(vb_ctx* c, int reps, double* avg_us, double* alg_bytes) {
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   *avg_us = 1000.0 * ms / reps;
-  // Algorithmic bytes of one launch.  FP64: per row length 4, row flag 1, diagonal 8, the gathered
-  // vector 8 (each element once), r_bar 8, v written 8; per slot the 2-byte column offset and the
-  // 8-byte value on compact rows, 4 + 8 on full rows.  Mixed, stored values: the same per row (row
-  // sum for the diagonal) and 2 + 4 bytes per compact slot.
+  // Algorithmic bytes of one launch, per nonzero whatever the storage: per row length 4, row flag 1,
+  // diagonal 8, the gathered vector 8 (each element once), r_bar 8, v written 8; per off-diagonal
+  // nonzero a 2-byte relative column and the value (8 bytes FP64, 4 mixed) on compact rows, a 4-byte
+  // column and an 8-byte value on full rows.  (The column blocks store one 2-byte start per block
+  // instead of a column per nonzero, and zeros where a block column is not a neighbour.)
   const double nnz = counts[0], nnz_full = A.rowflag ? counts[1] : counts[0];
   *alg_bytes = 37.0 * n + (mixed ? 6.0 : 10.0) * (nnz - nnz_full) + 12.0 * nnz_full;
   return VB_OK;
