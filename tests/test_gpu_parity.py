@@ -111,6 +111,83 @@ def test_step_phases_against_oracle(gpu_lib, name):
                 assert _rel(a["u"], b["u"]) <= 1e-5 and _rel(a["v"], b["v"]) <= 1e-5
 
 
+def _column_blocks(cnt, link, npfb):
+    """Host count of the 8-column blocks with even starts that cover each row's neighbour columns
+    in column order (the solver's compact-row storage); -1 for rows with a mirror neighbour."""
+    nb = np.full(npfb, -1)
+    for i in range(npfb):
+        cols = np.sort(np.asarray(link[i, :cnt[i]], dtype=np.int64) - 1)
+        if (cols >= npfb).any():
+            continue
+        end, b = -1, 0
+        for c in cols:
+            if c > end:
+                b += 1
+                end = (c & ~1) + 7
+        nb[i] = b
+    return nb
+
+
+@pytest.mark.parametrize("mix", ["shuffled", "swapped"])
+def test_renumbered_particles_against_oracle(gpu_lib, mix):
+    """The solver's SpMV stores a row as blocks of consecutive columns when it fits in 12 of them and
+    keeps the other rows full.  With the fluid particles renumbered at random ("shuffled"), almost
+    no row fits; with 10% of them swapped pairwise ("swapped"), fitting rows (with padding blocks)
+    and full rows share the same wavefronts.  Two steps against the oracle on the same renumbered
+    start, with the bars of test_step_phases_against_oracle."""
+    case = SMALL["poiseuille"]()
+    s = solver.initial_state(case)
+    rng = np.random.default_rng(11)
+    nb, n = case.nbS, case.npfb
+    perm = np.arange(n)
+    if mix == "shuffled":
+        perm[nb:] = nb + rng.permutation(n - nb)
+    else:
+        pick = nb + rng.choice(n - nb, size=(n - nb) // 20 * 2, replace=False)
+        perm[pick[0::2]], perm[pick[1::2]] = pick[1::2], pick[0::2]
+    s = {k: np.asarray(v)[perm].copy() for k, v in s.items()}
+    G = solver.Solver(case, state=s); O = ow.OracleSolver(case, state=s)
+    for step in range(2):
+        first = step == 0
+        for k in range(1, 13):
+            G.phase(k); O.phase(k)
+            if k == 4:
+                cg, lg = G.neighbours(); co, lo = O.neighbours()
+                _eq(cg, co); _eq(lg, lo)
+                if first:
+                    blocks = _column_blocks(co, lo, n)
+                    over = np.mean(blocks[blocks >= 0] > 12)
+                    print(f"{mix}: {over:.2f} of the rows without mirrors need more than 12 blocks",
+                          flush=True)
+                    assert over > (0.9 if mix == "shuffled" else 0.05)
+                    if mix == "swapped":
+                        assert np.mean((blocks >= 0) & (blocks <= 12)) > 0.5
+            if k == 10:
+                so, sg = O.scalars(), G.scalars()
+                w = so["maxneighbours"]
+                diag, col, val, rhs = G.ppe_ell(w)
+                a_g, ija_g = solver.ell_to_nr(diag, col, val, G.neighbours()[0])
+                a_o, ija_o, b_o = O.ppe()
+                _eq(ija_g, ija_o)
+                if first:
+                    _eq(a_g, a_o)
+                assert sg["LS_residual"] <= 1e-8 and 1 <= sg["LS_iters"] <= 1000
+                assert _rel(G.get_state()["p"], O.get_state()["p"]) <= 1e-3
+            if k == 12:
+                a, b = G.get_state(), O.get_state()
+                assert _rel(a["x"], b["x"]) <= 1e-10 and _rel(a["y"], b["y"]) <= 1e-10
+                assert _rel(a["u"], b["u"]) <= 1e-5 and _rel(a["v"], b["v"]) <= 1e-5
+    # the solved pressure satisfies the assembled system in FP64 (the SpMV of every row kind)
+    diag, col, val, rhs = G.ppe_ell(O.scalars()["maxneighbours"])
+    r = rhs - G.ppe_operator(G.get_state()["p"])
+    r = r - r.mean()
+    assert np.linalg.norm(r) <= 1e-7 * np.linalg.norm(rhs)
+    # and the solver's own rows (blocks, padding blocks, full rows) are A x
+    x = rng.standard_normal(n)
+    y_op, y_solver = G.ppe_operator(x), G.ppe_operator(x, solver_rows=True)
+    assert np.max(np.abs(y_solver - y_op)) <= 1e-13 * np.max(np.abs(y_op))
+
+
 @pytest.mark.parametrize("name", list(SMALL))
 def test_bicgstab_bit_exact_in_serial_order(gpu_lib, name):
     """The GPU BiCGSTAB on the oracle's assembled PPE, with its global sums taken in index order (the
