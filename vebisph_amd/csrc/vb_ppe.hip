@@ -386,39 +386,57 @@ __device__ inline void block_offsets(const Mat& A, size_t c, int lane, int nb, s
 }
 typedef double xd2 __attribute__((ext_vector_type(2)));    // two gathered x, 16-byte aligned
 
+// The chunk of row i, as a wavefront-uniform value.  In every caller a wavefront's rows are one
+// 64-row chunk (rows come in runs of a multiple of 64 from block_rows or blockIdx * 256), so the
+// chunk and its block count can live in scalar registers: the block loop then unrolls with scalar
+// exits instead of a per-lane bound, register rotation and a wait for every load at each step.
+__device__ inline int uniform_chunk(int i) { return __builtin_amdgcn_readfirstlane(i / kChunk); }
+
+// The block steps of a compact row as straight-line code: step B issues block B+1's loads, then
+// sums block B, for B < nb (wavefront-uniform, so every exit is a scalar branch).  Written as a
+// template recursion rather than a loop: with a loop the compiler kept a rolled two-block body whose
+// back edge rotates the in-flight registers and waits for every outstanding load at each turn.
+template <int B, class Load, class Sum>
+__device__ __forceinline__ void block_steps(int nb, Load&& load, Sum&& sum) {
+  if constexpr (B < kMaxBlk) {
+    if (B >= nb) return;
+    if (B + 1 < nb) load(B + 1);
+    sum(B);
+    block_steps<B + 1>(nb, load, sum);
+  }
+}
+
 // A compact FP64 row from its blocks (val: bv64 for A, bv64T for the compact rows of A^T):
 // acc + sum over blocks and columns, in column order, of a * x (no fused multiply-add, as the FP64
 // path everywhere).  Two-stage pipeline: block b+1's values and gathers are in flight while block b
-// is summed.  The block loop is unrolled to kMaxBlk with a wavefront-uniform exit, so every block's
-// start comes from a register chosen at compile time (measured at the 1M cylinder, mixed row: 59.5 us
-// per SpMV against 76.5 us for the rolled loop, which picks o[b + 1] by compares).
+// is summed.
 __device__ inline double row_blk64(const Mat& A, const double2* __restrict__ bv, int i,
                                    const double* __restrict__ x, double acc) {
-  const size_t c = (size_t)i / kChunk;
-  const int lane = i % kChunk, nb = A.chunkb[c];
+  const size_t c = (size_t)uniform_chunk(i);
+  const int lane = i % kChunk, nb = __builtin_amdgcn_readfirstlane(A.chunkb[c]);
   short o[kMaxBlk];
   block_offsets(A, c, lane, nb, o);
   auto vs = [&](int b, int h) { return ((c * kMaxBlk + b) * 4 + h) * kChunk + lane; };
-  double2 a0 = ld_nt(bv + vs(0, 0)), a1 = ld_nt(bv + vs(0, 1)), a2 = ld_nt(bv + vs(0, 2)), a3 = ld_nt(bv + vs(0, 3));
-  const xd2* xb = reinterpret_cast<const xd2*>(x + i + o[0]);
-  xd2 x0 = xb[0], x1 = xb[1], x2 = xb[2], x3 = xb[3];
+  double2 va[2][4];
+  xd2 xg[2][4];
+  auto load = [&](int b) {
+    const int t = b & 1;
 #pragma unroll
-  for (int b = 0; b < kMaxBlk; b++) {
-    if (b >= nb) break;
-    double2 n0, n1, n2, n3;
-    xd2 y0, y1, y2, y3;
-    if (b + 1 < nb && b + 1 < kMaxBlk) {
-      n0 = ld_nt(bv + vs(b + 1, 0)); n1 = ld_nt(bv + vs(b + 1, 1));
-      n2 = ld_nt(bv + vs(b + 1, 2)); n3 = ld_nt(bv + vs(b + 1, 3));
-      const xd2* xn = reinterpret_cast<const xd2*>(x + i + o[b + 1 < kMaxBlk ? b + 1 : 0]);
-      y0 = xn[0]; y1 = xn[1]; y2 = xn[2]; y3 = xn[3];
+    for (int h = 0; h < 4; h++) va[t][h] = ld_nt(bv + vs(b, h));
+    const xd2* xn = reinterpret_cast<const xd2*>(x + i + o[b]);
+#pragma unroll
+    for (int h = 0; h < 4; h++) xg[t][h] = xn[h];
+  };
+  auto sum = [&](int b) {
+    const int t = b & 1;
+#pragma unroll
+    for (int h = 0; h < 4; h++) {
+      acc = acc + va[t][h].x * xg[t][h].x;
+      acc = acc + va[t][h].y * xg[t][h].y;
     }
-    acc = acc + a0.x * x0.x; acc = acc + a0.y * x0.y;
-    acc = acc + a1.x * x1.x; acc = acc + a1.y * x1.y;
-    acc = acc + a2.x * x2.x; acc = acc + a2.y * x2.y;
-    acc = acc + a3.x * x3.x; acc = acc + a3.y * x3.y;
-    a0 = n0; a1 = n1; a2 = n2; a3 = n3; x0 = y0; x1 = y1; x2 = y2; x3 = y3;
-  }
+  };
+  load(0);
+  block_steps<0>(nb, load, sum);
   return acc;
 }
 
@@ -432,30 +450,29 @@ __device__ inline double row_blk64(const Mat& A, const double2* __restrict__ bv,
 // one of its workgroups, made the SpMV 1.5x slower than the separate projection kernels.)
 // ------------------------------------------------------------------------------------------------
 __device__ inline double row_blk32(const Mat& A, int i, const double* __restrict__ x, double acc) {
-  const size_t c = (size_t)i / kChunk;
-  const int lane = i % kChunk, nb = A.chunkb[c];
+  const size_t c = (size_t)uniform_chunk(i);
+  const int lane = i % kChunk, nb = __builtin_amdgcn_readfirstlane(A.chunkb[c]);
   short o[kMaxBlk];
   block_offsets(A, c, lane, nb, o);
   auto vs = [&](int b, int h) { return ((c * kMaxBlk + b) * 2 + h) * kChunk + lane; };
-  float4 a0 = ld_nt(A.bv32 + vs(0, 0)), b0 = ld_nt(A.bv32 + vs(0, 1));
-  const xd2* xb = reinterpret_cast<const xd2*>(x + i + o[0]);
-  xd2 x0 = xb[0], x1 = xb[1], x2 = xb[2], x3 = xb[3];
+  float4 va[2][2];
+  xd2 xg[2][4];
+  auto load = [&](int b) {
+    const int t = b & 1;
+    va[t][0] = ld_nt(A.bv32 + vs(b, 0)); va[t][1] = ld_nt(A.bv32 + vs(b, 1));
+    const xd2* xn = reinterpret_cast<const xd2*>(x + i + o[b]);
 #pragma unroll
-  for (int b = 0; b < kMaxBlk; b++) {
-    if (b >= nb) break;
-    float4 a1, b1;
-    xd2 y0, y1, y2, y3;
-    if (b + 1 < nb && b + 1 < kMaxBlk) {
-      a1 = ld_nt(A.bv32 + vs(b + 1, 0)); b1 = ld_nt(A.bv32 + vs(b + 1, 1));
-      const xd2* xn = reinterpret_cast<const xd2*>(x + i + o[b + 1 < kMaxBlk ? b + 1 : 0]);
-      y0 = xn[0]; y1 = xn[1]; y2 = xn[2]; y3 = xn[3];
-    }
-    acc = fma((double)a0.x, x0.x, acc); acc = fma((double)a0.y, x0.y, acc);
-    acc = fma((double)a0.z, x1.x, acc); acc = fma((double)a0.w, x1.y, acc);
-    acc = fma((double)b0.x, x2.x, acc); acc = fma((double)b0.y, x2.y, acc);
-    acc = fma((double)b0.z, x3.x, acc); acc = fma((double)b0.w, x3.y, acc);
-    a0 = a1; b0 = b1; x0 = y0; x1 = y1; x2 = y2; x3 = y3;
-  }
+    for (int h = 0; h < 4; h++) xg[t][h] = xn[h];
+  };
+  auto sum = [&](int b) {
+    const int t = b & 1;
+    acc = fma((double)va[t][0].x, xg[t][0].x, acc); acc = fma((double)va[t][0].y, xg[t][0].y, acc);
+    acc = fma((double)va[t][0].z, xg[t][1].x, acc); acc = fma((double)va[t][0].w, xg[t][1].y, acc);
+    acc = fma((double)va[t][1].x, xg[t][2].x, acc); acc = fma((double)va[t][1].y, xg[t][2].y, acc);
+    acc = fma((double)va[t][1].z, xg[t][3].x, acc); acc = fma((double)va[t][1].w, xg[t][3].y, acc);
+  };
+  load(0);
+  block_steps<0>(nb, load, sum);
   return acc;
 }
 
