@@ -298,9 +298,26 @@ __global__ void k_transpose_blocks(int npfb, const int* cnt, const double* tval,
   }
 }
 
+// The listed rows of A^T (their transposed map rows, c's slot order) copied in list order, as the
+// forward's listed rows are (xslot), for the adjoint solve's SpMV.
+__global__ void k_transpose_listed(int npfb, const int* rows, int nrows, const int* cnt, const double* tval,
+                                   const int* tcol, int* xtcol, double* xtval) {
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  if (t >= nrows) return;
+  const int i = rows[t], n = cnt[i];
+  for (int k = 0; k < n; k++) {
+    xtcol[xslot(t, k)] = tcol[(size_t)k * npfb + i];
+    xtval[xslot(t, k)] = tval[(size_t)k * npfb + i];
+  }
+}
+
 vb_status launch_transpose_blocks(vb_ctx* c) {
   VB_LAUNCH(c, k_transpose_blocks, grid_1d(c->npfb), c->npfb, c->cnt, c->op.tval, c->rowflag, c->chunkb,
             c->bpos, c->op.bv64T);
+  const int nx = c->scal_host->nexplicit;       // synchronised by the assembly
+  if (nx > 0)
+    VB_LAUNCH(c, k_transpose_listed, grid_1d(nx), c->npfb, c->xrows, nx, c->cnt, c->op.tval, c->op.tcol,
+              c->op.xtcol, c->op.xtval);
   return VB_OK;
 }
 
@@ -322,8 +339,10 @@ struct Mat {
   const double* xval;
   int nx, nxb;                     // listed rows; their workgroups (0: full rows inline)
   // the transposed map (SPMV_TRANSPOSE): real part slot-major like the lists, mirror buckets per column,
-  // and the compact rows' A^T values (the forward blocks' starts)
+  // and the compact rows' A^T values (the forward blocks' starts); the listed rows' A^T rows in list order
   const double2* bv64T;
+  const int* xtcol;
+  const double* xtval;
   const double *tval, *mval;
   const int *tcol, *moff, *mrow;
   int n;
@@ -500,13 +519,16 @@ __device__ inline double spmv_row_f32(const Mat& A, int i, const double* __restr
 }
 
 // Row c of A^T: rows without mirror neighbours as compact FP64 rows (the forward row's blocks, the
-// A^T value plane, column order), the others from the transposed map in c's slot order; then c's
-// mirror bucket.
-__device__ inline double spmv_row_transpose(const Mat& A, int c, const double* __restrict__ x) {
+// A^T value plane, column order), the others from the transposed map in c's slot order (from its
+// list-order copy when t >= 0 is the row's list position); then c's mirror bucket.
+__device__ inline double spmv_row_transpose(const Mat& A, int c, const double* __restrict__ x, int t = -1) {
   // the bucket bounds first: their loads overlap the row's chain instead of following it
   const int pb = A.moff[c], pe = A.moff[c + 1];
   double acc;
-  if (A.rowflag && !A.rowflag[c]) {
+  if (t >= 0) {
+    acc = gather_sum(A.diag[c] * x[c], A.len[c], [&](int k) { return A.xtval[xslot(t, k)]; },
+                     [&](int k) { return A.xtcol[xslot(t, k)]; }, x);
+  } else if (A.rowflag && !A.rowflag[c]) {
     acc = row_blk64(A, A.bv64T, c, x, A.diag[c] * x[c]);
   } else {
     const size_t n = A.n;
@@ -532,15 +554,18 @@ template <int MODE, class Out>
 __device__ inline void spmv_rows(const Mat& A, const double* __restrict__ x, Out&& out) {
   if (A.nxb > 0) {
     if ((int)blockIdx.x < A.nxb) {
-      for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < A.nx; t += A.nxb * blockDim.x)
-        out(A.xrows[t], row_listed(A, t, A.xrows[t], x));
+      for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < A.nx; t += A.nxb * blockDim.x) {
+        const int i = A.xrows[t];
+        out(i, MODE == SPMV_TRANSPOSE ? spmv_row_transpose(A, i, x, t) : row_listed(A, t, i, x));
+      }
       return;
     }
     const RowRange rr = block_rows(A.n, (int)blockIdx.x - A.nxb, (int)gridDim.x - A.nxb);
     for (int i = rr.begin + threadIdx.x; i < rr.end; i += blockDim.x) {
       if (A.rowflag[i]) continue;
       const double d = A.diag[i] * x[i];
-      out(i, MODE == SPMV_MIXED ? row_blk32(A, i, x, d) : row_blk64(A, A.bv64, i, x, d));
+      if (MODE == SPMV_TRANSPOSE) out(i, spmv_row_transpose(A, i, x));
+      else out(i, MODE == SPMV_MIXED ? row_blk32(A, i, x, d) : row_blk64(A, A.bv64, i, x, d));
     }
     return;
   }
@@ -1011,12 +1036,12 @@ static Mat step_matrix(vb_ctx* c) {
   A.bv64 = (const double2*)c->bv64; A.bv32 = (const float4*)c->bv32; A.n = c->npfb;
   A.xrows = c->xrows; A.xcol = c->xcol; A.xval = c->xval; A.nx = 0; A.nxb = 0;
   A.tval = c->op.tval; A.mval = c->op.mval; A.tcol = c->op.tcol; A.moff = c->op.moff; A.mrow = c->op.mrow;
-  A.bv64T = (const double2*)c->op.bv64T;
+  A.bv64T = (const double2*)c->op.bv64T; A.xtcol = c->op.xtcol; A.xtval = c->op.xtval;
   return A;
 }
 
-// The forward solver's SpMVs take the assembled step matrix's listed full rows in workgroups of their
-// own (spmv_rows); nexplicit is the count the assembly synchronised.
+// The solvers' SpMVs (forward and adjoint) take the assembled step matrix's listed full rows in
+// workgroups of their own (spmv_rows); nexplicit is the count the assembly synchronised.
 static void split_listed_rows(vb_ctx* c, Mat& A) {
   if (!A.rowflag) return;
   A.nx = c->scal_host->nexplicit;
@@ -1250,6 +1275,8 @@ vb_status adjoint_solve(vb_ctx* c, double tol, int itermax, int* iters, double* 
   OperatorMap& op = c->op;
   const int n = c->npfb, nb = grid_for(n), open = c->g.open ? 1 : 0;
   Mat A = step_matrix(c);
+  Mat As = A;                                   // the solve's SpMVs: listed rows in workgroups of their own
+  if (!c->red) split_listed_rows(c, As);
   Vecs w{op.dref, op.bref, c->r, c->rb, c->vv, c->pn, c->pinv, c->ss, c->sinv, c->tt, open ? c->a_diag : nullptr};
   double h[2], gnorm = 0.0, rnorm = 0.0;
   int total = 0, st = VB_OK;
@@ -1271,7 +1298,7 @@ vb_status adjoint_solve(vb_ctx* c, double tol, int itermax, int* iters, double* 
     // 3 of margin, and at least a tenfold reduction) once the residual is close to it
     const double inner = std::min(0.1, std::max(kRefineInnerTol, 0.3 * tol * gnorm / rnorm));
     VB_LAUNCH(c, k_ref_scale, grid_1d(n), n, op.rref, op.refs, op.bref);
-    if ((s = solve<SPMV_TRANSPOSE>(c, A, w, 0, inner, itermax - total)) != VB_OK) return s;
+    if ((s = solve<SPMV_TRANSPOSE>(c, As, w, 0, inner, itermax - total)) != VB_OK) return s;
     total += std::min(c->scal_host->iter, itermax - total);
     if (c->scal_host->status == VB_WARN_BICGSTAB_BREAKDOWN) st = VB_WARN_BICGSTAB_BREAKDOWN;
     VB_LAUNCH(c, k_ref_update, grid_1d(n), n, op.lam, op.dref, op.refs, round == 0 ? 1 : 0);
