@@ -259,6 +259,53 @@ def adjoint_run_cost(workload, nsteps, every, precision="fp64"):
     return out
 
 
+def drift(case, nsteps, every):
+    """Mixed precision against FP64 over a run, beside the FP64 path's own spread: mixed, FP64, and
+    FP64 from the start with every fluid coordinate moved by one ulp (seeded) step in lockstep; every
+    `every` steps, max |a - FP64| / max |FP64| of each field for a = mixed ("mixed") and a = the
+    moved FP64 run ("spread"), positions in units of dx, and the force on each circle (the C3
+    configuration's drag, vb_circle_forces)."""
+    import numpy as np
+    from vebisph_amd import _lib, solver
+    moved = solver.initial_state(case)
+    rng = np.random.default_rng(7)
+    for key in ("x", "y"):
+        v = np.array(moved[key], dtype=np.float64)
+        up = rng.random(v.size - case.nbS) < 0.5
+        v[case.nbS:] = np.where(up, np.nextafter(v[case.nbS:], np.inf), np.nextafter(v[case.nbS:], -np.inf))
+        moved[key] = v
+    runs = {"mixed": solver.Solver(case, precision=_lib.PREC_MIXED),
+            "spread": solver.Solver(case, precision=_lib.PREC_FP64, state=moved)}
+    f = solver.Solver(case, precision=_lib.PREC_FP64)
+    out = []
+    rel = lambda a, b: float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+    for k in range(1, nsteps + 1):
+        f.div_free()
+        for r in runs.values():
+            r.div_free()
+        if k % every and k != nsteps:
+            continue
+        b = f.get_state()
+        rec = {"step": k, "iters_fp64": f.scalars()["LS_iters"]}
+        ff = f.circle_forces()[0] if case.nb_circles else None
+        for name, r in runs.items():
+            a = r.get_state()
+            d = {key: rel(a[key], b[key]) for key in ("u", "v", "p", "tau", "trace")}
+            d["pos_dx"] = float(max(np.max(np.abs(a["x"] - b["x"])), np.max(np.abs(a["y"] - b["y"]))) / case.dx)
+            if case.nb_circles:
+                d["force_rel"] = rel(r.circle_forces()[0], ff)
+            rec[name] = d
+        if case.nb_circles:
+            rec["force_fp64"] = ff.tolist()
+        out.append(rec)
+        log(f"drift step {k}: " + "; ".join(
+            f"{name} " + ", ".join(f"{key} {v:.1e}" for key, v in rec[name].items()) for name in runs))
+    f.close()
+    for r in runs.values():
+        r.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -276,6 +323,10 @@ def main():
                     help="the convection inversion's adjoint cost ('' skips it): mixed-precision forward")
     ap.add_argument("--adjoint-convection-steps", type=int, default=20)
     ap.add_argument("--adjoint-convection-every", type=int, default=10)
+    ap.add_argument("--drift-steps", type=int, default=0,
+                    help="also step mixed, FP64 and FP64 from a start one ulp away side by side this many "
+                         "steps and report their differences from FP64")
+    ap.add_argument("--drift-every", type=int, default=50)
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: the step has no multi-GPU path in this library (the box "
@@ -312,6 +363,8 @@ def main():
     if a.adjoint_convection_workload:
         adj_conv = adjoint_run_cost(a.adjoint_convection_workload, a.adjoint_convection_steps,
                                     a.adjoint_convection_every, precision="mixed")
+
+    drift_rec = drift(case, a.drift_steps, a.drift_every) if a.drift_steps > 0 else None
 
     cpu = None
     if not a.no_cpu_baseline:
@@ -366,6 +419,8 @@ def main():
         "adjoint": adj_run,
         # the same at the convection inversion's size, 1M particles, mixed-precision forward
         "adjoint_convection": adj_conv,
+        # mixed against FP64 over a longer run (--drift-steps; not run by default)
+        **({"drift": drift_rec} if drift_rec is not None else {}),
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
