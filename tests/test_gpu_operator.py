@@ -12,7 +12,7 @@ import numpy as np
 import pytest
 import scipy.sparse as sp
 
-from vebisph_amd import cases, solver
+from vebisph_amd import _lib, cases, solver
 
 pytestmark = pytest.mark.gpu
 
@@ -96,17 +96,20 @@ def test_ppe_operator_refused_for_a_callers_system(gpu_lib):
         G.ppe_operator(np.ones(case.npfb))
 
 
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
 @pytest.mark.parametrize("name", list(CASES))
-def test_ppe_adjoint_solve(gpu_lib, name):
+def test_ppe_adjoint_solve(gpu_lib, name, prec):
     """A^T lambda = g with the reference's BiCGSTAB on the transposed PPE: the residual meets the
     reference criterion on A^T itself (checked with scipy), and in the open domains, where A is
     nonsingular, lambda is scipy's direct solution to the accuracy the criterion allows
     (cond(A) * 1e-8: 1e-3 relative).  Closed domains (Poiseuille; the cylinder, whose circle mirrors
     make A unsymmetric) take g with zero mean (A^T maps into the mean-zero vectors); a g with a
-    constant added gives the same lambda, since the solve removes g's mean first."""
+    constant added gives the same lambda, since the solve removes g's mean first.  On the mixed path
+    the refinement's inner BiCGSTAB reads the compact rows' A^T values as FP32 (exact there, the
+    kernel weights being FP32) and its residual stays FP64: the same bars hold."""
     import scipy.sparse.linalg as spla
     case = CASES[name]()
-    G = solver.Solver(case)
+    G = solver.Solver(case, precision=_lib.PREC_MIXED if prec == "mixed" else _lib.PREC_FP64)
     G.div_free()
     for k in range(1, 11):
         G.phase(k)

@@ -19,6 +19,7 @@ struct OperatorMap {
   size_t cap_m = 0;
   double *g = nullptr, *lam = nullptr;   // adjoint solve: right-hand side and solution (npfb)
   double* bv64T = nullptr;            // A^T values of the compact rows (layout of vb_ctx::bv64)
+  float* bv32T = nullptr;             //   the same in FP32 (mixed path; exact: its weights are FP32)
   int* xtcol = nullptr;               // A^T rows of the listed (full) rows in list order (xslot)
   double* xtval = nullptr;
   int* rslot = nullptr;               // reverse slot: the slot of c in i's list, beside tcol

@@ -148,6 +148,8 @@ vb_status alloc_operator(vb_ctx* c, size_t bucket_cap) {
     VB_HIP(c, hipMalloc((void**)&op.mfill, ((size_t)npfb + 1) * sizeof(int)));
     VB_HIP(c, hipMalloc((void**)&op.nmiss, sizeof(int)));
     VB_HIP(c, hipMalloc((void**)&op.bv64T, ppe_block_values(npfb) * sizeof(double)));   // as vb_ctx::bv64
+    if (c->prm.precision == VB_PREC_MIXED)
+      VB_HIP(c, hipMalloc((void**)&op.bv32T, ppe_block_values(npfb) * sizeof(float)));   // as vb_ctx::bv32
     VB_HIP(c, hipMalloc((void**)&op.xtcol, ppe_list_slots(npfb) * sizeof(int)));         // as vb_ctx::xcol
     VB_HIP(c, hipMalloc((void**)&op.xtval, ppe_list_slots(npfb) * sizeof(double)));
   }
@@ -351,7 +353,7 @@ vb_status remove_mean(vb_ctx* c, double* v, int n) {
 
 void free_operator(vb_ctx* c) {
   void* ptrs[] = {c->op.tval, c->op.tcol, c->op.mcount, c->op.moff, c->op.mfill, c->op.mkey, c->op.mrow,
-                  c->op.mval, c->op.nmiss, c->op.g, c->op.lam, c->op.bv64T, c->op.xtcol, c->op.xtval,
+                  c->op.mval, c->op.nmiss, c->op.g, c->op.lam, c->op.bv64T, c->op.bv32T, c->op.xtcol, c->op.xtval,
                   c->op.rslot, c->op.ga,
                   c->op.rref, c->op.bref, c->op.dref, c->op.refs,
                   c->op.gb, c->op.gin, c->op.gout};

@@ -36,8 +36,9 @@ constexpr double kOnePlusEps = 1.0 + 1.0e-15;   // M(i) + 1e-15 with M = 1, line
 // quantities up to rounding; 24 bytes less traffic per row and iteration)
 constexpr double kInvOnePlusEps = 1.0 / kOnePlusEps;
 
-// SPMV_TRANSPOSE: y = A^T x through the transposed map (vb_operator.hip), for the adjoint solve
-enum SpmvMode { SPMV_FP64 = 0, SPMV_MIXED = 1, SPMV_TRANSPOSE = 2 };
+// SPMV_TRANSPOSE: y = A^T x through the transposed map (vb_operator.hip), for the adjoint solve;
+// SPMV_TRANSPOSE_MIXED: the same with the compact rows' A^T values read as FP32 (mixed path)
+enum SpmvMode { SPMV_FP64 = 0, SPMV_MIXED = 1, SPMV_TRANSPOSE = 2, SPMV_TRANSPOSE_MIXED = 3 };
 
 // ------------------------------------------------------------------------------------------------
 // RHS, ppe_solve_in_house_simple_dirichlet.F90:27-53 (DC removal for the closed domain only)
@@ -285,16 +286,21 @@ k_assemble_blocks(int npfb, const int* cnt, const int* link, const double* a_val
 // and takes a second FP64 value plane, bv64T, filled from the transposed map's real part at the
 // positions bpos recorded.
 __global__ void k_transpose_blocks(int npfb, const int* cnt, const double* tval, const uint8_t* rowflag,
-                                   const int* chunkb, const uint8_t* bpos, double* bv64T) {
+                                   const int* chunkb, const uint8_t* bpos, double* bv64T, float* bv32T) {
   const int t = threadIdx.x, i = blockIdx.x * kBlock + t, lane = t & (kChunk - 1);
   if (i >= npfb || rowflag[i]) return;
   const size_t c = (size_t)i / kChunk;
   const int nbc = chunkb[c], n = cnt[i];
   for (int b = 0; b < nbc; b++)
-    for (int e = 0; e < kBlkCols; e++) bv64T[blk64(c, b, e, lane)] = 0.0;
+    for (int e = 0; e < kBlkCols; e++) {
+      bv64T[blk64(c, b, e, lane)] = 0.0;
+      if (bv32T) bv32T[blk32(c, b, e, lane)] = 0.f;
+    }
   for (int k = 0; k < n; k++) {
     const int p = bpos[(size_t)k * npfb + i];
-    bv64T[blk64(c, p / kBlkCols, p % kBlkCols, lane)] = tval[(size_t)k * npfb + i];
+    const double v = tval[(size_t)k * npfb + i];
+    bv64T[blk64(c, p / kBlkCols, p % kBlkCols, lane)] = v;
+    if (bv32T) bv32T[blk32(c, p / kBlkCols, p % kBlkCols, lane)] = (float)v;   // exact: FP32 weights
   }
 }
 
@@ -313,7 +319,7 @@ __global__ void k_transpose_listed(int npfb, const int* rows, int nrows, const i
 
 vb_status launch_transpose_blocks(vb_ctx* c) {
   VB_LAUNCH(c, k_transpose_blocks, grid_1d(c->npfb), c->npfb, c->cnt, c->op.tval, c->rowflag, c->chunkb,
-            c->bpos, c->op.bv64T);
+            c->bpos, c->op.bv64T, c->op.bv32T);
   const int nx = c->scal_host->nexplicit;       // synchronised by the assembly
   if (nx > 0)
     VB_LAUNCH(c, k_transpose_listed, grid_1d(nx), c->npfb, c->xrows, nx, c->cnt, c->op.tval, c->op.tcol,
@@ -341,6 +347,7 @@ struct Mat {
   // the transposed map (SPMV_TRANSPOSE): real part slot-major like the lists, mirror buckets per column,
   // and the compact rows' A^T values (the forward blocks' starts); the listed rows' A^T rows in list order
   const double2* bv64T;
+  const float4* bv32T;             // the same in FP32 (mixed path)
   const int* xtcol;
   const double* xtval;
   const double *tval, *mval;
@@ -471,7 +478,9 @@ __device__ inline double row_blk64(const Mat& A, const double2* __restrict__ bv,
 // values; folding the p and s projections into the SpMV, which then folds the partial sums in every
 // one of its workgroups, made the SpMV 1.5x slower than the separate projection kernels.)
 // ------------------------------------------------------------------------------------------------
-__device__ inline double row_blk32(const Mat& A, int i, const double* __restrict__ x, double acc) {
+// (bv: bv32 for A, bv32T for the compact rows of A^T)
+__device__ inline double row_blk32(const Mat& A, const float4* __restrict__ bv, int i,
+                                   const double* __restrict__ x, double acc) {
   const size_t c = (size_t)uniform_chunk(i);
   const int lane = i % kChunk, nb = __builtin_amdgcn_readfirstlane(A.chunkb[c]);
   short o[kMaxBlk];
@@ -481,7 +490,7 @@ __device__ inline double row_blk32(const Mat& A, int i, const double* __restrict
   xd2 xg[2][4];
   auto load = [&](int b) {
     const int t = b & 1;
-    va[t][0] = ld_nt(A.bv32 + vs(b, 0)); va[t][1] = ld_nt(A.bv32 + vs(b, 1));
+    va[t][0] = ld_nt(bv + vs(b, 0)); va[t][1] = ld_nt(bv + vs(b, 1));
     const xd2* xn = reinterpret_cast<const xd2*>(x + i + o[b]);
 #pragma unroll
     for (int h = 0; h < 4; h++) xg[t][h] = xn[h];
@@ -515,12 +524,13 @@ __device__ inline double spmv_row_fp64(const Mat& A, int i, const double* __rest
 __device__ inline double spmv_row_f32(const Mat& A, int i, const double* __restrict__ x) {
   const double xi = x[i];
   if (A.rowflag[i]) return row_full(A, i, A.len[i], xi, x);
-  return row_blk32(A, i, x, A.diag[i] * xi);
+  return row_blk32(A, A.bv32, i, x, A.diag[i] * xi);
 }
 
 // Row c of A^T: rows without mirror neighbours as compact FP64 rows (the forward row's blocks, the
 // A^T value plane, column order), the others from the transposed map in c's slot order (from its
 // list-order copy when t >= 0 is the row's list position); then c's mirror bucket.
+template <bool MIXED = false>
 __device__ inline double spmv_row_transpose(const Mat& A, int c, const double* __restrict__ x, int t = -1) {
   // the bucket bounds first: their loads overlap the row's chain instead of following it
   const int pb = A.moff[c], pe = A.moff[c + 1];
@@ -529,7 +539,8 @@ __device__ inline double spmv_row_transpose(const Mat& A, int c, const double* _
     acc = gather_sum(A.diag[c] * x[c], A.len[c], [&](int k) { return A.xtval[xslot(t, k)]; },
                      [&](int k) { return A.xtcol[xslot(t, k)]; }, x);
   } else if (A.rowflag && !A.rowflag[c]) {
-    acc = row_blk64(A, A.bv64T, c, x, A.diag[c] * x[c]);
+    const double d = A.diag[c] * x[c];
+    acc = MIXED ? row_blk32(A, A.bv32T, c, x, d) : row_blk64(A, A.bv64T, c, x, d);
   } else {
     const size_t n = A.n;
     acc = gather_sum(A.diag[c] * x[c], A.len[c], [&](int k) { return A.tval[(size_t)k * n + c]; },
@@ -541,6 +552,7 @@ __device__ inline double spmv_row_transpose(const Mat& A, int c, const double* _
 template <int MODE>
 __device__ inline double spmv_row(const Mat& A, int i, const double* __restrict__ x) {
   if (MODE == SPMV_TRANSPOSE) return spmv_row_transpose(A, i, x);
+  if (MODE == SPMV_TRANSPOSE_MIXED) return spmv_row_transpose<true>(A, i, x);
   return MODE == SPMV_MIXED ? spmv_row_f32(A, i, x) : spmv_row_fp64(A, i, x);
 }
 
@@ -556,7 +568,7 @@ __device__ inline void spmv_rows(const Mat& A, const double* __restrict__ x, Out
     if ((int)blockIdx.x < A.nxb) {
       for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < A.nx; t += A.nxb * blockDim.x) {
         const int i = A.xrows[t];
-        out(i, MODE == SPMV_TRANSPOSE ? spmv_row_transpose(A, i, x, t) : row_listed(A, t, i, x));
+        out(i, MODE >= SPMV_TRANSPOSE ? spmv_row_transpose(A, i, x, t) : row_listed(A, t, i, x));
       }
       return;
     }
@@ -564,8 +576,8 @@ __device__ inline void spmv_rows(const Mat& A, const double* __restrict__ x, Out
     for (int i = rr.begin + threadIdx.x; i < rr.end; i += blockDim.x) {
       if (A.rowflag[i]) continue;
       const double d = A.diag[i] * x[i];
-      if (MODE == SPMV_TRANSPOSE) out(i, spmv_row_transpose(A, i, x));
-      else out(i, MODE == SPMV_MIXED ? row_blk32(A, i, x, d) : row_blk64(A, A.bv64, i, x, d));
+      if (MODE >= SPMV_TRANSPOSE) out(i, spmv_row_transpose<MODE == SPMV_TRANSPOSE_MIXED>(A, i, x));
+      else out(i, MODE == SPMV_MIXED ? row_blk32(A, A.bv32, i, x, d) : row_blk64(A, A.bv64, i, x, d));
     }
     return;
   }
@@ -1036,7 +1048,8 @@ static Mat step_matrix(vb_ctx* c) {
   A.bv64 = (const double2*)c->bv64; A.bv32 = (const float4*)c->bv32; A.n = c->npfb;
   A.xrows = c->xrows; A.xcol = c->xcol; A.xval = c->xval; A.nx = 0; A.nxb = 0;
   A.tval = c->op.tval; A.mval = c->op.mval; A.tcol = c->op.tcol; A.moff = c->op.moff; A.mrow = c->op.mrow;
-  A.bv64T = (const double2*)c->op.bv64T; A.xtcol = c->op.xtcol; A.xtval = c->op.xtval;
+  A.bv64T = (const double2*)c->op.bv64T; A.bv32T = (const float4*)c->op.bv32T;
+  A.xtcol = c->op.xtcol; A.xtval = c->op.xtval;
   return A;
 }
 
@@ -1298,7 +1311,11 @@ vb_status adjoint_solve(vb_ctx* c, double tol, int itermax, int* iters, double* 
     // 3 of margin, and at least a tenfold reduction) once the residual is close to it
     const double inner = std::min(0.1, std::max(kRefineInnerTol, 0.3 * tol * gnorm / rnorm));
     VB_LAUNCH(c, k_ref_scale, grid_1d(n), n, op.rref, op.refs, op.bref);
-    if ((s = solve<SPMV_TRANSPOSE>(c, As, w, 0, inner, itermax - total)) != VB_OK) return s;
+    // the inner solve reads the compact rows' A^T values as FP32 on the mixed path (exact there); the
+    // refinement residual above is FP64
+    s = c->prm.precision == VB_PREC_MIXED ? solve<SPMV_TRANSPOSE_MIXED>(c, As, w, 0, inner, itermax - total)
+                                          : solve<SPMV_TRANSPOSE>(c, As, w, 0, inner, itermax - total);
+    if (s != VB_OK) return s;
     total += std::min(c->scal_host->iter, itermax - total);
     if (c->scal_host->status == VB_WARN_BICGSTAB_BREAKDOWN) st = VB_WARN_BICGSTAB_BREAKDOWN;
     VB_LAUNCH(c, k_ref_update, grid_1d(n), n, op.lam, op.dref, op.refs, round == 0 ? 1 : 0);
