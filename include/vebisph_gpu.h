@@ -186,8 +186,10 @@ vb_status vb_grad_operator(vb_ctx* ctx, int32_t transpose, int32_t n, const doub
  * removal of the mean of its right-hand side), SPARSE_PBICGSTAB in an open one.  Each round solves
  * A^T d = r / |r| and adds |r| d to lambda (from lambda = 0), then recomputes r = g - A^T lambda in
  * FP64, until |r| <= tol |g| (BiCGSTAB alone stalls near 3e-9 on A^T, which is not symmetric in
- * wall-bounded domains); tol <= 0 / itermax <= 0 select 1e-8 / 1000, itermax bounding the inner
- * iterations in total.  g, lambda host arrays of n = npfb; iters (inner iterations) and residual
+ * wall-bounded domains; on a VB_PREC_MIXED context the inner solve reads the compact rows' A^T values
+ * in FP32, which are exact there, while the residual stays FP64); tol <= 0 / itermax <= 0 select
+ * 1e-8 / 1000, itermax bounding the inner iterations in total.  g, lambda host arrays of n = npfb;
+ * iters (inner iterations) and residual
  * (|r| / |g| at the end) may be NULL; returns VB_WARN_* like vb_ppe_solve. */
 vb_status vb_ppe_adjoint_solve(vb_ctx* ctx, int32_t n, const double* g, double* lambda, double tol,
                                int32_t itermax, int32_t* iters, double* residual);
