@@ -7,11 +7,12 @@
 // list (col/val[k*npfb + i]) plus a diagonal: slot k of row i is the k-th neighbour of i with
 // mirrors folded onto their parents, zero slots included (duplicate columns accumulate into the
 // last slot, self-mirrors into the diagonal).  The ~98% of rows with no mirror neighbour (and no
-// neighbour further than 32767 rows away) are also stored compact, grouped by four slots so that a
-// thread loads four slots per instruction: 2-byte column offsets and FP64 values (10 bytes per slot)
-// or FP32 values in the mixed path (6 bytes per slot), in increasing column order (the gathers of a
-// wavefront then touch fewer cache lines; the row sum differs from the reference's slot order by
-// rounding).  Serial mode, whose solve reproduces the reference bit for bit, uses the full rows.
+// neighbour further than 32767 rows away) are also stored compact, as blocks of 8 consecutive
+// columns (layout at kChunk): one 2-byte start per block, FP64 values or FP32 values in the mixed
+// path, and four aligned 16-byte gathers of x per block; the row is summed in increasing column
+// order (a rounding-level difference from the reference's slot order).  The other rows are copied
+// in list order and taken by workgroups of their own.  Serial mode, whose solve reproduces the
+// reference bit for bit, uses the full rows.
 // (Recomputing w_L from the positions instead of storing it was measured: compute-bound on the FP64
 // divisions and square root, ~1.8x slower.)
 //
