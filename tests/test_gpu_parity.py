@@ -128,14 +128,9 @@ def _column_blocks(cnt, link, npfb):
     return nb
 
 
-@pytest.mark.parametrize("mix", ["shuffled", "swapped"])
-def test_renumbered_particles_against_oracle(gpu_lib, mix):
-    """The solver's SpMV stores a row as blocks of consecutive columns when it fits in 12 of them and
-    keeps the other rows full.  With the fluid particles renumbered at random ("shuffled"), almost
-    no row fits; with 10% of them swapped pairwise ("swapped"), fitting rows (with padding blocks)
-    and full rows share the same wavefronts.  Two steps against the oracle on the same renumbered
-    start, with the bars of test_step_phases_against_oracle."""
-    case = SMALL["poiseuille"]()
+def _renumbered_state(case, mix, ulp=False):
+    """The start state with the fluid particles renumbered: at random ("shuffled") or 10% of them
+    swapped pairwise ("swapped"); with ulp, every fluid coordinate is then moved by one ulp up."""
     s = solver.initial_state(case)
     rng = np.random.default_rng(11)
     nb, n = case.nbS, case.npfb
@@ -146,6 +141,23 @@ def test_renumbered_particles_against_oracle(gpu_lib, mix):
         pick = nb + rng.choice(n - nb, size=(n - nb) // 20 * 2, replace=False)
         perm[pick[0::2]], perm[pick[1::2]] = pick[1::2], pick[0::2]
     s = {k: np.asarray(v)[perm].copy() for k, v in s.items()}
+    if ulp:
+        for k in ("x", "y"):
+            s[k][nb:] = np.nextafter(s[k][nb:], np.inf)
+    return s
+
+
+@pytest.mark.parametrize("mix", ["shuffled", "swapped"])
+def test_renumbered_particles_against_oracle(gpu_lib, mix):
+    """The solver's SpMV stores a row as blocks of consecutive columns when it fits in 12 of them and
+    keeps the other rows full.  With the fluid particles renumbered at random ("shuffled"), almost
+    no row fits; with 10% of them swapped pairwise ("swapped"), fitting rows (with padding blocks)
+    and full rows share the same wavefronts.  Two steps against the oracle on the same renumbered
+    start, with the bars of test_step_phases_against_oracle."""
+    case = SMALL["poiseuille"]()
+    s = _renumbered_state(case, mix)
+    rng = np.random.default_rng(12)
+    n = case.npfb
     G = solver.Solver(case, state=s); O = ow.OracleSolver(case, state=s)
     for step in range(2):
         first = step == 0
@@ -186,6 +198,35 @@ def test_renumbered_particles_against_oracle(gpu_lib, mix):
     x = rng.standard_normal(n)
     y_op, y_solver = G.ppe_operator(x), G.ppe_operator(x, solver_rows=True)
     assert np.max(np.abs(y_solver - y_op)) <= 1e-13 * np.max(np.abs(y_op))
+
+
+@pytest.mark.parametrize("mix", ["shuffled", "swapped"])
+def test_renumbered_particles_mixed(gpu_lib, mix):
+    """The mixed path on renumbered particles (its SpMV's full rows, padding blocks and FP32 block
+    values in the same wavefronts): ten steps against the FP64 oracle with the bars of
+    test_mixed_precision_fields, and the final solve's FP64 residual on its own system."""
+    from vebisph_amd import _lib
+    case = SMALL["poiseuille"]()
+    s = _renumbered_state(case, mix)
+    O = ow.OracleSolver(case, state=s)
+    O2 = ow.OracleSolver(case, state=_renumbered_state(case, mix, ulp=True))
+    G = solver.Solver(case, precision=_lib.PREC_MIXED, state=s)
+    for _ in range(10):
+        O.step(); O2.step(); G.div_free()
+        assert G.scalars()["status"] in (0, 6)
+    a, b, b2 = G.get_state(), O.get_state(), O2.get_state()
+    for key in ("x", "y", "u", "v", "theta", "p", "tau", "trace"):
+        spread = _rel(b2[key], b[key])
+        err = _rel(a[key], b[key])
+        print(f"{mix} mixed {key}: vs oracle {err:.2e}, oracle one-ulp spread {spread:.2e}", flush=True)
+        assert err <= max(10.0 * spread, mixed_floor(key)), key
+    for k in range(1, 11):
+        G.phase(k)
+    w = G.scalars()["maxneighbours"]
+    diag, col, val, rhs = G.ppe_ell(w)
+    a_nr, ija = solver.ell_to_nr(diag, col, val, G.neighbours()[0])
+    r = rhs - ow.spmv_nr(a_nr, ija, G.get_state()["p"])
+    assert np.linalg.norm(r - r.mean()) <= 1e-5 * np.linalg.norm(rhs)
 
 
 @pytest.mark.parametrize("name", list(SMALL))
