@@ -385,7 +385,7 @@ __device__ inline double row_full(const Mat& A, int i, int m, double xi, const d
 
 // The compact matrix arrays are read once per SpMV: non-temporal loads keep them from displacing the
 // gathered vector in L2 (measured on MI355X: a copy kernel with non-temporal accesses reaches 6.2
-// TB/s, a plain one 4.6).
+// TB/s, a plain one 4.6).  The exception is the mixed row's FP32 value stream (row_blk32).
 typedef short nt_s4 __attribute__((ext_vector_type(4)));
 typedef double nt_d2 __attribute__((ext_vector_type(2)));
 typedef float nt_f4 __attribute__((ext_vector_type(4)));
@@ -474,10 +474,13 @@ __device__ inline double row_blk64(const Mat& A, const double2* __restrict__ bv,
 // Mixed path compact rows: y_i = diag_i x_i + sum a32 x_j in FP64, the values read as FP32 (4 bytes
 // per block column instead of 8).  The mixed path's kernel weights are FP32, so the values -w_L are
 // exact in FP32 and this is the FP64 compact row's sum on the same numbers (with fused multiply-adds).
-// Same two-stage pipeline as the FP64 row.  (Measured alternatives: recomputing the FP32 entries
-// from the pair distance, three FP64 gathers per slot, was 1.5x slower than reading the stored
-// values; folding the p and s projections into the SpMV, which then folds the partial sums in every
-// one of its workgroups, made the SpMV 1.5x slower than the separate projection kernels.)
+// Same two-stage pipeline as the FP64 row, but the FP32 values are read with ordinary loads: with
+// the non-temporal hint the mixed SpMV took 54.8 us at the 1M cylinder, without it 45.9 us (the same
+// with buffer loads of any cache scope), while the FP64 row is fastest with it (83 against 93.5 us).
+// (Measured alternatives: recomputing the FP32 entries from the pair distance, three FP64 gathers
+// per slot, was 1.5x slower than reading the stored values; folding the p and s projections into
+// the SpMV, which then folds the partial sums in every one of its workgroups, made the SpMV 1.5x
+// slower than the separate projection kernels.)
 // ------------------------------------------------------------------------------------------------
 // (bv: bv32 for A, bv32T for the compact rows of A^T)
 __device__ inline double row_blk32(const Mat& A, const float4* __restrict__ bv, int i,
@@ -491,7 +494,7 @@ __device__ inline double row_blk32(const Mat& A, const float4* __restrict__ bv, 
   xd2 xg[2][4];
   auto load = [&](int b) {
     const int t = b & 1;
-    va[t][0] = ld_nt(bv + vs(b, 0)); va[t][1] = ld_nt(bv + vs(b, 1));
+    va[t][0] = bv[vs(b, 0)]; va[t][1] = bv[vs(b, 1)];
     const xd2* xn = reinterpret_cast<const xd2*>(x + i + o[b]);
 #pragma unroll
     for (int h = 0; h < 4; h++) xg[t][h] = xn[h];
@@ -906,9 +909,24 @@ __device__ inline FinState finish_iteration(const double (&f)[3], int k, Scal* s
   return o;
 }
 
+// A vector element read for the last time in the iteration (NT: non-temporal) and the solution's
+// update (NT: non-temporal store: x is read again only in the next iteration's last kernel).  NT on
+// the mixed path, whose SpMVs read the FP32 matrix with ordinary loads: streaming the vectors that
+// are not read again keeps more of the matrix and of the next kernels' vectors in the caches
+// (measured at the 1M cylinder: 141.3 against 135.3 ms per mixed step; the FP64 path, whose matrix
+// is read non-temporally, 214.2 against 215.0 ms, so it keeps ordinary accesses).
+template <bool NT> __device__ __forceinline__ double ld_last(const double* p) {
+  if constexpr (NT) return __builtin_nontemporal_load(p);
+  else return *p;
+}
+template <bool NT> __device__ __forceinline__ void st_last(double v, double* p) {
+  if constexpr (NT) __builtin_nontemporal_store(v, p);
+  else *p = v;
+}
+
 // Iteration k; fin = 1: finish iteration k-1 here (its x,r partials), 0: k_fu_fin or k_fin_init did
-// (OPEN: the open domain's SPARSE_PBICGSTAB, Jacobi and no mean removed)
-template <bool OPEN>
+// (OPEN: the open domain's SPARSE_PBICGSTAB, Jacobi and no mean removed; NT: see ld_last)
+template <bool OPEN, bool NT>
 __global__ void k_fu_p(int n, Vecs w, RedBufs rb, Scal* sc, int k, int fin) {
   if (finished(sc)) return;
   RowRange rr = block_rows(n, gridDim.x);
@@ -917,7 +935,7 @@ __global__ void k_fu_p(int n, Vecs w, RedBufs rb, Scal* sc, int k, int fin) {
 #pragma unroll
   for (int u = 0; u < kPre; u++) {
     const int i = i0 + u * bd;
-    if (i < rr.end) { rv[u] = w.r[i]; pv[u] = w.pn[i]; vv[u] = w.v[i]; }
+    if (i < rr.end) { rv[u] = w.r[i]; pv[u] = ld_last<NT>(w.pn + i); vv[u] = ld_last<NT>(w.v + i); }
   }
   double beta, sum_r;
   if (fin) {
@@ -943,11 +961,11 @@ __global__ void k_fu_p(int n, Vecs w, RedBufs rb, Scal* sc, int k, int fin) {
   };
 #pragma unroll
   for (int u = 0; u < kPre; u++) if (i0 + u * bd < rr.end) row(i0 + u * bd, rv[u], pv[u], vv[u]);
-  for (int i = i0 + kPre * bd; i < rr.end; i += bd) row(i, w.r[i], w.pn[i], w.v[i]);
+  for (int i = i0 + kPre * bd; i < rr.end; i += bd) row(i, w.r[i], ld_last<NT>(w.pn + i), ld_last<NT>(w.v + i));
   R.store(rb.P(ST_S) + gridDim.x);
 }
 
-template <bool OPEN>
+template <bool OPEN, bool NT>
 __global__ void k_fu_s(int n, Vecs w, RedBufs rb, Scal* sc, int nspmv, int k) {
   if (finished(sc)) return;
   RowRange rr = block_rows(n, gridDim.x);
@@ -955,7 +973,7 @@ __global__ void k_fu_s(int n, Vecs w, RedBufs rb, Scal* sc, int nspmv, int k) {
   double rv[kPre], vv[kPre];
 #pragma unroll
   for (int u = 0; u < kPre; u++)
-    if (i0 + u * bd < rr.end) { rv[u] = w.r[i0 + u * bd]; vv[u] = w.v[i0 + u * bd]; }
+    if (i0 + u * bd < rr.end) { rv[u] = ld_last<NT>(w.r + i0 + u * bd); vv[u] = ld_last<NT>(w.v + i0 + u * bd); }
   double f[2];
   fold_n<2>(rb.P(ST_SPMV_P), nspmv, nullptr, n, f);
   const double alpha = sc->rho_pp[k & 1] / (f[0] + 1.0e-15);
@@ -971,12 +989,12 @@ __global__ void k_fu_s(int n, Vecs w, RedBufs rb, Scal* sc, int nspmv, int k) {
   };
 #pragma unroll
   for (int u = 0; u < kPre; u++) if (i0 + u * bd < rr.end) row(i0 + u * bd, rv[u], vv[u]);
-  for (int i = i0 + kPre * bd; i < rr.end; i += bd) row(i, w.r[i], w.v[i]);
+  for (int i = i0 + kPre * bd; i < rr.end; i += bd) row(i, ld_last<NT>(w.r + i), ld_last<NT>(w.v + i));
   R.store(rb.P(ST_S));
 }
 
 // (OPEN: the directions p_inv, s_inv are stored; closed: they are c p and c s, c = 1 / (1 + 1e-15))
-template <bool OPEN>
+template <bool OPEN, bool NT>
 __global__ void k_fu_xr(int n, Vecs w, RedBufs rb, Scal* sc, int nspmv, int k) {
   if (finished(sc)) return;
   RowRange rr = block_rows(n, gridDim.x);
@@ -987,7 +1005,8 @@ __global__ void k_fu_xr(int n, Vecs w, RedBufs rb, Scal* sc, int nspmv, int k) {
   for (int u = 0; u < kPre; u++) {
     const int i = i0 + u * bd;
     if (i < rr.end) {
-      xv[u] = w.x[i]; pv[u] = pd[i]; sv[u] = w.s[i]; tv[u] = w.t[i]; bv[u] = w.rb[i];
+      xv[u] = ld_last<NT>(w.x + i); pv[u] = ld_last<NT>(pd + i); sv[u] = ld_last<NT>(w.s + i);
+      tv[u] = ld_last<NT>(w.t + i); bv[u] = w.rb[i];
       qv[u] = OPEN ? w.sinv[i] : sv[u];
     }
   }
@@ -1004,7 +1023,7 @@ __global__ void k_fu_xr(int n, Vecs w, RedBufs rb, Scal* sc, int nspmv, int k) {
     sc->sum_p = fs[1];
   }
   if (n2s <= sc->tol * sc->tol * n2s0) {                     // :374-381
-    for (int i = i0; i < rr.end; i += bd) w.x[i] = w.x[i] + ca * pd[i];
+    for (int i = i0; i < rr.end; i += bd) st_last<NT>(ld_last<NT>(w.x + i) + ca * ld_last<NT>(pd + i), w.x + i);
     if (lead) { sc->resid = sqrt(n2s / n2s0); sc->exit_s = 1; }
     return;
   }
@@ -1013,7 +1032,7 @@ __global__ void k_fu_xr(int n, Vecs w, RedBufs rb, Scal* sc, int nspmv, int k) {
   if (lead) sc->omega = omega;
   Red<3> R(nullptr, n);
   auto row = [&](int i, double x, double pdir, double sdir, double s, double t, double rbi) {
-    w.x[i] = x + ca * pdir + co * sdir;
+    st_last<NT>(x + ca * pdir + co * sdir, w.x + i);
     const double r = s - omega * t;
     w.r[i] = r;
     R.add(0, i, r * r);
@@ -1024,7 +1043,8 @@ __global__ void k_fu_xr(int n, Vecs w, RedBufs rb, Scal* sc, int nspmv, int k) {
   for (int u = 0; u < kPre; u++)
     if (i0 + u * bd < rr.end) row(i0 + u * bd, xv[u], pv[u], qv[u], sv[u], tv[u], bv[u]);
   for (int i = i0 + kPre * bd; i < rr.end; i += bd)
-    row(i, w.x[i], pd[i], OPEN ? w.sinv[i] : w.s[i], w.s[i], w.t[i], w.rb[i]);
+    row(i, ld_last<NT>(w.x + i), ld_last<NT>(pd + i), OPEN ? w.sinv[i] : ld_last<NT>(w.s + i), ld_last<NT>(w.s + i),
+        ld_last<NT>(w.t + i), w.rb[i]);
   R.store(rb.P(ST_XR));
 }
 
@@ -1074,6 +1094,7 @@ static vb_status solve(vb_ctx* c, const Mat& A, Vecs w, int x_given, double tol,
   RedBufs rb = red_bufs(c, n);
   Scal* sc = c->scal;
   const bool fused = rb.rows == nullptr;   // serial mode: the reference's eight-kernel order
+  constexpr bool NT = MODE == SPMV_MIXED || MODE == SPMV_TRANSPOSE_MIXED;   // see ld_last
   const double small = w.jac ? (double)(10 - 6) : 1.0e-6;
   VB_LAUNCH(c, k_bi_init, nb, A, w, x_given, rb);
   VB_FIN(c, k_fin_init, rb, nb, tol, itermax, small, sc);
@@ -1082,14 +1103,14 @@ static vb_status solve(vb_ctx* c, const Mat& A, Vecs w, int x_given, double tol,
     for (int b = 0; b < batch; b++) {
       const int k = launched + b + 1;
       if (fused) {
-        if (w.jac) VB_LAUNCH(c, k_fu_p<true>, nb, n, w, rb, sc, k, b > 0 ? 1 : 0);
-        else VB_LAUNCH(c, k_fu_p<false>, nb, n, w, rb, sc, k, b > 0 ? 1 : 0);
+        if (w.jac) VB_LAUNCH(c, (k_fu_p<true, NT>), nb, n, w, rb, sc, k, b > 0 ? 1 : 0);
+        else VB_LAUNCH(c, (k_fu_p<false, NT>), nb, n, w, rb, sc, k, b > 0 ? 1 : 0);
         VB_LAUNCH_BS(c, (k_bi_spmv_p<MODE, true>), ns, kSpmvBlock, A, w, rb, sc);
-        if (w.jac) VB_LAUNCH(c, k_fu_s<true>, nb, n, w, rb, sc, ns, k);
-        else VB_LAUNCH(c, k_fu_s<false>, nb, n, w, rb, sc, ns, k);
+        if (w.jac) VB_LAUNCH(c, (k_fu_s<true, NT>), nb, n, w, rb, sc, ns, k);
+        else VB_LAUNCH(c, (k_fu_s<false, NT>), nb, n, w, rb, sc, ns, k);
         VB_LAUNCH_BS(c, (k_bi_spmv_s<MODE, true>), ns, kSpmvBlock, A, w, rb, sc);
-        if (w.jac) VB_LAUNCH(c, k_fu_xr<true>, nb, n, w, rb, sc, ns, k);
-        else VB_LAUNCH(c, k_fu_xr<false>, nb, n, w, rb, sc, ns, k);
+        if (w.jac) VB_LAUNCH(c, (k_fu_xr<true, NT>), nb, n, w, rb, sc, ns, k);
+        else VB_LAUNCH(c, (k_fu_xr<false, NT>), nb, n, w, rb, sc, ns, k);
         continue;
       }
       VB_LAUNCH(c, k_bi_p, nb, n, w, rb, sc);
