@@ -423,9 +423,13 @@ __device__ inline int uniform_chunk(int i) { return __builtin_amdgcn_readfirstla
 // sums block B, for B < nb (wavefront-uniform, so every exit is a scalar branch).  Written as a
 // template recursion rather than a loop: with a loop the compiler kept a rolled two-block body whose
 // back edge rotates the in-flight registers and waits for every outstanding load at each turn
-// (59.4 against 54.8 us, mixed, 1M cylinder).  Depth measured at 1M: loading one block per step
-// (64 VGPRs) gave 54.8 / 84.0 us (mixed / FP64) and ~2% slower steps, three blocks in flight 57.3 us
-// (mixed); two are kept.
+// (59.4 against 54.8 us, mixed, 1M cylinder).  The conditional load of block B+1 joins the sum of
+// block B, and at that join the compiler waits for all loads in flight, block B+1's too: each step
+// waits about one memory round trip.  Measured at 1M and not kept: separate paths for the last block
+// (the wait then covers block B only), the three start groups loaded unconditionally with the row's
+// first loads, and the values three blocks ahead: none faster (54.5-55.6 against 54.7 us, mixed;
+// the SpMV is bound by its streams' rate, not by one wavefront's latency); loading one block per
+// step (64 VGPRs) gave 54.8 / 84.0 us (mixed / FP64) and ~2% slower steps.
 template <int B, class Load, class Sum>
 __device__ __forceinline__ void block_steps(int nb, Load&& load, Sum&& sum) {
   if constexpr (B < kMaxBlk) {
@@ -438,8 +442,7 @@ __device__ __forceinline__ void block_steps(int nb, Load&& load, Sum&& sum) {
 
 // A compact FP64 row from its blocks (val: bv64 for A, bv64T for the compact rows of A^T):
 // acc + sum over blocks and columns, in column order, of a * x (no fused multiply-add, as the FP64
-// path everywhere).  Two-stage pipeline: block b+1's values and gathers are in flight while block b
-// is summed.
+// path everywhere).  Block b+1's values and gathers are issued before block b is summed.
 __device__ inline double row_blk64(const Mat& A, const double2* __restrict__ bv, int i,
                                    const double* __restrict__ x, double acc) {
   const size_t c = (size_t)uniform_chunk(i);
@@ -474,7 +477,7 @@ __device__ inline double row_blk64(const Mat& A, const double2* __restrict__ bv,
 // Mixed path compact rows: y_i = diag_i x_i + sum a32 x_j in FP64, the values read as FP32 (4 bytes
 // per block column instead of 8).  The mixed path's kernel weights are FP32, so the values -w_L are
 // exact in FP32 and this is the FP64 compact row's sum on the same numbers (with fused multiply-adds).
-// Same two-stage pipeline as the FP64 row, but the FP32 values are read with ordinary loads: with
+// Same block steps as the FP64 row, but the FP32 values are read with ordinary loads: with
 // the non-temporal hint the mixed SpMV took 54.8 us at the 1M cylinder, without it 45.9 us (the same
 // with buffer loads of any cache scope), while the FP64 row is fastest with it (83 against 93.5 us).
 // (Measured alternatives: recomputing the FP32 entries from the pair distance, three FP64 gathers
