@@ -917,7 +917,10 @@ __device__ inline FinState finish_iteration(const double (&f)[3], int k, Scal* s
 // the mixed path, whose SpMVs read the FP32 matrix with ordinary loads: streaming the vectors that
 // are not read again keeps more of the matrix and of the next kernels' vectors in the caches
 // (measured at the 1M cylinder: 141.3 against 135.3 ms per mixed step; the FP64 path, whose matrix
-// is read non-temporally, 214.2 against 215.0 ms, so it keeps ordinary accesses).
+// is read non-temporally, 214.2 against 215.0 ms, so it keeps ordinary accesses).  The mixed path's
+// vector kernels also load 2 * kPre rows per thread before their fold, all of a thread's rows at 1M
+// (132.1 against 135.2 ms per mixed step; FP64 215.6 against 213.8, so it keeps kPre).  256-thread
+// workgroups: the launch bound lets k_fu_xr hold its 16 rows' operands without spilling.
 template <bool NT> __device__ __forceinline__ double ld_last(const double* p) {
   if constexpr (NT) return __builtin_nontemporal_load(p);
   else return *p;
@@ -930,13 +933,14 @@ template <bool NT> __device__ __forceinline__ void st_last(double v, double* p) 
 // Iteration k; fin = 1: finish iteration k-1 here (its x,r partials), 0: k_fu_fin or k_fin_init did
 // (OPEN: the open domain's SPARSE_PBICGSTAB, Jacobi and no mean removed; NT: see ld_last)
 template <bool OPEN, bool NT>
-__global__ void k_fu_p(int n, Vecs w, RedBufs rb, Scal* sc, int k, int fin) {
+__global__ void __launch_bounds__(kBlock) k_fu_p(int n, Vecs w, RedBufs rb, Scal* sc, int k, int fin) {
+  constexpr int P = NT ? 2 * kPre : kPre;   // rows loaded before the fold, see ld_last
   if (finished(sc)) return;
   RowRange rr = block_rows(n, gridDim.x);
   const int i0 = rr.begin + threadIdx.x, bd = blockDim.x;
-  double rv[kPre], pv[kPre], vv[kPre];
+  double rv[P], pv[P], vv[P];
 #pragma unroll
-  for (int u = 0; u < kPre; u++) {
+  for (int u = 0; u < P; u++) {
     const int i = i0 + u * bd;
     if (i < rr.end) { rv[u] = w.r[i]; pv[u] = ld_last<NT>(w.pn + i); vv[u] = ld_last<NT>(w.v + i); }
   }
@@ -963,19 +967,20 @@ __global__ void k_fu_p(int n, Vecs w, RedBufs rb, Scal* sc, int k, int fin) {
     R.add(0, i, pp);
   };
 #pragma unroll
-  for (int u = 0; u < kPre; u++) if (i0 + u * bd < rr.end) row(i0 + u * bd, rv[u], pv[u], vv[u]);
-  for (int i = i0 + kPre * bd; i < rr.end; i += bd) row(i, w.r[i], ld_last<NT>(w.pn + i), ld_last<NT>(w.v + i));
+  for (int u = 0; u < P; u++) if (i0 + u * bd < rr.end) row(i0 + u * bd, rv[u], pv[u], vv[u]);
+  for (int i = i0 + P * bd; i < rr.end; i += bd) row(i, w.r[i], ld_last<NT>(w.pn + i), ld_last<NT>(w.v + i));
   R.store(rb.P(ST_S) + gridDim.x);
 }
 
 template <bool OPEN, bool NT>
-__global__ void k_fu_s(int n, Vecs w, RedBufs rb, Scal* sc, int nspmv, int k) {
+__global__ void __launch_bounds__(kBlock) k_fu_s(int n, Vecs w, RedBufs rb, Scal* sc, int nspmv, int k) {
+  constexpr int P = NT ? 2 * kPre : kPre;   // rows loaded before the fold, see ld_last
   if (finished(sc)) return;
   RowRange rr = block_rows(n, gridDim.x);
   const int i0 = rr.begin + threadIdx.x, bd = blockDim.x;
-  double rv[kPre], vv[kPre];
+  double rv[P], vv[P];
 #pragma unroll
-  for (int u = 0; u < kPre; u++)
+  for (int u = 0; u < P; u++)
     if (i0 + u * bd < rr.end) { rv[u] = ld_last<NT>(w.r + i0 + u * bd); vv[u] = ld_last<NT>(w.v + i0 + u * bd); }
   double f[2];
   fold_n<2>(rb.P(ST_SPMV_P), nspmv, nullptr, n, f);
@@ -991,21 +996,22 @@ __global__ void k_fu_s(int n, Vecs w, RedBufs rb, Scal* sc, int nspmv, int k) {
     if (OPEN) w.sinv[i] = inv_m_t<OPEN>(w, i, sp);   // closed: s_inv = c s is never stored
   };
 #pragma unroll
-  for (int u = 0; u < kPre; u++) if (i0 + u * bd < rr.end) row(i0 + u * bd, rv[u], vv[u]);
-  for (int i = i0 + kPre * bd; i < rr.end; i += bd) row(i, ld_last<NT>(w.r + i), ld_last<NT>(w.v + i));
+  for (int u = 0; u < P; u++) if (i0 + u * bd < rr.end) row(i0 + u * bd, rv[u], vv[u]);
+  for (int i = i0 + P * bd; i < rr.end; i += bd) row(i, ld_last<NT>(w.r + i), ld_last<NT>(w.v + i));
   R.store(rb.P(ST_S));
 }
 
 // (OPEN: the directions p_inv, s_inv are stored; closed: they are c p and c s, c = 1 / (1 + 1e-15))
 template <bool OPEN, bool NT>
-__global__ void k_fu_xr(int n, Vecs w, RedBufs rb, Scal* sc, int nspmv, int k) {
+__global__ void __launch_bounds__(kBlock) k_fu_xr(int n, Vecs w, RedBufs rb, Scal* sc, int nspmv, int k) {
+  constexpr int P = NT ? 2 * kPre : kPre;   // rows loaded before the fold, see ld_last
   if (finished(sc)) return;
   RowRange rr = block_rows(n, gridDim.x);
   const int i0 = rr.begin + threadIdx.x, bd = blockDim.x;
   const double* pd = OPEN ? w.pinv : w.pn;
-  double xv[kPre], pv[kPre], qv[kPre], sv[kPre], tv[kPre], bv[kPre];
+  double xv[P], pv[P], qv[P], sv[P], tv[P], bv[P];
 #pragma unroll
-  for (int u = 0; u < kPre; u++) {
+  for (int u = 0; u < P; u++) {
     const int i = i0 + u * bd;
     if (i < rr.end) {
       xv[u] = ld_last<NT>(w.x + i); pv[u] = ld_last<NT>(pd + i); sv[u] = ld_last<NT>(w.s + i);
@@ -1043,9 +1049,9 @@ __global__ void k_fu_xr(int n, Vecs w, RedBufs rb, Scal* sc, int nspmv, int k) {
     R.add(2, i, r);
   };
 #pragma unroll
-  for (int u = 0; u < kPre; u++)
+  for (int u = 0; u < P; u++)
     if (i0 + u * bd < rr.end) row(i0 + u * bd, xv[u], pv[u], qv[u], sv[u], tv[u], bv[u]);
-  for (int i = i0 + kPre * bd; i < rr.end; i += bd)
+  for (int i = i0 + P * bd; i < rr.end; i += bd)
     row(i, ld_last<NT>(w.x + i), ld_last<NT>(pd + i), OPEN ? w.sinv[i] : ld_last<NT>(w.s + i), ld_last<NT>(w.s + i),
         ld_last<NT>(w.t + i), w.rb[i]);
   R.store(rb.P(ST_XR));
