@@ -26,8 +26,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0      # MI355X HBM3E
-SPMV_KERNEL = {"mixed": "k_bi_spmv_p<1, true> (BiCGSTAB SpMV, FP32 values)",
-               "fp64": "k_bi_spmv_p<0, true> (BiCGSTAB SpMV, FP64 values)"}
+SPMV_KERNEL = {"mixed": "k_bi_spmv_p<1, true, true> (BiCGSTAB SpMV, FP32 values read cacheably; "
+                        "<1, true, false> where the value plane exceeds the Infinity Cache)",
+               "fp64": "k_bi_spmv_p<0, true, false> (BiCGSTAB SpMV, FP64 values)"}
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "spmv_traffic_cylinder_1M.json")
 
 

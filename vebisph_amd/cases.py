@@ -290,6 +290,10 @@ CONFIGS = {
     "poiseuille_2500": lambda: poiseuille(),
     "convection_80k": lambda: convection(),
     "cylinder_1M": lambda: cylinder(),
+    # the same flow at twice and four times the resolution (capacity and throughput at size)
+    "cylinder_2M": lambda: cylinder(nx=2026, ny=1013, name="cylinder_2M"),
+    "cylinder_4M": lambda: cylinder(nx=2864, ny=1432, name="cylinder_4M"),
+    "cylinder_16M": lambda: cylinder(nx=5728, ny=2864, name="cylinder_16M"),
     "poiseuille_250k": lambda: poiseuille(nx=1000, ny=250, L=4.0, H=1.0, seed=4, steps=200,
                                           name="poiseuille_250k"),
     "convection_1M": lambda: convection(nx=1414, ny=707, seed=5, Ra=5e4, Wi=1.0, beta=0.5, steps=500,

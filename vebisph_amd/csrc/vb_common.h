@@ -49,6 +49,11 @@ inline size_t ppe_chunks(int npfb) { return ((size_t)npfb + kChunk - 1) / kChunk
 inline size_t ppe_block_values(int npfb) { return ppe_chunks(npfb) * kChunk * kMaxBlk * kBlkCols; }
 inline size_t ppe_block_offsets(int npfb) { return ppe_chunks(npfb) * kChunk * kMaxBlk; }
 inline size_t ppe_list_slots(int npfb) { return ppe_chunks(npfb) * kChunk * kNplink; }
+// MI355X's Infinity Cache (the memory-side cache in front of HBM) holds 256 MiB.  The mixed SpMVs
+// read their FP32 value plane with ordinary (cacheable) loads when it takes at most 15/16 of that,
+// so that the two SpMVs of an iteration and the next iteration's find it there; a larger plane is
+// streamed with non-temporal loads (vb_ppe.hip: row_blk32, solve).
+constexpr double kCachedValueBytes = 256.0 * 1024 * 1024 * 15 / 16;
 constexpr double kEps = 2.220446049250313e-16;   // epsilon(1d0)
 
 // Boundary description and derived constants, passed by value to kernels (< 4 KiB).
@@ -105,6 +110,7 @@ struct Scal {
   // of the projected p, each recorded by the kernel that folds it
   double rho_pp[2], sum_r, sum_v, sum_p;
   int nexplicit;                           // PPE rows with mirror neighbours (full stored rows)
+  int nblocks;                             // column blocks of the compact rows, summed over chunks
   // output quantities: RMS velocity (secondary_properties), max |u|, |v|, |p| over rows 1..npfb
   double u_rms, fmax[3];
   // diagnostics: per circle Fx, Fy, boundary-particle count; the Nusselt number

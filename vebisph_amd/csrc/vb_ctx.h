@@ -81,6 +81,7 @@ struct vb_ctx {
   uint8_t* bpos;                     //   block position of each neighbour slot (for A^T)
 
   bool spmv_stored;                  // FP64 path: full stored rows everywhere (VB_SPMV=stored)
+  bool vcache = false;               // mixed path: FP32 value plane read cacheably (kCachedValueBytes)
   double *r, *rb, *vv, *pn, *pinv, *ss, *sinv, *tt, *xsys;
 
   double* partials;                  // reduction partials, 2 * kPartialStride per solver stage

@@ -217,6 +217,20 @@ __global__ void k_list_explicit(int npfb, const uint8_t* rowflag, const int* exc
   if (i == 0) sc->nexplicit = excl[npfb];
 }
 
+// The compact rows' column blocks summed over the chunks (one workgroup; integer sums, any order).
+__global__ void k_count_blocks(int nchunks, const int* chunkb, Scal* sc) {
+  __shared__ int part[kBlock];
+  int s = 0;
+  for (int c = threadIdx.x; c < nchunks; c += blockDim.x) s += chunkb[c];
+  part[threadIdx.x] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int t = 0;
+    for (int k = 0; k < (int)blockDim.x; k++) t += part[k];
+    sc->nblocks = t;
+  }
+}
+
 __global__ void k_flag_to_int(int n, const uint8_t* f, int* out) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i < n) out[i] = f[i];
@@ -477,15 +491,19 @@ __device__ inline double row_blk64(const Mat& A, const double2* __restrict__ bv,
 // Mixed path compact rows: y_i = diag_i x_i + sum a32 x_j in FP64, the values read as FP32 (4 bytes
 // per block column instead of 8).  The mixed path's kernel weights are FP32, so the values -w_L are
 // exact in FP32 and this is the FP64 compact row's sum on the same numbers (with fused multiply-adds).
-// Same block steps as the FP64 row, but the FP32 values are read with ordinary loads: with
-// the non-temporal hint the mixed SpMV took 54.8 us at the 1M cylinder, without it 45.9 us (the same
-// with buffer loads of any cache scope), while the FP64 row is fastest with it (83 against 93.5 us).
+// Same block steps as the FP64 row.  CACHED (the value plane fits the Infinity Cache,
+// kCachedValueBytes): the FP32 values are read with ordinary loads, so that the iteration's two SpMVs
+// find them there (1M cylinder, 226 MB: 45.9 against 54.8 us with the non-temporal hint, the same
+// with buffer loads of any cache scope; 139.6 against 156 ms per step with the vector kernels' own
+// hints, see ld_last); otherwise non-temporally (2M: 277 against 306 ms per step, 4M: 538 against
+// 582).  The FP64 row, 482 MB at 1M, is always non-temporal (83 against 93.5 us).
 // (Measured alternatives: recomputing the FP32 entries from the pair distance, three FP64 gathers
 // per slot, was 1.5x slower than reading the stored values; folding the p and s projections into
 // the SpMV, which then folds the partial sums in every one of its workgroups, made the SpMV 1.5x
 // slower than the separate projection kernels.)
 // ------------------------------------------------------------------------------------------------
 // (bv: bv32 for A, bv32T for the compact rows of A^T)
+template <bool CACHED = false>
 __device__ inline double row_blk32(const Mat& A, const float4* __restrict__ bv, int i,
                                    const double* __restrict__ x, double acc) {
   const size_t c = (size_t)uniform_chunk(i);
@@ -497,7 +515,11 @@ __device__ inline double row_blk32(const Mat& A, const float4* __restrict__ bv, 
   xd2 xg[2][4];
   auto load = [&](int b) {
     const int t = b & 1;
-    va[t][0] = bv[vs(b, 0)]; va[t][1] = bv[vs(b, 1)];
+    if constexpr (CACHED) {
+      va[t][0] = bv[vs(b, 0)]; va[t][1] = bv[vs(b, 1)];
+    } else {
+      va[t][0] = ld_nt(bv + vs(b, 0)); va[t][1] = ld_nt(bv + vs(b, 1));
+    }
     const xd2* xn = reinterpret_cast<const xd2*>(x + i + o[b]);
 #pragma unroll
     for (int h = 0; h < 4; h++) xg[t][h] = xn[h];
@@ -528,16 +550,17 @@ __device__ inline double spmv_row_fp64(const Mat& A, int i, const double* __rest
   return row_blk64(A, A.bv64, i, x, A.diag[i] * xi);
 }
 
+template <bool CACHED = false>
 __device__ inline double spmv_row_f32(const Mat& A, int i, const double* __restrict__ x) {
   const double xi = x[i];
   if (A.rowflag[i]) return row_full(A, i, A.len[i], xi, x);
-  return row_blk32(A, A.bv32, i, x, A.diag[i] * xi);
+  return row_blk32<CACHED>(A, A.bv32, i, x, A.diag[i] * xi);
 }
 
 // Row c of A^T: rows without mirror neighbours as compact FP64 rows (the forward row's blocks, the
 // A^T value plane, column order), the others from the transposed map in c's slot order (from its
 // list-order copy when t >= 0 is the row's list position); then c's mirror bucket.
-template <bool MIXED = false>
+template <bool MIXED = false, bool CACHED = false>
 __device__ inline double spmv_row_transpose(const Mat& A, int c, const double* __restrict__ x, int t = -1) {
   // the bucket bounds first: their loads overlap the row's chain instead of following it
   const int pb = A.moff[c], pe = A.moff[c + 1];
@@ -547,7 +570,7 @@ __device__ inline double spmv_row_transpose(const Mat& A, int c, const double* _
                      [&](int k) { return A.xtcol[xslot(t, k)]; }, x);
   } else if (A.rowflag && !A.rowflag[c]) {
     const double d = A.diag[c] * x[c];
-    acc = MIXED ? row_blk32(A, A.bv32T, c, x, d) : row_blk64(A, A.bv64T, c, x, d);
+    acc = MIXED ? row_blk32<CACHED>(A, A.bv32T, c, x, d) : row_blk64(A, A.bv64T, c, x, d);
   } else {
     const size_t n = A.n;
     acc = gather_sum(A.diag[c] * x[c], A.len[c], [&](int k) { return A.tval[(size_t)k * n + c]; },
@@ -556,11 +579,11 @@ __device__ inline double spmv_row_transpose(const Mat& A, int c, const double* _
   return gather_sum(acc, pe - pb, [&](int q) { return A.mval[pb + q]; }, [&](int q) { return A.mrow[pb + q]; }, x);
 }
 
-template <int MODE>
+template <int MODE, bool CACHED = false>
 __device__ inline double spmv_row(const Mat& A, int i, const double* __restrict__ x) {
   if (MODE == SPMV_TRANSPOSE) return spmv_row_transpose(A, i, x);
-  if (MODE == SPMV_TRANSPOSE_MIXED) return spmv_row_transpose<true>(A, i, x);
-  return MODE == SPMV_MIXED ? spmv_row_f32(A, i, x) : spmv_row_fp64(A, i, x);
+  if (MODE == SPMV_TRANSPOSE_MIXED) return spmv_row_transpose<true, CACHED>(A, i, x);
+  return MODE == SPMV_MIXED ? spmv_row_f32<CACHED>(A, i, x) : spmv_row_fp64(A, i, x);
 }
 
 // The rows of one SpMV launch, out(i, (A x)_i) for each.  With A.nxb > 0 the first nxb workgroups
@@ -569,7 +592,7 @@ __device__ inline double spmv_row(const Mat& A, int i, const double* __restrict_
 // is a chain of ~6 dependent batches of uncoalesced loads, and inside a wavefront of compact rows
 // it held the whole wavefront (measured at the 1M cylinder, 2% of the rows: ~19 us of an 80 us
 // SpMV).  With A.nxb = 0 every row is taken in place.
-template <int MODE, class Out>
+template <int MODE, bool CACHED, class Out>
 __device__ inline void spmv_rows(const Mat& A, const double* __restrict__ x, Out&& out) {
   if (A.nxb > 0) {
     if ((int)blockIdx.x < A.nxb) {
@@ -583,13 +606,13 @@ __device__ inline void spmv_rows(const Mat& A, const double* __restrict__ x, Out
     for (int i = rr.begin + threadIdx.x; i < rr.end; i += blockDim.x) {
       if (A.rowflag[i]) continue;
       const double d = A.diag[i] * x[i];
-      if (MODE >= SPMV_TRANSPOSE) out(i, spmv_row_transpose<MODE == SPMV_TRANSPOSE_MIXED>(A, i, x));
-      else out(i, MODE == SPMV_MIXED ? row_blk32(A, A.bv32, i, x, d) : row_blk64(A, A.bv64, i, x, d));
+      if (MODE >= SPMV_TRANSPOSE) out(i, spmv_row_transpose<MODE == SPMV_TRANSPOSE_MIXED, CACHED>(A, i, x));
+      else out(i, MODE == SPMV_MIXED ? row_blk32<CACHED>(A, A.bv32, i, x, d) : row_blk64(A, A.bv64, i, x, d));
     }
     return;
   }
   const RowRange rr = block_rows(A.n, gridDim.x);
-  for (int i = rr.begin + threadIdx.x; i < rr.end; i += blockDim.x) out(i, spmv_row<MODE>(A, i, x));
+  for (int i = rr.begin + threadIdx.x; i < rr.end; i += blockDim.x) out(i, spmv_row<MODE, CACHED>(A, i, x));
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -710,7 +733,7 @@ __global__ void k_bi_pinv(int n, Vecs w, RedBufs rb, Scal* sc) {
 
 // :353-354  v = A p_inv; the terms of r_bar . v
 // (FUSED: the five-kernel iteration, which also needs the terms of sum(v))
-template <int MODE, bool FUSED = false>
+template <int MODE, bool FUSED = false, bool CACHED = false>
 __global__ void k_bi_spmv_p(Mat A, Vecs w, RedBufs rb, const Scal* sc) {
   if (finished(sc)) return;
   const int n = A.n;
@@ -723,7 +746,7 @@ __global__ void k_bi_spmv_p(Mat A, Vecs w, RedBufs rb, const Scal* sc) {
     R.add(0, i, w.rb[i] * bb);
     if (FUSED) R.add(FUSED ? 1 : 0, i, bb);
   };
-  spmv_rows<MODE>(A, x, out);
+  spmv_rows<MODE, CACHED>(A, x, out);
   R.store(rb.P(ST_SPMV_P));
 }
 
@@ -782,7 +805,7 @@ __global__ void k_bi_sinv(int n, Vecs w, RedBufs rb, Scal* sc) {
 
 // :385-387  t = A s_inv; terms of t.s and t.t with the projected s
 // (FUSED: the five-kernel iteration stores s already projected)
-template <int MODE, bool FUSED = false>
+template <int MODE, bool FUSED = false, bool CACHED = false>
 __global__ void k_bi_spmv_s(Mat A, Vecs w, RedBufs rb, const Scal* sc) {
   if (finished(sc)) return;
   const int n = A.n;
@@ -797,7 +820,7 @@ __global__ void k_bi_spmv_s(Mat A, Vecs w, RedBufs rb, const Scal* sc) {
     R.add(0, i, bb * si);
     R.add(1, i, bb * bb);
   };
-  spmv_rows<MODE>(A, x, out);
+  spmv_rows<MODE, CACHED>(A, x, out);
   R.store(rb.P(ST_SPMV_S));
 }
 
@@ -913,14 +936,15 @@ __device__ inline FinState finish_iteration(const double (&f)[3], int k, Scal* s
 }
 
 // A vector element read for the last time in the iteration (NT: non-temporal) and the solution's
-// update (NT: non-temporal store: x is read again only in the next iteration's last kernel).  NT on
-// the mixed path, whose SpMVs read the FP32 matrix with ordinary loads: streaming the vectors that
+// update (NT: non-temporal store: x is read again only in the next iteration's last kernel).  NT
+// when the mixed SpMVs read their FP32 values cacheably (solve's CACHED): streaming the vectors that
 // are not read again keeps more of the matrix and of the next kernels' vectors in the caches
-// (measured at the 1M cylinder: 141.3 against 135.3 ms per mixed step; the FP64 path, whose matrix
-// is read non-temporally, 214.2 against 215.0 ms, so it keeps ordinary accesses).  The mixed path's
-// vector kernels also load 2 * kPre rows per thread before their fold, all of a thread's rows at 1M
-// (132.1 against 135.2 ms per mixed step; FP64 215.6 against 213.8, so it keeps kPre).  256-thread
-// workgroups: the launch bound lets k_fu_xr hold its 16 rows' operands without spilling.
+// (measured at the 1M cylinder: 141.3 against 135.3 ms per mixed step; where the matrix is read
+// non-temporally, FP64 at 1M: 214.2 against 215.0 ms, and mixed at 2M: 277 against 287 ms, the
+// vectors keep ordinary accesses).  With NT the vector kernels also load 2 * kPre rows per thread
+// before their fold, all of a thread's rows at 1M (132.1 against 135.2 ms per mixed step; FP64 215.6
+// against 213.8).  256-thread workgroups: the launch bound lets k_fu_xr hold its 16 rows' operands
+// without spilling.
 template <bool NT> __device__ __forceinline__ double ld_last(const double* p) {
   if constexpr (NT) return __builtin_nontemporal_load(p);
   else return *p;
@@ -1095,7 +1119,7 @@ static void split_listed_rows(vb_ctx* c, Mat& A) {
 
 static RedBufs red_bufs(vb_ctx* c, int n) { return RedBufs{c->partials, c->red, n}; }
 
-template <int MODE>
+template <int MODE, bool CACHED = false>
 static vb_status solve(vb_ctx* c, const Mat& A, Vecs w, int x_given, double tol, int itermax) {
   const int n = A.n;
   const int nb = grid_for(n);          // streaming vector kernels
@@ -1103,7 +1127,7 @@ static vb_status solve(vb_ctx* c, const Mat& A, Vecs w, int x_given, double tol,
   RedBufs rb = red_bufs(c, n);
   Scal* sc = c->scal;
   const bool fused = rb.rows == nullptr;   // serial mode: the reference's eight-kernel order
-  constexpr bool NT = MODE == SPMV_MIXED || MODE == SPMV_TRANSPOSE_MIXED;   // see ld_last
+  constexpr bool NT = CACHED;            // see ld_last
   const double small = w.jac ? (double)(10 - 6) : 1.0e-6;
   VB_LAUNCH(c, k_bi_init, nb, A, w, x_given, rb);
   VB_FIN(c, k_fin_init, rb, nb, tol, itermax, small, sc);
@@ -1114,10 +1138,10 @@ static vb_status solve(vb_ctx* c, const Mat& A, Vecs w, int x_given, double tol,
       if (fused) {
         if (w.jac) VB_LAUNCH(c, (k_fu_p<true, NT>), nb, n, w, rb, sc, k, b > 0 ? 1 : 0);
         else VB_LAUNCH(c, (k_fu_p<false, NT>), nb, n, w, rb, sc, k, b > 0 ? 1 : 0);
-        VB_LAUNCH_BS(c, (k_bi_spmv_p<MODE, true>), ns, kSpmvBlock, A, w, rb, sc);
+        VB_LAUNCH_BS(c, (k_bi_spmv_p<MODE, true, CACHED>), ns, kSpmvBlock, A, w, rb, sc);
         if (w.jac) VB_LAUNCH(c, (k_fu_s<true, NT>), nb, n, w, rb, sc, ns, k);
         else VB_LAUNCH(c, (k_fu_s<false, NT>), nb, n, w, rb, sc, ns, k);
-        VB_LAUNCH_BS(c, (k_bi_spmv_s<MODE, true>), ns, kSpmvBlock, A, w, rb, sc);
+        VB_LAUNCH_BS(c, (k_bi_spmv_s<MODE, true, CACHED>), ns, kSpmvBlock, A, w, rb, sc);
         if (w.jac) VB_LAUNCH(c, (k_fu_xr<true, NT>), nb, n, w, rb, sc, ns, k);
         else VB_LAUNCH(c, (k_fu_xr<false, NT>), nb, n, w, rb, sc, ns, k);
         continue;
@@ -1171,7 +1195,13 @@ vb_status assemble_ppe(vb_ctx* c) {
   vb_status ss;
   if ((ss = launch_exclusive_scan(c, c->xflag, c->xscan, c->npfb)) != VB_OK) return ss;
   VB_LAUNCH(c, k_list_explicit, grid_1d(c->npfb), c->npfb, c->rowflag, c->xscan, c->xrows, c->scal);
+  hipLaunchKernelGGL(k_count_blocks, dim3(1), dim3(kBlock), 0, c->stream, (int)ppe_chunks(c->npfb), c->chunkb,
+                     c->scal);
+  VB_HIP(c, hipGetLastError());
   if ((ss = sync_scalars(c)) != VB_OK) return ss;
+  // the mixed SpMVs' FP32 value plane as read: (blocks) x (64 lanes) x (8 columns) x 4 bytes
+  c->vcache = c->prm.precision == VB_PREC_MIXED &&
+              (double)c->scal_host->nblocks * kChunk * kBlkCols * sizeof(float) <= kCachedValueBytes;
   const int nx = c->scal_host->nexplicit;
   if (nx > 0) {
     sw = with_weights(c, [&](auto*, auto*, auto* wl) -> vb_status {
@@ -1195,7 +1225,8 @@ vb_status launch_ppe(vb_ctx* c, double tol, int itermax) {
   // FP64 from the stored rows in the reference's slot order: on request, and in serial mode
   if ((c->spmv_stored || c->red) && c->prm.precision != VB_PREC_MIXED) A.rowflag = nullptr;
   if (!c->red) split_listed_rows(c, A);
-  if (c->prm.precision == VB_PREC_MIXED) return solve<SPMV_MIXED>(c, A, w, 0, tol, itermax);
+  if (c->prm.precision == VB_PREC_MIXED)
+    return c->vcache ? solve<SPMV_MIXED, true>(c, A, w, 0, tol, itermax) : solve<SPMV_MIXED>(c, A, w, 0, tol, itermax);
   return solve<SPMV_FP64>(c, A, w, 0, tol, itermax);
 }
 
@@ -1344,8 +1375,9 @@ vb_status adjoint_solve(vb_ctx* c, double tol, int itermax, int* iters, double* 
     VB_LAUNCH(c, k_ref_scale, grid_1d(n), n, op.rref, op.refs, op.bref);
     // the inner solve reads the compact rows' A^T values as FP32 on the mixed path (exact there); the
     // refinement residual above is FP64
-    s = c->prm.precision == VB_PREC_MIXED ? solve<SPMV_TRANSPOSE_MIXED>(c, As, w, 0, inner, itermax - total)
-                                          : solve<SPMV_TRANSPOSE>(c, As, w, 0, inner, itermax - total);
+    s = c->prm.precision != VB_PREC_MIXED ? solve<SPMV_TRANSPOSE>(c, As, w, 0, inner, itermax - total)
+        : c->vcache ? solve<SPMV_TRANSPOSE_MIXED, true>(c, As, w, 0, inner, itermax - total)
+                    : solve<SPMV_TRANSPOSE_MIXED>(c, As, w, 0, inner, itermax - total);
     if (s != VB_OK) return s;
     total += std::min(c->scal_host->iter, itermax - total);
     if (c->scal_host->status == VB_WARN_BICGSTAB_BREAKDOWN) st = VB_WARN_BICGSTAB_BREAKDOWN;
@@ -1433,7 +1465,8 @@ vb_status profile_spmv(vb_ctx* c, int reps, double* avg_us, double* alg_bytes) {
     if (r == 1) VB_HIP(c, hipEventRecord(e0, c->stream));
     RedBufs rb{c->partials, nullptr, n};
     if (fused) {                     // the kernel the solve launches (five-kernel iteration)
-      if (mixed) VB_LAUNCH_BS(c, (k_bi_spmv_p<SPMV_MIXED, true>), nb, kSpmvBlock, A, w, rb, c->scal);
+      if (mixed && c->vcache) VB_LAUNCH_BS(c, (k_bi_spmv_p<SPMV_MIXED, true, true>), nb, kSpmvBlock, A, w, rb, c->scal);
+      else if (mixed) VB_LAUNCH_BS(c, (k_bi_spmv_p<SPMV_MIXED, true>), nb, kSpmvBlock, A, w, rb, c->scal);
       else VB_LAUNCH_BS(c, (k_bi_spmv_p<SPMV_FP64, true>), nb, kSpmvBlock, A, w, rb, c->scal);
     } else {
       if (mixed) VB_LAUNCH_BS(c, k_bi_spmv_p<SPMV_MIXED>, nb, kSpmvBlock, A, w, rb, c->scal);
