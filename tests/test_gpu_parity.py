@@ -229,6 +229,37 @@ def test_renumbered_particles_mixed(gpu_lib, mix):
     assert np.linalg.norm(r - r.mean()) <= 1e-5 * np.linalg.norm(rhs)
 
 
+def test_mixed_value_stream_hints_change_no_bits(gpu_lib, monkeypatch):
+    """The mixed solve has two cache policies: FP32 values read cacheably when the value plane fits
+    the Infinity Cache (every small case here), the vector kernels then streaming their last reads
+    and loading all of a thread's rows before their fold; or every value streamed, which large
+    matrices get (forced here with VB_SPMV=stream).  They differ in load hints and in when the same
+    rows are loaded, not in what is summed or in which order: three steps and one adjoint PPE solve
+    on the cylinder give the same bits."""
+    from vebisph_amd import _lib
+    case = SMALL["cylinder"]()
+
+    def run():
+        G = solver.Solver(case, precision=_lib.PREC_MIXED)
+        for _ in range(3):
+            G.div_free()
+        st, sc = G.get_state(), G.scalars()
+        for k in range(1, 11):
+            G.phase(k)
+        g = np.random.default_rng(5).standard_normal(case.npfb)
+        lam, it, res, status = G.ppe_adjoint_solve(g)
+        return st, sc, lam, it
+
+    a, sa, la, ia = run()
+    monkeypatch.setenv("VB_SPMV", "stream")
+    b, sb, lb, ib = run()
+    for key in a:
+        _eq(a[key], b[key])
+    assert sa["LS_iters"] == sb["LS_iters"] and sa["LS_residual"] == sb["LS_residual"]
+    _eq(la, lb)
+    assert ia == ib
+
+
 @pytest.mark.parametrize("name", list(SMALL))
 def test_bicgstab_bit_exact_in_serial_order(gpu_lib, name):
     """The GPU BiCGSTAB on the oracle's assembled PPE, with its global sums taken in index order (the

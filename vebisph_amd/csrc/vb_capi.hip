@@ -209,6 +209,9 @@ vb_status vb_create(const vb_params* prm, const vb_boundary* bnd, int32_t npfb, 
   // VB_SPMV=stored: every FP64 SpMV row from the stored matrix (measurement switch; same results).
   const char* sp = getenv("VB_SPMV");
   c->spmv_stored = sp && strcmp(sp, "stored") == 0;
+  // VB_SPMV=stream: the mixed SpMVs stream their FP32 values whatever the plane's size (measurement
+  // switch; the same results, bit for bit)
+  c->spmv_stream = sp && strcmp(sp, "stream") == 0;
   if (prm->precision == VB_PREC_MIXED) A(c->bv32, ppe_block_values(npfb));
   A(c->r, n); A(c->rb, n); A(c->vv, n); A(c->pn, n); A(c->pinv, n); A(c->ss, n); A(c->sinv, n);
   A(c->tt, n); A(c->xsys, n);

@@ -81,6 +81,7 @@ struct vb_ctx {
   uint8_t* bpos;                     //   block position of each neighbour slot (for A^T)
 
   bool spmv_stored;                  // FP64 path: full stored rows everywhere (VB_SPMV=stored)
+  bool spmv_stream = false;          // mixed path: FP32 values always streamed (VB_SPMV=stream)
   bool vcache = false;               // mixed path: FP32 value plane read cacheably (kCachedValueBytes)
   double *r, *rb, *vv, *pn, *pinv, *ss, *sinv, *tt, *xsys;
 

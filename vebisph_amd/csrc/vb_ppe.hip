@@ -1200,7 +1200,7 @@ vb_status assemble_ppe(vb_ctx* c) {
   VB_HIP(c, hipGetLastError());
   if ((ss = sync_scalars(c)) != VB_OK) return ss;
   // the mixed SpMVs' FP32 value plane as read: (blocks) x (64 lanes) x (8 columns) x 4 bytes
-  c->vcache = c->prm.precision == VB_PREC_MIXED &&
+  c->vcache = c->prm.precision == VB_PREC_MIXED && !c->spmv_stream &&
               (double)c->scal_host->nblocks * kChunk * kBlkCols * sizeof(float) <= kCachedValueBytes;
   const int nx = c->scal_host->nexplicit;
   if (nx > 0) {
